@@ -1,0 +1,1354 @@
+// C ABI of libbaorec_hip: context, device buffers and the reconstruction
+// pipelines built from the kernels in fft.hip, mas.hip and mg.hip.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/baorec.h"
+#include "kernels.hpp"
+
+using namespace baorec;
+
+namespace baorec {
+hipError_t launch_scatter_accum(const ScatterArgs &a, int accumulate, hipStream_t s);
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct baorec_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, DevBuf> bufs;
+    // recon.result_cache
+    int has_result = 0;
+    int result_kind = 0;
+    int32_t result_grid[3] = {0, 0, 0};
+    uint64_t result_gen = 0;
+    // displacement grid cached for the result (Psi_x, Psi_y, Psi_z, 0)
+    uint64_t disp_gen = 0;
+    float disp_box[6] = {0, 0, 0, 0, 0, 0};
+    int disp_valid = 0;
+};
+
+static thread_local std::string g_create_err;
+
+namespace {
+
+struct Fail {
+    int code;
+};
+
+[[noreturn]] void fail(baorec_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    throw Fail{code};
+}
+
+void hipchk(baorec_ctx *c, hipError_t e, const char *what) {
+    if (e == hipSuccess) return;
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+                         ? BAOREC_ERR_OUT_OF_MEMORY
+                         : BAOREC_ERR_HIP;
+    fail(c, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void *buf(baorec_ctx *c, const std::string &name, size_t bytes) {
+    DevBuf &b = c->bufs[name];
+    if (b.bytes < bytes) {
+        if (b.p) {
+            hipchk(c, hipStreamSynchronize(c->stream), "sync before realloc");
+            (void)hipFree(b.p);
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        void *p = nullptr;
+        hipchk(c, hipMalloc(&p, bytes < 256 ? 256 : bytes), ("hipMalloc " + name).c_str());
+        b.p = p;
+        b.bytes = bytes;
+    }
+    return b.p;
+}
+
+template <typename T>
+T *bufT(baorec_ctx *c, const std::string &name, size_t count) {
+    return static_cast<T *>(buf(c, name, sizeof(T) * count));
+}
+
+struct Grid {
+    int nx, ny, nz, P;
+    int64_t R() const { return (int64_t)nx * ny * nz; }
+    int64_t C() const { return (int64_t)P * ny * nz; }
+};
+
+Grid check_grid(baorec_ctx *c, const int32_t *g, bool fft) {
+    if (!g) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "grid is NULL");
+    Grid G{g[0], g[1], g[2], g[0] / 2 + 1};
+    for (int a = 0; a < 3; a++) {
+        if (g[a] <= 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "grid sides must be positive");
+        if (fft && !fft_side_ok(g[a]))
+            fail(c, BAOREC_ERR_UNSUPPORTED,
+                 "grid sides must be powers of two in [16, 1024] (got " + std::to_string(g[a]) + ")");
+    }
+    return G;
+}
+
+void check_params(baorec_ctx *c, const baorec_params *p) {
+    if (!p) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "params is NULL");
+    if (p->mas != BAOREC_MAS_CIC && p->mas != BAOREC_MAS_TSC)
+        fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mass-assignment scheme");
+}
+
+void check_box(baorec_ctx *c, const baorec_params *p) {
+    for (int a = 0; a < 3; a++)
+        if (!(p->box_size[a] > 0.f) || !isfinite(p->box_min[a]))
+            fail(c, BAOREC_ERR_INVALID_ARGUMENT, "box_size/box_min not set");
+}
+
+// Input array: device pointer used as is, host pointer staged into a named
+// device buffer.
+const float *stage_in(baorec_ctx *c, int mem, const float *src, int64_t n, const char *name) {
+    if (n == 0) return nullptr;
+    if (!src) fail(c, BAOREC_ERR_INVALID_ARGUMENT, std::string(name) + " is NULL");
+    if (mem == BAOREC_MEM_DEVICE) return src;
+    float *d = bufT<float>(c, name, n);
+    hipchk(c, hipMemcpyAsync(d, src, sizeof(float) * n, hipMemcpyHostToDevice, c->stream), "H2D");
+    return d;
+}
+
+float *stage_out(baorec_ctx *c, int mem, float *dst, int64_t n, const char *name) {
+    if (n == 0) return nullptr;
+    if (!dst) fail(c, BAOREC_ERR_INVALID_ARGUMENT, std::string(name) + " is NULL");
+    if (mem == BAOREC_MEM_DEVICE) return dst;
+    return bufT<float>(c, name, n);
+}
+
+void copy_out(baorec_ctx *c, int mem, float *host, const float *dev, int64_t n) {
+    if (mem == BAOREC_MEM_DEVICE || n == 0) return;
+    hipchk(c, hipMemcpyAsync(host, dev, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "D2H");
+}
+
+void sync(baorec_ctx *c) { hipchk(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
+
+// ----------------------------------------------------------- device scalars
+
+__global__ void k_slot_set(Slot *s, float b0, float b1, float b2, float m0, float m1, float m2,
+                           double count_r) {
+    if (threadIdx.x != 0) return;
+    s->box_size[0] = b0;
+    s->box_size[1] = b1;
+    s->box_size[2] = b2;
+    s->box_min[0] = m0;
+    s->box_min[1] = m1;
+    s->box_min[2] = m2;
+    s->sum[0] = 0.0;
+    s->sum[1] = 0.0;
+    s->count_r = count_r;
+    s->err = 0;
+}
+
+__global__ void k_slot_reset(Slot *s, double count_r) {
+    if (threadIdx.x != 0) return;
+    s->sum[0] = 0.0;
+    s->sum[1] = 0.0;
+    s->count_r = count_r;
+    s->err = 0;
+}
+
+Slot *slot_buf(baorec_ctx *c) { return bufT<Slot>(c, "slot", 4); }
+
+void slot_from_params(baorec_ctx *c, Slot *s, const baorec_params *p, double count_r) {
+    hipLaunchKernelGGL(k_slot_set, dim3(1), dim3(64), 0, c->stream, s, p->box_size[0],
+                       p->box_size[1], p->box_size[2], p->box_min[0], p->box_min[1], p->box_min[2],
+                       count_r);
+    hipchk(c, hipGetLastError(), "slot_set");
+}
+
+Slot read_slot(baorec_ctx *c, const Slot *s) {
+    Slot h;
+    hipchk(c, hipMemcpyAsync(&h, s, sizeof(Slot), hipMemcpyDeviceToHost, c->stream), "D2H slot");
+    sync(c);
+    return h;
+}
+
+// ----------------------------------------------------------------- scatter
+
+void tiles_for(const Grid &G, ScatterArgs &a) {
+    a.tx = G.nx < 32 ? G.nx : 32;
+    a.ty = G.ny < 16 ? G.ny : 16;
+    a.tz = G.nz < 16 ? G.nz : 16;
+    a.ntx = (G.nx + a.tx - 1) / a.tx;
+    a.nty = (G.ny + a.ty - 1) / a.ty;
+    a.ntz = (G.nz + a.tz - 1) / a.tz;
+}
+
+// cic! into rho (overwrite unless accumulate) for one catalog.
+void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, float *rho, float *x,
+             float *y, float *z, const float *w, int64_t n, int wrap, int accumulate,
+             const char *tag) {
+    ScatterArgs a{};
+    a.x = x;
+    a.y = y;
+    a.z = z;
+    a.w = w;
+    a.offsets = nullptr;
+    a.ncat = 1;
+    a.ntotal = n;
+    a.slot = slot;
+    a.slotStride = 0;
+    a.nx = G.nx;
+    a.ny = G.ny;
+    a.nz = G.nz;
+    a.wrap = wrap;
+    a.mutate = p->mutate_inputs;
+    a.tsc = p->mas == BAOREC_MAS_TSC;
+    tiles_for(G, a);
+    const int64_t ntiles = (int64_t)a.ntx * a.nty * a.ntz;
+    a.count = bufT<uint32_t>(c, "sc_count", ntiles);
+    a.cursor = bufT<uint32_t>(c, "sc_cursor", ntiles);
+    a.sorted = bufT<float4>(c, "sc_sorted", scatter_capacity(n > 0 ? n : 1, a.tsc));
+    a.err = &slot->err;
+    a.rho = rho;
+    a.rstride = G.R();
+    (void)tag;
+    hipchk(c, launch_scatter(a, nullptr, c->stream), "scatter");
+    hipchk(c, launch_scatter_accum(a, accumulate, c->stream), "scatter accumulate");
+}
+
+// ------------------------------------------------------------------ FFT
+
+struct FFTBufs {
+    float2 *cw;   // 11 complex grids
+    float *ktab, *xtab;
+};
+
+FFTBufs fft_bufs(baorec_ctx *c, const Grid &G) {
+    FFTBufs f;
+    f.cw = bufT<float2>(c, "cwork", (size_t)11 * G.C());
+    f.ktab = bufT<float>(c, "ktab", (size_t)4 * (G.P + G.ny + G.nz));
+    f.xtab = bufT<float>(c, "xtab", (size_t)4 * (G.nx + G.ny + G.nz));
+    return f;
+}
+
+void tables(baorec_ctx *c, const Grid &G, const Slot *slot, FFTBufs &f) {
+    hipchk(c, launch_tables(slot, f.ktab, f.xtab, G.nx, G.ny, G.nz, 1, c->stream), "tables");
+}
+
+StridedArgs sargs(const Grid &G, const FFTBufs &f, int axis, int mode, int mul) {
+    StridedArgs a{};
+    a.cstride = G.C();
+    a.ktab = f.ktab;
+    a.ktabStride = 0;
+    a.P = G.P;
+    a.ny = G.ny;
+    a.nz = G.nz;
+    a.axis = axis;
+    a.mode = mode;
+    a.mul = mul;
+    return a;
+}
+
+void x_fwd(baorec_ctx *c, const Grid &G, const float *in, float2 *out, int nbatch, double *sums,
+           int64_t sumStride) {
+    XFwdArgs a{};
+    a.in = in;
+    a.out = out;
+    a.rstride = G.R();
+    a.cstride = G.C();
+    a.nx = G.nx;
+    a.ny = G.ny;
+    a.nz = G.nz;
+    a.sums = sums;
+    a.sumStride = sumStride;
+    hipchk(c, launch_x_r2c(a, nbatch, c->stream), "x r2c");
+}
+
+void strided_plain(baorec_ctx *c, const Grid &G, const FFTBufs &f, int axis, bool inverse,
+                   float2 *data, int nbatch) {
+    StridedArgs a = sargs(G, f, axis, inverse ? 1 : 0, MUL_NONE);
+    a.job[0].in = data;
+    a.job[0].out[0] = data;
+    a.job[0].nout = 1;
+    a.job[0].pw[0] = 0;
+    hipchk(c, launch_strided(a, 1, nbatch, c->stream), inverse ? "strided inverse" : "strided fwd");
+}
+
+XInvArgs xargs(const Grid &G, const FFTBufs &f, const Slot *slot) {
+    XInvArgs a{};
+    a.cstride = G.C();
+    a.rstride = G.R();
+    a.nx = G.nx;
+    a.ny = G.ny;
+    a.nz = G.nz;
+    a.scale = (float)(1.0 / ((double)G.nx * G.ny * G.nz));
+    a.ktab = f.ktab;
+    a.ktabStride = 0;
+    a.xtab = f.xtab;
+    a.xtabStride = 0;
+    a.slot = slot;
+    return a;
+}
+
+// smooth! on nb contiguous real grids starting at rin, result in cw[0..nb)
+// (complex, fully transformed back along y and z; x inverse left to caller).
+void smooth_to_half(baorec_ctx *c, const Grid &G, FFTBufs &f, const float *rin, int nb, double *sums,
+                    float R) {
+    x_fwd(c, G, rin, f.cw, nb, sums, 1);
+    strided_plain(c, G, f, 1, false, f.cw, nb);
+    StridedArgs a = sargs(G, f, 2, 2, MUL_GAUSS);
+    a.R2 = R * R;
+    a.job[0].in = f.cw;
+    a.job[0].out[0] = f.cw;
+    a.job[0].nout = 1;
+    a.job[0].pw[0] = 0;
+    hipchk(c, launch_strided(a, 1, nb, c->stream), "z smooth");
+    strided_plain(c, G, f, 1, true, f.cw, nb);
+}
+
+// setup_overdensity! (src/recon.jl:42-91) -> delta (device).
+void overdensity(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, float *delta,
+                 float *x, float *y, float *z, const float *w, int64_t n, float *rx, float *ry,
+                 float *rz, const float *rw, int64_t nr) {
+    FFTBufs f = fft_bufs(c, G);
+    tables(c, G, slot, f);
+    float *rho = bufT<float>(c, "rho2", (size_t)2 * G.R());
+    const bool rand = nr > 0;
+    scatter(c, G, p, slot, rho, x, y, z, w, n, rand ? 0 : 1, 0, "data");
+    if (rand) scatter(c, G, p, slot, rho + G.R(), rx, ry, rz, rw, nr, 0, 0, "randoms");
+    smooth_to_half(c, G, f, rho, rand ? 2 : 1, slot->sum, p->smoothing_radius);
+    XInvArgs a = xargs(G, f, slot);
+    a.in[0] = f.cw;
+    a.kxpow[0] = 0;
+    a.nin = 1;
+    a.bias = p->bias;
+    a.ran_min = p->ran_min;
+    a.out = delta;
+    if (rand) {
+        a.in[1] = f.cw + G.C();
+        a.kxpow[1] = 0;
+        a.nin = 2;
+    }
+    hipchk(c, launch_x_c2r(a, rand ? XE_OD_RAND : XE_OD_BOX, 1, c->stream), "x overdensity");
+}
+
+// n_iter x iterate! (src/iterative.jl:151-211): delta_s -> delta_r
+void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const float *ds,
+                 float *dr) {
+    FFTBufs f = fft_bufs(c, G);
+    const int64_t C = G.C();
+    float2 *C0 = f.cw, *Gb = f.cw + 2 * C, *Hb = f.cw + 5 * C;
+    if (p->n_iter <= 0) {
+        hipchk(c, hipMemcpyAsync(dr, ds, sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream),
+               "copy");
+        return;
+    }
+    const float beta = p->beta;
+    for (int it = 1; it <= p->n_iter; it++) {
+        x_fwd(c, G, it == 1 ? ds : dr, C0, 1, nullptr, 0);
+        strided_plain(c, G, f, 1, false, C0, 1);
+        if (!p->has_los) {
+            StridedArgs z = sargs(G, f, 2, 2, MUL_INVK2);
+            z.job[0].in = C0;
+            for (int m = 0; m < 3; m++) {
+                z.job[0].out[m] = Gb + m * C;
+                z.job[0].pw[m] = m;
+            }
+            z.job[0].nout = 3;
+            hipchk(c, launch_strided(z, 1, 1, c->stream), "z iterate");
+            // y inverse: G0 -> H00,H10,H20 ; G1 -> H01,H11 ; G2 -> H02
+            // H slots: 0:H00 1:H10 2:H01 3:H20 4:H11 5:H02 (x pass order)
+            StridedArgs y = sargs(G, f, 1, 1, MUL_NONE);
+            y.job[0].in = Gb;
+            y.job[0].nout = 3;
+            y.job[0].out[0] = Hb + 0 * C; y.job[0].pw[0] = 0;
+            y.job[0].out[1] = Hb + 1 * C; y.job[0].pw[1] = 1;
+            y.job[0].out[2] = Hb + 3 * C; y.job[0].pw[2] = 2;
+            y.job[1].in = Gb + C;
+            y.job[1].nout = 2;
+            y.job[1].out[0] = Hb + 2 * C; y.job[1].pw[0] = 0;
+            y.job[1].out[1] = Hb + 4 * C; y.job[1].pw[1] = 1;
+            y.job[2].in = Gb + 2 * C;
+            y.job[2].nout = 1;
+            y.job[2].out[0] = Hb + 5 * C; y.job[2].pw[0] = 0;
+            hipchk(c, launch_strided(y, 3, 1, c->stream), "y iterate");
+            XInvArgs a = xargs(G, f, slot);
+            const int kxp[6] = {2, 1, 1, 0, 0, 0};
+            const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
+            for (int q = 0; q < 6; q++) {
+                a.in[q] = Hb + q * C;
+                a.kxpow[q] = kxp[q];
+                // factor::T = (1. + T(i != j)) * beta  (src/iterative.jl:23-24)
+                float fac = (float)((1.0 + (double)(float)(pi[q] != pj[q])) * (double)beta);
+                if (it == 1) fac = fac / (1.0f + beta);
+                a.fac[q] = fac;
+            }
+            a.nin = 6;
+            a.aux = ds;
+            a.out = dr;
+            hipchk(c, launch_x_c2r(a, XE_ITER_LOCAL, 1, c->stream), "x iterate");
+        } else {
+            StridedArgs z = sargs(G, f, 2, 2, MUL_FIXEDLOS);
+            z.los[0] = p->los[0];
+            z.los[1] = p->los[1];
+            z.los[2] = p->los[2];
+            z.job[0].in = C0;
+            z.job[0].out[0] = C0;
+            z.job[0].pw[0] = 0;
+            z.job[0].nout = 1;
+            hipchk(c, launch_strided(z, 1, 1, c->stream), "z iterate fixed");
+            strided_plain(c, G, f, 1, true, C0, 1);
+            XInvArgs a = xargs(G, f, slot);
+            a.in[0] = C0;
+            a.kxpow[0] = 0;
+            a.nin = 1;
+            float fac = beta;
+            if (it == 1) fac = fac / (1.0f + beta);
+            a.fac[0] = fac;
+            a.aux = ds;
+            a.out = dr;
+            hipchk(c, launch_x_c2r(a, XE_ITER_FIXED, 1, c->stream), "x iterate fixed");
+        }
+    }
+}
+
+// compute_displacements grids: float4 (Psi_x, Psi_y, Psi_z, 0)
+void displacement_grid(baorec_ctx *c, const Grid &G, int kind, Slot *slot, const float *field,
+                       float4 *disp) {
+    FFTBufs f = fft_bufs(c, G);
+    tables(c, G, slot, f);
+    const int64_t C = G.C();
+    float2 *C0 = f.cw, *Gb = f.cw + 2 * C, *Hb = f.cw + 5 * C;
+    x_fwd(c, G, field, C0, 1, nullptr, 0);
+    strided_plain(c, G, f, 1, false, C0, 1);
+    StridedArgs z = sargs(G, f, 2, 2, kind == BAOREC_MULTIGRID ? MUL_I : MUL_I_INVK2);
+    z.job[0].in = C0;
+    z.job[0].out[0] = Gb;
+    z.job[0].pw[0] = 0;
+    z.job[0].out[1] = Gb + C;
+    z.job[0].pw[1] = 1;
+    z.job[0].nout = 2;
+    hipchk(c, launch_strided(z, 1, 1, c->stream), "z displacement");
+    StridedArgs y = sargs(G, f, 1, 1, MUL_NONE);
+    y.job[0].in = Gb;
+    y.job[0].nout = 2;
+    y.job[0].out[0] = Hb; y.job[0].pw[0] = 0;
+    y.job[0].out[1] = Hb + C; y.job[0].pw[1] = 1;
+    y.job[1].in = Gb + C;
+    y.job[1].nout = 1;
+    y.job[1].out[0] = Hb + 2 * C; y.job[1].pw[0] = 0;
+    hipchk(c, launch_strided(y, 2, 1, c->stream), "y displacement");
+    XInvArgs a = xargs(G, f, slot);
+    a.in[0] = Hb; a.kxpow[0] = 1;
+    a.in[1] = Hb + C; a.kxpow[1] = 0;
+    a.in[2] = Hb + 2 * C; a.kxpow[2] = 0;
+    a.nin = 3;
+    a.out = reinterpret_cast<float *>(disp);
+    hipchk(c, launch_x_c2r(a, XE_DISP, 1, c->stream), "x displacement");
+}
+
+// ---------------------------------------------------------------- multigrid
+
+struct MGPyr {
+    std::vector<Grid> g;
+    std::vector<float *> A, B, F;      // FP32 levels
+    std::vector<double *> Ad, Bd, Fd;  // FP64 levels (RBGS coarse)
+    std::vector<MGLevel> L;
+    std::vector<int> cur;              // 0: A holds v, 1: B holds v
+    std::vector<int> dbl;
+};
+
+bool mg_recurse(const Grid &G) {
+    return G.nx > 4 && G.nx % 2 == 0 && G.ny > 4 && G.ny % 2 == 0 && G.nz > 4 && G.nz % 2 == 0;
+}
+
+MGPyr mg_build(baorec_ctx *c, const Grid &G0, const baorec_params *p, Slot *slot, float *v0,
+               const float *f0, int fp64_max) {
+    MGPyr P;
+    Grid g = G0;
+    int l = 0;
+    while (true) {
+        P.g.push_back(g);
+        if (!mg_recurse(g)) break;
+        g = Grid{g.nx / 2, g.ny / 2, g.nz / 2, 0};
+        l++;
+    }
+    const int nl = (int)P.g.size();
+    size_t xvtot = 0;
+    for (auto &q : P.g) xvtot += q.nx + q.ny + q.nz;
+    float *xv = bufT<float>(c, "mg_xv", xvtot);
+    size_t off = 0;
+    for (int i = 0; i < nl; i++) {
+        const Grid &q = P.g[i];
+        const bool d = i > 0 && q.nx <= fp64_max && q.ny <= fp64_max && q.nz <= fp64_max;
+        P.dbl.push_back(d);
+        std::string s = std::to_string(i);
+        if (i == 0) {
+            P.A.push_back(v0);
+            P.B.push_back(bufT<float>(c, "mg_B0", q.R()));
+            P.F.push_back(const_cast<float *>(f0));
+        } else if (!d) {
+            P.A.push_back(bufT<float>(c, "mg_A" + s, q.R()));
+            P.B.push_back(bufT<float>(c, "mg_B" + s, q.R()));
+            P.F.push_back(bufT<float>(c, "mg_F" + s, q.R()));
+        } else {
+            P.A.push_back(nullptr);
+            P.B.push_back(nullptr);
+            P.F.push_back(nullptr);
+        }
+        if (d) {
+            P.Ad.push_back(bufT<double>(c, "mg_Ad" + s, q.R()));
+            P.Bd.push_back(bufT<double>(c, "mg_Bd" + s, q.R()));
+            P.Fd.push_back(bufT<double>(c, "mg_Fd" + s, q.R()));
+        } else {
+            P.Ad.push_back(nullptr);
+            P.Bd.push_back(nullptr);
+            P.Fd.push_back(nullptr);
+        }
+        MGLevel L{};
+        L.n[0] = q.nx;
+        L.n[1] = q.ny;
+        L.n[2] = q.nz;
+        for (int a = 0; a < 3; a++) {
+            // src/multigrid.jl:196-199 (FP32 arithmetic)
+            const float cell = p->box_size[a] / (float)L.n[a];
+            L.cell[a] = cell;
+            L.cell2[a] = cell * cell;
+            L.icell2[a] = 1.0f / L.cell2[a];
+            L.losn[a] = p->has_los ? p->los[a] / cell : 0.f;
+        }
+        L.xv = xv + off;
+        hipchk(c, launch_mg_xvec(xv + off, slot, q.nx, q.ny, q.nz, c->stream), "mg xvec");
+        off += q.nx + q.ny + q.nz;
+        P.L.push_back(L);
+        P.cur.push_back(0);
+    }
+    return P;
+}
+
+float *vcur(MGPyr &P, int l) { return P.cur[l] ? P.B[l] : P.A[l]; }
+float *vother(MGPyr &P, int l) { return P.cur[l] ? P.A[l] : P.B[l]; }
+double *vcurd(MGPyr &P, int l) { return P.cur[l] ? P.Bd[l] : P.Ad[l]; }
+double *votherd(MGPyr &P, int l) { return P.cur[l] ? P.Ad[l] : P.Bd[l]; }
+
+void mg_jacobi_n(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int n) {
+    for (int i = 0; i < n; i++) {
+        hipchk(c, launch_mg_jacobi(vother(P, l), vcur(P, l), P.F[l], P.L[l], p->beta,
+                                   p->jacobi_damping_factor, p->has_los, c->stream),
+               "jacobi");
+        P.cur[l] ^= 1;
+    }
+}
+
+void zero_level(baorec_ctx *c, MGPyr &P, int l) {
+    const int64_t n = P.g[l].R();
+    if (P.dbl[l]) {
+        hipchk(c, hipMemsetAsync(vcurd(P, l), 0, sizeof(double) * n, c->stream), "memset");
+    } else {
+        hipchk(c, launch_fill(vcur(P, l), n, 0.f, c->stream), "fill");
+    }
+}
+
+// vcycle! (src/multigrid.jl:689-719), reference schedule, FP32
+void mg_vcycle_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
+    mg_jacobi_n(c, P, l, p, p->jacobi_niterations);
+    if (l + 1 < (int)P.g.size()) {
+        float *r = vother(P, l);
+        hipchk(c, launch_mg_residual(r, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
+               "residual");
+        const int fn[3] = {P.g[l].nx, P.g[l].ny, P.g[l].nz};
+        hipchk(c, launch_mg_reduce(P.F[l + 1], r, fn, c->stream), "reduce");
+        zero_level(c, P, l + 1);
+        mg_vcycle_ref(c, P, l + 1, p);
+        const int cn[3] = {P.g[l + 1].nx, P.g[l + 1].ny, P.g[l + 1].nz};
+        hipchk(c, launch_mg_prolong(vcur(P, l), vcur(P, l + 1), cn, 1, c->stream), "prolong");
+    }
+    mg_jacobi_n(c, P, l, p, p->jacobi_niterations);
+}
+
+// fmg (src/multigrid.jl:722-752), reference schedule
+void mg_fmg_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
+    if (l + 1 < (int)P.g.size()) {
+        const int fn[3] = {P.g[l].nx, P.g[l].ny, P.g[l].nz};
+        hipchk(c, launch_mg_reduce(P.F[l + 1], P.F[l], fn, c->stream), "reduce");
+        mg_fmg_ref(c, P, l + 1, p);
+        const int cn[3] = {P.g[l + 1].nx, P.g[l + 1].ny, P.g[l + 1].nz};
+        hipchk(c, launch_mg_prolong(vcur(P, l), vcur(P, l + 1), cn, 0, c->stream), "prolong");
+    } else {
+        zero_level(c, P, l);
+    }
+    for (int i = 0; i < p->vcycle_niterations; i++) mg_vcycle_ref(c, P, l, p);
+}
+
+// Multicolour Gauss-Seidel V-cycle (RBGS schedule) with FP64 coarse levels.
+void mg_smooth_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int sweeps) {
+    for (int s = 0; s < sweeps; s++)
+        for (int col = 0; col < 4; col++) {
+            if (P.dbl[l])
+                hipchk(c, launch_mg_rbgs<double>(vcurd(P, l), P.Fd[l], P.L[l], p->beta, p->has_los, col,
+                                                 c->stream),
+                       "mcgs64");
+            else
+                hipchk(c, launch_mg_rbgs<float>(vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, col,
+                                                c->stream),
+                       "mcgs");
+        }
+}
+
+void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
+    const int nl = (int)P.g.size();
+    const bool coarsest = l + 1 >= nl;
+    mg_smooth_rb(c, P, l, p, coarsest ? 16 * p->rb_sweeps : p->rb_sweeps);
+    if (!coarsest) {
+        const int fn[3] = {P.g[l].nx, P.g[l].ny, P.g[l].nz};
+        const int cn[3] = {P.g[l + 1].nx, P.g[l + 1].ny, P.g[l + 1].nz};
+        if (P.dbl[l]) {
+            double *r = votherd(P, l);
+            hipchk(c, launch_mg_residual_t<double>(r, vcurd(P, l), P.Fd[l], P.L[l], p->beta, p->has_los,
+                                                   c->stream), "residual64");
+            hipchk(c, launch_mg_reduce_t<double, double>(P.Fd[l + 1], r, fn, c->stream), "reduce64");
+        } else {
+            float *r = vother(P, l);
+            hipchk(c, launch_mg_residual(r, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
+                   "residual");
+            if (P.dbl[l + 1])
+                hipchk(c, launch_mg_reduce_t<double, float>(P.Fd[l + 1], r, fn, c->stream), "reduce");
+            else
+                hipchk(c, launch_mg_reduce(P.F[l + 1], r, fn, c->stream), "reduce");
+        }
+        zero_level(c, P, l + 1);
+        mg_vcycle_rb(c, P, l + 1, p);
+        if (P.dbl[l])
+            hipchk(c, launch_mg_prolong_t<double, double>(vcurd(P, l), vcurd(P, l + 1), cn, 1, c->stream),
+                   "prolong64");
+        else if (P.dbl[l + 1])
+            hipchk(c, launch_mg_prolong_t<float, double>(vcur(P, l), vcurd(P, l + 1), cn, 1, c->stream),
+                   "prolong");
+        else
+            hipchk(c, launch_mg_prolong(vcur(P, l), vcur(P, l + 1), cn, 1, c->stream), "prolong");
+    }
+    mg_smooth_rb(c, P, l, p, coarsest ? 16 * p->rb_sweeps : p->rb_sweeps);
+}
+
+// Solves A phi = f on level 0.  v0 (device) receives the potential.
+void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot, float *v0,
+                     const float *f0) {
+    if (p->mg_schedule == BAOREC_MG_REFERENCE) {
+        MGPyr P = mg_build(c, G, p, slot, v0, f0, 0);
+        mg_fmg_ref(c, P, 0, p);
+        if (P.cur[0])
+            hipchk(c, hipMemcpyAsync(v0, P.B[0], sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream),
+                   "copy");
+        return;
+    }
+    if (p->mg_schedule != BAOREC_MG_RBGS) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mg_schedule");
+    // RBGS: zero-mean right-hand side (the periodic operator annihilates
+    // constants), V-cycles from v = 0 until ||f - A v|| <= tol ||f||.
+    float *fz = bufT<float>(c, "mg_fz", G.R());
+    double *red = bufT<double>(c, "mg_red", 8);
+    hipchk(c, hipMemsetAsync(red, 0, sizeof(double) * 8, c->stream), "memset");
+    hipchk(c, launch_sum<float>(f0, G.R(), red, c->stream), "sum f");
+    hipchk(c, hipMemcpyAsync(fz, f0, sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream), "copy");
+    hipchk(c, launch_add_const<float>(fz, G.R(), red, (double)G.R(), -1.0, c->stream), "demean");
+    hipchk(c, launch_sumsq<float>(fz, G.R(), red + 1, c->stream), "norm f");
+    MGPyr P = mg_build(c, G, p, slot, v0, fz, p->fp64_coarse_max);
+    zero_level(c, P, 0);
+    double h[8];
+    hipchk(c, hipMemcpyAsync(h, red, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
+    sync(c);
+    const double fn = sqrt(h[1]);
+    int cyc = 0;
+    double rel = fn > 0 ? 1.0 : 0.0;
+    while (rel > p->tolerance && cyc < p->max_vcycles) {
+        mg_vcycle_rb(c, P, 0, p);
+        cyc++;
+        float *r = vother(P, 0);
+        hipchk(c, launch_mg_residual(r, vcur(P, 0), P.F[0], P.L[0], p->beta, p->has_los, c->stream),
+               "residual");
+        hipchk(c, hipMemsetAsync(red + 2, 0, sizeof(double), c->stream), "memset");
+        hipchk(c, launch_sumsq<float>(r, G.R(), red + 2, c->stream), "norm r");
+        hipchk(c, hipMemcpyAsync(h + 2, red + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        sync(c);
+        rel = fn > 0 ? sqrt(h[2]) / fn : 0.0;
+    }
+    p->vcycles_done = cyc;
+    p->final_residual = rel;
+    if (P.cur[0])
+        hipchk(c, hipMemcpyAsync(v0, P.B[0], sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream),
+               "copy");
+}
+
+// ------------------------------------------------------------ entry helpers
+
+struct Inputs {
+    float *x, *y, *z;
+    const float *w;
+    float *rx, *ry, *rz;
+    const float *rw;
+};
+
+Inputs stage_catalogs(baorec_ctx *c, int mem, float *x, float *y, float *z, const float *w, int64_t n,
+                      const float *rx, const float *ry, const float *rz, const float *rw, int64_t nr) {
+    Inputs in{};
+    in.x = const_cast<float *>(stage_in(c, mem, x, n, "in_x"));
+    in.y = const_cast<float *>(stage_in(c, mem, y, n, "in_y"));
+    in.z = const_cast<float *>(stage_in(c, mem, z, n, "in_z"));
+    in.w = stage_in(c, mem, w, n, "in_w");
+    in.rx = const_cast<float *>(stage_in(c, mem, rx, nr, "in_rx"));
+    in.ry = const_cast<float *>(stage_in(c, mem, ry, nr, "in_ry"));
+    in.rz = const_cast<float *>(stage_in(c, mem, rz, nr, "in_rz"));
+    in.rw = stage_in(c, mem, rw, nr, "in_rw");
+    return in;
+}
+
+void check_err_flag(baorec_ctx *c, const Slot *slot) {
+    Slot h = read_slot(c, slot);
+    if (h.err) fail(c, BAOREC_ERR_OUT_OF_BOX, "a particle indexes outside the grid (BoundsError)");
+}
+
+void mutate_back(baorec_ctx *c, int mem, const baorec_params *p, int wrap, float *x, float *y, float *z,
+                 const Inputs &in, int64_t n) {
+    if (mem == BAOREC_MEM_DEVICE || !wrap || !p->mutate_inputs) return;
+    copy_out(c, mem, x, in.x, n);
+    copy_out(c, mem, y, in.y, n);
+    copy_out(c, mem, z, in.z, n);
+}
+
+// run! pieces shared by both recon kinds: box + overdensity into delta
+void run_common(baorec_ctx *c, baorec_params *p, const Grid &G, int mem, float *x, float *y, float *z,
+                const float *w, int64_t n, const float *rx, const float *ry, const float *rz,
+                const float *rw, int64_t nr, float *delta, bool do_setup_box, Inputs &in, Slot *slot) {
+    if (n < 0 || nr < 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "negative particle count");
+    in = stage_catalogs(c, mem, x, y, z, w, n, rx, ry, rz, rw, nr);
+    if (nr > 0 && do_setup_box) {
+        if (!(p->box_pad >= 0.f)) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "box_pad");
+        slot_from_params(c, slot, p, (double)nr);
+        hipchk(c, launch_setup_box(in.rx, in.ry, in.rz, nr, slot, p->box_pad, c->stream), "setup_box");
+        Slot h = read_slot(c, slot);
+        for (int a = 0; a < 3; a++) {
+            p->box_size[a] = h.box_size[a];
+            p->box_min[a] = h.box_min[a];
+        }
+    } else {
+        check_box(c, p);
+        slot_from_params(c, slot, p, (double)nr);
+    }
+    overdensity(c, G, p, slot, delta, in.x, in.y, in.z, in.w, n, in.rx, in.ry, in.rz, in.rw, nr);
+}
+
+template <class F>
+int guarded(baorec_ctx *c, F &&fn) {
+    if (!c) return BAOREC_ERR_INVALID_ARGUMENT;
+    try {
+        hipError_t e = hipSetDevice(c->device);
+        if (e != hipSuccess) {
+            c->err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+            return BAOREC_ERR_HIP;
+        }
+        fn();
+        c->err.clear();
+        return BAOREC_OK;
+    } catch (const Fail &f) {
+        (void)hipStreamSynchronize(c->stream);
+        return f.code;
+    } catch (const std::bad_alloc &) {
+        c->err = "host allocation failed";
+        return BAOREC_ERR_OUT_OF_MEMORY;
+    }
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+
+extern "C" {
+
+void baorec_params_default(baorec_params *p, float bias, float f, float smoothing_radius) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->bias = bias;
+    p->f = f;
+    p->smoothing_radius = smoothing_radius;
+    p->n_iter = 3;
+    p->beta = f / bias;
+    p->jacobi_damping_factor = 0.4f;
+    p->jacobi_niterations = 5;
+    p->vcycle_niterations = 6;
+    p->mas = BAOREC_MAS_CIC;
+    p->mg_schedule = BAOREC_MG_REFERENCE;
+    p->tolerance = 1e-6;
+    p->max_vcycles = 200;
+    p->rb_sweeps = 2;
+    p->fp64_coarse_max = 8;
+    p->box_pad = 500.0f;
+    p->ran_min = 0.01f;
+    p->mutate_inputs = 1;
+}
+
+const char *baorec_version(void) { return "baorec_hip 0.1.0 (gfx950)"; }
+
+int baorec_create(baorec_ctx **out, int device) {
+    if (!out) return BAOREC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        g_create_err = "no HIP device available";
+        return BAOREC_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_create_err = "device index out of range";
+        return BAOREC_ERR_INVALID_ARGUMENT;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        g_create_err = "hipGetDeviceProperties failed";
+        return BAOREC_ERR_HIP;
+    }
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_create_err = std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950";
+        return BAOREC_ERR_NO_DEVICE;
+    }
+    baorec_ctx *c = new (std::nothrow) baorec_ctx();
+    if (!c) return BAOREC_ERR_OUT_OF_MEMORY;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_err = "stream creation failed";
+        delete c;
+        return BAOREC_ERR_HIP;
+    }
+    std::vector<float2> tw(4096);
+    fill_twiddles(tw.data());
+    set_twiddle_table(tw.data());
+    if (hipDeviceSynchronize() != hipSuccess) {
+        g_create_err = "twiddle upload failed";
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return BAOREC_ERR_HIP;
+    }
+    *out = c;
+    return BAOREC_OK;
+}
+
+int baorec_destroy(baorec_ctx *c) {
+    if (!c) return BAOREC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->bufs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BAOREC_OK;
+}
+
+const char *baorec_last_error(const baorec_ctx *c) {
+    if (!c) return g_create_err.c_str();
+    return c->err.c_str();
+}
+
+static int run_impl(baorec_ctx *c, int kind, baorec_params *p, const int32_t grid[3], int mem,
+                    float *x, float *y, float *z, const float *w, int64_t n, const float *rx,
+                    const float *ry, const float *rz, const float *rw, int64_t nr) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        Grid G = check_grid(c, grid, true);
+        if (mem != BAOREC_MEM_HOST && mem != BAOREC_MEM_DEVICE)
+            fail(c, BAOREC_ERR_INVALID_ARGUMENT, "mem");
+        Slot *slot = slot_buf(c);
+        float *result = bufT<float>(c, "result", G.R());
+        float *ds = bufT<float>(c, "delta_s", G.R());
+        Inputs in;
+        c->has_result = 0;
+        c->disp_valid = 0;
+        run_common(c, p, G, mem, x, y, z, w, n, rx, ry, rz, rw, nr, ds, true, in, slot);
+        if (kind == BAOREC_ITERATIVE) {
+            iterate_all(c, G, p, slot, ds, result);
+        } else {
+            multigrid_solve(c, G, p, slot, result, ds);
+        }
+        check_err_flag(c, slot);
+        mutate_back(c, mem, p, nr == 0, x, y, z, in, n);
+        sync(c);
+        c->has_result = 1;
+        c->result_kind = kind;
+        for (int a = 0; a < 3; a++) c->result_grid[a] = grid[a];
+        c->result_gen++;
+    });
+}
+
+int baorec_run_iterative(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int mem, float *x,
+                         float *y, float *z, const float *w, int64_t n, const float *rx,
+                         const float *ry, const float *rz, const float *rw, int64_t nr) {
+    return run_impl(c, BAOREC_ITERATIVE, p, grid, mem, x, y, z, w, n, rx, ry, rz, rw, nr);
+}
+
+int baorec_run_multigrid(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int mem, float *x,
+                         float *y, float *z, const float *w, int64_t n, const float *rx,
+                         const float *ry, const float *rz, const float *rw, int64_t nr) {
+    return run_impl(c, BAOREC_MULTIGRID, p, grid, mem, x, y, z, w, n, rx, ry, rz, rw, nr);
+}
+
+int baorec_result_info(const baorec_ctx *c, int32_t grid[3], int32_t *kind) {
+    if (!c || !grid || !kind) return BAOREC_ERR_INVALID_ARGUMENT;
+    if (!c->has_result) return BAOREC_ERR_NO_RESULT;
+    for (int a = 0; a < 3; a++) grid[a] = c->result_grid[a];
+    *kind = c->result_kind;
+    return BAOREC_OK;
+}
+
+int baorec_result_copy(baorec_ctx *c, int mem, float *dst) {
+    return guarded(c, [&] {
+        if (!c->has_result) fail(c, BAOREC_ERR_NO_RESULT, "no cached result (run! first)");
+        if (!dst) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "dst is NULL");
+        const int64_t N = (int64_t)c->result_grid[0] * c->result_grid[1] * c->result_grid[2];
+        float *r = bufT<float>(c, "result", N);
+        hipchk(c, hipMemcpyAsync(dst, r, sizeof(float) * N,
+                                 mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                 c->stream), "copy result");
+        sync(c);
+    });
+}
+
+int baorec_setup_overdensity(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem,
+                             float *delta, float *x, float *y, float *z, const float *w, int64_t n,
+                             const float *rx, const float *ry, const float *rz, const float *rw,
+                             int64_t nr) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        Grid G = check_grid(c, grid, true);
+        baorec_params q = *p;
+        Slot *slot = slot_buf(c) + 1;
+        float *d = stage_out(c, mem, delta, G.R(), "out_grid");
+        Inputs in;
+        run_common(c, &q, G, mem, x, y, z, w, n, rx, ry, rz, rw, nr, d, false, in, slot);
+        check_err_flag(c, slot);
+        copy_out(c, mem, delta, d, G.R());
+        mutate_back(c, mem, p, nr == 0, x, y, z, in, n);
+        sync(c);
+    });
+}
+
+int baorec_reconstructed_overdensity(baorec_ctx *c, const baorec_params *p, const int32_t grid[3],
+                                     int mem, float *delta, float *x, float *y, float *z,
+                                     const float *w, int64_t n, const float *rx, const float *ry,
+                                     const float *rz, const float *rw, int64_t nr) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        Grid G = check_grid(c, grid, true);
+        baorec_params q = *p;
+        Slot *slot = slot_buf(c) + 1;
+        float *d = stage_out(c, mem, delta, G.R(), "out_grid");
+        float *ds = bufT<float>(c, "delta_s2", G.R());
+        Inputs in;
+        run_common(c, &q, G, mem, x, y, z, w, n, rx, ry, rz, rw, nr, ds, false, in, slot);
+        iterate_all(c, G, &q, slot, ds, d);
+        check_err_flag(c, slot);
+        copy_out(c, mem, delta, d, G.R());
+        mutate_back(c, mem, p, nr == 0, x, y, z, in, n);
+        sync(c);
+    });
+}
+
+int baorec_reconstructed_potential(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int mem,
+                                   float *phi, float *x, float *y, float *z, const float *w,
+                                   int64_t n, const float *rx, const float *ry, const float *rz,
+                                   const float *rw, int64_t nr) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        Grid G = check_grid(c, grid, true);
+        Slot *slot = slot_buf(c) + 1;
+        float *d = stage_out(c, mem, phi, G.R(), "out_grid");
+        float *ds = bufT<float>(c, "delta_s2", G.R());
+        Inputs in;
+        run_common(c, p, G, mem, x, y, z, w, n, rx, ry, rz, rw, nr, ds, false, in, slot);
+        multigrid_solve(c, G, p, slot, d, ds);
+        check_err_flag(c, slot);
+        copy_out(c, mem, phi, d, G.R());
+        mutate_back(c, mem, p, nr == 0, x, y, z, in, n);
+        sync(c);
+    });
+}
+
+static int shifts_impl(baorec_ctx *c, const baorec_params *p, int mem, const float *x, const float *y,
+                       const float *z, int64_t n, const float *field3d, const int32_t grid[3], int kind,
+                       int field, float *ox, float *oy, float *oz, int positions) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        if (field < 0 || field > 2) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "field must be disp, rsd or sum");
+        if (n < 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "negative particle count");
+        Grid G;
+        float4 *disp;
+        Slot *slot;
+        if (!field3d) {
+            if (!c->has_result) fail(c, BAOREC_ERR_NO_RESULT, "no cached result (run! first)");
+            G = check_grid(c, c->result_grid, true);
+            kind = c->result_kind;
+            slot = slot_buf(c);
+            disp = bufT<float4>(c, "disp_cache", G.R());
+            const float key[6] = {p->box_size[0], p->box_size[1], p->box_size[2],
+                                  p->box_min[0], p->box_min[1], p->box_min[2]};
+            if (!c->disp_valid || c->disp_gen != c->result_gen || memcmp(key, c->disp_box, sizeof key)) {
+                slot_from_params(c, slot, p, 0.0);
+                displacement_grid(c, G, kind, slot, bufT<float>(c, "result", G.R()), disp);
+                c->disp_valid = 1;
+                c->disp_gen = c->result_gen;
+                memcpy(c->disp_box, key, sizeof key);
+            } else {
+                slot_from_params(c, slot, p, 0.0);
+            }
+        } else {
+            G = check_grid(c, grid, true);
+            if (kind != BAOREC_ITERATIVE && kind != BAOREC_MULTIGRID)
+                fail(c, BAOREC_ERR_INVALID_ARGUMENT, "kind");
+            slot = slot_buf(c) + 1;
+            slot_from_params(c, slot, p, 0.0);
+            const float *fd = stage_in(c, mem, field3d, G.R(), "in_field");
+            disp = bufT<float4>(c, "disp_tmp", G.R());
+            displacement_grid(c, G, kind, slot, fd, disp);
+        }
+        const float *dx = stage_in(c, mem, x, n, "sh_x");
+        const float *dy = stage_in(c, mem, y, n, "sh_y");
+        const float *dz = stage_in(c, mem, z, n, "sh_z");
+        float *o1 = stage_out(c, mem, ox, n, "sh_ox");
+        float *o2 = stage_out(c, mem, oy, n, "sh_oy");
+        float *o3 = stage_out(c, mem, oz, n, "sh_oz");
+        GatherArgs a{};
+        a.x = dx;
+        a.y = dy;
+        a.z = dz;
+        a.n = n;
+        a.disp = disp;
+        a.slot = slot;
+        a.nx = G.nx;
+        a.ny = G.ny;
+        a.nz = G.nz;
+        a.field = field;
+        a.positions = positions;
+        a.has_los = p->has_los;
+        a.los[0] = p->los[0];
+        a.los[1] = p->los[1];
+        a.los[2] = p->los[2];
+        a.f = p->f;
+        a.ox = o1;
+        a.oy = o2;
+        a.oz = o3;
+        a.err = &slot->err;
+        hipchk(c, launch_gather(a, c->stream), "gather");
+        check_err_flag(c, slot);
+        copy_out(c, mem, ox, o1, n);
+        copy_out(c, mem, oy, o2, n);
+        copy_out(c, mem, oz, o3, n);
+        sync(c);
+    });
+}
+
+int baorec_read_shifts(baorec_ctx *c, const baorec_params *p, int mem, const float *x, const float *y,
+                       const float *z, int64_t n, const float *field3d, const int32_t grid[3], int kind,
+                       int field, float *sx, float *sy, float *sz) {
+    return shifts_impl(c, p, mem, x, y, z, n, field3d, grid, kind, field, sx, sy, sz, 0);
+}
+
+int baorec_reconstructed_positions(baorec_ctx *c, const baorec_params *p, int mem, const float *x,
+                                   const float *y, const float *z, int64_t n, const float *field3d,
+                                   const int32_t grid[3], int kind, int field, float *ox, float *oy,
+                                   float *oz) {
+    return shifts_impl(c, p, mem, x, y, z, n, field3d, grid, kind, field, ox, oy, oz, 1);
+}
+
+int baorec_setup_box(baorec_ctx *c, int mem, const float *x, const float *y, const float *z, int64_t n,
+                     float pad, float box_size[3], float box_min[3]) {
+    return guarded(c, [&] {
+        if (n <= 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "setup_box needs at least one particle");
+        if (!box_size || !box_min) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "output is NULL");
+        const float *dx = stage_in(c, mem, x, n, "in_rx");
+        const float *dy = stage_in(c, mem, y, n, "in_ry");
+        const float *dz = stage_in(c, mem, z, n, "in_rz");
+        Slot *slot = slot_buf(c) + 1;
+        hipchk(c, launch_setup_box(dx, dy, dz, n, slot, pad, c->stream), "setup_box");
+        Slot h = read_slot(c, slot);
+        for (int a = 0; a < 3; a++) {
+            box_size[a] = h.box_size[a];
+            box_min[a] = h.box_min[a];
+        }
+    });
+}
+
+int baorec_cic(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, float *rho,
+               float *x, float *y, float *z, const float *w, int64_t n, int wrap) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, false);
+        if (n < 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "negative particle count");
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        Inputs in = stage_catalogs(c, mem, x, y, z, w, n, nullptr, nullptr, nullptr, nullptr, 0);
+        float *d = mem == BAOREC_MEM_DEVICE ? rho : const_cast<float *>(stage_in(c, mem, rho, G.R(), "out_grid"));
+        scatter(c, G, p, slot, d, in.x, in.y, in.z, in.w, n, wrap, 1, "cic");
+        check_err_flag(c, slot);
+        copy_out(c, mem, rho, d, G.R());
+        mutate_back(c, mem, p, wrap, x, y, z, in, n);
+        sync(c);
+    });
+}
+
+int baorec_read_cic(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, float *out,
+                    const float *field, const float *x, const float *y, const float *z, int64_t n) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, false);
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        const float *fd = stage_in(c, mem, field, G.R(), "in_field");
+        const float *dx = stage_in(c, mem, x, n, "sh_x");
+        const float *dy = stage_in(c, mem, y, n, "sh_y");
+        const float *dz = stage_in(c, mem, z, n, "sh_z");
+        float *o = stage_out(c, mem, out, n, "sh_ox");
+        hipchk(c, launch_read_cic(o, fd, dx, dy, dz, n, slot, G.nx, G.ny, G.nz, &slot->err, c->stream),
+               "read_cic");
+        check_err_flag(c, slot);
+        copy_out(c, mem, out, o, n);
+        sync(c);
+    });
+}
+
+int baorec_smooth(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, float *field) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, true);
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        float *d = const_cast<float *>(stage_in(c, mem, field, G.R(), "in_field"));
+        if (mem == BAOREC_MEM_DEVICE) d = field;
+        FFTBufs f = fft_bufs(c, G);
+        tables(c, G, slot, f);
+        smooth_to_half(c, G, f, d, 1, nullptr, p->smoothing_radius);
+        XInvArgs a = xargs(G, f, slot);
+        a.in[0] = f.cw;
+        a.kxpow[0] = 0;
+        a.nin = 1;
+        a.out = d;
+        hipchk(c, launch_x_c2r(a, XE_PLAIN, 1, c->stream), "x inverse");
+        copy_out(c, mem, field, d, G.R());
+        sync(c);
+    });
+}
+
+int baorec_rfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *in, float *out_complex) {
+    return guarded(c, [&] {
+        Grid G = check_grid(c, grid, true);
+        const float *d = stage_in(c, mem, in, G.R(), "in_field");
+        FFTBufs f = fft_bufs(c, G);
+        float2 *o = mem == BAOREC_MEM_DEVICE ? reinterpret_cast<float2 *>(out_complex) : f.cw;
+        if (!out_complex) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "out is NULL");
+        x_fwd(c, G, d, o, 1, nullptr, 0);
+        strided_plain(c, G, f, 1, false, o, 1);
+        strided_plain(c, G, f, 2, false, o, 1);
+        if (mem != BAOREC_MEM_DEVICE)
+            hipchk(c, hipMemcpyAsync(out_complex, o, sizeof(float2) * G.C(), hipMemcpyDeviceToHost, c->stream),
+                   "D2H");
+        sync(c);
+    });
+}
+
+int baorec_irfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *in_complex, float *out) {
+    return guarded(c, [&] {
+        Grid G = check_grid(c, grid, true);
+        if (!in_complex || !out) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "NULL argument");
+        FFTBufs f = fft_bufs(c, G);
+        float2 *w = f.cw;  // work copy: the caller's spectrum is not modified
+        hipchk(c, hipMemcpyAsync(w, in_complex, sizeof(float2) * G.C(),
+                                 mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                 c->stream), "copy spectrum");
+        Slot *slot = slot_buf(c) + 1;
+        strided_plain(c, G, f, 2, true, w, 1);
+        strided_plain(c, G, f, 1, true, w, 1);
+        float *o = stage_out(c, mem, out, G.R(), "out_grid");
+        XInvArgs a = xargs(G, f, slot);
+        a.in[0] = w;
+        a.kxpow[0] = 0;
+        a.nin = 1;
+        a.out = o;
+        hipchk(c, launch_x_c2r(a, XE_PLAIN, 1, c->stream), "x inverse");
+        copy_out(c, mem, out, o, G.R());
+        sync(c);
+    });
+}
+
+int baorec_k_vec(const int32_t grid[3], const float box_size[3], float *kx, float *ky, float *kz) {
+    if (!grid || !box_size || !kx || !ky || !kz) return BAOREC_ERR_INVALID_ARGUMENT;
+    float *out[3] = {kx, ky, kz};
+    for (int a = 0; a < 3; a++) {
+        const int n = grid[a];
+        if (n <= 0 || !(box_size[a] > 0.f)) return BAOREC_ERR_INVALID_ARGUMENT;
+        const float fs = (float)((2.0 * M_PI * (double)n) / (double)box_size[a]);
+        const float mult = fs / (float)n;
+        const int m = a == 0 ? n / 2 + 1 : n;
+        for (int i = 0; i < m; i++) {
+            const int ii = a == 0 ? i : (i < (n + 1) / 2 ? i : i - n);
+            out[a][i] = (float)ii * mult;
+        }
+    }
+    return BAOREC_OK;
+}
+
+int baorec_x_vec(const int32_t grid[3], const float box_size[3], const float box_min[3], float *xx,
+                 float *xy, float *xz) {
+    if (!grid || !box_size || !box_min || !xx || !xy || !xz) return BAOREC_ERR_INVALID_ARGUMENT;
+    float *out[3] = {xx, xy, xz};
+    for (int a = 0; a < 3; a++) {
+        if (grid[a] <= 0) return BAOREC_ERR_INVALID_ARGUMENT;
+        const float h = box_size[a] / (float)grid[a];
+        for (int i = 0; i < grid[a]; i++)
+            out[a][i] = (float)((double)box_min[a] + ((double)i + 0.5) * (double)h);
+    }
+    return BAOREC_OK;
+}
+
+// ---------------------------------------------------------------- MG pieces
+
+static MGLevel level_for(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot) {
+    MGLevel L{};
+    L.n[0] = G.nx;
+    L.n[1] = G.ny;
+    L.n[2] = G.nz;
+    for (int a = 0; a < 3; a++) {
+        const float cell = p->box_size[a] / (float)L.n[a];
+        L.cell[a] = cell;
+        L.cell2[a] = cell * cell;
+        L.icell2[a] = 1.0f / L.cell2[a];
+        L.losn[a] = p->has_los ? p->los[a] / cell : 0.f;
+    }
+    float *xv = bufT<float>(c, "mgp_xv", G.nx + G.ny + G.nz);
+    hipchk(c, launch_mg_xvec(xv, slot, G.nx, G.ny, G.nz, c->stream), "xvec");
+    L.xv = xv;
+    return L;
+}
+
+int baorec_mg_jacobi(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, float *v,
+                     const float *f) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, false);
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        float *dv = const_cast<float *>(stage_in(c, mem, v, G.R(), "mgp_v"));
+        if (mem == BAOREC_MEM_DEVICE) dv = v;
+        const float *df = stage_in(c, mem, f, G.R(), "mgp_f");
+        float *tmp = bufT<float>(c, "mgp_t", G.R());
+        MGLevel L = level_for(c, G, p, slot);
+        float *a = dv, *b = tmp;
+        for (int i = 0; i < p->jacobi_niterations; i++) {
+            hipchk(c, launch_mg_jacobi(b, a, df, L, p->beta, p->jacobi_damping_factor, p->has_los, c->stream),
+                   "jacobi");
+            std::swap(a, b);
+        }
+        if (a != dv)
+            hipchk(c, hipMemcpyAsync(dv, a, sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream), "copy");
+        copy_out(c, mem, v, dv, G.R());
+        sync(c);
+    });
+}
+
+int baorec_mg_residual(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, float *r,
+                       const float *v, const float *f) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, false);
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        const float *dv = stage_in(c, mem, v, G.R(), "mgp_v");
+        const float *df = stage_in(c, mem, f, G.R(), "mgp_f");
+        float *dr = stage_out(c, mem, r, G.R(), "mgp_r");
+        MGLevel L = level_for(c, G, p, slot);
+        hipchk(c, launch_mg_residual(dr, dv, df, L, p->beta, p->has_los, c->stream), "residual");
+        copy_out(c, mem, r, dr, G.R());
+        sync(c);
+    });
+}
+
+int baorec_mg_reduce(baorec_ctx *c, const int32_t fine_grid[3], int mem, float *coarse, const float *fine) {
+    return guarded(c, [&] {
+        Grid G = check_grid(c, fine_grid, false);
+        if (G.nx % 2 || G.ny % 2 || G.nz % 2) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "fine grid must be even");
+        const int64_t nc = G.R() / 8;
+        const float *df = stage_in(c, mem, fine, G.R(), "mgp_v");
+        float *dc = stage_out(c, mem, coarse, nc, "mgp_r");
+        const int fn[3] = {G.nx, G.ny, G.nz};
+        hipchk(c, launch_mg_reduce(dc, df, fn, c->stream), "reduce");
+        copy_out(c, mem, coarse, dc, nc);
+        sync(c);
+    });
+}
+
+int baorec_mg_prolong(baorec_ctx *c, const int32_t coarse_grid[3], int mem, float *fine,
+                      const float *coarse, int add) {
+    return guarded(c, [&] {
+        Grid G = check_grid(c, coarse_grid, false);
+        const int64_t nf = G.R() * 8;
+        const float *dc = stage_in(c, mem, coarse, G.R(), "mgp_v");
+        float *df = const_cast<float *>(add ? stage_in(c, mem, fine, nf, "mgp_r") : stage_out(c, mem, fine, nf, "mgp_r"));
+        const int cn[3] = {G.nx, G.ny, G.nz};
+        hipchk(c, launch_mg_prolong(df, dc, cn, add, c->stream), "prolong");
+        copy_out(c, mem, fine, df, nf);
+        sync(c);
+    });
+}
+
+int baorec_mg_vcycle(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, float *v,
+                     const float *f) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, false);
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        float *dv = mem == BAOREC_MEM_DEVICE ? v : const_cast<float *>(stage_in(c, mem, v, G.R(), "mgp_vv"));
+        const float *df = stage_in(c, mem, f, G.R(), "mgp_ff");
+        MGPyr P = mg_build(c, G, p, slot, dv, df, 0);
+        mg_vcycle_ref(c, P, 0, p);
+        if (P.cur[0])
+            hipchk(c, hipMemcpyAsync(dv, P.B[0], sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream),
+                   "copy");
+        copy_out(c, mem, v, dv, G.R());
+        sync(c);
+    });
+}
+
+int baorec_mg_fmg(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int mem, float *v, const float *f) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, false);
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        float *dv = mem == BAOREC_MEM_DEVICE ? v : bufT<float>(c, "mgp_vv", G.R());
+        const float *df = stage_in(c, mem, f, G.R(), "mgp_ff");
+        multigrid_solve(c, G, p, slot, dv, df);
+        copy_out(c, mem, v, dv, G.R());
+        sync(c);
+    });
+}
+
+int baorec_run_batch(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int mem, int ncat,
+                     const int64_t *offsets, float *x, float *y, float *z, const float *w, float *ox,
+                     float *oy, float *oz) {
+    return guarded(c, [&] {
+        (void)p; (void)grid; (void)mem; (void)ncat; (void)offsets; (void)x; (void)y; (void)z;
+        (void)w; (void)ox; (void)oy; (void)oz;
+        fail(c, BAOREC_ERR_UNSUPPORTED, "batched reconstruction is not built yet");
+    });
+}
+
+}  // extern "C"

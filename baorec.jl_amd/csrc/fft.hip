@@ -1,0 +1,495 @@
+// Stockham R2C / C2R 3D FFT passes for gfx950 with fused k-space multipliers
+// and real-space epilogues.
+//
+// A 3D real transform is three passes over HBM:
+//   x pass:  R2C along the contiguous axis (n real = n/2 complex, packed trick)
+//   y pass:  complex FFT along y, B consecutive kx columns per workgroup
+//   z pass:  complex FFT along z, B consecutive kx columns per workgroup
+// The reference's k-space kernels (gaussian_filter_kernel!, inv_laplace_kernel!,
+// hessian_kernel!, displacement_kernel!, mg_gradient_kernel!) and real-space
+// broadcasts are folded into these passes instead of running as extra sweeps.
+#include <math.h>
+
+#include "kernels.hpp"
+
+namespace baorec {
+
+__device__ float2 g_tw[4096];
+
+void fill_twiddles(float2 *tw) {
+    tw[0] = make_float2(1.f, 0.f);
+    for (int S = 1; S <= 2048; S *= 2)
+        for (int i = 0; i < S; i++) {
+            double a = -2.0 * M_PI * (double)i / (double)S;
+            tw[S + i] = make_float2((float)cos(a), (float)sin(a));
+        }
+}
+
+bool fft_side_ok(int n) { return n >= 16 && n <= kMaxSide && (n & (n - 1)) == 0; }
+
+// ------------------------------------------------------------ block sum
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int NT>
+__device__ void block_sum_atomic(double v, double *dst) {
+    __shared__ double red[NT / 64];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int i = 0; i < NT / 64; i++) s += red[i];
+        atomicAdd(dst, s);
+    }
+}
+
+// ------------------------------------------------------------ x: R2C
+
+template <int M>
+struct XShape {
+    static constexpr int TPL = M / 8;                   // threads per line
+    static constexpr int LPW = TPL >= 256 ? 1 : 256 / TPL;  // lines per workgroup
+    static constexpr int LS = M + M / 8;                // padded LDS line stride
+    static constexpr int NT = TPL * LPW;
+};
+
+template <int M>
+__global__ void __launch_bounds__(XShape<M>::NT) k_x_r2c(XFwdArgs a) {
+    using S = XShape<M>;
+    using PF = Plan<M, false>;
+    __shared__ float2 lds[S::LPW * S::LS];
+    const int tid = threadIdx.x;
+    const int li = tid / S::TPL, t = tid % S::TPL;
+    const int64_t nlines = (int64_t)a.ny * a.nz;
+    const int64_t line = (int64_t)blockIdx.x * S::LPW + li;
+    const bool valid = line < nlines;
+    const int b = blockIdx.y;
+    const float2 *src = reinterpret_cast<const float2 *>(a.in + b * a.rstride) + line * M;
+    float2 v[8];
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        v[i] = valid ? src[PF::in_pos(t, i)] : make_float2(0.f, 0.f);
+        s += (double)v[i].x + (double)v[i].y;
+    }
+    LayLine lay{li * S::LS};
+    fft_line<M, false>(v, lds, lay, t, g_tw);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) lds[lay(PF::out_pos(t, i))] = v[i];
+    __syncthreads();
+    float2 *dst = a.out + b * a.cstride + line * (M + 1);
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int k = t + i * S::TPL;
+            float2 zk = lds[lay(k)];
+            float2 zc = cconj(lds[lay((M - k) & (M - 1))]);
+            float2 e = cscale(cadd(zk, zc), 0.5f);
+            float2 d = csub(zk, zc);
+            float2 o = make_float2(0.5f * d.y, -0.5f * d.x);  // (zk - zc) / (2i)
+            dst[k] = cadd(e, cmul(g_tw[2 * M + k], o));
+        }
+        if (t == 0) {
+            float2 z0 = lds[lay(0)];
+            dst[M] = make_float2(z0.x - z0.y, 0.f);
+        }
+    }
+    if (a.sums) block_sum_atomic<S::NT>(valid ? s : 0.0, a.sums + b * a.sumStride);
+}
+
+// ------------------------------------------------------------ x: C2R
+
+template <int M, int EPI>
+__global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
+    using S = XShape<M>;
+    using PI = Plan<M, true>;
+    __shared__ float2 lds[S::LPW * S::LS];
+    const int tid = threadIdx.x;
+    const int li = tid / S::TPL, t = tid % S::TPL;
+    const int64_t nlines = (int64_t)a.ny * a.nz;
+    const int64_t line = (int64_t)blockIdx.x * S::LPW + li;
+    const bool valid = line < nlines;
+    const int b = blockIdx.y;
+    const int iy = (int)(line % a.ny), iz = (int)(line / a.ny);
+    const float *kx = a.ktab + b * a.ktabStride;
+    const int64_t rbase = b * a.rstride + line * (2 * M);
+    LayLine lay{li * S::LS};
+
+    float2 acc[8];
+    float x2[8][2];
+    const float *xv = a.xtab ? a.xtab + b * a.xtabStride : nullptr;
+    float xy = 0.f, xz = 0.f;
+    if constexpr (EPI == XE_ITER_LOCAL) {
+        xy = xv[2 * M + iy];
+        xz = xv[2 * M + a.ny + iz];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int m = PI::out_pos(t, i);
+        if constexpr (EPI == XE_ITER_LOCAL || EPI == XE_ITER_FIXED) {
+            acc[i] = valid ? *reinterpret_cast<const float2 *>(a.aux + rbase + 2 * m)
+                           : make_float2(0.f, 0.f);
+        } else {
+            acc[i] = make_float2(0.f, 0.f);
+        }
+        if constexpr (EPI == XE_ITER_LOCAL) {
+            float x0 = xv[2 * m], x1 = xv[2 * m + 1];
+            x2[i][0] = (x0 * x0 + xy * xy) + xz * xz;
+            x2[i][1] = (x1 * x1 + xy * xy) + xz * xz;
+        }
+    }
+    float4 disp[8][2];
+
+    for (int q = 0; q < a.nin; q++) {
+        const float2 *src = a.in[q] + b * a.cstride + line * (M + 1);
+        const int pw = a.kxpow[q];
+        float2 v[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int k = PI::in_pos(t, i);
+            float2 xk = valid ? src[k] : make_float2(0.f, 0.f);
+            float2 xm = valid ? src[M - k] : make_float2(0.f, 0.f);
+            if (pw) {
+                xk = cscale(xk, kpow(kx[k], pw));
+                xm = cscale(xm, kpow(kx[M - k], pw));
+            }
+            if (k == 0) {  // self-conjugate bins: imaginary parts ignored (FFTW/pocketfft C2R)
+                xk.y = 0.f;
+                xm.y = 0.f;
+            }
+            float2 xc = cconj(xm);
+            float2 s = cadd(xk, xc);
+            float2 d = csub(xk, xc);
+            float2 w = cconj(g_tw[2 * M + k]);   // exp(+2 pi i k / N)
+            float2 wd = cmul(w, d);
+            v[i] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i * w * d
+        }
+        fft_line<M, true>(v, lds, lay, t, g_tw);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float r0 = v[i].x * a.scale, r1 = v[i].y * a.scale;
+            if constexpr (EPI == XE_PLAIN) {
+                acc[i] = make_float2(r0, r1);
+            } else if constexpr (EPI == XE_OD_BOX || EPI == XE_OD_RAND) {
+                if (q == 0) acc[i] = make_float2(r0, r1);
+                else {
+                    disp[i][0].x = r0;  // randoms grid
+                    disp[i][1].x = r1;
+                }
+            } else if constexpr (EPI == XE_ITER_FIXED) {
+                const float fac = a.fac[0];
+                acc[i].x = acc[i].x - fac * r0;
+                acc[i].y = acc[i].y - fac * r1;
+            } else if constexpr (EPI == XE_ITER_LOCAL) {
+                const int m = PI::out_pos(t, i);
+                const float fac = a.fac[q];
+                const float dv[2] = {r0, r1};
+                float *ac = &acc[i].x;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const float xx = xv[2 * m + h];
+                    // pair order of src/iterative.jl:20-21: xx, xy, xz, yy, yz, zz
+                    float xi, xj;
+                    switch (q) {
+                        case 0: xi = xx; xj = xx; break;
+                        case 1: xi = xx; xj = xy; break;
+                        case 2: xi = xx; xj = xz; break;
+                        case 3: xi = xy; xj = xy; break;
+                        case 4: xi = xy; xj = xz; break;
+                        default: xi = xz; xj = xz; break;
+                    }
+                    const float xsq = x2[i][h];
+                    ac[h] = xsq > 0.f ? ac[h] - (((fac * dv[h]) * xi) * xj) / xsq : 0.f;
+                }
+            } else if constexpr (EPI == XE_DISP) {
+                if (q == 0) { disp[i][0].x = r0; disp[i][1].x = r1; }
+                else if (q == 1) { disp[i][0].y = r0; disp[i][1].y = r1; }
+                else { disp[i][0].z = r0; disp[i][1].z = r1; }
+            }
+        }
+    }
+    if (!valid) return;
+
+    if constexpr (EPI == XE_DISP) {
+        float4 *o4 = reinterpret_cast<float4 *>(a.out) + rbase;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int m = PI::out_pos(t, i);
+            disp[i][0].w = 0.f;
+            disp[i][1].w = 0.f;
+            o4[2 * m] = disp[i][0];
+            o4[2 * m + 1] = disp[i][1];
+        }
+        return;
+    }
+    if constexpr (EPI == XE_OD_BOX) {
+        const double ncell = (double)a.nx * a.ny * a.nz;
+        const float mean = (float)(a.slot[b].sum[0] / ncell);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            acc[i].x = ((acc[i].x / mean) - 1.0f) / a.bias;
+            acc[i].y = ((acc[i].y / mean) - 1.0f) / a.bias;
+        }
+    }
+    if constexpr (EPI == XE_OD_RAND) {
+        const Slot &sl = a.slot[b];
+        const float alpha = (float)(sl.sum[0] / sl.sum[1]);
+        const double thr = (double)a.ran_min * sl.sum[1] / sl.count_r;
+        const float ba = a.bias * alpha;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            float *ac = &acc[i].x;
+            const float rr[2] = {disp[i][0].x, disp[i][1].x};
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const float d = ac[h] - alpha * rr[h];
+                ac[h] = ((double)rr[h] > thr) ? d / (ba * rr[h]) : 0.f;
+            }
+        }
+    }
+    float2 *o2 = reinterpret_cast<float2 *>(a.out + rbase);
+#pragma unroll
+    for (int i = 0; i < 8; i++) o2[PI::out_pos(t, i)] = acc[i];
+}
+
+// ------------------------------------------------------------ strided y / z
+
+template <int M>
+struct SShape {
+    static constexpr int B = M >= 1024 ? 8 : 16;  // kx columns per workgroup
+    static constexpr int NT = B * M / 8;
+};
+
+__device__ __forceinline__ float2 apply_mul(float2 F, int mul, float kx, float ky, float kz,
+                                            float R2, float l0, float l1, float l2) {
+    const float k2 = (kx * kx + ky * ky) + kz * kz;
+    switch (mul) {
+        case MUL_GAUSS: {
+            const double g = exp(-0.5 * (double)R2 * (double)k2);
+            return make_float2((float)((double)F.x * g), (float)((double)F.y * g));
+        }
+        case MUL_INVK2:
+            return k2 > 0.f ? make_float2(F.x / k2, F.y / k2) : make_float2(0.f, 0.f);
+        case MUL_FIXEDLOS: {
+            if (!(k2 > 0.f)) return make_float2(0.f, 0.f);
+            const float m = ((kx * kx) * l0 + (ky * ky) * l1) + (kz * kz) * l2;
+            return cscale(make_float2(F.x / k2, F.y / k2), m);
+        }
+        case MUL_I_INVK2:
+            return k2 > 0.f ? make_float2(-F.y / k2, F.x / k2) : make_float2(0.f, 0.f);
+        case MUL_I:
+            return make_float2(-F.y, F.x);
+        default:
+            return F;
+    }
+}
+
+template <int M>
+__global__ void __launch_bounds__(SShape<M>::NT) k_strided(StridedArgs a) {
+    constexpr int B = SShape<M>::B;
+    using PF = Plan<M, false>;
+    using PI = Plan<M, true>;
+    __shared__ float2 lds[B * M];
+    const int tid = threadIdx.x;
+    const int col = tid % B, t = tid / B;
+    const int ncg = (a.P + B - 1) / B;
+    const int cg = blockIdx.x % ncg, other = blockIdx.x / ncg;
+    const int kxi = cg * B + col;
+    const bool valid = kxi < a.P;
+    const StridedJob &J = a.job[blockIdx.y];
+    const int64_t b = blockIdx.z;
+    int64_t base, rs;
+    if (a.axis == 1) {
+        base = (int64_t)other * a.P * a.ny + kxi;
+        rs = a.P;
+    } else {
+        base = (int64_t)other * a.P + kxi;
+        rs = (int64_t)a.P * a.ny;
+    }
+    base += b * a.cstride;
+    const float *kt = a.ktab + b * a.ktabStride;
+    const float *kax = a.axis == 1 ? kt + a.P : kt + a.P + a.ny;
+    LayStrided lay{col, B};
+    float2 v[8];
+
+    if (a.mode == 0) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            v[i] = valid ? J.in[base + PF::in_pos(t, i) * rs] : make_float2(0.f, 0.f);
+        fft_line<M, false>(v, lds, lay, t, g_tw);
+        if (valid) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) J.out[0][base + PF::out_pos(t, i) * rs] = v[i];
+        }
+        return;
+    }
+    if (a.mode == 2) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            v[i] = valid ? J.in[base + PF::in_pos(t, i) * rs] : make_float2(0.f, 0.f);
+        fft_line<M, false>(v, lds, lay, t, g_tw);
+        const float kxv = valid ? kt[kxi] : 0.f;
+        const float kother = a.axis == 1 ? kt[a.P + a.ny + other] : kt[a.P + other];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float kal = kax[PF::out_pos(t, i)];
+            const float ky = a.axis == 1 ? kal : kother;
+            const float kz = a.axis == 1 ? kother : kal;
+            v[i] = apply_mul(v[i], a.mul, kxv, ky, kz, a.R2, a.los[0], a.los[1], a.los[2]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            v[i] = valid ? J.in[base + PI::in_pos(t, i) * rs] : make_float2(0.f, 0.f);
+    }
+    float2 f[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) f[i] = v[i];
+    for (int o = 0; o < J.nout; o++) {
+        const int pw = J.pw[o];
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = pw ? cscale(f[i], kpow(kax[PI::in_pos(t, i)], pw)) : f[i];
+        fft_line<M, true>(v, lds, lay, t, g_tw);
+        if (valid) {
+            float2 *dst = J.out[o];
+#pragma unroll
+            for (int i = 0; i < 8; i++) dst[base + PI::out_pos(t, i) * rs] = v[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------ launchers
+
+template <int M>
+static hipError_t x_r2c_m(const XFwdArgs &a, int nbatch, hipStream_t s) {
+    using S = XShape<M>;
+    const int64_t nlines = (int64_t)a.ny * a.nz;
+    dim3 grid((unsigned)((nlines + S::LPW - 1) / S::LPW), nbatch);
+    hipLaunchKernelGGL(k_x_r2c<M>, grid, dim3(S::NT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_x_r2c(const XFwdArgs &a, int nbatch, hipStream_t s) {
+    switch (a.nx / 2) {
+        case 8: return x_r2c_m<8>(a, nbatch, s);
+        case 16: return x_r2c_m<16>(a, nbatch, s);
+        case 32: return x_r2c_m<32>(a, nbatch, s);
+        case 64: return x_r2c_m<64>(a, nbatch, s);
+        case 128: return x_r2c_m<128>(a, nbatch, s);
+        case 256: return x_r2c_m<256>(a, nbatch, s);
+        case 512: return x_r2c_m<512>(a, nbatch, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int M, int EPI>
+static hipError_t x_c2r_me(const XInvArgs &a, int nbatch, hipStream_t s) {
+    using S = XShape<M>;
+    const int64_t nlines = (int64_t)a.ny * a.nz;
+    dim3 grid((unsigned)((nlines + S::LPW - 1) / S::LPW), nbatch);
+    hipLaunchKernelGGL((k_x_c2r<M, EPI>), grid, dim3(S::NT), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int M>
+static hipError_t x_c2r_m(const XInvArgs &a, int epi, int nbatch, hipStream_t s) {
+    switch (epi) {
+        case XE_PLAIN: return x_c2r_me<M, XE_PLAIN>(a, nbatch, s);
+        case XE_OD_BOX: return x_c2r_me<M, XE_OD_BOX>(a, nbatch, s);
+        case XE_OD_RAND: return x_c2r_me<M, XE_OD_RAND>(a, nbatch, s);
+        case XE_ITER_LOCAL: return x_c2r_me<M, XE_ITER_LOCAL>(a, nbatch, s);
+        case XE_ITER_FIXED: return x_c2r_me<M, XE_ITER_FIXED>(a, nbatch, s);
+        case XE_DISP: return x_c2r_me<M, XE_DISP>(a, nbatch, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_x_c2r(const XInvArgs &a, int epi, int nbatch, hipStream_t s) {
+    switch (a.nx / 2) {
+        case 8: return x_c2r_m<8>(a, epi, nbatch, s);
+        case 16: return x_c2r_m<16>(a, epi, nbatch, s);
+        case 32: return x_c2r_m<32>(a, epi, nbatch, s);
+        case 64: return x_c2r_m<64>(a, epi, nbatch, s);
+        case 128: return x_c2r_m<128>(a, epi, nbatch, s);
+        case 256: return x_c2r_m<256>(a, epi, nbatch, s);
+        case 512: return x_c2r_m<512>(a, epi, nbatch, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int M>
+static hipError_t strided_m(const StridedArgs &a, int njobs, int nbatch, hipStream_t s) {
+    using S = SShape<M>;
+    const int ncg = (a.P + S::B - 1) / S::B;
+    const int nother = a.axis == 1 ? a.nz : a.ny;
+    dim3 grid((unsigned)(ncg * nother), njobs, nbatch);
+    hipLaunchKernelGGL(k_strided<M>, grid, dim3(S::NT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_strided(const StridedArgs &a, int njobs, int nbatch, hipStream_t s) {
+    const int M = a.axis == 1 ? a.ny : a.nz;
+    switch (M) {
+        case 16: return strided_m<16>(a, njobs, nbatch, s);
+        case 32: return strided_m<32>(a, njobs, nbatch, s);
+        case 64: return strided_m<64>(a, njobs, nbatch, s);
+        case 128: return strided_m<128>(a, njobs, nbatch, s);
+        case 256: return strided_m<256>(a, njobs, nbatch, s);
+        case 512: return strided_m<512>(a, njobs, nbatch, s);
+        case 1024: return strided_m<1024>(a, njobs, nbatch, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+void set_twiddle_table(const float2 *host_tw) {
+    (void)hipMemcpyToSymbol(HIP_SYMBOL(g_tw), host_tw, sizeof(float2) * 4096, 0,
+                            hipMemcpyHostToDevice);
+}
+
+// ------------------------------------------------------------ tables
+
+// k_vec (src/utils.jl:3-10) and x_vec (src/utils.jl:21-25) per batch item.
+__global__ void k_tables(const Slot *slot, float *ktab, float *xtab, int nx, int ny, int nz) {
+    const int b = blockIdx.y;
+    const int P = nx / 2 + 1;
+    const Slot &sl = slot[b];
+    float *kt = ktab + (int64_t)b * (P + ny + nz);
+    float *xt = xtab + (int64_t)b * (nx + ny + nz);
+    const int n3[3] = {nx, ny, nz};
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < P + ny + nz;
+         idx += gridDim.x * blockDim.x) {
+        int ax, i;
+        if (idx < P) { ax = 0; i = idx; }
+        else if (idx < P + ny) { ax = 1; i = idx - P; }
+        else { ax = 2; i = idx - P - ny; }
+        const int n = n3[ax];
+        const float L = sl.box_size[ax];
+        const float fs = (float)((2.0 * M_PI * (double)n) / (double)L);
+        const float mult = fs / (float)n;
+        const int ii = (ax == 0) ? i : (i < (n + 1) / 2 ? i : i - n);
+        kt[idx] = (float)ii * mult;
+    }
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nx + ny + nz;
+         idx += gridDim.x * blockDim.x) {
+        int ax, i;
+        if (idx < nx) { ax = 0; i = idx; }
+        else if (idx < nx + ny) { ax = 1; i = idx - nx; }
+        else { ax = 2; i = idx - nx - ny; }
+        const float h = sl.box_size[ax] / (float)n3[ax];
+        xt[idx] = (float)((double)sl.box_min[ax] + ((double)i + 0.5) * (double)h);
+    }
+}
+
+hipError_t launch_tables(const Slot *slot, float *ktab, float *xtab, int nx, int ny, int nz,
+                         int nbatch, hipStream_t s) {
+    hipLaunchKernelGGL(k_tables, dim3(4, nbatch), dim3(256), 0, s, slot, ktab, xtab, nx, ny, nz);
+    return hipGetLastError();
+}
+
+}  // namespace baorec

@@ -1,0 +1,194 @@
+// Host-side launch interfaces of the HIP kernels (internal to libbaorec_hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace baorec {
+
+// ------------------------------------------------------------------- FFT
+
+// k-space multipliers applied between the forward and inverse transform of
+// a fused strided pass.
+enum MulKind : int {
+    MUL_NONE = 0,
+    MUL_GAUSS = 1,      // exp(-R^2 k^2 / 2), exponent in FP64 (src/utils.jl:52,81)
+    MUL_INVK2 = 2,      // 1/k^2, 0 at k = 0 (src/iterative.jl:10-14,145-149)
+    MUL_FIXEDLOS = 3,   // (sum_i k_i^2 los_i)/k^2 (src/iterative.jl:43-62)
+    MUL_I_INVK2 = 4,    // i/k^2, 0 at k = 0 (src/iterative.jl:219-223)
+    MUL_I = 5,          // i (src/multigrid.jl:776-779)
+};
+
+constexpr int kMaxJobs = 3;
+constexpr int kMaxOut = 3;
+
+struct StridedJob {
+    const float2 *in;
+    float2 *out[kMaxOut];
+    int pw[kMaxOut];   // power of the along-axis wavenumber for each output
+    int nout;
+};
+
+struct StridedArgs {
+    StridedJob job[kMaxJobs];  // blockIdx.y
+    int64_t cstride;           // complex elements between batch items (blockIdx.z)
+    const float *ktab;         // per batch item: kx[P] | ky[ny] | kz[nz]
+    int64_t ktabStride;        // floats between batch items (0: shared)
+    int P, ny, nz;
+    int axis;                  // 1: y, 2: z
+    int mode;                  // 0 forward, 1 inverse, 2 forward*mul*inverse
+    int mul;                   // MulKind (mode 2)
+    float R2;                  // smoothing radius squared (FP32)
+    float los[3];
+};
+
+enum XEpi : int {
+    XE_PLAIN = 0,     // out = scale * c2r(in0)
+    XE_OD_BOX = 1,    // delta = ((rho/mean) - 1)/bias (src/recon.jl:54-55)
+    XE_OD_RAND = 2,   // data - alpha*randoms normalisation (src/recon.jl:78-85)
+    XE_ITER_LOCAL = 3,// delta_r = delta_s - sum_ij fac_ij D_ij x_i x_j / x^2
+    XE_ITER_FIXED = 4,// delta_r = delta_s - fac * D
+    XE_DISP = 5,      // float4 (Psi_x, Psi_y, Psi_z, 0)
+};
+
+constexpr int kMaxXIn = 6;
+
+struct XInvArgs {
+    const float2 *in[kMaxXIn];
+    int kxpow[kMaxXIn];
+    int nin;
+    int64_t cstride;      // complex elements between batch items
+    int64_t rstride;      // real elements between batch items
+    float *out;           // real output (float4 for XE_DISP)
+    const float *aux;     // delta_s for the iterative updates
+    int nx, ny, nz;
+    float scale;          // 1/(nx ny nz)
+    const float *ktab;    // kx at ktab[b*ktabStride + kx]
+    int64_t ktabStride;
+    const float *xtab;    // per batch item: x[nx] | y[ny] | z[nz] cell centres
+    int64_t xtabStride;
+    const Slot *slot;     // per batch item scalars (sums, counts)
+    float bias;
+    float fac[kMaxXIn];   // update factors (iterative)
+    float ran_min;
+};
+
+struct XFwdArgs {
+    const float *in;
+    float2 *out;
+    int64_t rstride, cstride;
+    int nx, ny, nz;
+    double *sums;         // optional FP64 total of each input (per batch item)
+    int64_t sumStride;
+};
+
+hipError_t launch_x_r2c(const XFwdArgs &a, int nbatch, hipStream_t s);
+hipError_t launch_x_c2r(const XInvArgs &a, int epi, int nbatch, hipStream_t s);
+hipError_t launch_strided(const StridedArgs &a, int njobs, int nbatch, hipStream_t s);
+bool fft_side_ok(int n);
+
+// ---------------------------------------------------------------- tables
+
+// Builds, for each batch item, the k tables (kx[P] | ky[ny] | kz[nz]) and the
+// cell-centre tables (x[nx] | y[ny] | z[nz]) from the item's box in Slot.
+hipError_t launch_tables(const Slot *slot, float *ktab, float *xtab, int nx, int ny, int nz,
+                         int nbatch, hipStream_t s);
+// Twiddle table (4096 float2) on the host.
+void fill_twiddles(float2 *tw);
+void set_twiddle_table(const float2 *dev_tw);
+
+// ------------------------------------------------------------- mass assignment
+
+struct ScatterArgs {
+    float *x, *y, *z;           // positions (mutated when wrap && mutate)
+    const float *w;
+    const int64_t *offsets;     // catalog c owns [offsets[c], offsets[c+1])
+    int ncat;
+    int64_t ntotal;
+    const Slot *slot;           // box per catalog
+    int64_t slotStride;         // Slot entries between catalogs (0: shared)
+    int nx, ny, nz;
+    int wrap, mutate, tsc;
+    int tx, ty, tz;             // tile extent (cells)
+    int ntx, nty, ntz;          // tiles per axis
+    uint32_t *count;            // per (catalog, tile)
+    uint32_t *cursor;           // per (catalog, tile): start, advanced to end
+    float4 *sorted;             // (u_x, u_y, u_z, w) per placement
+    int32_t *err;               // set to 1 on an out-of-grid index
+    float *rho;                 // output grids, catalog c at c*rstride
+    int64_t rstride;
+};
+
+hipError_t launch_scatter(ScatterArgs &a, uint32_t *scan_tmp, hipStream_t s);
+// Capacity (placements) the sorted buffer must hold for n particles.
+int64_t scatter_capacity(int64_t n, int tsc);
+
+struct GatherArgs {
+    const float *x, *y, *z;
+    int64_t n;
+    const float4 *disp;         // float4 displacement grid
+    const Slot *slot;
+    int nx, ny, nz;
+    int field;                  // 0 disp, 1 rsd, 2 sum
+    int positions;              // 1: write pos - shift
+    int has_los;
+    float los[3];
+    float f;
+    float *ox, *oy, *oz;
+    int32_t *err;
+};
+hipError_t launch_gather(const GatherArgs &a, hipStream_t s);
+
+// Scalar read-back (read_cic!) of one real grid.
+hipError_t launch_read_cic(float *out, const float *field, const float *x, const float *y,
+                           const float *z, int64_t n, const Slot *slot, int nx, int ny, int nz,
+                           int32_t *err, hipStream_t s);
+
+// setup_box min/max reduction into slot->minmax and the box.
+hipError_t launch_setup_box(const float *x, const float *y, const float *z, int64_t n, Slot *slot,
+                            float pad, hipStream_t s);
+
+// ----------------------------------------------------------------- multigrid
+
+struct MGLevel {
+    int n[3];
+    float cell[3], cell2[3], icell2[3], losn[3];
+    const float *xv;   // x[n0] | y[n1] | z[n2] cell centres of this level
+};
+
+hipError_t launch_mg_jacobi(float *vout, const float *vin, const float *f, const MGLevel &L,
+                            float beta, float omega, int has_los, hipStream_t s);
+hipError_t launch_mg_residual(float *r, const float *v, const float *f, const MGLevel &L,
+                              float beta, int has_los, hipStream_t s);
+hipError_t launch_mg_reduce(float *coarse, const float *fine, const int fine_n[3], hipStream_t s);
+hipError_t launch_mg_prolong(float *fine, const float *coarse, const int coarse_n[3], int add,
+                             hipStream_t s);
+hipError_t launch_mg_xvec(float *xv, const Slot *slot, int nx, int ny, int nz, hipStream_t s);
+
+// FP64-capable red-black Gauss-Seidel pieces (RBGS schedule)
+template <typename T>
+hipError_t launch_mg_rbgs(T *v, const T *f, const MGLevel &L, float beta, int has_los, int color,
+                          hipStream_t s);
+template <typename T>
+hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, float beta,
+                                int has_los, hipStream_t s);
+template <typename T, typename U>
+hipError_t launch_mg_reduce_t(T *coarse, const U *fine, const int fine_n[3], hipStream_t s);
+template <typename T, typename U>
+hipError_t launch_mg_prolong_t(T *fine, const U *coarse, const int coarse_n[3], int add,
+                               hipStream_t s);
+// sum of squares of a grid into out (FP64, atomically accumulated)
+template <typename T>
+hipError_t launch_sumsq(const T *a, int64_t n, double *out, hipStream_t s);
+template <typename T>
+hipError_t launch_sum(const T *a, int64_t n, double *out, hipStream_t s);
+template <typename T>
+hipError_t launch_add_const(T *a, int64_t n, const double *num, double den, double sign,
+                            hipStream_t s);
+template <typename T, typename U>
+hipError_t launch_convert(T *dst, const U *src, int64_t n, hipStream_t s);
+hipError_t launch_fill(float *a, int64_t n, float v, hipStream_t s);
+
+}  // namespace baorec

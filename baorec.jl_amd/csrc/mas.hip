@@ -1,0 +1,515 @@
+// Mass assignment for gfx950: CIC (src/mas.jl:1-107) and TSC scatter with
+// particles pre-binned by grid tile and accumulated in LDS, the fused
+// trilinear displacement gather + RSD shift (src/mas.jl:271-327 and
+// src/recon.jl:308-364), read_cic!, and the setup_box reduction
+// (src/utils.jl:100-109).
+//
+// Scatter = four launches over a catalog batch:
+//   count   per particle: apply the reference's one-sided wrap (and write it
+//           back), locate the 1..8 tiles its corners touch, count them;
+//   scan    exclusive prefix sum of the per-tile counts (one workgroup);
+//   place   per particle again: append (u_x, u_y, u_z, w) to each touched tile;
+//   accum   one workgroup per tile: LDS float atomics for every corner inside
+//           the tile, then one coalesced write of the tile (no HBM atomics).
+// u is the reference's FP32 grid coordinate, so the corner indices and
+// weights recomputed in accum are bit-identical to the reference's.
+#include <math.h>
+
+#include "kernels.hpp"
+
+namespace baorec {
+
+__device__ __forceinline__ int find_cat(const int64_t *off, int ncat, int64_t i) {
+    if (ncat <= 1) return 0;
+    int lo = 0, hi = ncat - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Per-axis CIC corners of src/mas.jl:62-84 (0-based).  Returns false when an
+// index leaves 0..n-1 (the reference throws a BoundsError).
+__device__ __forceinline__ bool cic_axis(float u, int n, int wrap, int &i0, int &i1, float &w0,
+                                         float &w1) {
+    if (!(u >= -1.0e9f && u <= 1.0e9f)) return false;  // also rejects NaN
+    const float fl = floorf(u);
+    int x0 = (int)fl;
+    w1 = u - fl;
+    w0 = 1.0f - w1;
+    if (x0 == n + 1) x0 = 1;
+    const int x1 = (x0 == n && wrap) ? 1 : x0 + 1;
+    if (x0 < 1 || x0 > n || x1 < 1 || x1 > n) return false;
+    i0 = x0 - 1;
+    i1 = x1 - 1;
+    return true;
+}
+
+// TSC (extension, no reference counterpart): nodes at integer u - 1 (the same
+// nodes as CIC), nearest node c, weights 0.5(0.5-d)^2, 0.75-d^2, 0.5(0.5+d)^2.
+__device__ __forceinline__ bool tsc_axis(float u, int n, int wrap, int (&ii)[3], float (&ww)[3]) {
+    if (!(u >= -1.0e9f && u <= 1.0e9f)) return false;
+    const float g = u - 1.0f;
+    const float c = floorf(g + 0.5f);
+    const float d = g - c;
+    ww[0] = 0.5f * (0.5f - d) * (0.5f - d);
+    ww[1] = 0.75f - d * d;
+    ww[2] = 0.5f * (0.5f + d) * (0.5f + d);
+    const int ci = (int)c;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        int j = ci - 1 + k;
+        if (wrap) {
+            if (j < 0) j += n;
+            else if (j >= n) j -= n;
+        }
+        if (j < 0 || j >= n) return false;
+        ii[k] = j;
+    }
+    return true;
+}
+
+struct PartU {
+    float u[3];
+    float w;
+    int cat;
+    bool ok;
+};
+
+__device__ __forceinline__ PartU load_particle(const ScatterArgs &a, int64_t i, bool apply_wrap,
+                                               bool write_back) {
+    PartU p;
+    p.cat = find_cat(a.offsets, a.ncat, i);
+    const Slot &sl = a.slot[p.cat * a.slotStride];
+    float x = a.x[i], y = a.y[i], z = a.z[i];
+    if (apply_wrap) {  // src/mas.jl:56-60: one-sided, box_min[1]/box_size[1] for all axes
+        const float b0 = sl.box_min[0], L0 = sl.box_size[0];
+        x = (x - b0) > L0 ? x - L0 : x;
+        y = (y - b0) > L0 ? y - L0 : y;
+        z = (z - b0) > L0 ? z - L0 : z;
+        if (write_back) {
+            a.x[i] = x;
+            a.y[i] = y;
+            a.z[i] = z;
+        }
+    }
+    const float pp[3] = {x, y, z};
+    const int nn[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+    for (int k = 0; k < 3; k++)  // src/mas.jl:62-64
+        p.u[k] = (pp[k] - sl.box_min[k]) * (float)nn[k] / sl.box_size[k] + 1.0f;
+    p.w = a.w[i];
+    p.ok = true;
+    return p;
+}
+
+// Tiles touched along one axis: up to 3 distinct tile indices.
+__device__ __forceinline__ int axis_tiles(const ScatterArgs &a, float u, int n, int t, int nt,
+                                          int (&tl)[3], bool &ok) {
+    int cnt = 0;
+    int idx[3];
+    int m;
+    if (a.tsc) {
+        float ww[3];
+        ok = tsc_axis(u, n, a.wrap, idx, ww) && ok;
+        m = 3;
+    } else {
+        float w0, w1;
+        int i0 = 0, i1 = 0;
+        ok = cic_axis(u, n, a.wrap, i0, i1, w0, w1) && ok;
+        idx[0] = i0;
+        idx[1] = i1;
+        m = 2;
+    }
+    if (!ok) return 0;
+    for (int k = 0; k < m; k++) {
+        int tt = idx[k] / t;
+        bool seen = false;
+        for (int q = 0; q < cnt; q++) seen |= (tl[q] == tt);
+        if (!seen) tl[cnt++] = tt;
+    }
+    (void)nt;
+    return cnt;
+}
+
+__global__ void k_scatter_count(ScatterArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.ntotal) return;
+    PartU p = load_particle(a, i, a.wrap != 0, a.wrap && a.mutate);
+    bool ok = true;
+    int tx[3], ty[3], tz[3];
+    const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
+    const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
+    const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
+    if (!ok) {
+        atomicOr(a.err, 1);
+        return;
+    }
+    const int64_t tbase = (int64_t)p.cat * a.ntx * a.nty * a.ntz;
+    for (int k = 0; k < cz; k++)
+        for (int j = 0; j < cy; j++)
+            for (int l = 0; l < cx; l++)
+                atomicAdd(&a.count[tbase + tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
+}
+
+__global__ void k_scatter_place(ScatterArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.ntotal) return;
+    // positions were already wrapped (and written back) by the count pass
+    PartU p = load_particle(a, i, a.wrap && !a.mutate, false);
+    bool ok = true;
+    int tx[3], ty[3], tz[3];
+    const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
+    const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
+    const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
+    if (!ok) return;
+    const int64_t tbase = (int64_t)p.cat * a.ntx * a.nty * a.ntz;
+    const float4 rec = make_float4(p.u[0], p.u[1], p.u[2], p.w);
+    for (int k = 0; k < cz; k++)
+        for (int j = 0; j < cy; j++)
+            for (int l = 0; l < cx; l++) {
+                const uint32_t slot =
+                    atomicAdd(&a.cursor[tbase + tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
+                a.sorted[slot] = rec;
+            }
+}
+
+// Exclusive scan of n counts into cursor (single workgroup of 1024 threads).
+__global__ void __launch_bounds__(1024) k_scan(const uint32_t *count, uint32_t *cursor, int64_t n) {
+    __shared__ uint32_t part[1024];
+    const int tid = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t lo = tid * per, hi = min(n, lo + per);
+    uint32_t s = 0;
+    for (int64_t i = lo; i < hi; i++) s += count[i];
+    part[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        uint32_t v = tid >= o ? part[tid - o] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = tid ? part[tid - 1] : 0u;
+    for (int64_t i = lo; i < hi; i++) {
+        cursor[i] = run;
+        run += count[i];
+    }
+}
+
+constexpr int kAccumThreads = 256;
+constexpr int kTileMax = 32 * 16 * 16;
+
+template <int TSC>
+__global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, int accumulate) {
+    __shared__ float tile[kTileMax];
+    const int64_t tid_global = blockIdx.x;  // (catalog, tile)
+    const int64_t ntiles = (int64_t)a.ntx * a.nty * a.ntz;
+    const int cat = (int)(tid_global / ntiles);
+    const int64_t tt = tid_global % ntiles;
+    const int tix = (int)(tt % a.ntx), tiy = (int)((tt / a.ntx) % a.nty), tiz = (int)(tt / ((int64_t)a.ntx * a.nty));
+    const int x0 = tix * a.tx, y0 = tiy * a.ty, z0 = tiz * a.tz;
+    const int tcells = a.tx * a.ty * a.tz;
+    for (int c = threadIdx.x; c < tcells; c += kAccumThreads) tile[c] = 0.f;
+    __syncthreads();
+    const uint32_t end = a.cursor[tid_global];
+    const uint32_t beg = end - a.count[tid_global];
+    for (uint32_t s = beg + threadIdx.x; s < end; s += kAccumThreads) {
+        const float4 r = a.sorted[s];
+        if constexpr (TSC == 0) {
+            int ix0, ix1, iy0, iy1, iz0, iz1;
+            float wx0, wx1, wy0, wy1, wz0, wz1;
+            cic_axis(r.x, a.nx, a.wrap, ix0, ix1, wx0, wx1);
+            cic_axis(r.y, a.ny, a.wrap, iy0, iy1, wy0, wy1);
+            cic_axis(r.z, a.nz, a.wrap, iz0, iz1, wz0, wz1);
+            wx0 *= r.w;  // src/mas.jl:86-87
+            wx1 *= r.w;
+            const int lx0 = ix0 - x0, lx1 = ix1 - x0, ly0 = iy0 - y0, ly1 = iy1 - y0;
+            const int lz0 = iz0 - z0, lz1 = iz1 - z0;
+            const bool bx0 = (unsigned)lx0 < (unsigned)a.tx, bx1 = (unsigned)lx1 < (unsigned)a.tx;
+            const bool by0 = (unsigned)ly0 < (unsigned)a.ty, by1 = (unsigned)ly1 < (unsigned)a.ty;
+            const bool bz0 = (unsigned)lz0 < (unsigned)a.tz, bz1 = (unsigned)lz1 < (unsigned)a.tz;
+#define TI(lx, ly, lz) ((lx) + a.tx * ((ly) + a.ty * (lz)))
+            // corner order and products of src/mas.jl:90-97
+            if (bx0 && by0 && bz0) atomicAdd(&tile[TI(lx0, ly0, lz0)], wx0 * wy0 * wz0);
+            if (bx1 && by0 && bz0) atomicAdd(&tile[TI(lx1, ly0, lz0)], wx1 * wy0 * wz0);
+            if (bx0 && by1 && bz0) atomicAdd(&tile[TI(lx0, ly1, lz0)], wx0 * wy1 * wz0);
+            if (bx0 && by0 && bz1) atomicAdd(&tile[TI(lx0, ly0, lz1)], wx0 * wy0 * wz1);
+            if (bx1 && by1 && bz0) atomicAdd(&tile[TI(lx1, ly1, lz0)], wx1 * wy1 * wz0);
+            if (bx1 && by0 && bz1) atomicAdd(&tile[TI(lx1, ly0, lz1)], wx1 * wy0 * wz1);
+            if (bx0 && by1 && bz1) atomicAdd(&tile[TI(lx0, ly1, lz1)], wx0 * wy1 * wz1);
+            if (bx1 && by1 && bz1) atomicAdd(&tile[TI(lx1, ly1, lz1)], wx1 * wy1 * wz1);
+        } else {
+            int ix[3], iy[3], iz[3];
+            float wx[3], wy[3], wz[3];
+            tsc_axis(r.x, a.nx, a.wrap, ix, wx);
+            tsc_axis(r.y, a.ny, a.wrap, iy, wy);
+            tsc_axis(r.z, a.nz, a.wrap, iz, wz);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int lz = iz[k] - z0;
+                if ((unsigned)lz >= (unsigned)a.tz) continue;
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int ly = iy[j] - y0;
+                    if ((unsigned)ly >= (unsigned)a.ty) continue;
+#pragma unroll
+                    for (int l = 0; l < 3; l++) {
+                        const int lx = ix[l] - x0;
+                        if ((unsigned)lx >= (unsigned)a.tx) continue;
+                        atomicAdd(&tile[TI(lx, ly, lz)], ((r.w * wx[l]) * wy[j]) * wz[k]);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    float *rho = a.rho + (int64_t)cat * a.rstride;
+    for (int c = threadIdx.x; c < tcells; c += kAccumThreads) {
+        const int lx = c % a.tx, ly = (c / a.tx) % a.ty, lz = c / (a.tx * a.ty);
+        if (x0 + lx >= a.nx || y0 + ly >= a.ny || z0 + lz >= a.nz) continue;  // partial edge tile
+        const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
+        rho[g] = accumulate ? rho[g] + tile[c] : tile[c];
+    }
+#undef TI
+}
+
+int64_t scatter_capacity(int64_t n, int tsc) {
+    // worst case: every particle touches 8 (CIC) or 27 (TSC) tiles; in
+    // practice ~1.2x for 32x16x16 tiles.  The bound used is exact-safe.
+    return n * (tsc ? 27 : 8);
+}
+
+hipError_t launch_scatter(ScatterArgs &a, uint32_t * /*scan_tmp*/, hipStream_t s) {
+    const int64_t ntiles = (int64_t)a.ntx * a.nty * a.ntz * a.ncat;
+    hipError_t e = hipMemsetAsync(a.count, 0, sizeof(uint32_t) * ntiles, s);
+    if (e != hipSuccess) return e;
+    if (a.ntotal > 0) {
+        const unsigned nb = (unsigned)((a.ntotal + 255) / 256);
+        hipLaunchKernelGGL(k_scatter_count, dim3(nb), dim3(256), 0, s, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.count, a.cursor, ntiles);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (a.ntotal > 0) {
+        const unsigned nb = (unsigned)((a.ntotal + 255) / 256);
+        hipLaunchKernelGGL(k_scatter_place, dim3(nb), dim3(256), 0, s, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_scatter_accum(const ScatterArgs &a, int accumulate, hipStream_t s) {
+    const int64_t ntiles = (int64_t)a.ntx * a.nty * a.ntz * a.ncat;
+    if (a.tsc)
+        hipLaunchKernelGGL(k_scatter_accum<1>, dim3((unsigned)ntiles), dim3(kAccumThreads), 0, s, a,
+                           accumulate);
+    else
+        hipLaunchKernelGGL(k_scatter_accum<0>, dim3((unsigned)ntiles), dim3(kAccumThreads), 0, s, a,
+                           accumulate);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- gather
+
+// 1-based index arithmetic of src/mas.jl:276-292, returned 0-based.
+__device__ __forceinline__ bool gather_axis(float p, float bmin, float L, int n, int &d, int &u,
+                                            float &wu, float &wd) {
+    const float dist = (p - bmin) * (float)n / L;
+    if (!(dist >= -1.0e9f && dist <= 1.0e9f)) return false;
+    const float fl = floorf(dist);
+    const int di = (int)fl;
+    wu = dist - fl;
+    wd = 1.0f - wu;
+    int id = di + 1;
+    if (id > n) id -= n;
+    int iu = id + 1;
+    if (iu > n) iu -= n;
+    if (id < 1 || id > n || iu < 1 || iu > n) return false;
+    d = id - 1;
+    u = iu - 1;
+    return true;
+}
+
+__global__ void k_gather(GatherArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const Slot &sl = a.slot[0];
+    const float px = a.x[i], py = a.y[i], pz = a.z[i];
+    int dx, ux, dy, uy, dz, uz;
+    float wux, wdx, wuy, wdy, wuz, wdz;
+    bool ok = gather_axis(px, sl.box_min[0], sl.box_size[0], a.nx, dx, ux, wux, wdx);
+    ok = gather_axis(py, sl.box_min[1], sl.box_size[1], a.ny, dy, uy, wuy, wdy) && ok;
+    ok = gather_axis(pz, sl.box_min[2], sl.box_size[2], a.nz, dz, uz, wuz, wdz) && ok;
+    if (!ok) {
+        atomicOr(a.err, 1);
+        return;
+    }
+    const int64_t nx = a.nx, nxy = (int64_t)a.nx * a.ny;
+#define G(ix, iy, iz) a.disp[(ix) + nx * (iy) + nxy * (iz)]
+    // term order of src/mas.jl:295-302
+    const float4 f0 = G(dx, dy, dz), f1 = G(dx, dy, uz), f2 = G(dx, uy, dz), f3 = G(dx, uy, uz);
+    const float4 f4 = G(ux, dy, dz), f5 = G(ux, dy, uz), f6 = G(ux, uy, dz), f7 = G(ux, uy, uz);
+#undef G
+    float s[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float g0 = c == 0 ? f0.x : c == 1 ? f0.y : f0.z;
+        const float g1 = c == 0 ? f1.x : c == 1 ? f1.y : f1.z;
+        const float g2 = c == 0 ? f2.x : c == 1 ? f2.y : f2.z;
+        const float g3 = c == 0 ? f3.x : c == 1 ? f3.y : f3.z;
+        const float g4 = c == 0 ? f4.x : c == 1 ? f4.y : f4.z;
+        const float g5 = c == 0 ? f5.x : c == 1 ? f5.y : f5.z;
+        const float g6 = c == 0 ? f6.x : c == 1 ? f6.y : f6.z;
+        const float g7 = c == 0 ? f7.x : c == 1 ? f7.y : f7.z;
+        s[c] = g0 * wdx * wdy * wdz + g1 * wdx * wdy * wuz + g2 * wdx * wuy * wdz +
+               g3 * wdx * wuy * wuz + g4 * wux * wdy * wdz + g5 * wux * wdy * wuz +
+               g6 * wux * wuy * wdz + g7 * wux * wuy * wuz;
+    }
+    float out[3] = {s[0], s[1], s[2]};
+    if (a.field != 0) {
+        float lx, ly, lz;
+        if (a.has_los) {  // src/recon.jl:322-330
+            lx = a.los[0];
+            ly = a.los[1];
+            lz = a.los[2];
+        } else {          // src/recon.jl:308-320
+            const float dist = sqrtf((px * px + py * py) + pz * pz);
+            lx = px / dist;
+            ly = py / dist;
+            lz = pz / dist;
+        }
+        const float dot = ((s[0] * lx) + (s[1] * ly)) + (s[2] * lz);
+        const float fd = a.f * dot;
+        const float r[3] = {fd * lx, fd * ly, fd * lz};
+        if (a.field == 1) {
+            out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
+        } else {  // :sum, src/recon.jl:357-361
+            out[0] = s[0] + r[0]; out[1] = s[1] + r[1]; out[2] = s[2] + r[2];
+        }
+    }
+    if (a.positions) {  // src/recon.jl:376-378
+        out[0] = px - out[0];
+        out[1] = py - out[1];
+        out[2] = pz - out[2];
+    }
+    a.ox[i] = out[0];
+    a.oy[i] = out[1];
+    a.oz[i] = out[2];
+}
+
+hipError_t launch_gather(const GatherArgs &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void k_read_cic(float *out, const float *field, const float *x, const float *y,
+                           const float *z, int64_t n, const Slot *slot, int nx, int ny, int nz,
+                           int32_t *err) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Slot &sl = slot[0];
+    int dx, ux, dy, uy, dz, uz;
+    float wux, wdx, wuy, wdy, wuz, wdz;
+    bool ok = gather_axis(x[i], sl.box_min[0], sl.box_size[0], nx, dx, ux, wux, wdx);
+    ok = gather_axis(y[i], sl.box_min[1], sl.box_size[1], ny, dy, uy, wuy, wdy) && ok;
+    ok = gather_axis(z[i], sl.box_min[2], sl.box_size[2], nz, dz, uz, wuz, wdz) && ok;
+    if (!ok) {
+        atomicOr(err, 1);
+        return;
+    }
+    const int64_t sx = nx, sxy = (int64_t)nx * ny;
+#define F(ix, iy, iz) field[(ix) + sx * (iy) + sxy * (iz)]
+    out[i] = F(dx, dy, dz) * wdx * wdy * wdz + F(dx, dy, uz) * wdx * wdy * wuz +
+             F(dx, uy, dz) * wdx * wuy * wdz + F(dx, uy, uz) * wdx * wuy * wuz +
+             F(ux, dy, dz) * wux * wdy * wdz + F(ux, dy, uz) * wux * wdy * wuz +
+             F(ux, uy, dz) * wux * wuy * wdz + F(ux, uy, uz) * wux * wuy * wuz;
+#undef F
+}
+
+hipError_t launch_read_cic(float *out, const float *field, const float *x, const float *y,
+                           const float *z, int64_t n, const Slot *slot, int nx, int ny, int nz,
+                           int32_t *err, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_read_cic, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, field, x,
+                       y, z, n, slot, nx, ny, nz, err);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- setup_box
+
+__device__ __forceinline__ uint32_t fkey(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fdekey(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ void k_box_init(Slot *slot) {
+    uint32_t *mm = reinterpret_cast<uint32_t *>(slot->minmax);
+    if (threadIdx.x < 3) mm[threadIdx.x] = 0xffffffffu;
+    else if (threadIdx.x < 6) mm[threadIdx.x] = 0u;
+}
+
+__global__ void k_box_reduce(const float *x, const float *y, const float *z, int64_t n, Slot *slot) {
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t k[3] = {fkey(x[i]), fkey(y[i]), fkey(z[i])};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = min(mn[a], k[a]);
+            mx[a] = max(mx[a], k[a]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        for (int o = 32; o > 0; o >>= 1) {
+            mn[a] = min(mn[a], (uint32_t)__shfl_xor((int)mn[a], o, 64));
+            mx[a] = max(mx[a], (uint32_t)__shfl_xor((int)mx[a], o, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        uint32_t *mm = reinterpret_cast<uint32_t *>(slot->minmax);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            atomicMin(&mm[a], mn[a]);
+            atomicMax(&mm[3 + a], mx[a]);
+        }
+    }
+}
+
+// src/utils.jl:100-109 in FP32: min - pad/2, max + pad/2, cubic side.
+__global__ void k_box_final(Slot *slot, float pad) {
+    if (threadIdx.x != 0) return;
+    const uint32_t *mm = reinterpret_cast<const uint32_t *>(slot->minmax);
+    float bmin[3], bmax[3];
+    const float h = pad / 2.0f;
+    for (int a = 0; a < 3; a++) {
+        bmin[a] = fdekey(mm[a]) - h;
+        bmax[a] = fdekey(mm[3 + a]) + h;
+    }
+    float L = bmax[0] - bmin[0];
+    for (int a = 1; a < 3; a++) L = fmaxf(L, bmax[a] - bmin[a]);
+    for (int a = 0; a < 3; a++) {
+        slot->box_min[a] = bmin[a];
+        slot->box_size[a] = L;
+    }
+}
+
+hipError_t launch_setup_box(const float *x, const float *y, const float *z, int64_t n, Slot *slot,
+                            float pad, hipStream_t s) {
+    hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, s, slot);
+    int64_t nb = (n + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_box_reduce, dim3((unsigned)nb), dim3(256), 0, s, x, y, z, n, slot);
+    hipLaunchKernelGGL(k_box_final, dim3(1), dim3(64), 0, s, slot, pad);
+    return hipGetLastError();
+}
+
+}  // namespace baorec

@@ -1,0 +1,359 @@
+// Multigrid kernels for gfx950: the anisotropic 19-point operator of
+// src/multigrid.jl (damped Jacobi, residual, 27-point restriction, trilinear
+// prolongation) and the multicolour Gauss-Seidel smoother of the RBGS
+// schedule.  The FP32 kernels keep the reference's operation order; the
+// templated ones also run FP64 coarse levels.
+#include <math.h>
+
+#include "kernels.hpp"
+
+namespace baorec {
+
+template <typename T>
+struct Coef {
+    T px, py, pz, g, gpx2, gpy2, gpz2;
+};
+
+// src/multigrid.jl:161-168 (per-cell coefficients)
+template <typename T>
+__device__ __forceinline__ Coef<T> coef(const MGLevel &L, int ix, int iy, int iz, T beta, int has_los) {
+    Coef<T> c;
+    if (has_los) {
+        c.px = (T)L.losn[0];
+        c.py = (T)L.losn[1];
+        c.pz = (T)L.losn[2];
+    } else {
+        c.px = (T)L.xv[ix] / (T)L.cell[0];
+        c.py = (T)L.xv[L.n[0] + iy] / (T)L.cell[1];
+        c.pz = (T)L.xv[L.n[0] + L.n[1] + iz] / (T)L.cell[2];
+    }
+    c.g = beta / ((T)L.cell2[0] * (c.px * c.px) + (T)L.cell2[1] * (c.py * c.py) +
+                  (T)L.cell2[2] * (c.pz * c.pz));
+    c.gpx2 = (T)L.icell2[0] + c.g * (c.px * c.px);
+    c.gpy2 = (T)L.icell2[1] + c.g * (c.py * c.py);
+    c.gpz2 = (T)L.icell2[2] + c.g * (c.pz * c.pz);
+    return c;
+}
+
+template <typename T>
+struct Nbr {
+    T A, B, C, X, G;
+};
+
+// Neighbour sums of src/multigrid.jl:170-185 in the reference's order.
+template <typename T>
+__device__ __forceinline__ Nbr<T> nbr(const T *__restrict__ v, const MGLevel &L, int ix, int iy, int iz,
+                                      const Coef<T> &c) {
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const int ixp = ix + 1 == nx ? 0 : ix + 1, ixm = ix == 0 ? nx - 1 : ix - 1;
+    const int iyp = iy + 1 == ny ? 0 : iy + 1, iym = iy == 0 ? ny - 1 : iy - 1;
+    const int izp = iz + 1 == nz ? 0 : iz + 1, izm = iz == 0 ? nz - 1 : iz - 1;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+#define V(a, b, d) v[(a) + sy * (b) + sz * (d)]
+    const T vxp = V(ixp, iy, iz), vxm = V(ixm, iy, iz);
+    const T vyp = V(ix, iyp, iz), vym = V(ix, iym, iz);
+    const T vzp = V(ix, iy, izp), vzm = V(ix, iy, izm);
+    Nbr<T> r;
+    r.A = c.gpx2 * (vxp + vxm);
+    r.B = c.gpy2 * (vyp + vym);
+    r.C = c.gpz2 * (vzp + vzm);
+    r.X = (c.g / (T)2) *
+          ((c.px * c.py) * (V(ixp, iyp, iz) + V(ixm, iym, iz) - V(ixm, iyp, iz) - V(ixp, iym, iz)) +
+           (c.px * c.pz) * (V(ixp, iy, izp) + V(ixm, iy, izm) - V(ixm, iy, izp) - V(ixp, iy, izm)) +
+           (c.py * c.pz) * (V(ix, iyp, izp) + V(ix, iym, izm) - V(ix, iym, izp) - V(ix, iyp, izm)));
+    r.G = c.g * (c.px * (vxp - vxm) + c.py * (vyp - vym) + c.pz * (vzp - vzm));
+#undef V
+    return r;
+}
+
+// One damped Jacobi sweep (src/multigrid.jl:137-191 and the broadcast at :207),
+// out of place: vout = (1 - w) vin + w jac(vin).
+__global__ void k_mg_jacobi(float *__restrict__ vout, const float *__restrict__ vin,
+                            const float *__restrict__ f, MGLevel L, float beta, float omega,
+                            int has_los) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= L.n[0]) return;
+    const Coef<float> c = coef<float>(L, ix, iy, iz, beta, has_los);
+    const Nbr<float> s = nbr<float>(vin, L, ix, iy, iz, c);
+    const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
+    float j = f[I] + s.A + s.B + s.C + s.X;
+    if (!has_los) j += s.G;
+    j /= 2.0f * (c.gpx2 + c.gpy2 + c.gpz2);
+    vout[I] = (1.0f - omega) * vin[I] + omega * j;
+}
+
+// r = f - A v (src/multigrid.jl:328-376 and the broadcast at :393).
+template <typename T>
+__global__ void k_mg_residual(T *__restrict__ r, const T *__restrict__ v, const T *__restrict__ f,
+                              MGLevel L, float beta, int has_los) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= L.n[0]) return;
+    const Coef<T> c = coef<T>(L, ix, iy, iz, (T)beta, has_los);
+    const Nbr<T> s = nbr<T>(v, L, ix, iy, iz, c);
+    const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
+    T t = (T)2 * (c.gpx2 + c.gpy2 + c.gpz2) * v[I] - (s.A + s.B + s.C + s.X);
+    if (!has_los) t -= s.G;
+    r[I] = f[I] - t;
+}
+
+// Multicolour Gauss-Seidel (RBGS schedule): colour = ((ix+iy)&1) + 2((iy+iz)&1)
+// separates every face and edge neighbour of the 19-point stencil, so one
+// colour updates in place without races.
+template <typename T>
+__global__ void k_mg_mcgs(T *__restrict__ v, const T *__restrict__ f, MGLevel L, float beta,
+                          int has_los, int color) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= L.n[0]) return;
+    if ((((ix + iy) & 1) + 2 * ((iy + iz) & 1)) != color) return;
+    const Coef<T> c = coef<T>(L, ix, iy, iz, (T)beta, has_los);
+    const Nbr<T> s = nbr<T>(v, L, ix, iy, iz, c);
+    const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
+    T j = f[I] + s.A + s.B + s.C + s.X;
+    if (!has_los) j += s.G;
+    v[I] = j / ((T)2 * (c.gpx2 + c.gpy2 + c.gpz2));
+}
+
+// 27-point full weighting around fine cell 2c+1 (src/multigrid.jl:586-640).
+template <typename T, typename U>
+__global__ void k_mg_reduce(T *__restrict__ coarse, const U *__restrict__ w, int fx, int fy, int fz) {
+    const int cx = fx / 2, cy = fy / 2;
+    const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ky = blockIdx.y, kz = blockIdx.z;
+    if (kx >= cx) return;
+    const int ix = 2 * kx + 1, iy = 2 * ky + 1, iz = 2 * kz + 1;
+    const int ixp = ix + 1 >= fx ? ix + 1 - fx : ix + 1, ixm = ix - 1;
+    const int iyp = iy + 1 >= fy ? iy + 1 - fy : iy + 1, iym = iy - 1;
+    const int izp = iz + 1 >= fz ? iz + 1 - fz : iz + 1, izm = iz - 1;
+    const int64_t sy = fx, sz = (int64_t)fx * fy;
+#define W(a, b, d) ((T)w[(a) + sy * (b) + sz * (d)])
+    const T s = (T)8 * W(ix, iy, iz) +
+                (T)4 * (W(ixp, iy, iz) + W(ixm, iy, iz) + W(ix, iyp, iz) + W(ix, iym, iz) +
+                        W(ix, iy, izp) + W(ix, iy, izm)) +
+                (T)2 * (W(ixp, iyp, iz) + W(ixm, iyp, iz) + W(ixp, iym, iz) + W(ixm, iym, iz) +
+                        W(ixp, iy, izp) + W(ixm, iy, izp) + W(ixp, iy, izm) + W(ixm, iy, izm) +
+                        W(ix, iyp, izp) + W(ix, iym, izp) + W(ix, iyp, izm) + W(ix, iym, izm)) +
+                W(ixp, iyp, izp) + W(ixm, iyp, izp) + W(ixp, iym, izp) + W(ixm, iym, izp) +
+                W(ixp, iyp, izm) + W(ixm, iyp, izm) + W(ixp, iym, izm) + W(ixm, iym, izm);
+#undef W
+    coarse[kx + (int64_t)cx * (ky + (int64_t)cy * kz)] = (T)((double)s / 64.0);
+}
+
+// Trilinear prolongation (src/multigrid.jl:456-509) evaluated per fine cell:
+// odd fine index 2c+1 copies coarse c, even index 2c+2 averages c and c+1;
+// the sums keep the reference's term order for each of the 8 cases.
+template <typename T, typename U>
+__global__ void k_mg_prolong(T *__restrict__ fine, const U *__restrict__ v, int cx, int cy, int cz,
+                             int add) {
+    const int fx = 2 * cx, fy = 2 * cy;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= fx) return;
+    const int tx = (ix & 1) == 0, ty = (iy & 1) == 0, tz = (iz & 1) == 0;
+    const int ax = tx ? (ix == 0 ? cx - 1 : ix / 2 - 1) : (ix - 1) / 2;
+    const int ay = ty ? (iy == 0 ? cy - 1 : iy / 2 - 1) : (iy - 1) / 2;
+    const int az = tz ? (iz == 0 ? cz - 1 : iz / 2 - 1) : (iz - 1) / 2;
+    const int bx = ax + 1 == cx ? 0 : ax + 1, by = ay + 1 == cy ? 0 : ay + 1;
+    const int bz = az + 1 == cz ? 0 : az + 1;
+    const int64_t sy = cx, sz = (int64_t)cx * cy;
+#define C(a, b, d) v[(a) + sy * (b) + sz * (d)]
+    const U c000 = C(ax, ay, az);
+    U val;
+    const int code = tx + 2 * ty + 4 * tz;
+    switch (code) {
+        case 0: val = c000; break;
+        case 1: val = (c000 + C(bx, ay, az)) / (U)2; break;
+        case 2: val = (c000 + C(ax, by, az)) / (U)2; break;
+        case 4: val = (c000 + C(ax, ay, bz)) / (U)2; break;
+        case 3: val = (c000 + C(bx, ay, az) + C(ax, by, az) + C(bx, by, az)) / (U)4; break;
+        case 6: val = (c000 + C(ax, by, az) + C(ax, ay, bz) + C(ax, by, bz)) / (U)4; break;
+        case 5: val = (c000 + C(bx, ay, az) + C(ax, ay, bz) + C(bx, ay, bz)) / (U)4; break;
+        default:
+            val = (c000 + C(bx, ay, az) + C(ax, by, az) + C(ax, ay, bz) + C(bx, by, az) +
+                   C(bx, ay, bz) + C(ax, by, bz) + C(bx, by, bz)) / (U)8;
+            break;
+    }
+#undef C
+    const int64_t I = ix + (int64_t)fx * (iy + (int64_t)fy * iz);
+    fine[I] = add ? (T)(fine[I] + (T)val) : (T)val;
+}
+
+// Cell centres of every level (x_vec per level, src/multigrid.jl:656).
+__global__ void k_mg_xvec(float *xv, const Slot *slot, int nx, int ny, int nz) {
+    const int n3[3] = {nx, ny, nz};
+    const int tot = nx + ny + nz;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += gridDim.x * blockDim.x) {
+        int ax = idx < nx ? 0 : (idx < nx + ny ? 1 : 2);
+        int i = idx - (ax == 0 ? 0 : (ax == 1 ? nx : nx + ny));
+        const float h = slot->box_size[ax] / (float)n3[ax];
+        xv[idx] = (float)((double)slot->box_min[ax] + ((double)i + 0.5) * (double)h);
+    }
+}
+
+template <typename T>
+__global__ void k_sumsq(const T *a, int64_t n, double *out) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)a[i];
+        s += v * v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+template <typename T>
+__global__ void k_sum(const T *a, int64_t n, double *out) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        s += (double)a[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+// a += sign * (*num) / den  (mean removal with a device-side FP64 total)
+template <typename T>
+__global__ void k_add_const(T *a, int64_t n, const double *num, double den, double sign) {
+    const T c = (T)(sign * (*num) / den);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        a[i] += c;
+}
+
+template <typename T, typename U>
+__global__ void k_convert(T *dst, const U *src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (T)src[i];
+}
+
+__global__ void k_fill(float *a, int64_t n, float v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+// ------------------------------------------------------------ launchers
+
+static inline dim3 cell_grid(int nx, int ny, int nz, int bx) {
+    return dim3((unsigned)((nx + bx - 1) / bx), (unsigned)ny, (unsigned)nz);
+}
+static inline int bx_for(int nx) { return nx >= 256 ? 256 : (nx >= 64 ? 64 : nx); }
+static inline unsigned red_blocks(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    return (unsigned)(b > 1024 ? 1024 : (b < 1 ? 1 : b));
+}
+
+hipError_t launch_mg_jacobi(float *vout, const float *vin, const float *f, const MGLevel &L,
+                            float beta, float omega, int has_los, hipStream_t s) {
+    const int bx = bx_for(L.n[0]);
+    hipLaunchKernelGGL(k_mg_jacobi, cell_grid(L.n[0], L.n[1], L.n[2], bx), dim3(bx), 0, s, vout,
+                       vin, f, L, beta, omega, has_los);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_residual(float *r, const float *v, const float *f, const MGLevel &L,
+                              float beta, int has_los, hipStream_t s) {
+    return launch_mg_residual_t<float>(r, v, f, L, beta, has_los, s);
+}
+
+template <typename T>
+hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, float beta,
+                                int has_los, hipStream_t s) {
+    const int bx = bx_for(L.n[0]);
+    hipLaunchKernelGGL(k_mg_residual<T>, cell_grid(L.n[0], L.n[1], L.n[2], bx), dim3(bx), 0, s, r, v,
+                       f, L, beta, has_los);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_mg_rbgs(T *v, const T *f, const MGLevel &L, float beta, int has_los, int color,
+                          hipStream_t s) {
+    const int bx = bx_for(L.n[0]);
+    hipLaunchKernelGGL(k_mg_mcgs<T>, cell_grid(L.n[0], L.n[1], L.n[2], bx), dim3(bx), 0, s, v, f, L,
+                       beta, has_los, color);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_reduce(float *coarse, const float *fine, const int fine_n[3], hipStream_t s) {
+    return launch_mg_reduce_t<float, float>(coarse, fine, fine_n, s);
+}
+
+template <typename T, typename U>
+hipError_t launch_mg_reduce_t(T *coarse, const U *fine, const int fine_n[3], hipStream_t s) {
+    const int cx = fine_n[0] / 2, cy = fine_n[1] / 2, cz = fine_n[2] / 2;
+    const int bx = bx_for(cx);
+    hipLaunchKernelGGL((k_mg_reduce<T, U>), cell_grid(cx, cy, cz, bx), dim3(bx), 0, s, coarse, fine,
+                       fine_n[0], fine_n[1], fine_n[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_prolong(float *fine, const float *coarse, const int coarse_n[3], int add,
+                             hipStream_t s) {
+    return launch_mg_prolong_t<float, float>(fine, coarse, coarse_n, add, s);
+}
+
+template <typename T, typename U>
+hipError_t launch_mg_prolong_t(T *fine, const U *coarse, const int coarse_n[3], int add,
+                               hipStream_t s) {
+    const int fx = 2 * coarse_n[0], fy = 2 * coarse_n[1], fz = 2 * coarse_n[2];
+    const int bx = bx_for(fx);
+    hipLaunchKernelGGL((k_mg_prolong<T, U>), cell_grid(fx, fy, fz, bx), dim3(bx), 0, s, fine, coarse,
+                       coarse_n[0], coarse_n[1], coarse_n[2], add);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_xvec(float *xv, const Slot *slot, int nx, int ny, int nz, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_xvec, dim3(4), dim3(256), 0, s, xv, slot, nx, ny, nz);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_sumsq(const T *a, int64_t n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sumsq<T>, dim3(red_blocks(n)), dim3(256), 0, s, a, n, out);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_sum(const T *a, int64_t n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum<T>, dim3(red_blocks(n)), dim3(256), 0, s, a, n, out);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_add_const(T *a, int64_t n, const double *num, double den, double sign,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_add_const<T>, dim3(red_blocks(n)), dim3(256), 0, s, a, n, num, den, sign);
+    return hipGetLastError();
+}
+template <typename T, typename U>
+hipError_t launch_convert(T *dst, const U *src, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<T, U>), dim3(red_blocks(n)), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+hipError_t launch_fill(float *a, int64_t n, float v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill, dim3(red_blocks(n)), dim3(256), 0, s, a, n, v);
+    return hipGetLastError();
+}
+
+template hipError_t launch_mg_residual_t<float>(float *, const float *, const float *, const MGLevel &, float, int, hipStream_t);
+template hipError_t launch_mg_residual_t<double>(double *, const double *, const double *, const MGLevel &, float, int, hipStream_t);
+template hipError_t launch_mg_rbgs<float>(float *, const float *, const MGLevel &, float, int, int, hipStream_t);
+template hipError_t launch_mg_rbgs<double>(double *, const double *, const MGLevel &, float, int, int, hipStream_t);
+template hipError_t launch_mg_reduce_t<float, float>(float *, const float *, const int *, hipStream_t);
+template hipError_t launch_mg_reduce_t<double, float>(double *, const float *, const int *, hipStream_t);
+template hipError_t launch_mg_reduce_t<double, double>(double *, const double *, const int *, hipStream_t);
+template hipError_t launch_mg_prolong_t<float, float>(float *, const float *, const int *, int, hipStream_t);
+template hipError_t launch_mg_prolong_t<float, double>(float *, const double *, const int *, int, hipStream_t);
+template hipError_t launch_mg_prolong_t<double, double>(double *, const double *, const int *, int, hipStream_t);
+template hipError_t launch_sumsq<float>(const float *, int64_t, double *, hipStream_t);
+template hipError_t launch_sumsq<double>(const double *, int64_t, double *, hipStream_t);
+template hipError_t launch_sum<float>(const float *, int64_t, double *, hipStream_t);
+template hipError_t launch_sum<double>(const double *, int64_t, double *, hipStream_t);
+template hipError_t launch_add_const<float>(float *, int64_t, const double *, double, double, hipStream_t);
+template hipError_t launch_add_const<double>(double *, int64_t, const double *, double, double, hipStream_t);
+template hipError_t launch_convert<double, float>(double *, const float *, int64_t, hipStream_t);
+template hipError_t launch_convert<float, double>(float *, const double *, int64_t, hipStream_t);
+
+}  // namespace baorec

@@ -1,0 +1,37 @@
+"""baorec_hip: MI355X (gfx950) reconstruction hot path of BAOrec.jl.
+
+The compute runs in libbaorec_hip.so (hand-written HIP); this package is the
+Python mirror of the Julia binding that drives its C ABI.
+"""
+from ._lib import LIB_PATH, load  # noqa: F401
+from .api import (  # noqa: F401
+    BaorecError,
+    BoundsError,
+    Context,
+    IterativeRecon,
+    MultigridRecon,
+    ResultCache,
+    cic_,
+    default_context,
+    fmg,
+    irfft3d,
+    jacobi_,
+    k_vec,
+    prolong_,
+    read_cic_,
+    read_shifts,
+    reconstructed_overdensity_,
+    reconstructed_positions,
+    reconstructed_potential_,
+    reduce_,
+    residual_,
+    rfft3d,
+    run_,
+    setup_box,
+    setup_overdensity_,
+    smooth_,
+    vcycle_,
+    x_vec,
+)
+
+__version__ = "0.1.0"

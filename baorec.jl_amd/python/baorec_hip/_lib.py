@@ -1,0 +1,118 @@
+"""ctypes binding of libbaorec_hip.so (declarations in include/baorec.h).
+
+This is the Python mirror of the Julia ``ccall`` binding: one function per
+C entry point, same argument order.  Loading fails loudly when the library is
+missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+LIB_PATH = os.environ.get("BAOREC_HIP_LIB", os.path.join(_PKG_ROOT, "libbaorec_hip.so"))
+
+OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_NO_DEVICE = -2
+ERR_HIP = -3
+ERR_OUT_OF_MEMORY = -4
+ERR_OUT_OF_BOX = -5
+ERR_NO_RESULT = -6
+ERR_UNSUPPORTED = -7
+
+MEM_HOST = 0
+MEM_DEVICE = 1
+ITERATIVE = 0
+MULTIGRID = 1
+FIELD = {"disp": 0, "rsd": 1, "sum": 2}
+MAS = {"cic": 0, "tsc": 1}
+MG_SCHEDULE = {"reference": 0, "rbgs": 1}
+
+
+class Params(ctypes.Structure):
+    """struct baorec_params (include/baorec.h)."""
+    _fields_ = [
+        ("bias", ctypes.c_float),
+        ("f", ctypes.c_float),
+        ("smoothing_radius", ctypes.c_float),
+        ("box_size", ctypes.c_float * 3),
+        ("box_min", ctypes.c_float * 3),
+        ("has_los", ctypes.c_int32),
+        ("los", ctypes.c_float * 3),
+        ("n_iter", ctypes.c_int32),
+        ("beta", ctypes.c_float),
+        ("jacobi_damping_factor", ctypes.c_float),
+        ("jacobi_niterations", ctypes.c_int32),
+        ("vcycle_niterations", ctypes.c_int32),
+        ("mas", ctypes.c_int32),
+        ("mg_schedule", ctypes.c_int32),
+        ("tolerance", ctypes.c_double),
+        ("max_vcycles", ctypes.c_int32),
+        ("rb_sweeps", ctypes.c_int32),
+        ("fp64_coarse_max", ctypes.c_int32),
+        ("box_pad", ctypes.c_float),
+        ("ran_min", ctypes.c_float),
+        ("mutate_inputs", ctypes.c_int32),
+        ("vcycles_done", ctypes.c_int32),
+        ("final_residual", ctypes.c_double),
+    ]
+
+
+P = ctypes.c_void_p
+I32P = ctypes.POINTER(ctypes.c_int32)
+FP = ctypes.c_void_p  # float* (host or device address)
+I64 = ctypes.c_int64
+INT = ctypes.c_int
+PP = ctypes.POINTER(Params)
+
+# name -> (restype, argtypes); mirrors include/baorec.h
+SIGNATURES = {
+    "baorec_params_default": (None, [PP, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "baorec_create": (INT, [ctypes.POINTER(P), INT]),
+    "baorec_destroy": (INT, [P]),
+    "baorec_last_error": (ctypes.c_char_p, [P]),
+    "baorec_version": (ctypes.c_char_p, []),
+    "baorec_run_iterative": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),
+    "baorec_run_multigrid": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),
+    "baorec_result_info": (INT, [P, I32P, I32P]),
+    "baorec_result_copy": (INT, [P, INT, FP]),
+    "baorec_setup_overdensity": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),
+    "baorec_reconstructed_overdensity": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),
+    "baorec_reconstructed_potential": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),
+    "baorec_read_shifts": (INT, [P, PP, INT, FP, FP, FP, I64, FP, I32P, INT, INT, FP, FP, FP]),
+    "baorec_reconstructed_positions": (INT, [P, PP, INT, FP, FP, FP, I64, FP, I32P, INT, INT, FP, FP, FP]),
+    "baorec_setup_box": (INT, [P, INT, FP, FP, FP, I64, ctypes.c_float, FP, FP]),
+    "baorec_cic": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, FP, I64, INT]),
+    "baorec_read_cic": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, FP, I64]),
+    "baorec_smooth": (INT, [P, PP, I32P, INT, FP]),
+    "baorec_rfft3d": (INT, [P, I32P, INT, FP, FP]),
+    "baorec_irfft3d": (INT, [P, I32P, INT, FP, FP]),
+    "baorec_k_vec": (INT, [I32P, FP, FP, FP, FP]),
+    "baorec_x_vec": (INT, [I32P, FP, FP, FP, FP, FP]),
+    "baorec_mg_jacobi": (INT, [P, PP, I32P, INT, FP, FP]),
+    "baorec_mg_residual": (INT, [P, PP, I32P, INT, FP, FP, FP]),
+    "baorec_mg_reduce": (INT, [P, I32P, INT, FP, FP]),
+    "baorec_mg_prolong": (INT, [P, I32P, INT, FP, FP, INT]),
+    "baorec_mg_vcycle": (INT, [P, PP, I32P, INT, FP, FP]),
+    "baorec_mg_fmg": (INT, [P, PP, I32P, INT, FP, FP]),
+    "baorec_run_batch": (INT, [P, PP, I32P, INT, INT, FP, FP, FP, FP, FP, FP, FP, FP]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libbaorec_hip.so (raises OSError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"libbaorec_hip.so not found at {LIB_PATH}: build it first "
+                          "(python -c 'import __graft_entry__ as g; g.build()')")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib

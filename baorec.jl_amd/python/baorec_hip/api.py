@@ -1,0 +1,490 @@
+"""Python mirror of BAOrec.jl's reconstruction API over libbaorec_hip.
+
+Names follow the Julia package (``!`` becomes a trailing ``_``):
+
+    IterativeRecon, MultigridRecon          src/recon.jl:2-33
+    run_(recon, grid, x,y,z,w[, rx..rw])    run!, src/recon.jl:134-261
+    setup_overdensity_, reconstructed_overdensity_, reconstructed_potential_
+    read_shifts, reconstructed_positions    src/recon.jl:264-388
+    k_vec, x_vec, setup_box                 src/utils.jl:3-25,100-109
+    cic_, read_cic_, smooth_                src/mas.jl, src/utils.jl
+    jacobi_, residual_, reduce_, prolong_, vcycle_, fmg   src/multigrid.jl
+
+Particle arrays are float32 numpy arrays (host) or CUDA/HIP torch tensors
+(device, used in place).  Grids are numpy arrays of shape (nz, ny, nx), i.e.
+the bytes of Julia's column-major (nx, ny, nz) arrays.  Like the reference,
+the periodic-box path (no randoms) wraps and overwrites input positions.
+Every call runs on the GPU; errors raise BaorecError.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from dataclasses import dataclass, field as dfield
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+F32 = np.float32
+
+
+class BaorecError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"baorec error {code}: {msg}")
+        self.code = code
+
+
+class BoundsError(BaorecError, IndexError):
+    """A particle indexes outside the grid (the reference throws BoundsError)."""
+
+
+class Context:
+    """One GPU context (stream + cached buffers + result_cache)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.baorec_create(ctypes.byref(h), int(device))
+        if rc != L.OK:
+            raise BaorecError(rc, self.lib.baorec_last_error(None).decode())
+        self.handle = h
+        self.device = device
+
+    def check(self, rc: int):
+        if rc != L.OK:
+            msg = self.lib.baorec_last_error(self.handle).decode()
+            if rc == L.ERR_OUT_OF_BOX:
+                raise BoundsError(rc, msg)
+            raise BaorecError(rc, msg)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.baorec_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_tls = threading.local()
+
+
+def default_context() -> Context:
+    ctx = getattr(_tls, "ctx", None)
+    if ctx is None:
+        ctx = Context(0)
+        _tls.ctx = ctx
+    return ctx
+
+
+# ------------------------------------------------------------------ recon types
+
+@dataclass
+class _Recon:
+    bias: float
+    f: float
+    smoothing_radius: float
+    box_size: Optional[Sequence[float]] = None
+    box_min: Optional[Sequence[float]] = None
+    los: Optional[Sequence[float]] = None
+    beta: Optional[float] = None
+    # extensions (defaults reproduce the reference)
+    mas: str = "cic"
+    box_pad: float = 500.0
+    ran_min: float = 0.01
+    mutate_inputs: bool = True
+    result_cache: object = None
+    ctx: Optional[Context] = dfield(default=None, repr=False)
+
+    def __post_init__(self):
+        if self.beta is None:
+            self.beta = float(F32(F32(self.f) / F32(self.bias)))
+
+    def context(self) -> Context:
+        if self.ctx is None:
+            self.ctx = default_context()
+        return self.ctx
+
+    def params(self) -> L.Params:
+        p = L.Params()
+        self.context().lib.baorec_params_default(ctypes.byref(p), float(self.bias), float(self.f),
+                                                 float(self.smoothing_radius))
+        p.beta = float(self.beta)
+        if self.box_size is not None:
+            for a in range(3):
+                p.box_size[a] = float(self.box_size[a])
+        if self.box_min is not None:
+            for a in range(3):
+                p.box_min[a] = float(self.box_min[a])
+        if self.los is not None:
+            p.has_los = 1
+            for a in range(3):
+                p.los[a] = float(self.los[a])
+        p.mas = L.MAS[self.mas]
+        p.box_pad = float(self.box_pad)
+        p.ran_min = float(self.ran_min)
+        p.mutate_inputs = int(bool(self.mutate_inputs))
+        return p
+
+    def _take_box(self, p: L.Params):
+        self.box_size = np.array(list(p.box_size), dtype=F32)
+        self.box_min = np.array(list(p.box_min), dtype=F32)
+
+
+@dataclass
+class IterativeRecon(_Recon):
+    """IterativeRecon, src/recon.jl:2-16."""
+    n_iter: int = 3
+
+    kind = L.ITERATIVE
+
+    def params(self):
+        p = super().params()
+        p.n_iter = int(self.n_iter)
+        return p
+
+
+@dataclass
+class MultigridRecon(_Recon):
+    """MultigridRecon, src/recon.jl:18-33 (+ RBGS schedule extension)."""
+    jacobi_damping_factor: float = 0.4
+    jacobi_niterations: int = 5
+    vcycle_niterations: int = 6
+    schedule: str = "reference"
+    tolerance: float = 1e-6
+    max_vcycles: int = 200
+    rb_sweeps: int = 2
+    fp64_coarse_max: int = 8
+    vcycles_done: int = 0
+    final_residual: float = float("nan")
+
+    kind = L.MULTIGRID
+
+    def params(self):
+        p = super().params()
+        p.jacobi_damping_factor = float(self.jacobi_damping_factor)
+        p.jacobi_niterations = int(self.jacobi_niterations)
+        p.vcycle_niterations = int(self.vcycle_niterations)
+        p.mg_schedule = L.MG_SCHEDULE[self.schedule]
+        p.tolerance = float(self.tolerance)
+        p.max_vcycles = int(self.max_vcycles)
+        p.rb_sweeps = int(self.rb_sweeps)
+        p.fp64_coarse_max = int(self.fp64_coarse_max)
+        return p
+
+
+@dataclass
+class ResultCache:
+    """recon.result_cache: the field lives on the GPU inside the context."""
+    ctx: Context
+    grid: tuple
+    kind: int
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.grid[::-1], dtype=F32)
+        self.ctx.check(self.ctx.lib.baorec_result_copy(self.ctx.handle, L.MEM_HOST, _ptr(out)))
+        return out
+
+
+# ------------------------------------------------------------------ arrays
+
+def _is_device(a) -> bool:
+    return hasattr(a, "is_cuda") and a.is_cuda
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if _is_device(a):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _mem(*arrays) -> int:
+    dev = [_is_device(a) for a in arrays if a is not None]
+    if any(dev) and not all(dev):
+        raise ValueError("mix of host and device arrays")
+    return L.MEM_DEVICE if dev and dev[0] else L.MEM_HOST
+
+
+def _f32(a, writable=False):
+    if a is None:
+        return None
+    if _is_device(a):
+        import torch
+        if a.dtype != torch.float32 or not a.is_contiguous():
+            raise TypeError("device arrays must be contiguous float32 tensors")
+        return a
+    if not (isinstance(a, np.ndarray) and a.dtype == F32 and a.flags.c_contiguous):
+        if writable:
+            raise TypeError("positions must be contiguous float32 numpy arrays (they may be wrapped in place)")
+        a = np.ascontiguousarray(a, dtype=F32)
+    return a
+
+
+def _grid(g) -> ctypes.Array:
+    g = tuple(int(v) for v in g)
+    if len(g) != 3:
+        raise ValueError("grid_size must have 3 entries")
+    return (ctypes.c_int32 * 3)(*g)
+
+
+def _cat(x, y, z, w, rx=None, ry=None, rz=None, rw=None):
+    x, y, z = (_f32(a, writable=True) for a in (x, y, z))
+    w = _f32(w)
+    n = len(x)
+    if not (len(y) == n and len(z) == n and len(w) == n):
+        raise ValueError("x, y, z, w lengths differ")
+    if rx is not None:
+        rx, ry, rz, rw = (_f32(a) for a in (rx, ry, rz, rw))
+        nr = len(rx)
+        if not (len(ry) == nr and len(rz) == nr and len(rw) == nr):
+            raise ValueError("randoms lengths differ")
+    else:
+        nr = 0
+    return x, y, z, w, rx, ry, rz, rw, n, nr
+
+
+def _new_grid(grid, like):
+    nx, ny, nz = grid
+    if _is_device(like):
+        import torch
+        return torch.empty((nz, ny, nx), dtype=torch.float32, device=like.device)
+    return np.zeros((nz, ny, nx), dtype=F32)
+
+
+# ------------------------------------------------------------------ run!
+
+def run_(recon: _Recon, grid_size, x, y, z, w, rx=None, ry=None, rz=None, rw=None, copy=True):
+    """run!(recon, grid_size, x,y,z,w[, rx,ry,rz,rw]), src/recon.jl:134-261.
+    Returns the result grid (delta for iterative, phi for multigrid) as a host
+    array when copy=True; the result also stays cached on the GPU."""
+    ctx = recon.context()
+    x, y, z, w, rx, ry, rz, rw, n, nr = _cat(x, y, z, w, rx, ry, rz, rw)
+    mem = _mem(x, y, z, w, rx, ry, rz, rw)
+    p = recon.params()
+    fn = ctx.lib.baorec_run_iterative if recon.kind == L.ITERATIVE else ctx.lib.baorec_run_multigrid
+    ctx.check(fn(ctx.handle, ctypes.byref(p), _grid(grid_size), mem, _ptr(x), _ptr(y), _ptr(z), _ptr(w),
+                 n, _ptr(rx), _ptr(ry), _ptr(rz), _ptr(rw), nr))
+    if nr > 0:
+        recon._take_box(p)
+    if isinstance(recon, MultigridRecon):
+        recon.vcycles_done = p.vcycles_done
+        recon.final_residual = p.final_residual
+    recon.result_cache = ResultCache(ctx, tuple(int(v) for v in grid_size), recon.kind)
+    return recon.result_cache.numpy() if copy else recon.result_cache
+
+
+def _grid_call(name, recon, delta, x, y, z, w, rx, ry, rz, rw, pass_p_mut=False):
+    ctx = recon.context()
+    x, y, z, w, rx, ry, rz, rw, n, nr = _cat(x, y, z, w, rx, ry, rz, rw)
+    mem = _mem(delta, x, y, z, w, rx, ry, rz, rw)
+    grid = tuple(delta.shape[::-1])
+    p = recon.params()
+    ctx.check(getattr(ctx.lib, name)(ctx.handle, ctypes.byref(p), _grid(grid), mem, _ptr(delta), _ptr(x),
+                                     _ptr(y), _ptr(z), _ptr(w), n, _ptr(rx), _ptr(ry), _ptr(rz), _ptr(rw), nr))
+    if isinstance(recon, MultigridRecon):
+        recon.vcycles_done = p.vcycles_done
+        recon.final_residual = p.final_residual
+    return delta
+
+
+def setup_overdensity_(delta, recon, x, y, z, w, rx=None, ry=None, rz=None, rw=None):
+    """setup_overdensity!, src/recon.jl:42-91 (delta is overwritten)."""
+    return _grid_call("baorec_setup_overdensity", recon, delta, x, y, z, w, rx, ry, rz, rw)
+
+
+def reconstructed_overdensity_(delta, recon: IterativeRecon, x, y, z, w, rx=None, ry=None, rz=None, rw=None):
+    """reconstructed_overdensity!, src/recon.jl:93-132."""
+    return _grid_call("baorec_reconstructed_overdensity", recon, delta, x, y, z, w, rx, ry, rz, rw)
+
+
+def reconstructed_potential_(phi, recon: MultigridRecon, x, y, z, w, rx=None, ry=None, rz=None, rw=None):
+    """reconstructed_potential!, src/recon.jl:184-212."""
+    return _grid_call("baorec_reconstructed_potential", recon, phi, x, y, z, w, rx, ry, rz, rw)
+
+
+# ------------------------------------------------------------------ shifts
+
+def _shifts(name, recon, x, y, z, field3d, field):
+    ctx = recon.context()
+    x, y, z = (_f32(a) for a in (x, y, z))
+    n = len(x)
+    mem = _mem(x, y, z, field3d)
+    if _is_device(x):
+        import torch
+        outs = [torch.empty(n, dtype=torch.float32, device=x.device) for _ in range(3)]
+    else:
+        outs = [np.empty(n, dtype=F32) for _ in range(3)]
+    if field3d is None or isinstance(field3d, ResultCache):
+        fptr, grid = None, (ctypes.c_int32 * 3)(0, 0, 0)
+    else:
+        field3d = _f32(field3d)
+        fptr, grid = _ptr(field3d), _grid(field3d.shape[::-1])
+    p = recon.params()
+    ctx.check(getattr(ctx.lib, name)(ctx.handle, ctypes.byref(p), mem, _ptr(x), _ptr(y), _ptr(z), n, fptr,
+                                     grid, recon.kind, L.FIELD[field], *(_ptr(o) for o in outs)))
+    return tuple(outs)
+
+
+def read_shifts(recon, x, y, z, field3d=None, field="disp"):
+    """read_shifts(recon, x,y,z[, field3d]; field), src/recon.jl:264-364."""
+    return _shifts("baorec_read_shifts", recon, x, y, z, field3d, field)
+
+
+def reconstructed_positions(recon, x, y, z, field3d=None, field="disp"):
+    """reconstructed_positions, src/recon.jl:366-388."""
+    return _shifts("baorec_reconstructed_positions", recon, x, y, z, field3d, field)
+
+
+# ------------------------------------------------------------------ utils
+
+def k_vec(grid, box_size):
+    """k_vec, src/utils.jl:3-10 (host)."""
+    lib = L.load()
+    nx, ny, nz = (int(v) for v in grid)
+    out = [np.empty(nx // 2 + 1, F32), np.empty(ny, F32), np.empty(nz, F32)]
+    bs = np.asarray(box_size, dtype=F32)
+    rc = lib.baorec_k_vec(_grid(grid), _ptr(bs), *(_ptr(o) for o in out))
+    if rc:
+        raise BaorecError(rc, "k_vec")
+    return tuple(out)
+
+
+def x_vec(grid, box_size, box_min):
+    """x_vec, src/utils.jl:21-25 (host)."""
+    lib = L.load()
+    out = [np.empty(int(n), F32) for n in grid]
+    bs = np.asarray(box_size, dtype=F32)
+    bm = np.asarray(box_min, dtype=F32)
+    rc = lib.baorec_x_vec(_grid(grid), _ptr(bs), _ptr(bm), *(_ptr(o) for o in out))
+    if rc:
+        raise BaorecError(rc, "x_vec")
+    return tuple(out)
+
+
+def setup_box(x, y, z, box_pad, ctx: Optional[Context] = None):
+    """setup_box, src/utils.jl:100-109 (GPU reduction)."""
+    ctx = ctx or default_context()
+    x, y, z = (_f32(a) for a in (x, y, z))
+    bs = np.zeros(3, F32)
+    bm = np.zeros(3, F32)
+    ctx.check(ctx.lib.baorec_setup_box(ctx.handle, _mem(x, y, z), _ptr(x), _ptr(y), _ptr(z), len(x),
+                                       float(box_pad), _ptr(bs), _ptr(bm)))
+    return bs, bm
+
+
+def cic_(rho, x, y, z, w, recon_or_box, wrap=True):
+    """cic!(rho, x,y,z,w, box_size, box_min; wrap), src/mas.jl:1-107 (adds into rho)."""
+    recon = recon_or_box
+    ctx = recon.context()
+    x, y, z = (_f32(a, writable=True) for a in (x, y, z))
+    w = _f32(w)
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_cic(ctx.handle, ctypes.byref(p), _grid(rho.shape[::-1]), _mem(rho, x, y, z, w),
+                                 _ptr(rho), _ptr(x), _ptr(y), _ptr(z), _ptr(w), len(x), int(bool(wrap))))
+    return rho
+
+
+def read_cic_(out, field, x, y, z, recon):
+    """read_cic!(out, field, x,y,z, box_size, box_min), src/mas.jl:271-327."""
+    ctx = recon.context()
+    x, y, z = (_f32(a) for a in (x, y, z))
+    field = _f32(field)
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_read_cic(ctx.handle, ctypes.byref(p), _grid(field.shape[::-1]),
+                                      _mem(out, field, x, y, z), _ptr(out), _ptr(field), _ptr(x), _ptr(y),
+                                      _ptr(z), len(x)))
+    return out
+
+
+def smooth_(field, recon):
+    """smooth!(field, R, box_size, plan), src/utils.jl:85-96 (in place)."""
+    ctx = recon.context()
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_smooth(ctx.handle, ctypes.byref(p), _grid(field.shape[::-1]), _mem(field),
+                                    _ptr(field)))
+    return field
+
+
+def rfft3d(field, ctx: Optional[Context] = None):
+    """fft_plan * field (R2C over all three axes, x halved)."""
+    ctx = ctx or default_context()
+    field = _f32(field)
+    nz, ny, nx = field.shape
+    out = np.empty((nz, ny, nx // 2 + 1), dtype=np.complex64)
+    ctx.check(ctx.lib.baorec_rfft3d(ctx.handle, _grid((nx, ny, nz)), L.MEM_HOST, _ptr(field), _ptr(out)))
+    return out
+
+
+def irfft3d(spec, grid, ctx: Optional[Context] = None):
+    """ldiv!(field, fft_plan, spec): normalised C2R."""
+    ctx = ctx or default_context()
+    spec = np.ascontiguousarray(spec, dtype=np.complex64)
+    nx, ny, nz = grid
+    out = np.empty((nz, ny, nx), dtype=F32)
+    ctx.check(ctx.lib.baorec_irfft3d(ctx.handle, _grid(grid), L.MEM_HOST, _ptr(spec), _ptr(out)))
+    return out
+
+
+# ------------------------------------------------------------------ multigrid
+
+def jacobi_(v, f, recon: MultigridRecon):
+    """jacobi!, src/multigrid.jl:193-210 (in place on v)."""
+    ctx = recon.context()
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_mg_jacobi(ctx.handle, ctypes.byref(p), _grid(v.shape[::-1]), _mem(v, f), _ptr(v),
+                                       _ptr(_f32(f))))
+    return v
+
+
+def residual_(r, v, f, recon: MultigridRecon):
+    """residual!, src/multigrid.jl:378-395."""
+    ctx = recon.context()
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_mg_residual(ctx.handle, ctypes.byref(p), _grid(v.shape[::-1]), _mem(r, v, f),
+                                         _ptr(r), _ptr(_f32(v)), _ptr(_f32(f))))
+    return r
+
+
+def reduce_(coarse, fine, ctx: Optional[Context] = None):
+    """reduce!, src/multigrid.jl:641-651."""
+    ctx = ctx or default_context()
+    ctx.check(ctx.lib.baorec_mg_reduce(ctx.handle, _grid(fine.shape[::-1]), _mem(coarse, fine), _ptr(coarse),
+                                       _ptr(_f32(fine))))
+    return coarse
+
+
+def prolong_(fine, coarse, add=False, ctx: Optional[Context] = None):
+    """prolong!, src/multigrid.jl:511-518 (add=True adds into fine)."""
+    ctx = ctx or default_context()
+    ctx.check(ctx.lib.baorec_mg_prolong(ctx.handle, _grid(coarse.shape[::-1]), _mem(fine, coarse), _ptr(fine),
+                                        _ptr(_f32(coarse)), int(bool(add))))
+    return fine
+
+
+def vcycle_(v, f, recon: MultigridRecon):
+    """vcycle!, src/multigrid.jl:689-719 (in place on v)."""
+    ctx = recon.context()
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_mg_vcycle(ctx.handle, ctypes.byref(p), _grid(v.shape[::-1]), _mem(v, f), _ptr(v),
+                                       _ptr(_f32(f))))
+    return v
+
+
+def fmg(f, recon: MultigridRecon, v=None):
+    """fmg, src/multigrid.jl:722-752 (returns v)."""
+    ctx = recon.context()
+    if v is None:
+        v = _new_grid(f.shape[::-1], f)
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_mg_fmg(ctx.handle, ctypes.byref(p), _grid(f.shape[::-1]), _mem(v, f), _ptr(v),
+                                    _ptr(_f32(f))))
+    recon.vcycles_done = p.vcycles_done
+    recon.final_residual = p.final_residual
+    return v

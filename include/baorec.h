@@ -1,0 +1,274 @@
+/*
+ * baorec.h — C ABI of the MI355X (gfx950) reconstruction library
+ * libbaorec_hip.so.
+ *
+ * It replaces the reconstruction entry points of BAOrec.jl (a Julia package
+ * whose GPU path is CUDA.jl + KernelAbstractions + CUFFT).  Each declaration
+ * names the Julia method it stands in for (file:line in BAOrec.jl).  A Julia
+ * host calls these through `ccall`; the Python package baorec_hip mirrors that
+ * binding through ctypes.
+ *
+ * Conventions
+ *  - Grids are FP32, x fastest: element (i, j, k) of an (nx, ny, nz) grid is at
+ *    i + nx*(j + ny*k), which is Julia's column-major Array{Float32,3}.
+ *  - Particles are FP32 structure-of-arrays (x, y, z, w).
+ *  - `mem` says where every pointer argument of the call lives:
+ *    BAOREC_MEM_HOST (Julia Array) or BAOREC_MEM_DEVICE (a device pointer on
+ *    the context's GPU, e.g. a ROCArray).  Host inputs are copied in, host
+ *    outputs copied out; device pointers are used in place.
+ *  - Every call is synchronous on return and returns 0 or a negative
+ *    baorec_status; it never aborts.  baorec_last_error() gives the message.
+ *  - One context per host thread; calls on one context are serialised.
+ */
+#ifndef BAOREC_H
+#define BAOREC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct baorec_ctx baorec_ctx;
+
+typedef enum {
+    BAOREC_OK = 0,
+    BAOREC_ERR_INVALID_ARGUMENT = -1, /* null pointer, bad size, bad enum */
+    BAOREC_ERR_NO_DEVICE = -2,        /* no usable gfx950 GPU */
+    BAOREC_ERR_HIP = -3,              /* a HIP runtime call failed */
+    BAOREC_ERR_OUT_OF_MEMORY = -4,
+    BAOREC_ERR_OUT_OF_BOX = -5,       /* a particle indexes outside the grid:
+                                         the reference throws BoundsError */
+    BAOREC_ERR_NO_RESULT = -6,        /* no cached result for this call */
+    BAOREC_ERR_UNSUPPORTED = -7       /* e.g. a grid side that is not 2^k */
+} baorec_status;
+
+typedef enum { BAOREC_MEM_HOST = 0, BAOREC_MEM_DEVICE = 1 } baorec_mem;
+
+typedef enum { BAOREC_ITERATIVE = 0, BAOREC_MULTIGRID = 1 } baorec_recon_kind;
+
+/* read_shifts(...; field = :disp | :rsd | :sum), src/recon.jl:264-306 */
+typedef enum {
+    BAOREC_FIELD_DISP = 0,
+    BAOREC_FIELD_RSD = 1,
+    BAOREC_FIELD_SUM = 2
+} baorec_field;
+
+/* Mass-assignment scheme.  CIC is the reference's (src/mas.jl:1-99); TSC is
+ * an extension with no reference counterpart. */
+typedef enum { BAOREC_MAS_CIC = 0, BAOREC_MAS_TSC = 1 } baorec_mas;
+
+/* Multigrid schedule.  REFERENCE is fmg with damped Jacobi, a fixed number of
+ * V-cycles per level (src/multigrid.jl:654-752).  RBGS is an extension:
+ * red-black Gauss-Seidel V-cycles until the relative residual (FP64 norm)
+ * drops below `tolerance`, with levels of side <= fp64_coarse_max in FP64. */
+typedef enum { BAOREC_MG_REFERENCE = 0, BAOREC_MG_RBGS = 1 } baorec_mg_schedule;
+
+/*
+ * Fields of IterativeRecon (src/recon.jl:2-16) and MultigridRecon
+ * (src/recon.jl:18-33), plus extension fields.  Fill with
+ * baorec_params_default() and override.
+ */
+typedef struct baorec_params {
+    float bias;
+    float f;
+    float smoothing_radius;
+    float box_size[3];
+    float box_min[3];
+    int32_t has_los;          /* 0: los = nothing (local radial LOS) */
+    float los[3];
+    int32_t n_iter;           /* IterativeRecon.n_iter, default 3 */
+    float beta;               /* f / bias */
+    float jacobi_damping_factor; /* 0.4 */
+    int32_t jacobi_niterations;  /* 5 */
+    int32_t vcycle_niterations;  /* 6 */
+    /* ---- extensions (defaults reproduce the reference) ---- */
+    int32_t mas;              /* baorec_mas, default CIC */
+    int32_t mg_schedule;      /* baorec_mg_schedule, default REFERENCE */
+    double tolerance;         /* RBGS: relative residual target, 1e-6 */
+    int32_t max_vcycles;      /* RBGS: cap on V-cycles, 200 */
+    int32_t rb_sweeps;        /* RBGS: pre- and post-smoothing sweeps, 2 */
+    int32_t fp64_coarse_max;  /* RBGS: levels of side <= this run in FP64, 8 */
+    float box_pad;            /* run with randoms: setup_box pad, 500 */
+    float ran_min;            /* randoms threshold factor, 0.01 */
+    int32_t mutate_inputs;    /* write wrapped positions back (wrap=true), 1 */
+    /* ---- outputs of the last run ---- */
+    int32_t vcycles_done;     /* RBGS: V-cycles used on the finest level */
+    double final_residual;    /* RBGS: last relative residual */
+} baorec_params;
+
+/* Defaults of the Julia constructors (src/recon.jl:2-33); beta = f/bias in
+ * FP32, box and los unset. */
+void baorec_params_default(baorec_params *p, float bias, float f,
+                           float smoothing_radius);
+
+/* ------------------------------------------------------------- context */
+
+/* Opens a context on HIP device `device` (one stream, cached work buffers).
+ * Returns BAOREC_ERR_NO_DEVICE when there is no GPU. */
+int baorec_create(baorec_ctx **ctx, int device);
+int baorec_destroy(baorec_ctx *ctx);
+/* Message of the last failed call on ctx (or of the last failed
+ * baorec_create on this thread when ctx is NULL). */
+const char *baorec_last_error(const baorec_ctx *ctx);
+/* Library version string. */
+const char *baorec_version(void);
+
+/* ------------------------------------------------------------- run! */
+
+/*
+ * run!(::IterativeRecon, grid_size, x,y,z,w[, rx,ry,rz,rw])
+ *   src/recon.jl:134-153 (periodic box) and :155-180 (with randoms).
+ * Without randoms (rx == NULL, nr == 0) the box in `p` is used and CIC wraps:
+ * x, y, z are mutated like the reference when p->mutate_inputs.  With randoms,
+ * setup_box(randoms, p->box_pad) overwrites p->box_size / p->box_min.
+ * The result (the reconstructed real-space overdensity) is kept in the
+ * context as recon.result_cache.
+ */
+int baorec_run_iterative(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3],
+                         int mem, float *x, float *y, float *z, const float *w,
+                         int64_t n, const float *rx, const float *ry,
+                         const float *rz, const float *rw, int64_t nr);
+
+/* run!(::MultigridRecon, ...), src/recon.jl:215-234 and :236-261; caches the
+ * potential phi. */
+int baorec_run_multigrid(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3],
+                         int mem, float *x, float *y, float *z, const float *w,
+                         int64_t n, const float *rx, const float *ry,
+                         const float *rz, const float *rw, int64_t nr);
+
+/* recon.result_cache: dimensions, kind, and a copy of the cached grid. */
+int baorec_result_info(const baorec_ctx *ctx, int32_t grid[3], int32_t *kind);
+int baorec_result_copy(baorec_ctx *ctx, int mem, float *dst);
+
+/* ------------------------------------------------ grid-level entry points */
+
+/* setup_overdensity!(delta, recon, x,y,z,w[, rx..rw]), src/recon.jl:42-57 and
+ * :60-91; writes delta (nx*ny*nz floats). Uses the box in p. */
+int baorec_setup_overdensity(baorec_ctx *ctx, const baorec_params *p,
+                             const int32_t grid[3], int mem, float *delta,
+                             float *x, float *y, float *z, const float *w,
+                             int64_t n, const float *rx, const float *ry,
+                             const float *rz, const float *rw, int64_t nr);
+
+/* reconstructed_overdensity!(delta, recon::IterativeRecon, ...),
+ * src/recon.jl:93-132. */
+int baorec_reconstructed_overdensity(baorec_ctx *ctx, const baorec_params *p,
+                                     const int32_t grid[3], int mem, float *delta,
+                                     float *x, float *y, float *z, const float *w,
+                                     int64_t n, const float *rx, const float *ry,
+                                     const float *rz, const float *rw, int64_t nr);
+
+/* reconstructed_potential!(phi, recon::MultigridRecon, ...),
+ * src/recon.jl:184-212. */
+int baorec_reconstructed_potential(baorec_ctx *ctx, baorec_params *p,
+                                   const int32_t grid[3], int mem, float *phi,
+                                   float *x, float *y, float *z, const float *w,
+                                   int64_t n, const float *rx, const float *ry,
+                                   const float *rz, const float *rw, int64_t nr);
+
+/* ------------------------------------------------------ shifts / positions */
+
+/*
+ * read_shifts(recon, x,y,z[, field3d]; field), src/recon.jl:264-306 (CPU) and
+ * :333-364 (GPU).  `field3d` == NULL uses the context's cached result (whose
+ * displacement grids are kept, so repeated calls only gather).  Otherwise
+ * field3d (grid[0]*grid[1]*grid[2] floats in `mem`) is the field and `kind`
+ * says how to read it: BAOREC_ITERATIVE (delta, Psi_k = i k delta_k / k^2,
+ * src/iterative.jl:229-275) or BAOREC_MULTIGRID (phi, Psi_k = i k phi_k,
+ * src/multigrid.jl:755-799).  Writes n shifts per component.
+ */
+int baorec_read_shifts(baorec_ctx *ctx, const baorec_params *p, int mem,
+                       const float *x, const float *y, const float *z, int64_t n,
+                       const float *field3d, const int32_t grid[3], int kind,
+                       int field, float *sx, float *sy, float *sz);
+
+/* reconstructed_positions(recon, x,y,z[, field3d]; field),
+ * src/recon.jl:366-388: out = pos - shift. */
+int baorec_reconstructed_positions(baorec_ctx *ctx, const baorec_params *p, int mem,
+                                   const float *x, const float *y, const float *z,
+                                   int64_t n, const float *field3d,
+                                   const int32_t grid[3], int kind, int field,
+                                   float *ox, float *oy, float *oz);
+
+/* ------------------------------------------------------------- primitives */
+
+/* setup_box(x, y, z, pad), src/utils.jl:100-109. */
+int baorec_setup_box(baorec_ctx *ctx, int mem, const float *x, const float *y,
+                     const float *z, int64_t n, float pad, float box_size[3],
+                     float box_min[3]);
+
+/* cic!(rho, x,y,z,w, box_size, box_min; wrap), src/mas.jl:1-107: ADDS the
+ * mass of n particles to rho (p->mas selects CIC or TSC).  wrap mutates x,y,z
+ * when p->mutate_inputs. */
+int baorec_cic(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],
+               int mem, float *rho, float *x, float *y, float *z, const float *w,
+               int64_t n, int wrap);
+
+/* read_cic!(out, field, x,y,z, box_size, box_min), src/mas.jl:271-327
+ * (wrap = true). */
+int baorec_read_cic(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],
+                    int mem, float *out, const float *field, const float *x,
+                    const float *y, const float *z, int64_t n);
+
+/* smooth!(field, R, box_size, plan), src/utils.jl:85-96 (in place). */
+int baorec_smooth(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],
+                  int mem, float *field);
+
+/* fft_plan * field and ldiv!(field, fft_plan, field_k) (setup_fft!,
+ * src/recon.jl:35-37): forward R2C into (nx/2+1)*ny*nz interleaved complex
+ * (re, im) floats, and normalised inverse C2R (the input is not modified). */
+int baorec_rfft3d(baorec_ctx *ctx, const int32_t grid[3], int mem, const float *in,
+                  float *out_complex);
+int baorec_irfft3d(baorec_ctx *ctx, const int32_t grid[3], int mem,
+                   const float *in_complex, float *out);
+
+/* k_vec / x_vec (src/utils.jl:3-10, 21-25) as used by the kernels:
+ * kx has nx/2+1 entries, ky ny, kz nz; xv* hold the cell centres. Host only. */
+int baorec_k_vec(const int32_t grid[3], const float box_size[3], float *kx,
+                 float *ky, float *kz);
+int baorec_x_vec(const int32_t grid[3], const float box_size[3],
+                 const float box_min[3], float *xx, float *xy, float *xz);
+
+/* Multigrid pieces on one level (all in place / out of place as named):
+ *   jacobi!   src/multigrid.jl:193-210    v <- niter damped sweeps
+ *   residual! src/multigrid.jl:378-395    r = f - A v
+ *   reduce!   src/multigrid.jl:641-651    coarse (grid/2) <- fine
+ *   prolong!  src/multigrid.jl:511-518    fine <- coarse (grid = coarse side),
+ *                                          add != 0 adds instead of overwriting
+ *   vcycle!   src/multigrid.jl:689-719
+ *   fmg       src/multigrid.jl:722-752    v overwritten
+ * The box, beta, los, damping and sweep counts come from p. */
+int baorec_mg_jacobi(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],
+                     int mem, float *v, const float *f);
+int baorec_mg_residual(baorec_ctx *ctx, const baorec_params *p,
+                       const int32_t grid[3], int mem, float *r, const float *v,
+                       const float *f);
+int baorec_mg_reduce(baorec_ctx *ctx, const int32_t fine_grid[3], int mem,
+                     float *coarse, const float *fine);
+int baorec_mg_prolong(baorec_ctx *ctx, const int32_t coarse_grid[3], int mem,
+                      float *fine, const float *coarse, int add);
+int baorec_mg_vcycle(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],
+                     int mem, float *v, const float *f);
+int baorec_mg_fmg(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3], int mem,
+                  float *v, const float *f);
+
+/* --------------------------------------------------------------- batching */
+
+/*
+ * Extension (no reference counterpart): reconstruct `ncat` periodic-box
+ * catalogs of uneven sizes with one captured launch sequence (hipGraph per
+ * grid/catalog-size class).  Catalog c occupies particles
+ * [offsets[c], offsets[c+1]) of x,y,z,w (device or host per `mem`); all share
+ * the grid and params.  Outputs: reconstructed positions (field = SUM) for
+ * every particle into ox, oy, oz.
+ */
+int baorec_run_batch(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3],
+                     int mem, int ncat, const int64_t *offsets, float *x, float *y,
+                     float *z, const float *w, float *ox, float *oy, float *oz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BAOREC_H */

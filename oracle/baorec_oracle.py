@@ -1,0 +1,495 @@
+"""CPU oracle for the BAOrec.jl reconstruction hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module: as the checker the HIP
+path is compared against, or as the timed CPU restatement.  The product path
+(``baorec.jl_amd``) never imports it.
+
+It restates the reference algorithm step by step (file:line of BAOrec.jl given
+per function).  Grids are numpy arrays of shape ``(nz, ny, nx)`` in C order,
+i.e. the same bytes as the reference's column-major ``(nx, ny, nz)`` Julia
+arrays (x fastest).  ``dtype=np.float32`` keeps the reference's FP32 operation
+order; ``dtype=np.float64`` is the same algorithm in FP64 (used to state the
+error of the FP32 paths).  FFTs use scipy's pocketfft (FFTW is not installed);
+loops that are not FFTs (CIC scatter/gather, multigrid) live in the C library
+``_build/liboracle.so`` built from ``baorec_oracle.c``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field as dfield
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+import scipy.fft as sfft
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+F32 = np.float32
+F64 = np.float64
+
+
+def build() -> str:
+    """Compile the C part of the oracle in-tree (``oracle/_build``)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "_build", "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "_build", "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f3(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=F32).reshape(3))
+
+
+def _sfx(dtype):
+    return "_f32" if np.dtype(dtype) == np.float32 else "_f64"
+
+
+def _workers():
+    return int(os.environ.get("BAOREC_ORACLE_WORKERS", os.cpu_count() or 1))
+
+
+# --------------------------------------------------------------- parameters
+
+@dataclass
+class IterativeRecon:
+    """Mirror of ``IterativeRecon`` (src/recon.jl:2-16)."""
+    bias: float
+    f: float
+    smoothing_radius: float
+    box_size: Optional[Sequence[float]] = None
+    box_min: Optional[Sequence[float]] = None
+    los: Optional[Sequence[float]] = None
+    n_iter: int = 3
+    beta: Optional[float] = None
+    result_cache: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        if self.beta is None:
+            self.beta = float(F32(F32(self.f) / F32(self.bias)))
+
+
+@dataclass
+class MultigridRecon:
+    """Mirror of ``MultigridRecon`` (src/recon.jl:18-33)."""
+    bias: float
+    f: float
+    smoothing_radius: float
+    box_size: Optional[Sequence[float]] = None
+    box_min: Optional[Sequence[float]] = None
+    los: Optional[Sequence[float]] = None
+    jacobi_damping_factor: float = 0.4
+    jacobi_niterations: int = 5
+    vcycle_niterations: int = 6
+    beta: Optional[float] = None
+    result_cache: Optional[np.ndarray] = None
+
+    def __post_init__(self):
+        if self.beta is None:
+            self.beta = float(F32(F32(self.f) / F32(self.bias)))
+
+
+# ------------------------------------------------------------- grid vectors
+
+def k_vec(dims: Sequence[int], box_size, dtype=F32) -> Tuple[np.ndarray, ...]:
+    """src/utils.jl:3-10.  sample_rate = fl32(2*pi*n/L) (computed in Float64),
+    step = fl32(sample_rate/n), k[i] = fl32(i*step); x uses rfftfreq (0..n/2),
+    y and z use fftfreq (Nyquist negative)."""
+    out = []
+    for a, n in enumerate(dims):
+        L = F32(box_size[a])
+        if dtype == F32:
+            fs = F32(2 * np.pi * n / float(L))
+            mult = F32(fs / F32(n))
+        else:
+            mult = 2 * np.pi / float(L)
+        if a == 0:
+            idx = np.arange(n // 2 + 1)
+        else:
+            idx = np.arange(n)
+            idx = np.where(idx < (n + 1) // 2, idx, idx - n)
+        out.append((idx.astype(dtype) * dtype(mult)).astype(dtype))
+    return tuple(out)
+
+
+def x_vec(dims: Sequence[int], box_size, box_min, dtype=F32) -> Tuple[np.ndarray, ...]:
+    """src/utils.jl:21-25: cell centres b + (i+1/2)*fl32(L/n), rounded to T."""
+    out = []
+    fn = getattr(lib(), "oracle_xvec" + _sfx(dtype))
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float]
+    for a, n in enumerate(dims):
+        v = np.zeros(n, dtype=dtype)
+        fn(_p(v), int(n), float(box_size[a]), float(box_min[a]))
+        out.append(v)
+    return tuple(out)
+
+
+def setup_box(x, y, z, box_pad):
+    """src/utils.jl:100-109: cubic box, min - pad/2 .. max + pad/2 (Float32)."""
+    pad = F32(box_pad)
+    bmin = np.array([np.min(x), np.min(y), np.min(z)], dtype=F32) - pad / F32(2)
+    bmax = np.array([np.max(x), np.max(y), np.max(z)], dtype=F32) + pad / F32(2)
+    L = np.max((bmax - bmin).astype(F32))
+    return np.full(3, L, dtype=F32), bmin.astype(F32)
+
+
+# -------------------------------------------------------------- mass assign
+
+def cic(rho: np.ndarray, x, y, z, w, box_size, box_min, wrap=True) -> np.ndarray:
+    """cic!, src/mas.jl:1-52 (serial; mutates x,y,z in place when wrap)."""
+    nz, ny, nx = rho.shape
+    fn = getattr(lib(), "oracle_cic" + _sfx(rho.dtype))
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + \
+        [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    for a in (x, y, z, w):
+        assert a.dtype == F32 and a.flags.c_contiguous
+    assert rho.flags.c_contiguous
+    bs, bm = _f3(box_size), _f3(box_min)
+    rc = fn(_p(rho), nx, ny, nz, _p(x), _p(y), _p(z), _p(w), len(x), _p(bs), _p(bm), int(bool(wrap)))
+    if rc != 0:
+        raise IndexError("cic!: particle outside the grid (reference BoundsError)")
+    return rho
+
+
+def read_cic(field: np.ndarray, x, y, z, box_size, box_min, gpu_formula=True) -> np.ndarray:
+    """read_cic!, src/mas.jl:218-269 (CPU) / :271-318 (GPU formula)."""
+    nz, ny, nx = field.shape
+    out = np.zeros(len(x), dtype=field.dtype)
+    fn = getattr(lib(), "oracle_read_cic" + _sfx(field.dtype))
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + \
+        [ctypes.c_void_p] * 3 + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    x, y, z = (np.ascontiguousarray(a, dtype=F32) for a in (x, y, z))
+    field = np.ascontiguousarray(field)
+    bs, bm = _f3(box_size), _f3(box_min)
+    rc = fn(_p(out), _p(field), nx, ny, nz, _p(x), _p(y), _p(z), len(x), _p(bs), _p(bm), int(gpu_formula))
+    if rc != 0:
+        raise IndexError("read_cic!: particle outside the grid")
+    return out
+
+
+# -------------------------------------------------------------------- FFTs
+
+def _rfft(a):
+    return sfft.rfftn(a, workers=_workers())
+
+
+def _irfft(ak, shape):
+    return sfft.irfftn(ak, s=shape, workers=_workers())
+
+
+def _k2(kv, dtype):
+    kx, ky, kz = kv
+    return (kx[None, None, :] ** 2 + ky[None, :, None] ** 2).astype(dtype) + kz[:, None, None] ** 2
+
+
+def smooth(field: np.ndarray, R, box_size) -> np.ndarray:
+    """smooth!, src/utils.jl:45-56: F_k *= exp(-0.5 R^2 k^2) with the exponent
+    in Float64 (the -0.5 literal promotes), product rounded back to T."""
+    dt = field.dtype.type
+    kv = k_vec(field.shape[::-1], box_size, dt)
+    fk = _rfft(field)
+    k2 = _k2(kv, dt)
+    R2 = dt(R) * dt(R)
+    fac = np.exp(-0.5 * np.float64(R2) * k2.astype(F64))
+    if dt == F32:
+        fk = (fk.astype(np.complex128) * fac).astype(np.complex64)
+    else:
+        fk = fk * fac
+    field[...] = _irfft(fk, field.shape)
+    return field
+
+
+# -------------------------------------------------------------- overdensity
+
+def setup_overdensity(delta: np.ndarray, recon, x, y, z, w, wrap=True) -> np.ndarray:
+    """setup_overdensity!, src/recon.jl:42-57 (periodic box, no randoms)."""
+    dt = delta.dtype.type
+    cic(delta, x, y, z, w, recon.box_size, recon.box_min, wrap=wrap)
+    smooth(delta, recon.smoothing_radius, recon.box_size)
+    mean = dt(np.mean(delta, dtype=dt))
+    delta[...] = ((delta / mean) - dt(1)) / dt(recon.bias)
+    return delta
+
+
+def setup_overdensity_randoms(delta: np.ndarray, recon, x, y, z, w, rx, ry, rz, rw,
+                              ran_min=0.01) -> np.ndarray:
+    """setup_overdensity!, src/recon.jl:60-91 (data minus alpha*randoms)."""
+    dt = delta.dtype.type
+    dran = np.zeros_like(delta)
+    cic(delta, x, y, z, w, recon.box_size, recon.box_min, wrap=False)
+    cic(dran, rx, ry, rz, rw, recon.box_size, recon.box_min, wrap=False)
+    smooth(delta, recon.smoothing_radius, recon.box_size)
+    smooth(dran, recon.smoothing_radius, recon.box_size)
+    sd = dt(np.sum(delta, dtype=dt))
+    sr = dt(np.sum(dran, dtype=dt))
+    alpha = dt(sd / sr)
+    thr = np.float64(ran_min) * np.float64(sr) / len(rx)
+    delta -= alpha * dran
+    den = (dt(recon.bias) * alpha) * dran
+    with np.errstate(divide="ignore", invalid="ignore"):
+        delta[...] = np.where(dran > thr, delta / den, dt(0))
+    return delta
+
+
+# ------------------------------------------------------------------ iterate
+
+def iterate(delta_r, delta_s, kv, it, beta, los=None, xv=None):
+    """iterate!, src/iterative.jl:5-67 (CPU semantics: axes with los_i == 0 are
+    skipped, x^2 == 0 cells are set to 0)."""
+    dt = delta_r.dtype.type
+    shape = delta_r.shape
+    dk = _rfft(delta_r)
+    k2 = _k2(kv, dt)
+    k2 = np.where(k2 == 0, dt(1), k2)
+    dk = (dk / k2).astype(dk.dtype)
+    dk[0, 0, 0] = 0
+    delta_r[...] = delta_s
+    beta = dt(beta)
+    K = (kv[0][None, None, :], kv[1][None, :, None], kv[2][:, None, None])
+    if los is None:
+        X = (xv[0][None, None, :], xv[1][None, :, None], xv[2][:, None, None])
+        x2 = (X[0] ** 2 + X[1] ** 2).astype(dt) + X[2] ** 2
+        for i in range(3):
+            for j in range(i, 3):
+                fac = dt((1.0 + float(dt(i != j))) * float(beta))
+                if it == 1:
+                    fac = dt(fac / (dt(1) + beta))
+                d = _irfft(((K[i] * K[j]).astype(dt) * dk).astype(dk.dtype), shape).astype(dt)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    upd = (((fac * d) * X[i]) * X[j]) / x2
+                    delta_r[...] = np.where(x2 > 0, delta_r - upd, dt(0))
+    else:
+        los = [dt(v) for v in los]
+        for i in range(3):
+            if los[i] == 0:
+                continue
+            fac = beta
+            if it == 1:
+                fac = dt(fac / (dt(1) + beta))
+            d = _irfft(((K[i] ** 2).astype(dt) * los[i] * dk).astype(dk.dtype), shape).astype(dt)
+            delta_r -= fac * d
+    return delta_r
+
+
+def reconstructed_overdensity(delta, recon, x, y, z, w, rx=None, ry=None, rz=None, rw=None):
+    """reconstructed_overdensity!, src/recon.jl:93-132."""
+    dt = delta.dtype.type
+    if rx is None:
+        setup_overdensity(delta, recon, x, y, z, w)
+    else:
+        setup_overdensity_randoms(delta, recon, x, y, z, w, rx, ry, rz, rw)
+    ds = delta.copy()
+    dims = delta.shape[::-1]
+    kv = k_vec(dims, recon.box_size, dt)
+    xv = x_vec(dims, recon.box_size, recon.box_min, dt) if recon.los is None else None
+    for it in range(1, recon.n_iter + 1):
+        iterate(delta, ds, kv, it, recon.beta, los=recon.los, xv=xv)
+    return delta
+
+
+def run_iterative(recon: IterativeRecon, grid_size, x, y, z, w, rx=None, ry=None, rz=None,
+                  rw=None, dtype=F32, box_pad=500.0):
+    """run!(::IterativeRecon), src/recon.jl:134-180."""
+    nx, ny, nz = grid_size
+    delta = np.zeros((nz, ny, nx), dtype=dtype)
+    if rx is not None:
+        recon.box_size, recon.box_min = setup_box(rx, ry, rz, box_pad)
+    reconstructed_overdensity(delta, recon, x, y, z, w, rx, ry, rz, rw)
+    recon.result_cache = delta
+    return delta
+
+
+# ---------------------------------------------------------------- multigrid
+
+def _mgfn(name, dtype):
+    return getattr(lib(), "oracle_mg_" + name + _sfx(dtype))
+
+
+def _los_args(los):
+    if los is None:
+        return 0, _f3([0, 0, 0])
+    return 1, _f3(los)
+
+
+def fmg(f: np.ndarray, box_size, box_min, beta, omega, niter, ncycles, los=None) -> np.ndarray:
+    """fmg, src/multigrid.jl:722-752 (returns a fresh v)."""
+    dt = f.dtype
+    nz, ny, nx = f.shape
+    v = np.zeros_like(f)
+    fn = _mgfn("fmg", dt)
+    R = ctypes.c_float if dt == F32 else ctypes.c_double
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 3 + \
+        [ctypes.c_void_p, ctypes.c_void_p, R, R, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    hl, lv = _los_args(los)
+    bs, bm = _f3(box_size), _f3(box_min)
+    fn(_p(v), _p(np.ascontiguousarray(f)), nx, ny, nz, _p(bs), _p(bm), float(beta), float(omega),
+       int(niter), int(ncycles), hl, _p(lv))
+    return v
+
+
+def mg_jacobi(v, f, box_size, box_min, beta, omega, niter, los=None):
+    """jacobi!, src/multigrid.jl:14-135 (in place on v)."""
+    dt = v.dtype
+    nz, ny, nx = v.shape
+    fn = _mgfn("jacobi", dt)
+    R = ctypes.c_float if dt == F32 else ctypes.c_double
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 3 + \
+        [ctypes.c_void_p, ctypes.c_void_p, R, R, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    hl, lv = _los_args(los)
+    bs, bm = _f3(box_size), _f3(box_min)
+    fn(_p(v), _p(np.ascontiguousarray(f)), nx, ny, nz, _p(bs), _p(bm), float(beta), float(omega),
+       int(niter), hl, _p(lv))
+    return v
+
+
+def mg_residual(v, f, box_size, box_min, beta, los=None):
+    """residual!, src/multigrid.jl:213-326."""
+    dt = v.dtype
+    nz, ny, nx = v.shape
+    r = np.zeros_like(v)
+    fn = _mgfn("residual", dt)
+    R = ctypes.c_float if dt == F32 else ctypes.c_double
+    fn.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 3 + \
+        [ctypes.c_void_p, ctypes.c_void_p, R, ctypes.c_int, ctypes.c_void_p]
+    hl, lv = _los_args(los)
+    bs, bm = _f3(box_size), _f3(box_min)
+    fn(_p(r), _p(np.ascontiguousarray(v)), _p(np.ascontiguousarray(f)), nx, ny, nz, _p(bs), _p(bm),
+       float(beta), hl, _p(lv))
+    return r
+
+
+def mg_prolong(v2h: np.ndarray) -> np.ndarray:
+    """prolong!, src/multigrid.jl:398-453 (fresh fine array)."""
+    cz, cy, cx = v2h.shape
+    v1h = np.zeros((2 * cz, 2 * cy, 2 * cx), dtype=v2h.dtype)
+    fn = _mgfn("prolong", v2h.dtype)
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    fn(_p(v1h), _p(np.ascontiguousarray(v2h)), cx, cy, cz)
+    return v1h
+
+
+def mg_reduce(v1h: np.ndarray) -> np.ndarray:
+    """reduce!, src/multigrid.jl:520-584."""
+    fz, fy, fx = v1h.shape
+    v2h = np.zeros((fz // 2, fy // 2, fx // 2), dtype=v1h.dtype)
+    fn = _mgfn("reduce", v1h.dtype)
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    fn(_p(v2h), _p(np.ascontiguousarray(v1h)), fx, fy, fz)
+    return v2h
+
+
+def mg_vcycle(v, f, box_size, box_min, beta, omega, niter, los=None):
+    """vcycle!, src/multigrid.jl:654-687 (in place on v)."""
+    dt = v.dtype
+    nz, ny, nx = v.shape
+    fn = _mgfn("vcycle", dt)
+    R = ctypes.c_float if dt == F32 else ctypes.c_double
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 3 + \
+        [ctypes.c_void_p, ctypes.c_void_p, R, R, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    hl, lv = _los_args(los)
+    bs, bm = _f3(box_size), _f3(box_min)
+    fn(_p(v), _p(np.ascontiguousarray(f)), nx, ny, nz, _p(bs), _p(bm), float(beta), float(omega),
+       int(niter), hl, _p(lv))
+    return v
+
+
+def reconstructed_potential(phi, recon: MultigridRecon, x, y, z, w, rx=None, ry=None, rz=None, rw=None):
+    """reconstructed_potential!, src/recon.jl:184-212."""
+    delta = np.zeros_like(phi)
+    if rx is None:
+        setup_overdensity(delta, recon, x, y, z, w)
+    else:
+        setup_overdensity_randoms(delta, recon, x, y, z, w, rx, ry, rz, rw)
+    phi[...] = fmg(delta, recon.box_size, recon.box_min, recon.beta, recon.jacobi_damping_factor,
+                   recon.jacobi_niterations, recon.vcycle_niterations, los=recon.los)
+    return phi
+
+
+def run_multigrid(recon: MultigridRecon, grid_size, x, y, z, w, rx=None, ry=None, rz=None,
+                  rw=None, dtype=F32, box_pad=500.0):
+    """run!(::MultigridRecon), src/recon.jl:215-261."""
+    nx, ny, nz = grid_size
+    phi = np.zeros((nz, ny, nx), dtype=dtype)
+    if rx is not None:
+        recon.box_size, recon.box_min = setup_box(rx, ry, rz, box_pad)
+    reconstructed_potential(phi, recon, x, y, z, w, rx, ry, rz, rw)
+    recon.result_cache = phi
+    return phi
+
+
+# ------------------------------------------------------------ displacements
+
+def displacement_grids(field: np.ndarray, recon) -> Tuple[np.ndarray, ...]:
+    """The three real-space displacement grids of compute_displacements:
+    iterative (src/iterative.jl:253-275): Psi_k = i k_i delta_k / k^2 (0 at k=0);
+    multigrid (src/multigrid.jl:755-773): Psi_k = i k_i phi_k."""
+    dt = field.dtype.type
+    kv = k_vec(field.shape[::-1], recon.box_size, dt)
+    fk = _rfft(field)
+    K = (kv[0][None, None, :], kv[1][None, :, None], kv[2][:, None, None])
+    out = []
+    if isinstance(recon, MultigridRecon):
+        for i in range(3):
+            pk = (1j * K[i]).astype(fk.dtype) * fk
+            out.append(_irfft(pk, field.shape).astype(dt))
+    else:
+        k2 = _k2(kv, dt)
+        for i in range(3):
+            with np.errstate(divide="ignore", invalid="ignore"):
+                pk = np.where(k2 > 0, ((1j * K[i]).astype(fk.dtype) * fk) / k2, 0).astype(fk.dtype)
+            pk[0, 0, 0] = 0
+            out.append(_irfft(pk, field.shape).astype(dt))
+    return tuple(out)
+
+
+def compute_displacements(field, x, y, z, recon, gpu_formula=True):
+    grids = displacement_grids(field, recon)
+    return tuple(read_cic(g, x, y, z, recon.box_size, recon.box_min, gpu_formula) for g in grids)
+
+
+def read_shifts(recon, x, y, z, field=None, kind="disp", gpu_formula=True):
+    """read_shifts, src/recon.jl:264-306 (:disp, :rsd or :sum)."""
+    if field is None:
+        field = recon.result_cache
+    dt = field.dtype.type
+    disp = compute_displacements(field, x, y, z, recon, gpu_formula)
+    if kind == "disp":
+        return disp
+    f = dt(recon.f)
+    if recon.los is None:
+        xx, yy, zz = (np.asarray(a, dtype=dt) for a in (x, y, z))
+        dist = np.sqrt((xx * xx + yy * yy) + zz * zz).astype(dt)
+        l = (xx / dist, yy / dist, zz / dist)
+    else:
+        l = tuple(np.full(len(x), dt(v), dtype=dt) for v in recon.los)
+    dot = ((disp[0] * l[0]) + (disp[1] * l[1])) + (disp[2] * l[2])
+    rsd = tuple(((f * dot) * l[j]).astype(dt) for j in range(3))
+    if kind == "rsd":
+        return rsd
+    if kind == "sum":
+        return tuple((disp[j] + rsd[j]).astype(dt) for j in range(3))
+    raise ValueError(kind)
+
+
+def reconstructed_positions(recon, x, y, z, field=None, kind="disp", gpu_formula=True):
+    """reconstructed_positions, src/recon.jl:366-388: pos - shift."""
+    sh = read_shifts(recon, x, y, z, field, kind, gpu_formula)
+    dt = sh[0].dtype.type
+    return tuple((np.asarray(p, dtype=dt) - s).astype(dt) for p, s in zip((x, y, z), sh))

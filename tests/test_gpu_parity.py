@@ -1,0 +1,331 @@
+"""GPU parity: every HIP entry point against the CPU oracle on seeded inputs.
+
+Tolerances (north-star): density grids within 1e-5 relative L2 (FP32);
+displacements and shifted positions within 1e-4 Mpc/h RMS.  Primitives that
+are pure FP32 arithmetic in the reference's order (scatter, gather, Jacobi,
+residual, restriction, prolongation) are held to tighter bounds.
+"""
+import numpy as np
+import pytest
+
+import baorec_hip as bh
+import baorec_oracle as orc
+from baorec_hip import synth
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def rms3(a, b):
+    return float(np.sqrt(np.mean([np.mean((np.asarray(p, np.float64) - np.asarray(q, np.float64)) ** 2)
+                                  for p, q in zip(a, b)])))
+
+
+def box_kw(L=1000.0, los=(0.0, 0.0, 1.0)):
+    return dict(bias=2.0, f=0.8, smoothing_radius=15.0, box_size=[L] * 3, box_min=[0.0] * 3,
+                los=None if los is None else list(los))
+
+
+# ------------------------------------------------------------------ FFT
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (16, 32, 64), (64, 16, 32), (128, 128, 128), (32, 256, 16)])
+def test_rfft3d_matches_numpy(ctx, shape):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((nz, ny, nx)).astype(F32)
+    got = bh.rfft3d(a, ctx)
+    ref = np.fft.rfftn(a.astype(np.float64))
+    assert rel_l2(got, ref) < 2e-6
+    back = bh.irfft3d(got, (nx, ny, nz), ctx)
+    assert rel_l2(back, a) < 2e-6
+
+
+def test_irfft3d_ignores_selfconjugate_imaginary_parts(ctx):
+    nx = ny = nz = 16
+    rng = np.random.default_rng(2)
+    spec = (rng.standard_normal((nz, ny, nx // 2 + 1)) + 1j * rng.standard_normal((nz, ny, nx // 2 + 1)))
+    spec = spec.astype(np.complex64)
+    got = bh.irfft3d(spec, (nx, ny, nz), ctx)
+    ref = np.fft.irfftn(spec.astype(np.complex128), s=(nz, ny, nx))
+    assert rel_l2(got, ref) < 2e-6
+
+
+# ------------------------------------------------------------------ mass assignment
+
+@pytest.mark.parametrize("grid", [(32, 32, 32), (64, 32, 16), (40, 24, 20)])
+def test_cic_matches_oracle(ctx, grid):
+    nx, ny, nz = grid
+    rng = np.random.default_rng(3)
+    n = 20000
+    L = 1000.0
+    x, y, z = (rng.random(n).astype(F32) * F32(L) for _ in range(3))
+    # a few particles exactly on the upper face and beyond it (one-sided wrap)
+    x[:5] = F32(L)
+    y[5:10] = F32(L + 3.0)
+    w = (rng.random(n) + 0.5).astype(F32)
+    rec = bh.IterativeRecon(**box_kw(L))
+    rho = np.zeros((nz, ny, nx), F32)
+    gx, gy, gz = x.copy(), y.copy(), z.copy()
+    bh.cic_(rho, gx, gy, gz, w, rec, wrap=True)
+    ref = np.zeros((nz, ny, nx), F32)
+    ox, oy, oz = x.copy(), y.copy(), z.copy()
+    orc.cic(ref, ox, oy, oz, w, rec.box_size, rec.box_min, wrap=True)
+    assert rel_l2(rho, ref) < 1e-6
+    assert abs(rho.sum(dtype=np.float64) - w.sum(dtype=np.float64)) < 1e-4 * w.sum()
+    # input mutation identical to the reference
+    np.testing.assert_array_equal(gx, ox)
+    np.testing.assert_array_equal(gy, oy)
+
+
+def test_cic_nowrap_out_of_box_raises(ctx):
+    rec = bh.IterativeRecon(**box_kw(1000.0))
+    rho = np.zeros((16, 16, 16), F32)
+    x = np.array([999.9], F32)  # last cell: the +1 corner is outside without wrap
+    y = np.array([10.0], F32)
+    z = np.array([10.0], F32)
+    with pytest.raises(bh.BoundsError):
+        bh.cic_(rho, x, y, z, np.ones(1, F32), rec, wrap=False)
+
+
+def test_tsc_mass_and_symmetry(ctx):
+    rec = bh.IterativeRecon(**box_kw(32.0), mas="tsc")
+    rho = np.zeros((32, 32, 32), F32)
+    rng = np.random.default_rng(4)
+    n = 5000
+    x, y, z = (rng.random(n).astype(F32) * F32(32.0) for _ in range(3))
+    bh.cic_(rho, x, y, z, np.ones(n, F32), rec, wrap=True)
+    assert abs(rho.sum(dtype=np.float64) - n) < 1e-3
+    # single particle on a node: weights 1/8, 3/4, 1/8 per axis
+    rho = np.zeros((32, 32, 32), F32)
+    p = np.array([5.0], F32)
+    bh.cic_(rho, p.copy(), p.copy(), p.copy(), np.ones(1, F32), rec, wrap=True)
+    w1 = np.array([0.125, 0.75, 0.125])
+    ref = np.zeros((32, 32, 32))
+    ref[4:7, 4:7, 4:7] = w1[:, None, None] * w1[None, :, None] * w1[None, None, :]
+    np.testing.assert_allclose(rho, ref, atol=1e-7)
+
+
+def test_read_cic_matches_oracle(ctx):
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 32, 16, 24
+    field = rng.standard_normal((nz, ny, nx)).astype(F32)
+    n = 10000
+    L = 500.0
+    x, y, z = (rng.random(n).astype(F32) * F32(L) for _ in range(3))
+    rec = bh.IterativeRecon(**box_kw(L))
+    out = np.zeros(n, F32)
+    bh.read_cic_(out, field, x, y, z, rec)
+    ref = orc.read_cic(field, x, y, z, rec.box_size, rec.box_min, gpu_formula=True)
+    assert np.max(np.abs(out - ref)) < 1e-5
+
+
+def test_setup_box_exact(ctx):
+    rng = np.random.default_rng(6)
+    x, y, z = (rng.standard_normal(100001).astype(F32) * 300 + 50 for _ in range(3))
+    bs, bm = bh.setup_box(x, y, z, 500.0, ctx)
+    rs, rm = orc.setup_box(x, y, z, 500.0)
+    np.testing.assert_array_equal(bs, rs)
+    np.testing.assert_array_equal(bm, rm)
+
+
+# ------------------------------------------------------------------ smoothing / overdensity
+
+def test_smooth_matches_oracle(ctx):
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal((32, 32, 32)).astype(F32)
+    rec = bh.IterativeRecon(**box_kw(300.0))
+    got = bh.smooth_(a.copy(), rec)
+    ref = orc.smooth(a.copy(), 15.0, rec.box_size)
+    assert rel_l2(got, ref) < 1e-5
+
+
+def test_setup_overdensity_box(ctx):
+    x, y, z, w = synth.box_catalog(50000, L=1000.0, mesh=64, seed=11)
+    rec = bh.IterativeRecon(**box_kw())
+    d = np.zeros((64, 64, 64), F32)
+    bh.setup_overdensity_(d, rec, x.copy(), y.copy(), z.copy(), w)
+    o = orc.IterativeRecon(**box_kw())
+    ref = orc.setup_overdensity(np.zeros((64, 64, 64), F32), o, x.copy(), y.copy(), z.copy(), w)
+    assert rel_l2(d, ref) < 1e-5
+
+
+def test_setup_overdensity_randoms(ctx):
+    x, y, z, w = synth.lightcone_catalog(40000, r_min=300.0, r_max=600.0, mesh=64, seed=12)
+    rx, ry, rz, rw = synth.lightcone_randoms(200000, r_min=300.0, r_max=600.0, seed=13)
+    bs, bm = orc.setup_box(rx, ry, rz, 200.0)
+    kw = dict(bias=2.2, f=0.75, smoothing_radius=15.0, box_size=list(bs), box_min=list(bm), los=None)
+    rec = bh.IterativeRecon(**kw)
+    d = np.zeros((64, 64, 64), F32)
+    bh.setup_overdensity_(d, rec, x, y, z, w, rx, ry, rz, rw)
+    ref = orc.setup_overdensity_randoms(np.zeros((64, 64, 64), F32), orc.IterativeRecon(**kw), x, y, z, w,
+                                        rx, ry, rz, rw)
+    assert rel_l2(d, ref) < 1e-5
+
+
+# ------------------------------------------------------------------ iterative recon end to end
+
+@pytest.mark.parametrize("los", [(0.0, 0.0, 1.0), (1.0, 0.0, 0.0), None])
+def test_run_iterative_box(ctx, los):
+    n = 64
+    x, y, z, w = synth.box_catalog(100000, L=1000.0, mesh=64, seed=21)
+    kw = box_kw(los=los)
+    rec = bh.IterativeRecon(**kw)
+    gx, gy, gz = x.copy(), y.copy(), z.copy()
+    d = bh.run_(rec, (n, n, n), gx, gy, gz, w)
+    o = orc.IterativeRecon(**kw)
+    ox, oy, oz = x.copy(), y.copy(), z.copy()
+    ref = orc.run_iterative(o, (n, n, n), ox, oy, oz, w)
+    assert rel_l2(d, ref) < 1e-5
+    for kind in ("disp", "rsd", "sum"):
+        got = bh.read_shifts(rec, gx, gy, gz, field=kind)
+        exp = orc.read_shifts(o, ox, oy, oz, kind=kind)
+        assert rms3(got, exp) < 1e-4, kind
+    got = bh.reconstructed_positions(rec, gx, gy, gz, field="sum")
+    exp = orc.reconstructed_positions(o, ox, oy, oz, kind="sum")
+    assert rms3(got, exp) < 1e-4
+
+
+def test_run_iterative_lightcone(ctx):
+    n = 64
+    x, y, z, w = synth.lightcone_catalog(50000, r_min=500.0, r_max=900.0, mesh=64, seed=31)
+    rx, ry, rz, rw = synth.lightcone_randoms(250000, r_min=500.0, r_max=900.0, seed=32)
+    kw = dict(bias=2.2, f=0.75, smoothing_radius=15.0, los=None)
+    rec = bh.IterativeRecon(**kw, box_pad=200.0)
+    d = bh.run_(rec, (n, n, n), x, y, z, w, rx, ry, rz, rw)
+    o = orc.IterativeRecon(**kw)
+    ref = orc.run_iterative(o, (n, n, n), x, y, z, w, rx, ry, rz, rw, box_pad=200.0)
+    np.testing.assert_array_equal(rec.box_size, o.box_size)
+    np.testing.assert_array_equal(rec.box_min, o.box_min)
+    assert rel_l2(d, ref) < 1e-5
+    got = bh.reconstructed_positions(rec, x, y, z, field="sum")
+    exp = orc.reconstructed_positions(o, x, y, z, kind="sum")
+    assert rms3(got, exp) < 1e-4
+    # randoms read-outs reuse the cached displacement grids
+    got = bh.reconstructed_positions(rec, rx, ry, rz, field="disp")
+    exp = orc.reconstructed_positions(o, rx, ry, rz, kind="disp")
+    assert rms3(got, exp) < 1e-4
+
+
+def test_read_shifts_explicit_field(ctx):
+    rng = np.random.default_rng(41)
+    field = rng.standard_normal((32, 32, 32)).astype(F32)
+    kw = box_kw(500.0)
+    x, y, z = (rng.random(5000).astype(F32) * F32(500.0) for _ in range(3))
+    for cls, ocls in ((bh.IterativeRecon, orc.IterativeRecon), (bh.MultigridRecon, orc.MultigridRecon)):
+        rec = cls(**kw)
+        got = bh.read_shifts(rec, x, y, z, field, field="sum")
+        exp = orc.read_shifts(ocls(**kw), x, y, z, field=field, kind="sum")
+        scale = max(np.abs(exp[0]).max(), 1e-3)
+        assert rms3(got, exp) < 1e-5 * scale + 1e-6
+
+
+# ------------------------------------------------------------------ multigrid
+
+def mg_rec(los=None, L=1000.0):
+    return dict(bias=2.0, f=0.8, smoothing_radius=15.0, box_size=[L] * 3, box_min=[-L / 2, -L / 2, -L / 2],
+                los=None if los is None else list(los))
+
+
+@pytest.mark.parametrize("los", [None, (0.0, 0.0, 1.0)])
+def test_mg_primitives(ctx, los):
+    rng = np.random.default_rng(51)
+    kw = mg_rec(los)
+    rec = bh.MultigridRecon(**kw)
+    o = orc.MultigridRecon(**kw)
+    v = rng.standard_normal((16, 32, 32)).astype(F32)
+    f = rng.standard_normal((16, 32, 32)).astype(F32)
+    got = bh.jacobi_(v.copy(), f, rec)
+    exp = orc.mg_jacobi(v.copy(), f, o.box_size, o.box_min, o.beta, F32(0.4), 5, los=o.los)
+    assert np.max(np.abs(got - exp)) < 1e-6 * np.abs(exp).max()
+    r = np.zeros_like(v)
+    bh.residual_(r, v, f, rec)
+    exp = orc.mg_residual(v, f, o.box_size, o.box_min, o.beta, los=o.los)
+    assert np.max(np.abs(r - exp)) < 1e-6 * np.abs(exp).max()
+    c = np.zeros((8, 16, 16), F32)
+    bh.reduce_(c, v, ctx)
+    np.testing.assert_allclose(c, orc.mg_reduce(v), rtol=0, atol=1e-6)
+    fine = np.zeros((16, 32, 32), F32)
+    bh.prolong_(fine, c, add=False, ctx=ctx)
+    np.testing.assert_array_equal(fine, orc.mg_prolong(c))
+    fine2 = v.copy()
+    bh.prolong_(fine2, c, add=True, ctx=ctx)
+    np.testing.assert_allclose(fine2, v + orc.mg_prolong(c), rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("los", [None, (0.0, 0.0, 1.0)])
+def test_mg_vcycle_and_fmg(ctx, los):
+    rng = np.random.default_rng(52)
+    kw = mg_rec(los)
+    rec = bh.MultigridRecon(**kw)
+    o = orc.MultigridRecon(**kw)
+    f = rng.standard_normal((32, 32, 32)).astype(F32)
+    v = rng.standard_normal((32, 32, 32)).astype(F32)
+    got = bh.vcycle_(v.copy(), f, rec)
+    exp = orc.mg_vcycle(v.copy(), f, o.box_size, o.box_min, o.beta, F32(0.4), 5, los=o.los)
+    assert rel_l2(got, exp) < 1e-5
+    got = bh.fmg(f, rec)
+    exp = orc.fmg(f, o.box_size, o.box_min, o.beta, F32(0.4), 5, 6, los=o.los)
+    assert rel_l2(got, exp) < 1e-5
+
+
+def test_run_multigrid_box(ctx):
+    n = 64
+    x, y, z, w = synth.box_catalog(100000, L=1000.0, mesh=64, seed=61)
+    kw = box_kw(los=None)
+    kw["box_min"] = [0.0, 0.0, 0.0]
+    rec = bh.MultigridRecon(**kw)
+    gx, gy, gz = x.copy(), y.copy(), z.copy()
+    phi = bh.run_(rec, (n, n, n), gx, gy, gz, w)
+    o = orc.MultigridRecon(**kw)
+    ox, oy, oz = x.copy(), y.copy(), z.copy()
+    ref = orc.run_multigrid(o, (n, n, n), ox, oy, oz, w)
+    assert rel_l2(phi, ref) < 1e-5
+    got = bh.reconstructed_positions(rec, gx, gy, gz, field="sum")
+    exp = orc.reconstructed_positions(o, ox, oy, oz, kind="sum")
+    assert rms3(got, exp) < 1e-4
+
+
+def test_run_multigrid_lightcone(ctx):
+    n = 64
+    x, y, z, w = synth.lightcone_catalog(50000, r_min=500.0, r_max=900.0, mesh=64, seed=71)
+    rx, ry, rz, rw = synth.lightcone_randoms(250000, r_min=500.0, r_max=900.0, seed=72)
+    kw = dict(bias=2.2, f=0.75, smoothing_radius=15.0, los=None)
+    rec = bh.MultigridRecon(**kw, box_pad=200.0)
+    phi = bh.run_(rec, (n, n, n), x, y, z, w, rx, ry, rz, rw)
+    o = orc.MultigridRecon(**kw)
+    ref = orc.run_multigrid(o, (n, n, n), x, y, z, w, rx, ry, rz, rw, box_pad=200.0)
+    assert rel_l2(phi, ref) < 1e-5
+    got = bh.reconstructed_positions(rec, x, y, z, field="sum")
+    exp = orc.reconstructed_positions(o, x, y, z, kind="sum")
+    assert rms3(got, exp) < 1e-4
+
+
+def test_device_pointer_path(ctx):
+    import torch
+    n = 32
+    x, y, z, w = synth.box_catalog(20000, L=1000.0, mesh=32, seed=81)
+    kw = box_kw()
+    rec = bh.IterativeRecon(**kw)
+    hx, hy, hz = x.copy(), y.copy(), z.copy()
+    dh = bh.run_(rec, (n, n, n), hx, hy, hz, w)
+    ph = bh.reconstructed_positions(rec, hx, hy, hz, field="sum")
+    dev = [torch.from_numpy(a.copy()).cuda() for a in (x, y, z, w)]
+    rec2 = bh.IterativeRecon(**kw)
+    dd = bh.run_(rec2, (n, n, n), *dev)
+    pd = bh.reconstructed_positions(rec2, dev[0], dev[1], dev[2], field="sum")
+    assert rel_l2(dd, dh) < 1e-6
+    assert rms3([t.cpu().numpy() for t in pd], ph) < 1e-5
+    np.testing.assert_array_equal(dev[0].cpu().numpy(), hx)
+
+
+def test_no_result_error(ctx):
+    c = bh.Context(0)
+    rec = bh.IterativeRecon(**box_kw(), ctx=c)
+    with pytest.raises(bh.BaorecError):
+        bh.reconstructed_positions(rec, np.ones(3, F32), np.ones(3, F32), np.ones(3, F32))
+    c.close()
