@@ -36,6 +36,20 @@ struct baorec_ctx {
     uint64_t disp_gen = 0;
     float disp_box[6] = {0, 0, 0, 0, 0, 0};
     int disp_valid = 0;
+    // optional per-launch timing (HIP events on `stream`)
+    int prof_on = 0;
+    struct Pending {
+        const char *name;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    struct Agg {
+        int64_t count = 0;
+        double ms = 0.0, bytes = 0.0;
+    };
+    std::map<std::string, Agg> prof;
 };
 
 static thread_local std::string g_create_err;
@@ -134,7 +148,58 @@ void copy_out(baorec_ctx *c, int mem, float *host, const float *dev, int64_t n) 
     hipchk(c, hipMemcpyAsync(host, dev, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream), "D2H");
 }
 
-void sync(baorec_ctx *c) { hipchk(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
+hipEvent_t take_event(baorec_ctx *c) {
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipchk(c, hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+void prof_resolve(baorec_ctx *c) {
+    for (auto &p : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto &g = c->prof[p.name];
+            g.count++;
+            g.ms += ms;
+            g.bytes += p.bytes;
+        }
+        c->event_pool.push_back(p.a);
+        c->event_pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+void sync(baorec_ctx *c) {
+    hipchk(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (!c->pending.empty()) prof_resolve(c);
+}
+
+// Brackets one launch with events when profiling is on; `bytes` is the
+// launch's algorithmic HBM traffic (compulsory reads + writes).
+struct ProfScope {
+    baorec_ctx *c;
+    hipEvent_t a = nullptr, b = nullptr;
+    const char *name;
+    double bytes;
+    ProfScope(baorec_ctx *c_, const char *n, double by) : c(c_), name(n), bytes(by) {
+        if (c->prof_on) {
+            a = take_event(c);
+            b = take_event(c);
+            hipchk(c, hipEventRecord(a, c->stream), "hipEventRecord");
+        }
+    }
+    ~ProfScope() {
+        if (a) {
+            (void)hipEventRecord(b, c->stream);
+            c->pending.push_back({name, bytes, a, b});
+        }
+    }
+};
 
 // ----------------------------------------------------------- device scalars
 
@@ -217,8 +282,14 @@ void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, f
     a.rho = rho;
     a.rstride = G.R();
     (void)tag;
-    hipchk(c, launch_scatter(a, nullptr, c->stream), "scatter");
-    hipchk(c, launch_scatter_accum(a, accumulate, c->stream), "scatter accumulate");
+    {
+        ProfScope ps(c, "scatter_bin", (double)n * (a.mutate && wrap ? 40.0 : 28.0) + 16.0 * n * 1.2);
+        hipchk(c, launch_scatter(a, nullptr, c->stream), "scatter");
+    }
+    {
+        ProfScope ps(c, "scatter_accum", 16.0 * n * 1.2 + 4.0 * G.R() * (accumulate ? 2 : 1));
+        hipchk(c, launch_scatter_accum(a, accumulate, c->stream), "scatter accumulate");
+    }
 }
 
 // ------------------------------------------------------------------ FFT
@@ -238,6 +309,20 @@ FFTBufs fft_bufs(baorec_ctx *c, const Grid &G) {
 
 void tables(baorec_ctx *c, const Grid &G, const Slot *slot, FFTBufs &f) {
     hipchk(c, launch_tables(slot, f.ktab, f.xtab, G.nx, G.ny, G.nz, 1, c->stream), "tables");
+}
+
+void strided(baorec_ctx *c, const Grid &G, const StridedArgs &a, int njobs, int nbatch, const char *what) {
+    double bytes = 0.0;
+    for (int j = 0; j < njobs; j++) bytes += 8.0 * G.C() * (1 + a.job[j].nout);
+    ProfScope ps(c, a.axis == 1 ? "fft_y" : "fft_z", bytes * nbatch);
+    hipchk(c, launch_strided(a, njobs, nbatch, c->stream), what);
+}
+
+void x_inv(baorec_ctx *c, const Grid &G, const XInvArgs &a, int epi, int nbatch, const char *what) {
+    double out = epi == XE_DISP ? 16.0 : 4.0;
+    double aux = (epi == XE_ITER_LOCAL || epi == XE_ITER_FIXED) ? 4.0 : 0.0;
+    ProfScope ps(c, "fft_x_c2r", (double)nbatch * (8.0 * G.C() * a.nin + (out + aux) * G.R()));
+    hipchk(c, launch_x_c2r(a, epi, nbatch, c->stream), what);
 }
 
 StridedArgs sargs(const Grid &G, const FFTBufs &f, int axis, int mode, int mul) {
@@ -266,6 +351,7 @@ void x_fwd(baorec_ctx *c, const Grid &G, const float *in, float2 *out, int nbatc
     a.nz = G.nz;
     a.sums = sums;
     a.sumStride = sumStride;
+    ProfScope ps(c, "fft_x_r2c", (double)nbatch * (4.0 * G.R() + 8.0 * G.C()));
     hipchk(c, launch_x_r2c(a, nbatch, c->stream), "x r2c");
 }
 
@@ -276,7 +362,7 @@ void strided_plain(baorec_ctx *c, const Grid &G, const FFTBufs &f, int axis, boo
     a.job[0].out[0] = data;
     a.job[0].nout = 1;
     a.job[0].pw[0] = 0;
-    hipchk(c, launch_strided(a, 1, nbatch, c->stream), inverse ? "strided inverse" : "strided fwd");
+    strided(c, G, a, 1, nbatch, inverse ? "strided inverse" : "strided fwd");
 }
 
 XInvArgs xargs(const Grid &G, const FFTBufs &f, const Slot *slot) {
@@ -307,7 +393,7 @@ void smooth_to_half(baorec_ctx *c, const Grid &G, FFTBufs &f, const float *rin, 
     a.job[0].out[0] = f.cw;
     a.job[0].nout = 1;
     a.job[0].pw[0] = 0;
-    hipchk(c, launch_strided(a, 1, nb, c->stream), "z smooth");
+    strided(c, G, a, 1, nb, "z smooth");
     strided_plain(c, G, f, 1, true, f.cw, nb);
 }
 
@@ -334,7 +420,7 @@ void overdensity(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
         a.kxpow[1] = 0;
         a.nin = 2;
     }
-    hipchk(c, launch_x_c2r(a, rand ? XE_OD_RAND : XE_OD_BOX, 1, c->stream), "x overdensity");
+    x_inv(c, G, a, rand ? XE_OD_RAND : XE_OD_BOX, 1, "x overdensity");
 }
 
 // n_iter x iterate! (src/iterative.jl:151-211): delta_s -> delta_r
@@ -360,7 +446,7 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
                 z.job[0].pw[m] = m;
             }
             z.job[0].nout = 3;
-            hipchk(c, launch_strided(z, 1, 1, c->stream), "z iterate");
+            strided(c, G, z, 1, 1, "z iterate");
             // y inverse: G0 -> H00,H10,H20 ; G1 -> H01,H11 ; G2 -> H02
             // H slots: 0:H00 1:H10 2:H01 3:H20 4:H11 5:H02 (x pass order)
             StridedArgs y = sargs(G, f, 1, 1, MUL_NONE);
@@ -376,7 +462,7 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             y.job[2].in = Gb + 2 * C;
             y.job[2].nout = 1;
             y.job[2].out[0] = Hb + 5 * C; y.job[2].pw[0] = 0;
-            hipchk(c, launch_strided(y, 3, 1, c->stream), "y iterate");
+            strided(c, G, y, 3, 1, "y iterate");
             XInvArgs a = xargs(G, f, slot);
             const int kxp[6] = {2, 1, 1, 0, 0, 0};
             const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
@@ -391,7 +477,7 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             a.nin = 6;
             a.aux = ds;
             a.out = dr;
-            hipchk(c, launch_x_c2r(a, XE_ITER_LOCAL, 1, c->stream), "x iterate");
+            x_inv(c, G, a, XE_ITER_LOCAL, 1, "x iterate");
         } else {
             StridedArgs z = sargs(G, f, 2, 2, MUL_FIXEDLOS);
             z.los[0] = p->los[0];
@@ -401,7 +487,7 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             z.job[0].out[0] = C0;
             z.job[0].pw[0] = 0;
             z.job[0].nout = 1;
-            hipchk(c, launch_strided(z, 1, 1, c->stream), "z iterate fixed");
+            strided(c, G, z, 1, 1, "z iterate fixed");
             strided_plain(c, G, f, 1, true, C0, 1);
             XInvArgs a = xargs(G, f, slot);
             a.in[0] = C0;
@@ -412,7 +498,7 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             a.fac[0] = fac;
             a.aux = ds;
             a.out = dr;
-            hipchk(c, launch_x_c2r(a, XE_ITER_FIXED, 1, c->stream), "x iterate fixed");
+            x_inv(c, G, a, XE_ITER_FIXED, 1, "x iterate fixed");
         }
     }
 }
@@ -433,7 +519,7 @@ void displacement_grid(baorec_ctx *c, const Grid &G, int kind, Slot *slot, const
     z.job[0].out[1] = Gb + C;
     z.job[0].pw[1] = 1;
     z.job[0].nout = 2;
-    hipchk(c, launch_strided(z, 1, 1, c->stream), "z displacement");
+    strided(c, G, z, 1, 1, "z displacement");
     StridedArgs y = sargs(G, f, 1, 1, MUL_NONE);
     y.job[0].in = Gb;
     y.job[0].nout = 2;
@@ -442,14 +528,14 @@ void displacement_grid(baorec_ctx *c, const Grid &G, int kind, Slot *slot, const
     y.job[1].in = Gb + C;
     y.job[1].nout = 1;
     y.job[1].out[0] = Hb + 2 * C; y.job[1].pw[0] = 0;
-    hipchk(c, launch_strided(y, 2, 1, c->stream), "y displacement");
+    strided(c, G, y, 2, 1, "y displacement");
     XInvArgs a = xargs(G, f, slot);
     a.in[0] = Hb; a.kxpow[0] = 1;
     a.in[1] = Hb + C; a.kxpow[1] = 0;
     a.in[2] = Hb + 2 * C; a.kxpow[2] = 0;
     a.nin = 3;
     a.out = reinterpret_cast<float *>(disp);
-    hipchk(c, launch_x_c2r(a, XE_DISP, 1, c->stream), "x displacement");
+    x_inv(c, G, a, XE_DISP, 1, "x displacement");
 }
 
 // ---------------------------------------------------------------- multigrid
@@ -538,6 +624,7 @@ double *votherd(MGPyr &P, int l) { return P.cur[l] ? P.Ad[l] : P.Bd[l]; }
 
 void mg_jacobi_n(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int n) {
     for (int i = 0; i < n; i++) {
+        ProfScope ps(c, "mg_jacobi", 12.0 * P.g[l].R());
         hipchk(c, launch_mg_jacobi(vother(P, l), vcur(P, l), P.F[l], P.L[l], p->beta,
                                    p->jacobi_damping_factor, p->has_los, c->stream),
                "jacobi");
@@ -559,14 +646,23 @@ void mg_vcycle_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
     mg_jacobi_n(c, P, l, p, p->jacobi_niterations);
     if (l + 1 < (int)P.g.size()) {
         float *r = vother(P, l);
-        hipchk(c, launch_mg_residual(r, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
-               "residual");
+        {
+            ProfScope ps(c, "mg_residual", 12.0 * P.g[l].R());
+            hipchk(c, launch_mg_residual(r, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
+                   "residual");
+        }
         const int fn[3] = {P.g[l].nx, P.g[l].ny, P.g[l].nz};
-        hipchk(c, launch_mg_reduce(P.F[l + 1], r, fn, c->stream), "reduce");
+        {
+            ProfScope ps(c, "mg_reduce", 4.5 * P.g[l].R());
+            hipchk(c, launch_mg_reduce(P.F[l + 1], r, fn, c->stream), "reduce");
+        }
         zero_level(c, P, l + 1);
         mg_vcycle_ref(c, P, l + 1, p);
         const int cn[3] = {P.g[l + 1].nx, P.g[l + 1].ny, P.g[l + 1].nz};
-        hipchk(c, launch_mg_prolong(vcur(P, l), vcur(P, l + 1), cn, 1, c->stream), "prolong");
+        {
+            ProfScope ps(c, "mg_prolong", 8.5 * P.g[l].R());
+            hipchk(c, launch_mg_prolong(vcur(P, l), vcur(P, l + 1), cn, 1, c->stream), "prolong");
+        }
     }
     mg_jacobi_n(c, P, l, p, p->jacobi_niterations);
 }
@@ -589,6 +685,7 @@ void mg_fmg_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
 void mg_smooth_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int sweeps) {
     for (int s = 0; s < sweeps; s++)
         for (int col = 0; col < 4; col++) {
+            ProfScope ps(c, P.dbl[l] ? "mg_mcgs64" : "mg_mcgs", (P.dbl[l] ? 8.0 : 4.0) * 2.5 * P.g[l].R());
             if (P.dbl[l])
                 hipchk(c, launch_mg_rbgs<double>(vcurd(P, l), P.Fd[l], P.L[l], p->beta, p->has_los, col,
                                                  c->stream),
@@ -1040,7 +1137,10 @@ static int shifts_impl(baorec_ctx *c, const baorec_params *p, int mem, const flo
         a.oy = o2;
         a.oz = o3;
         a.err = &slot->err;
-        hipchk(c, launch_gather(a, c->stream), "gather");
+        {
+            ProfScope ps(c, "gather", (double)n * (24.0 + 128.0));
+            hipchk(c, launch_gather(a, c->stream), "gather");
+        }
         check_err_flag(c, slot);
         copy_out(c, mem, ox, o1, n);
         copy_out(c, mem, oy, o2, n);
@@ -1137,7 +1237,7 @@ int baorec_smooth(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], 
         a.kxpow[0] = 0;
         a.nin = 1;
         a.out = d;
-        hipchk(c, launch_x_c2r(a, XE_PLAIN, 1, c->stream), "x inverse");
+        x_inv(c, G, a, XE_PLAIN, 1, "x inverse");
         copy_out(c, mem, field, d, G.R());
         sync(c);
     });
@@ -1178,7 +1278,7 @@ int baorec_irfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *i
         a.kxpow[0] = 0;
         a.nin = 1;
         a.out = o;
-        hipchk(c, launch_x_c2r(a, XE_PLAIN, 1, c->stream), "x inverse");
+        x_inv(c, G, a, XE_PLAIN, 1, "x inverse");
         copy_out(c, mem, out, o, G.R());
         sync(c);
     });
@@ -1339,6 +1439,39 @@ int baorec_mg_fmg(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int me
         copy_out(c, mem, v, dv, G.R());
         sync(c);
     });
+}
+
+int baorec_profile(baorec_ctx *c, int enable) {
+    if (!c) return BAOREC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    prof_resolve(c);
+    c->prof.clear();
+    c->prof_on = enable ? 1 : 0;
+    return BAOREC_OK;
+}
+
+int64_t baorec_profile_report(baorec_ctx *c, char *buf, int64_t len) {
+    if (!c) return BAOREC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    prof_resolve(c);
+    std::string js = "{";
+    bool first = true;
+    for (auto &kv : c->prof) {
+        char tmp[256];
+        snprintf(tmp, sizeof tmp, "%s\"%s\": {\"count\": %lld, \"ms\": %.6f, \"bytes\": %.1f}",
+                 first ? "" : ", ", kv.first.c_str(), (long long)kv.second.count, kv.second.ms,
+                 kv.second.bytes);
+        js += tmp;
+        first = false;
+    }
+    js += "}";
+    if (buf && len > 0) {
+        strncpy(buf, js.c_str(), (size_t)len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int64_t)js.size() + 1;
 }
 
 int baorec_run_batch(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int mem, int ncat,

@@ -97,6 +97,8 @@ SIGNATURES = {
     "baorec_mg_vcycle": (INT, [P, PP, I32P, INT, FP, FP]),
     "baorec_mg_fmg": (INT, [P, PP, I32P, INT, FP, FP]),
     "baorec_run_batch": (INT, [P, PP, I32P, INT, INT, FP, FP, FP, FP, FP, FP, FP, FP]),
+    "baorec_profile": (INT, [P, INT]),
+    "baorec_profile_report": (I64, [P, ctypes.c_char_p, I64]),
 }
 
 _lib = None

@@ -194,7 +194,11 @@ class ResultCache:
 # ------------------------------------------------------------------ arrays
 
 def _is_device(a) -> bool:
-    return hasattr(a, "is_cuda") and a.is_cuda
+    return bool(getattr(a, "is_cuda", False))
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
 
 
 def _ptr(a):
@@ -216,9 +220,12 @@ def _f32(a, writable=False):
     if a is None:
         return None
     if _is_device(a):
-        import torch
-        if a.dtype != torch.float32 or not a.is_contiguous():
-            raise TypeError("device arrays must be contiguous float32 tensors")
+        if _is_torch(a):
+            import torch
+            if a.dtype != torch.float32 or not a.is_contiguous():
+                raise TypeError("device arrays must be contiguous float32 tensors")
+        elif a.dtype != F32:
+            raise TypeError("device arrays must be float32")
         return a
     if not (isinstance(a, np.ndarray) and a.dtype == F32 and a.flags.c_contiguous):
         if writable:
@@ -253,9 +260,16 @@ def _cat(x, y, z, w, rx=None, ry=None, rz=None, rw=None):
 def _new_grid(grid, like):
     nx, ny, nz = grid
     if _is_device(like):
-        import torch
-        return torch.empty((nz, ny, nx), dtype=torch.float32, device=like.device)
+        return _dev_empty((nz, ny, nx), like)
     return np.zeros((nz, ny, nx), dtype=F32)
+
+
+def _dev_empty(shape, like):
+    if _is_torch(like):
+        import torch
+        return torch.empty(shape, dtype=torch.float32, device=like.device)
+    from .devmem import DeviceArray
+    return DeviceArray(shape, F32, like.device)
 
 
 # ------------------------------------------------------------------ run!
@@ -317,8 +331,7 @@ def _shifts(name, recon, x, y, z, field3d, field):
     n = len(x)
     mem = _mem(x, y, z, field3d)
     if _is_device(x):
-        import torch
-        outs = [torch.empty(n, dtype=torch.float32, device=x.device) for _ in range(3)]
+        outs = [_dev_empty((n,), x) for _ in range(3)]
     else:
         outs = [np.empty(n, dtype=F32) for _ in range(3)]
     if field3d is None or isinstance(field3d, ResultCache):

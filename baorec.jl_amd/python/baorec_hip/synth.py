@@ -35,7 +35,7 @@ def gaussian_field(n: int, L: float, rng, amp=2e4, slope=-1.5, kmin=0.005, kmax=
         pk = np.where((k >= kmin) & (k <= kmax), amp * (k / 0.1) ** slope, 0.0)
     vcell = (L / n) ** 3
     wk *= np.sqrt(pk / vcell)
-    return np.fft.irfftn(wk, s=(n, n, n))
+    return np.fft.irfftn(wk, s=(n, n, n), axes=(0, 1, 2))
 
 
 def lognormal(g):

@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Reconstructions per second of the BAOrec.jl hot path on one MI355X.
+
+One step = one reconstruction as the reference's GPU benchmark times it
+(test/benchmarks_gpu.jl:114-126): run!(IterativeRecon, 512^3, data, randoms)
+followed by reconstructed_positions(recon, data; field=:sum), with the
+catalogs already resident in HBM.
+
+Default workload (the 512^3 lightcone configuration): 2,000,000 lognormal
+tracers in the shell r in [1000, 1800] Mpc/h over a 90x90 deg cap with
+n(r) ~ r^-0.5, 20,000,000 unclustered randoms in the same geometry, box padded
+10 %, local radial LOS, CIC, FP32, R = 15 Mpc/h, b = 2.2, f = 0.75,
+3 iterations.  Synthetic data (the reference's mocks are not available).
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" fields).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "baorec.jl_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(args):
+    from baorec_hip import synth
+    t0 = time.perf_counter()
+    if args.workload == "lightcone512":
+        data = synth.lightcone_catalog(args.ndata, r_min=1000.0, r_max=1800.0, mesh=256, seed=2)
+        rand = synth.lightcone_randoms(args.nrand, r_min=1000.0, r_max=1800.0, seed=3)
+        ext = max(float(np.max(a) - np.min(a)) for a in rand[:3])
+        cfg = dict(grid=(512, 512, 512), bias=2.2, f=0.75, R=15.0, los=None, pad=0.1 * ext)
+    elif args.workload == "box128":
+        data = synth.box_catalog(1_000_000, L=1000.0, mesh=256, seed=1)
+        rand = None
+        cfg = dict(grid=(128, 128, 128), bias=2.0, f=0.8, R=15.0, los=(0.0, 0.0, 1.0), pad=0.0)
+    else:
+        raise SystemExit(f"unknown workload {args.workload}")
+    log(f"synthetic catalogs in {time.perf_counter() - t0:.1f} s")
+    return data, rand, cfg
+
+
+def gpu_bench(args, data, rand, cfg):
+    import baorec_hip as bh
+    from baorec_hip import _lib as L
+    from baorec_hip.devmem import DeviceArray, hip
+
+    ctx = bh.Context(0)
+    lib = ctx.lib
+    p = L.Params()
+    lib.baorec_params_default(ctypes.byref(p), cfg["bias"], cfg["f"], cfg["R"])
+    p.box_pad = cfg["pad"]
+    if cfg["los"] is None:
+        p.has_los = 0
+    else:
+        p.has_los = 1
+        for a in range(3):
+            p.los[a] = cfg["los"][a]
+        for a in range(3):
+            p.box_size[a] = 1000.0
+            p.box_min[a] = 0.0
+    grid = (ctypes.c_int32 * 3)(*cfg["grid"])
+    d = [DeviceArray.from_numpy(a) for a in data]
+    r = [DeviceArray.from_numpy(a) for a in rand] if rand is not None else [None] * 4
+    n = len(data[0])
+    nr = len(rand[0]) if rand is not None else 0
+    out = [DeviceArray((n,)) for _ in range(3)]
+    # the periodic-box path wraps positions in place: keep pristine copies
+    pristine = [DeviceArray.from_numpy(a) for a in data[:3]] if nr == 0 else None
+    vp = lambda a: None if a is None else ctypes.c_void_p(a.ptr)  # noqa: E731
+
+    def step():
+        rc = lib.baorec_run_iterative(ctx.handle, ctypes.byref(p), grid, L.MEM_DEVICE, vp(d[0]), vp(d[1]),
+                                      vp(d[2]), vp(d[3]), n, vp(r[0]), vp(r[1]), vp(r[2]), vp(r[3]), nr)
+        ctx.check(rc)
+        rc = lib.baorec_reconstructed_positions(ctx.handle, ctypes.byref(p), L.MEM_DEVICE, vp(d[0]), vp(d[1]),
+                                                vp(d[2]), n, None, grid, L.ITERATIVE, L.FIELD["sum"],
+                                                vp(out[0]), vp(out[1]), vp(out[2]))
+        ctx.check(rc)
+
+    h = hip()
+    for i in range(args.warmup):
+        step()
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    h.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    h.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {dt:.3f} s")
+
+    # one profiled step: per-kernel HIP-event durations with algorithmic bytes
+    lib.baorec_profile(ctx.handle, 1)
+    step()
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib.baorec_profile_report(ctx.handle, buf, len(buf))
+    lib.baorec_profile(ctx.handle, 0)
+    prof = json.loads(buf.value.decode())
+    pos = [o.numpy() for o in out]
+    finite = all(np.isfinite(a).all() for a in pos)
+    return dict(dt=dt, prof=prof, finite=finite, box=(list(p.box_size), list(p.box_min)))
+
+
+def cpu_baseline(args, data, rand, cfg):
+    """FP32 CPU restatement (oracle) of the reference path on a bounded
+    sample: setup (CIC + smoothing + normalisation), one of the three
+    iterations, and the shifted-position read-out, all at the full size; the
+    remaining iterations are extrapolated from the timed one."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import baorec_oracle as orc
+    workers = int(os.environ.get("BAOREC_ORACLE_WORKERS", os.cpu_count() or 1))
+    os.environ["BAOREC_ORACLE_WORKERS"] = str(workers)
+    nx, ny, nz = cfg["grid"]
+    x, y, z, w = (a.copy() for a in data)
+    rec = orc.IterativeRecon(bias=cfg["bias"], f=cfg["f"], smoothing_radius=cfg["R"], los=cfg["los"])
+    t0 = time.perf_counter()
+    if rand is not None:
+        rec.box_size, rec.box_min = orc.setup_box(rand[0], rand[1], rand[2], cfg["pad"])
+        delta = np.zeros((nz, ny, nx), np.float32)
+        orc.setup_overdensity_randoms(delta, rec, x, y, z, w, *rand)
+    else:
+        rec.box_size, rec.box_min = [1000.0] * 3, [0.0] * 3
+        delta = np.zeros((nz, ny, nx), np.float32)
+        orc.setup_overdensity(delta, rec, x, y, z, w)
+    t_setup = time.perf_counter() - t0
+    ds = delta.copy()
+    kv = orc.k_vec((nx, ny, nz), rec.box_size)
+    xv = orc.x_vec((nx, ny, nz), rec.box_size, rec.box_min) if rec.los is None else None
+    t0 = time.perf_counter()
+    orc.iterate(delta, ds, kv, 1, rec.beta, los=rec.los, xv=xv)
+    t_iter = time.perf_counter() - t0
+    rec.result_cache = delta
+    t0 = time.perf_counter()
+    orc.reconstructed_positions(rec, x, y, z, kind="sum")
+    t_read = time.perf_counter() - t0
+    total = t_setup + rec.n_iter * t_iter + t_read
+    log(f"cpu oracle: setup {t_setup:.1f} s, iteration {t_iter:.1f} s, read-out {t_read:.1f} s")
+    return dict(value=1.0 / total, unit="recons/s", cores=workers, kind="port",
+                sample=(f"FP32 numpy/C oracle at the full {nx}^3 workload: setup {t_setup:.1f} s + "
+                        f"1 of {rec.n_iter} iterations {t_iter:.1f} s (x{rec.n_iter} extrapolated) + "
+                        f"read-out {t_read:.1f} s"))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="lightcone512")
+    ap.add_argument("--ndata", type=int, default=2_000_000)
+    ap.add_argument("--nrand", type=int, default=20_000_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    data, rand, cfg = make_workload(args)
+    g = gpu_bench(args, data, rand, cfg)
+    prof = g["prof"]
+    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    name, st = dom
+    achieved = st["bytes"] / (st["ms"] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None, "launches": st["count"],
+            "avg_launch_ms": round(st["ms"] / st["count"], 4),
+            "bytes_per_launch": st["bytes"] / st["count"]}
+    total_ms = sum(v["ms"] for v in prof.values())
+    total_bytes = sum(v["bytes"] for v in prof.values())
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, data, rand, cfg)
+    ms = g["dt"] / args.steps * 1e3
+    nx, ny, nz = cfg["grid"]
+    line = {
+        "metric": "recons/s at 512^3 grid, 2M particles (plain iterative-FFT recon)",
+        "value": round(args.steps / g["dt"], 4),
+        "unit": "recons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "grid": [nx, ny, nz], "n_data": len(data[0]),
+                   "n_randoms": 0 if rand is None else len(rand[0]), "recon": "iterative",
+                   "n_iter": 3, "los": "local" if cfg["los"] is None else "fixed",
+                   "box_pad": round(cfg["pad"], 2)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": {k: {"count": v["count"], "ms": round(v["ms"], 3),
+                        "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()},
+        "profiled_step": {"kernel_ms": round(total_ms, 3), "algorithmic_GB": round(total_bytes / 1e9, 3)},
+        "outputs_finite": g["finite"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -267,6 +267,17 @@ int baorec_run_batch(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3],
                      int mem, int ncat, const int64_t *offsets, float *x, float *y,
                      float *z, const float *w, float *ox, float *oy, float *oz);
 
+/* ------------------------------------------------------------- profiling */
+
+/* Extension, no reference counterpart (the reference times whole calls with
+ * BenchmarkTools): when enabled, every kernel launch on ctx is bracketed with
+ * HIP events on the context's stream and aggregated per kernel family with
+ * its algorithmic HBM bytes.  Enabling or disabling clears the tallies. */
+int baorec_profile(baorec_ctx *ctx, int enable);
+/* JSON {"name": {"count": c, "ms": total, "bytes": total}, ...} into buf;
+ * returns the length needed including the terminating NUL. */
+int64_t baorec_profile_report(baorec_ctx *ctx, char *buf, int64_t len);
+
 #ifdef __cplusplus
 }
 #endif
