@@ -95,10 +95,15 @@ T *bufT(baorec_ctx *c, const std::string &name, size_t count) {
     return static_cast<T *>(buf(c, name, sizeof(T) * count));
 }
 
+// P: valid kx columns of the half spectrum (nx/2 + 1); PP: row pitch of the
+// work spectra, rounded up to 16 complex (128 B) so that every column
+// segment a strided pass touches starts on a cache line.
 struct Grid {
     int nx, ny, nz, P;
+    int PP() const { return (P + 15) / 16 * 16; }
     int64_t R() const { return (int64_t)nx * ny * nz; }
-    int64_t C() const { return (int64_t)P * ny * nz; }
+    int64_t C() const { return (int64_t)PP() * ny * nz; }
+    int64_t Cdense() const { return (int64_t)P * ny * nz; }
 };
 
 Grid check_grid(baorec_ctx *c, const int32_t *g, bool fft) {
@@ -330,7 +335,8 @@ StridedArgs sargs(const Grid &G, const FFTBufs &f, int axis, int mode, int mul) 
     a.cstride = G.C();
     a.ktab = f.ktab;
     a.ktabStride = 0;
-    a.P = G.P;
+    a.P = G.PP();
+    a.Pv = G.P;
     a.ny = G.ny;
     a.nz = G.nz;
     a.axis = axis;
@@ -346,6 +352,7 @@ void x_fwd(baorec_ctx *c, const Grid &G, const float *in, float2 *out, int nbatc
     a.out = out;
     a.rstride = G.R();
     a.cstride = G.C();
+    a.P = G.PP();
     a.nx = G.nx;
     a.ny = G.ny;
     a.nz = G.nz;
@@ -369,6 +376,7 @@ XInvArgs xargs(const Grid &G, const FFTBufs &f, const Slot *slot) {
     XInvArgs a{};
     a.cstride = G.C();
     a.rstride = G.R();
+    a.P = G.PP();
     a.nx = G.nx;
     a.ny = G.ny;
     a.nz = G.nz;
@@ -1246,16 +1254,17 @@ int baorec_smooth(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], 
 int baorec_rfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *in, float *out_complex) {
     return guarded(c, [&] {
         Grid G = check_grid(c, grid, true);
+        if (!out_complex) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "out is NULL");
         const float *d = stage_in(c, mem, in, G.R(), "in_field");
         FFTBufs f = fft_bufs(c, G);
-        float2 *o = mem == BAOREC_MEM_DEVICE ? reinterpret_cast<float2 *>(out_complex) : f.cw;
-        if (!out_complex) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "out is NULL");
+        float2 *o = f.cw;
         x_fwd(c, G, d, o, 1, nullptr, 0);
         strided_plain(c, G, f, 1, false, o, 1);
         strided_plain(c, G, f, 2, false, o, 1);
-        if (mem != BAOREC_MEM_DEVICE)
-            hipchk(c, hipMemcpyAsync(out_complex, o, sizeof(float2) * G.C(), hipMemcpyDeviceToHost, c->stream),
-                   "D2H");
+        hipchk(c, hipMemcpy2DAsync(out_complex, sizeof(float2) * G.P, o, sizeof(float2) * G.PP(),
+                                   sizeof(float2) * G.P, (size_t)G.ny * G.nz,
+                                   mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                   c->stream), "copy spectrum");
         sync(c);
     });
 }
@@ -1266,9 +1275,10 @@ int baorec_irfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *i
         if (!in_complex || !out) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "NULL argument");
         FFTBufs f = fft_bufs(c, G);
         float2 *w = f.cw;  // work copy: the caller's spectrum is not modified
-        hipchk(c, hipMemcpyAsync(w, in_complex, sizeof(float2) * G.C(),
-                                 mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                                 c->stream), "copy spectrum");
+        hipchk(c, hipMemcpy2DAsync(w, sizeof(float2) * G.PP(), in_complex, sizeof(float2) * G.P,
+                                   sizeof(float2) * G.P, (size_t)G.ny * G.nz,
+                                   mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                   c->stream), "copy spectrum");
         Slot *slot = slot_buf(c) + 1;
         strided_plain(c, G, f, 2, true, w, 1);
         strided_plain(c, G, f, 1, true, w, 1);

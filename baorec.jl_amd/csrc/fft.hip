@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_r2c(XFwdArgs a) {
 #pragma unroll
     for (int i = 0; i < 8; i++) lds[lay(PF::out_pos(t, i))] = v[i];
     __syncthreads();
-    float2 *dst = a.out + b * a.cstride + line * (M + 1);
+    float2 *dst = a.out + b * a.cstride + line * a.P;
     if (valid) {
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -139,15 +139,19 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
             acc[i] = make_float2(0.f, 0.f);
         }
         if constexpr (EPI == XE_ITER_LOCAL) {
+            // 1/x^2 once per cell (the reference divides by x^2 per pair,
+            // src/iterative.jl:33-34; the product differs by at most 1 ulp)
             float x0 = xv[2 * m], x1 = xv[2 * m + 1];
-            x2[i][0] = (x0 * x0 + xy * xy) + xz * xz;
-            x2[i][1] = (x1 * x1 + xy * xy) + xz * xz;
+            const float s0 = (x0 * x0 + xy * xy) + xz * xz;
+            const float s1 = (x1 * x1 + xy * xy) + xz * xz;
+            x2[i][0] = s0 > 0.f ? 1.0f / s0 : 0.f;
+            x2[i][1] = s1 > 0.f ? 1.0f / s1 : 0.f;
         }
     }
     float4 disp[8][2];
 
     for (int q = 0; q < a.nin; q++) {
-        const float2 *src = a.in[q] + b * a.cstride + line * (M + 1);
+        const float2 *src = a.in[q] + b * a.cstride + line * a.P;
         const int pw = a.kxpow[q];
         float2 v[8];
 #pragma unroll
@@ -204,8 +208,8 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
                         case 4: xi = xy; xj = xz; break;
                         default: xi = xz; xj = xz; break;
                     }
-                    const float xsq = x2[i][h];
-                    ac[h] = xsq > 0.f ? ac[h] - (((fac * dv[h]) * xi) * xj) / xsq : 0.f;
+                    const float ix2 = x2[i][h];
+                    ac[h] = ix2 > 0.f ? ac[h] - (((fac * dv[h]) * xi) * xj) * ix2 : 0.f;
                 }
             } else if constexpr (EPI == XE_DISP) {
                 if (q == 0) { disp[i][0].x = r0; disp[i][1].x = r1; }
@@ -298,10 +302,10 @@ __global__ void __launch_bounds__(SShape<M>::NT) k_strided(StridedArgs a) {
     __shared__ float2 lds[B * M];
     const int tid = threadIdx.x;
     const int col = tid % B, t = tid / B;
-    const int ncg = (a.P + B - 1) / B;
+    const int ncg = (a.Pv + B - 1) / B;
     const int cg = blockIdx.x % ncg, other = blockIdx.x / ncg;
     const int kxi = cg * B + col;
-    const bool valid = kxi < a.P;
+    const bool valid = kxi < a.Pv;
     const StridedJob &J = a.job[blockIdx.y];
     const int64_t b = blockIdx.z;
     int64_t base, rs;
@@ -314,7 +318,7 @@ __global__ void __launch_bounds__(SShape<M>::NT) k_strided(StridedArgs a) {
     }
     base += b * a.cstride;
     const float *kt = a.ktab + b * a.ktabStride;
-    const float *kax = a.axis == 1 ? kt + a.P : kt + a.P + a.ny;
+    const float *kax = a.axis == 1 ? kt + a.Pv : kt + a.Pv + a.ny;
     LayStrided lay{col, B};
     float2 v[8];
 
@@ -335,7 +339,7 @@ __global__ void __launch_bounds__(SShape<M>::NT) k_strided(StridedArgs a) {
             v[i] = valid ? J.in[base + PF::in_pos(t, i) * rs] : make_float2(0.f, 0.f);
         fft_line<M, false>(v, lds, lay, t, g_tw);
         const float kxv = valid ? kt[kxi] : 0.f;
-        const float kother = a.axis == 1 ? kt[a.P + a.ny + other] : kt[a.P + other];
+        const float kother = a.axis == 1 ? kt[a.Pv + a.ny + other] : kt[a.Pv + other];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const float kal = kax[PF::out_pos(t, i)];
@@ -426,7 +430,7 @@ hipError_t launch_x_c2r(const XInvArgs &a, int epi, int nbatch, hipStream_t s) {
 template <int M>
 static hipError_t strided_m(const StridedArgs &a, int njobs, int nbatch, hipStream_t s) {
     using S = SShape<M>;
-    const int ncg = (a.P + S::B - 1) / S::B;
+    const int ncg = (a.Pv + S::B - 1) / S::B;
     const int nother = a.axis == 1 ? a.nz : a.ny;
     dim3 grid((unsigned)(ncg * nother), njobs, nbatch);
     hipLaunchKernelGGL(k_strided<M>, grid, dim3(S::NT), 0, s, a);

@@ -36,7 +36,9 @@ struct StridedArgs {
     int64_t cstride;           // complex elements between batch items (blockIdx.z)
     const float *ktab;         // per batch item: kx[P] | ky[ny] | kz[nz]
     int64_t ktabStride;        // floats between batch items (0: shared)
-    int P, ny, nz;
+    int P;                     // complex pitch of a kx row (>= Pv, multiple of 16)
+    int Pv;                    // valid kx columns, nx/2 + 1
+    int ny, nz;
     int axis;                  // 1: y, 2: z
     int mode;                  // 0 forward, 1 inverse, 2 forward*mul*inverse
     int mul;                   // MulKind (mode 2)
@@ -61,6 +63,7 @@ struct XInvArgs {
     int nin;
     int64_t cstride;      // complex elements between batch items
     int64_t rstride;      // real elements between batch items
+    int P;                // complex pitch of a kx row
     float *out;           // real output (float4 for XE_DISP)
     const float *aux;     // delta_s for the iterative updates
     int nx, ny, nz;
@@ -79,6 +82,7 @@ struct XFwdArgs {
     const float *in;
     float2 *out;
     int64_t rstride, cstride;
+    int P;                // complex pitch of a kx row
     int nx, ny, nz;
     double *sums;         // optional FP64 total of each input (per batch item)
     int64_t sumStride;

@@ -504,8 +504,8 @@ __global__ void k_box_final(Slot *slot, float pad) {
 hipError_t launch_setup_box(const float *x, const float *y, const float *z, int64_t n, Slot *slot,
                             float pad, hipStream_t s) {
     hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, s, slot);
-    int64_t nb = (n + 255) / 256;
-    if (nb > 2048) nb = 2048;
+    int64_t nb = (n + 1023) / 1024;
+    if (nb > 65536) nb = 65536;
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_box_reduce, dim3((unsigned)nb), dim3(256), 0, s, x, y, z, n, slot);
     hipLaunchKernelGGL(k_box_final, dim3(1), dim3(64), 0, s, slot, pad);

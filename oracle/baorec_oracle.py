@@ -229,8 +229,11 @@ def setup_overdensity(delta: np.ndarray, recon, x, y, z, w, wrap=True) -> np.nda
 
 
 def setup_overdensity_randoms(delta: np.ndarray, recon, x, y, z, w, rx, ry, rz, rw,
-                              ran_min=0.01) -> np.ndarray:
-    """setup_overdensity!, src/recon.jl:60-91 (data minus alpha*randoms)."""
+                              ran_min=0.01, return_aux=False):
+    """setup_overdensity!, src/recon.jl:60-91 (data minus alpha*randoms).
+    return_aux=True also returns the smoothed randoms grid and the threshold
+    (cells with rho_r within rounding of the threshold are ambiguous: the
+    reference's own result there depends on its summation order)."""
     dt = delta.dtype.type
     dran = np.zeros_like(delta)
     cic(delta, x, y, z, w, recon.box_size, recon.box_min, wrap=False)
@@ -245,6 +248,8 @@ def setup_overdensity_randoms(delta: np.ndarray, recon, x, y, z, w, rx, ry, rz, 
     den = (dt(recon.bias) * alpha) * dran
     with np.errstate(divide="ignore", invalid="ignore"):
         delta[...] = np.where(dran > thr, delta / den, dt(0))
+    if return_aux:
+        return delta, dran, thr
     return delta
 
 
