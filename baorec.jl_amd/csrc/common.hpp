@@ -169,6 +169,80 @@ __device__ __forceinline__ void fft_line(float2 (&v)[8], float2 *lds, const Lay 
     fft_stages<M, INV, 0>(v, lds, lay, t, tw);
 }
 
+// Per-thread twiddles of every stage after the first, loaded once and reused
+// by kernels that run several transforms of the same length.
+template <int M, bool INV>
+struct TwCache {
+    using P = Plan<M, INV>;
+    static constexpr int NS1 = P::NST > 1 ? P::NST - 1 : 1;
+    float2 w[NS1][7];
+    __device__ __forceinline__ void load(int t, const float2 *__restrict__ tw) {
+        load_st<1>(t, tw);
+    }
+    template <int ST>
+    __device__ __forceinline__ void load_st(int t, const float2 *__restrict__ tw) {
+        if constexpr (ST < P::NST) {
+            constexpr int R = P::radix(ST), NS = P::ns(ST), NB = 8 / R;
+#pragma unroll
+            for (int s = 0; s < NB; s++) {
+                const int k = (t + s * (M / 8)) & (NS - 1);
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    float2 x = tw[NS * R + r * k];
+                    if (INV) x.y = -x.y;
+                    w[ST - 1][s * (R - 1) + r - 1] = x;
+                }
+            }
+            load_st<ST + 1>(t, tw);
+        }
+    }
+};
+
+template <int M, bool INV, int ST, class Lay>
+__device__ __forceinline__ void fft_stages_c(float2 (&v)[8], float2 *lds, const Lay &lay, int t,
+                                             const TwCache<M, INV> &tc) {
+    using P = Plan<M, INV>;
+    constexpr int R = P::radix(ST);
+    constexpr int NS = P::ns(ST);
+    constexpr int NB = 8 / R;
+    if constexpr (ST > 0) {
+        constexpr int RP = P::radix(ST - 1);
+        constexpr int NSP = P::ns(ST - 1);
+        constexpr int NBP = 8 / RP;
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NBP; s++) {
+            int j = t + s * (M / 8);
+            int k = j & (NSP - 1);
+            int base = (j - k) * RP + k;
+#pragma unroll
+            for (int q = 0; q < RP; q++) lds[lay(base + q * NSP)] = v[s * RP + q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NB; s++) {
+            int j = t + s * (M / 8);
+#pragma unroll
+            for (int r = 0; r < R; r++) v[s * R + r] = lds[lay(j + r * (M / R))];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NB; s++) {
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) v[s * R + r] = cmul(v[s * R + r], tc.w[ST - 1][s * (R - 1) + r - 1]);
+        }
+        dft<R, INV>(&v[s * R]);
+    }
+    if constexpr (ST + 1 < P::NST) fft_stages_c<M, INV, ST + 1>(v, lds, lay, t, tc);
+}
+
+template <int M, bool INV, class Lay>
+__device__ __forceinline__ void fft_line_c(float2 (&v)[8], float2 *lds, const Lay &lay, int t,
+                                           const TwCache<M, INV> &tc) {
+    fft_stages_c<M, INV, 0>(v, lds, lay, t, tc);
+}
+
 // LDS layouts
 struct LayStrided {  // [pos][col], B columns per row
     int col, B;

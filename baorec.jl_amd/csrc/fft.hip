@@ -105,13 +105,30 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_r2c(XFwdArgs a) {
 
 // ------------------------------------------------------------ x: C2R
 
+template <int EPI>
+struct XIn {
+    static constexpr int N = EPI == XE_ITER_LOCAL ? 6 : EPI == XE_DISP ? 3 : EPI == XE_OD_RAND ? 2 : 1;
+};
+
+// Loads X[k], k = t + i*M/8, of one spectrum line (times kx^pw) and, for
+// thread t == 0, the Nyquist bin X[M].
+template <int M>
+__device__ __forceinline__ void xline_load(const float2 *__restrict__ src, bool valid, int t,
+                                           float2 (&x)[8], float2 &xn) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = valid ? src[t + i * (M / 8)] : make_float2(0.f, 0.f);
+    xn = (valid && t == 0) ? src[M] : make_float2(0.f, 0.f);
+}
+
 template <int M, int EPI>
 __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
     using S = XShape<M>;
     using PI = Plan<M, true>;
+    constexpr int NIN = XIn<EPI>::N;
+    constexpr int TPL = S::TPL;
     __shared__ float2 lds[S::LPW * S::LS];
     const int tid = threadIdx.x;
-    const int li = tid / S::TPL, t = tid % S::TPL;
+    const int li = tid / TPL, t = tid % TPL;
     const int64_t nlines = (int64_t)a.ny * a.nz;
     const int64_t line = (int64_t)blockIdx.x * S::LPW + li;
     const bool valid = line < nlines;
@@ -120,9 +137,24 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
     const float *kx = a.ktab + b * a.ktabStride;
     const int64_t rbase = b * a.rstride + line * (2 * M);
     LayLine lay{li * S::LS};
+    // partner lane holding X[M - k] (k = t + i*M/8 -> thread TPL - t, index 7 - i)
+    const int lane = threadIdx.x & 63;
+    const int plane = t == 0 ? lane : lane - t + (TPL - t);
+
+    TwCache<M, true> tc;
+    tc.load(t, g_tw);
+    float2 wk[8];   // exp(+2 pi i k / N) of this thread's bins
+    float kxv[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int k = t + i * TPL;
+        wk[i] = cconj(g_tw[2 * M + k]);
+        kxv[i] = kx[k];
+    }
+    const float kxn = kx[M];
 
     float2 acc[8];
-    float x2[8][2];
+    float ix2[8][2];
     const float *xv = a.xtab ? a.xtab + b * a.xtabStride : nullptr;
     float xy = 0.f, xz = 0.f;
     if constexpr (EPI == XE_ITER_LOCAL) {
@@ -141,40 +173,46 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
         if constexpr (EPI == XE_ITER_LOCAL) {
             // 1/x^2 once per cell (the reference divides by x^2 per pair,
             // src/iterative.jl:33-34; the product differs by at most 1 ulp)
-            float x0 = xv[2 * m], x1 = xv[2 * m + 1];
+            const float x0 = xv[2 * m], x1 = xv[2 * m + 1];
             const float s0 = (x0 * x0 + xy * xy) + xz * xz;
             const float s1 = (x1 * x1 + xy * xy) + xz * xz;
-            x2[i][0] = s0 > 0.f ? 1.0f / s0 : 0.f;
-            x2[i][1] = s1 > 0.f ? 1.0f / s1 : 0.f;
+            ix2[i][0] = s0 > 0.f ? 1.0f / s0 : 0.f;
+            ix2[i][1] = s1 > 0.f ? 1.0f / s1 : 0.f;
         }
     }
     float4 disp[8][2];
+    float rr[8][2];
 
-    for (int q = 0; q < a.nin; q++) {
-        const float2 *src = a.in[q] + b * a.cstride + line * a.P;
+    float2 cur[8], cn;
+    xline_load<M>(a.in[0] + b * a.cstride + line * a.P, valid, t, cur, cn);
+#pragma unroll
+    for (int q = 0; q < NIN; q++) {
+        float2 nxt[8], nn;
+        if (q + 1 < NIN) xline_load<M>(a.in[q + 1] + b * a.cstride + line * a.P, valid, t, nxt, nn);
         const int pw = a.kxpow[q];
         float2 v[8];
+        float2 xk[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) xk[i] = pw ? cscale(cur[i], kpow(kxv[i], pw)) : cur[i];
+        const float2 xnyq = pw ? cscale(cn, kpow(kxn, pw)) : cn;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const int k = PI::in_pos(t, i);
-            float2 xk = valid ? src[k] : make_float2(0.f, 0.f);
-            float2 xm = valid ? src[M - k] : make_float2(0.f, 0.f);
-            if (pw) {
-                xk = cscale(xk, kpow(kx[k], pw));
-                xm = cscale(xm, kpow(kx[M - k], pw));
-            }
-            if (k == 0) {  // self-conjugate bins: imaginary parts ignored (FFTW/pocketfft C2R)
-                xk.y = 0.f;
+            float2 xm;
+            xm.x = __shfl(xk[7 - i].x, plane, 64);
+            xm.y = __shfl(xk[7 - i].y, plane, 64);
+            if (t == 0) xm = i == 0 ? xnyq : xk[(8 - i) & 7];
+            float2 xkk = xk[i];
+            if (t == 0 && i == 0) {  // self-conjugate bins: imaginary parts ignored (FFTW/pocketfft C2R)
+                xkk.y = 0.f;
                 xm.y = 0.f;
             }
-            float2 xc = cconj(xm);
-            float2 s = cadd(xk, xc);
-            float2 d = csub(xk, xc);
-            float2 w = cconj(g_tw[2 * M + k]);   // exp(+2 pi i k / N)
-            float2 wd = cmul(w, d);
+            const float2 xc = cconj(xm);
+            const float2 s = cadd(xkk, xc);
+            const float2 d = csub(xkk, xc);
+            const float2 wd = cmul(wk[i], d);
             v[i] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i * w * d
         }
-        fft_line<M, true>(v, lds, lay, t, g_tw);
+        fft_line_c<M, true>(v, lds, lay, t, tc);
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const float r0 = v[i].x * a.scale, r1 = v[i].y * a.scale;
@@ -183,8 +221,8 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
             } else if constexpr (EPI == XE_OD_BOX || EPI == XE_OD_RAND) {
                 if (q == 0) acc[i] = make_float2(r0, r1);
                 else {
-                    disp[i][0].x = r0;  // randoms grid
-                    disp[i][1].x = r1;
+                    rr[i][0] = r0;  // randoms grid
+                    rr[i][1] = r1;
                 }
             } else if constexpr (EPI == XE_ITER_FIXED) {
                 const float fac = a.fac[0];
@@ -199,23 +237,21 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
                 for (int h = 0; h < 2; h++) {
                     const float xx = xv[2 * m + h];
                     // pair order of src/iterative.jl:20-21: xx, xy, xz, yy, yz, zz
-                    float xi, xj;
-                    switch (q) {
-                        case 0: xi = xx; xj = xx; break;
-                        case 1: xi = xx; xj = xy; break;
-                        case 2: xi = xx; xj = xz; break;
-                        case 3: xi = xy; xj = xy; break;
-                        case 4: xi = xy; xj = xz; break;
-                        default: xi = xz; xj = xz; break;
-                    }
-                    const float ix2 = x2[i][h];
-                    ac[h] = ix2 > 0.f ? ac[h] - (((fac * dv[h]) * xi) * xj) * ix2 : 0.f;
+                    const float xi = q < 3 ? xx : (q < 5 ? xy : xz);
+                    const float xj = q == 0 ? xx : ((q == 1 || q == 3) ? xy : xz);
+                    const float c = ix2[i][h];
+                    ac[h] = c > 0.f ? ac[h] - (((fac * dv[h]) * xi) * xj) * c : 0.f;
                 }
             } else if constexpr (EPI == XE_DISP) {
                 if (q == 0) { disp[i][0].x = r0; disp[i][1].x = r1; }
                 else if (q == 1) { disp[i][0].y = r0; disp[i][1].y = r1; }
                 else { disp[i][0].z = r0; disp[i][1].z = r1; }
             }
+        }
+        if (q + 1 < NIN) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) cur[i] = nxt[i];
+            cn = nn;
         }
     }
     if (!valid) return;
@@ -249,11 +285,10 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             float *ac = &acc[i].x;
-            const float rr[2] = {disp[i][0].x, disp[i][1].x};
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const float d = ac[h] - alpha * rr[h];
-                ac[h] = ((double)rr[h] > thr) ? d / (ba * rr[h]) : 0.f;
+                const float d = ac[h] - alpha * rr[i][h];
+                ac[h] = ((double)rr[i][h] > thr) ? d / (ba * rr[i][h]) : 0.f;
             }
         }
     }
@@ -353,13 +388,19 @@ __global__ void __launch_bounds__(SShape<M>::NT) k_strided(StridedArgs a) {
             v[i] = valid ? J.in[base + PI::in_pos(t, i) * rs] : make_float2(0.f, 0.f);
     }
     float2 f[8];
+    float kal[8];
 #pragma unroll
-    for (int i = 0; i < 8; i++) f[i] = v[i];
+    for (int i = 0; i < 8; i++) {
+        f[i] = v[i];
+        kal[i] = kax[PI::in_pos(t, i)];
+    }
+    TwCache<M, true> tc;
+    tc.load(t, g_tw);
     for (int o = 0; o < J.nout; o++) {
         const int pw = J.pw[o];
 #pragma unroll
-        for (int i = 0; i < 8; i++) v[i] = pw ? cscale(f[i], kpow(kax[PI::in_pos(t, i)], pw)) : f[i];
-        fft_line<M, true>(v, lds, lay, t, g_tw);
+        for (int i = 0; i < 8; i++) v[i] = pw ? cscale(f[i], kpow(kal[i], pw)) : f[i];
+        fft_line_c<M, true>(v, lds, lay, t, tc);
         if (valid) {
             float2 *dst = J.out[o];
 #pragma unroll
