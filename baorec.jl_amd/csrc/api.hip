@@ -13,9 +13,6 @@
 
 using namespace baorec;
 
-namespace baorec {
-hipError_t launch_scatter_accum(const ScatterArgs &a, int accumulate, hipStream_t s);
-}
 
 struct DevBuf {
     void *p = nullptr;
@@ -287,7 +284,13 @@ void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, f
     a.rho = rho;
     a.rstride = G.R();
     (void)tag;
-    {
+    if (sort_fits(a)) {
+        const int nblk = sort_nblk(n);
+        const int T = a.ntx * a.nty * a.ntz;
+        uint32_t *H = bufT<uint32_t>(c, "sc_hist", (size_t)(nblk > 0 ? nblk : 1) * T);
+        ProfScope ps(c, "scatter_bin", (double)n * (a.mutate && wrap ? 40.0 : 28.0) + 16.0 * n * 1.2);
+        hipchk(c, launch_sort(a, nblk, H, nullptr, nullptr, nullptr, c->stream), "scatter sort");
+    } else {
         ProfScope ps(c, "scatter_bin", (double)n * (a.mutate && wrap ? 40.0 : 28.0) + 16.0 * n * 1.2);
         hipchk(c, launch_scatter(a, nullptr, c->stream), "scatter");
     }

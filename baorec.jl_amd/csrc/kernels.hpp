@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace baorec {
@@ -126,6 +128,14 @@ struct ScatterArgs {
 };
 
 hipError_t launch_scatter(ScatterArgs &a, uint32_t *scan_tmp, hipStream_t s);
+hipError_t launch_scatter_accum(const ScatterArgs &a, int accumulate, hipStream_t s);
+// Blocked counting sort (no global atomics); valid when sort_fits(a).
+bool sort_fits(const ScatterArgs &a);
+int sort_nblk(int64_t n);  // blocks of a single catalog (tables may then be NULL)
+int sort_blocks(int64_t n, const int64_t *offsets_host, int ncat, std::vector<int> &blk_cat,
+                std::vector<int64_t> &blk_beg, std::vector<int> &cat_blk0);
+hipError_t launch_sort(ScatterArgs &a, int nblk, uint32_t *H, const int *blk_cat, const int64_t *blk_beg,
+                       const int *cat_blk0, hipStream_t s);
 // Capacity (placements) the sorted buffer must hold for n particles.
 int64_t scatter_capacity(int64_t n, int tsc);
 

@@ -15,6 +15,8 @@
 // weights recomputed in accum are bit-identical to the reference's.
 #include <math.h>
 
+#include <vector>
+
 #include "kernels.hpp"
 
 namespace baorec {
@@ -216,8 +218,18 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
     __syncthreads();
     const uint32_t end = a.cursor[tid_global];
     const uint32_t beg = end - a.count[tid_global];
-    for (uint32_t s = beg + threadIdx.x; s < end; s += kAccumThreads) {
-        const float4 r = a.sorted[s];
+    constexpr int U = 4;  // placements in flight per thread
+    for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += U * kAccumThreads) {
+      float4 rbuf[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+          const uint32_t s = s0 + u * kAccumThreads;
+          rbuf[u] = s < end ? a.sorted[s] : make_float4(1.f, 1.f, 1.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (s0 + u * kAccumThreads >= end) break;
+        const float4 r = rbuf[u];
         if constexpr (TSC == 0) {
             int ix0, ix1, iy0, iy1, iz0, iz1;
             float wx0, wx1, wy0, wy1, wz0, wz1;
@@ -264,6 +276,7 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
                 }
             }
         }
+      }
     }
     __syncthreads();
     float *rho = a.rho + (int64_t)cat * a.rstride;
@@ -274,6 +287,151 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
         rho[g] = accumulate ? rho[g] + tile[c] : tile[c];
     }
 #undef TI
+}
+
+// ------------------------------------------------ blocked counting sort
+//
+// Binning without global atomics.  Particles are cut into blocks of
+// kSortBlock; block b builds an LDS histogram of the tiles its particles
+// touch and stores it as row H[b][*].  A column scan turns H into per-block
+// offsets inside each tile, a scan over tile totals gives the tile starts,
+// and the place pass ranks each particle within its block with LDS atomics.
+
+constexpr int kSortThreads = 1024;
+constexpr int kSortBlock = 32768;          // particles per block
+constexpr int kSortMaxTiles = 16384;        // LDS histogram capacity (64 KB)
+
+__global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint32_t *H,
+                                                            const int *blk_cat,
+                                                            const int64_t *blk_beg) {
+    __shared__ uint32_t hist[kSortMaxTiles];
+    const int blk = blockIdx.x;
+    const int cat = blk_cat ? blk_cat[blk] : 0;
+    const int64_t beg = blk_beg ? blk_beg[blk] : (int64_t)blk * kSortBlock;
+    const int64_t end = blk_beg ? blk_beg[blk + 1] : min(beg + (int64_t)kSortBlock, a.ntotal);
+    const int T = a.ntx * a.nty * a.ntz;
+    for (int i = threadIdx.x; i < T; i += kSortThreads) hist[i] = 0u;
+    __syncthreads();
+    for (int64_t i = beg + threadIdx.x; i < end; i += kSortThreads) {
+        PartU p = load_particle(a, i, a.wrap != 0, a.wrap && a.mutate);
+        bool ok = true;
+        int tx[3], ty[3], tz[3];
+        const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
+        const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
+        const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
+        if (!ok) {
+            atomicOr(a.err, 1);
+            continue;
+        }
+        for (int k = 0; k < cz; k++)
+            for (int j = 0; j < cy; j++)
+                for (int l = 0; l < cx; l++) atomicAdd(&hist[tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
+    }
+    (void)cat;
+    __syncthreads();
+    uint32_t *row = H + (int64_t)blk * T;
+    for (int i = threadIdx.x; i < T; i += kSortThreads) row[i] = hist[i];
+}
+
+// Column scan: for every (catalog, tile) an exclusive prefix over the blocks
+// of that catalog; totals go to count[].
+__global__ void k_sort_colscan(uint32_t *H, uint32_t *count, const int *cat_blk0, int ncat, int T,
+                               int nblk) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)ncat * T) return;
+    const int cat = (int)(g / T), t = (int)(g % T);
+    const int b0 = cat_blk0 ? cat_blk0[cat] : 0, b1 = cat_blk0 ? cat_blk0[cat + 1] : nblk;
+    uint32_t run = 0;
+    for (int b = b0; b < b1; b++) {
+        uint32_t *h = H + (int64_t)b * T + t;
+        const uint32_t c = *h;
+        *h = run;
+        run += c;
+    }
+    count[g] = run;
+}
+
+__global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, const uint32_t *H,
+                                                             const int *blk_cat,
+                                                             const int64_t *blk_beg) {
+    __shared__ uint32_t cur[kSortMaxTiles];
+    const int blk = blockIdx.x;
+    const int cat = blk_cat ? blk_cat[blk] : 0;
+    const int64_t beg = blk_beg ? blk_beg[blk] : (int64_t)blk * kSortBlock;
+    const int64_t end = blk_beg ? blk_beg[blk + 1] : min(beg + (int64_t)kSortBlock, a.ntotal);
+    const int T = a.ntx * a.nty * a.ntz;
+    const uint32_t *row = H + (int64_t)blk * T;
+    const uint32_t *start = a.cursor + (int64_t)cat * T;
+    for (int i = threadIdx.x; i < T; i += kSortThreads) cur[i] = start[i] + row[i];
+    __syncthreads();
+    for (int64_t i = beg + threadIdx.x; i < end; i += kSortThreads) {
+        PartU p = load_particle(a, i, a.wrap && !a.mutate, false);
+        bool ok = true;
+        int tx[3], ty[3], tz[3];
+        const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
+        const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
+        const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
+        if (!ok) continue;
+        const float4 rec = make_float4(p.u[0], p.u[1], p.u[2], p.w);
+        for (int k = 0; k < cz; k++)
+            for (int j = 0; j < cy; j++)
+                for (int l = 0; l < cx; l++) {
+                    const uint32_t slot = atomicAdd(&cur[tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
+                    a.sorted[slot] = rec;
+                }
+    }
+}
+
+// cursor[i] = start of tile i, then advanced to its end (what k_scatter_accum reads)
+__global__ void k_add_count(uint32_t *cursor, const uint32_t *count, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cursor[i] += count[i];
+}
+
+int sort_nblk(int64_t n) { return (int)((n + kSortBlock - 1) / kSortBlock); }
+
+int sort_blocks(int64_t n, const int64_t *offsets_host, int ncat, std::vector<int> &blk_cat,
+                std::vector<int64_t> &blk_beg, std::vector<int> &cat_blk0) {
+    blk_cat.clear();
+    blk_beg.clear();
+    cat_blk0.assign(ncat + 1, 0);
+    for (int c = 0; c < ncat; c++) {
+        const int64_t b0 = offsets_host ? offsets_host[c] : 0;
+        const int64_t b1 = offsets_host ? offsets_host[c + 1] : n;
+        cat_blk0[c] = (int)blk_cat.size();
+        for (int64_t s = b0; s < b1; s += kSortBlock) {
+            blk_cat.push_back(c);
+            blk_beg.push_back(s);
+        }
+    }
+    cat_blk0[ncat] = (int)blk_cat.size();
+    blk_beg.push_back(offsets_host ? offsets_host[ncat] : n);
+    return (int)blk_cat.size();
+}
+
+bool sort_fits(const ScatterArgs &a) { return a.ntx * a.nty * a.ntz <= kSortMaxTiles; }
+
+hipError_t launch_sort(ScatterArgs &a, int nblk, uint32_t *H, const int *blk_cat, const int64_t *blk_beg,
+                       const int *cat_blk0, hipStream_t s) {
+    const int T = a.ntx * a.nty * a.ntz;
+    const int64_t ntiles = (int64_t)T * a.ncat;
+    hipError_t e;
+    if (nblk > 0) {
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, a, H, blk_cat, blk_beg);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_sort_colscan, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, s, H, a.count,
+                       cat_blk0, a.ncat, T, nblk);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.count, a.cursor, ntiles);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (nblk > 0) {
+        hipLaunchKernelGGL(k_sort_place, dim3(nblk), dim3(kSortThreads), 0, s, a, H, blk_cat, blk_beg);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_add_count, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, s, a.cursor, a.count,
+                       ntiles);
+    return hipGetLastError();
 }
 
 int64_t scatter_capacity(int64_t n, int tsc) {
