@@ -294,9 +294,13 @@ void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, f
         ProfScope ps(c, "scatter_bin", (double)n * (a.mutate && wrap ? 40.0 : 28.0) + 16.0 * n * 1.2);
         hipchk(c, launch_scatter(a, nullptr, c->stream), "scatter");
     }
+    if (!accumulate) {
+        ProfScope ps(c, "zero", 4.0 * G.R());
+        hipchk(c, launch_zero(rho, G.R(), c->stream), "zero grid");
+    }
     {
         ProfScope ps(c, "scatter_accum", 16.0 * n * 1.2 + 4.0 * G.R() * (accumulate ? 2 : 1));
-        hipchk(c, launch_scatter_accum(a, accumulate, c->stream), "scatter accumulate");
+        hipchk(c, launch_scatter_accum(a, accumulate ? 1 : 2, c->stream), "scatter accumulate");
     }
 }
 

@@ -128,7 +128,10 @@ struct ScatterArgs {
 };
 
 hipError_t launch_scatter(ScatterArgs &a, uint32_t *scan_tmp, hipStream_t s);
+// accumulate: 0 overwrite, 1 add into rho, 2 rho already zero (empty tiles skipped)
 hipError_t launch_scatter_accum(const ScatterArgs &a, int accumulate, hipStream_t s);
+// Zero n floats with a fill kernel (graph-capture safe).
+hipError_t launch_zero(float *a, int64_t n, hipStream_t s);
 // Blocked counting sort (no global atomics); valid when sort_fits(a).
 bool sort_fits(const ScatterArgs &a);
 int sort_nblk(int64_t n);  // blocks of a single catalog (tables may then be NULL)

@@ -204,9 +204,11 @@ __global__ void __launch_bounds__(1024) k_scan(const uint32_t *count, uint32_t *
 constexpr int kAccumThreads = 256;
 constexpr int kTileMax = 32 * 16 * 16;
 
+// accumulate: 0 overwrite every tile, 1 add into rho, 2 rho is zero already
+// (empty tiles are skipped).
 template <int TSC>
 __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, int accumulate) {
-    __shared__ float tile[kTileMax];
+    __shared__ __align__(16) float tile[kTileMax];
     const int64_t tid_global = blockIdx.x;  // (catalog, tile)
     const int64_t ntiles = (int64_t)a.ntx * a.nty * a.ntz;
     const int cat = (int)(tid_global / ntiles);
@@ -214,10 +216,12 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
     const int tix = (int)(tt % a.ntx), tiy = (int)((tt / a.ntx) % a.nty), tiz = (int)(tt / ((int64_t)a.ntx * a.nty));
     const int x0 = tix * a.tx, y0 = tiy * a.ty, z0 = tiz * a.tz;
     const int tcells = a.tx * a.ty * a.tz;
-    for (int c = threadIdx.x; c < tcells; c += kAccumThreads) tile[c] = 0.f;
-    __syncthreads();
     const uint32_t end = a.cursor[tid_global];
     const uint32_t beg = end - a.count[tid_global];
+    // accumulate == 2: the grid was zero-filled beforehand, empty tiles stay
+    if (accumulate == 2 && beg == end) return;
+    for (int c = threadIdx.x; c < tcells; c += kAccumThreads) tile[c] = 0.f;
+    __syncthreads();
     constexpr int U = 4;  // placements in flight per thread
     for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += U * kAccumThreads) {
       float4 rbuf[U];
@@ -280,11 +284,22 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
     }
     __syncthreads();
     float *rho = a.rho + (int64_t)cat * a.rstride;
+    if (a.tx == 32 && a.nx % 32 == 0 && accumulate != 1 && (reinterpret_cast<uintptr_t>(a.rho) & 15) == 0) {
+        // full-width rows: 8 float4 per tile row, no edge masking in x
+        for (int c4 = threadIdx.x; c4 < tcells / 4; c4 += kAccumThreads) {
+            const int c = 4 * c4;
+            const int lx = c % 32, ly = (c / 32) % a.ty, lz = c / (32 * a.ty);
+            if (y0 + ly >= a.ny || z0 + lz >= a.nz) continue;
+            const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
+            *reinterpret_cast<float4 *>(rho + g) = *reinterpret_cast<const float4 *>(&tile[c]);
+        }
+        return;
+    }
     for (int c = threadIdx.x; c < tcells; c += kAccumThreads) {
         const int lx = c % a.tx, ly = (c / a.tx) % a.ty, lz = c / (a.tx * a.ty);
         if (x0 + lx >= a.nx || y0 + ly >= a.ny || z0 + lz >= a.nz) continue;  // partial edge tile
         const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
-        rho[g] = accumulate ? rho[g] + tile[c] : tile[c];
+        rho[g] = accumulate == 1 ? rho[g] + tile[c] : tile[c];
     }
 #undef TI
 }
@@ -334,21 +349,44 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
 }
 
 // Column scan: for every (catalog, tile) an exclusive prefix over the blocks
-// of that catalog; totals go to count[].
-__global__ void k_sort_colscan(uint32_t *H, uint32_t *count, const int *cat_blk0, int ncat, int T,
-                               int nblk) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (int64_t)ncat * T) return;
-    const int cat = (int)(g / T), t = (int)(g % T);
+// of that catalog; totals go to count[].  A workgroup takes 64 columns and
+// splits the block range into 16 groups (partial sums in LDS).
+constexpr int kScanCols = 64, kScanGroups = 16;
+
+__global__ void __launch_bounds__(kScanCols * kScanGroups) k_sort_colscan(uint32_t *H, uint32_t *count,
+                                                                        const int *cat_blk0, int ncat, int T,
+                                                                        int nblk) {
+    __shared__ uint32_t part[kScanGroups][kScanCols];
+    const int col = threadIdx.x % kScanCols, grp = threadIdx.x / kScanCols;
+    const int64_t g = (int64_t)blockIdx.x * kScanCols + col;
+    const bool ok = g < (int64_t)ncat * T;
+    const int cat = ok ? (int)(g / T) : 0, t = ok ? (int)(g % T) : 0;
     const int b0 = cat_blk0 ? cat_blk0[cat] : 0, b1 = cat_blk0 ? cat_blk0[cat + 1] : nblk;
-    uint32_t run = 0;
-    for (int b = b0; b < b1; b++) {
+    const int per = (b1 - b0 + kScanGroups - 1) / kScanGroups;
+    const int lo = b0 + grp * per, hi = min(b1, lo + per);
+    uint32_t s = 0;
+    if (ok)
+        for (int b = lo; b < hi; b++) s += H[(int64_t)b * T + t];
+    part[grp][col] = s;
+    __syncthreads();
+    if (grp == 0) {
+        uint32_t run = 0;
+        for (int q = 0; q < kScanGroups; q++) {
+            const uint32_t v = part[q][col];
+            part[q][col] = run;
+            run += v;
+        }
+        if (ok) count[g] = run;
+    }
+    __syncthreads();
+    if (!ok) return;
+    uint32_t run = part[grp][col];
+    for (int b = lo; b < hi; b++) {
         uint32_t *h = H + (int64_t)b * T + t;
         const uint32_t c = *h;
         *h = run;
         run += c;
     }
-    count[g] = run;
 }
 
 __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, const uint32_t *H,
@@ -420,8 +458,8 @@ hipError_t launch_sort(ScatterArgs &a, int nblk, uint32_t *H, const int *blk_cat
         hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, a, H, blk_cat, blk_beg);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_sort_colscan, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, s, H, a.count,
-                       cat_blk0, a.ncat, T, nblk);
+    hipLaunchKernelGGL(k_sort_colscan, dim3((unsigned)((ntiles + kScanCols - 1) / kScanCols)),
+                       dim3(kScanCols * kScanGroups), 0, s, H, a.count, cat_blk0, a.ncat, T, nblk);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.count, a.cursor, ntiles);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -431,6 +469,21 @@ hipError_t launch_sort(ScatterArgs &a, int nblk, uint32_t *H, const int *blk_cat
     }
     hipLaunchKernelGGL(k_add_count, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, s, a.cursor, a.count,
                        ntiles);
+    return hipGetLastError();
+}
+
+__global__ void k_zero4(float4 *a, int64_t n4) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+hipError_t launch_zero(float *a, int64_t n, hipStream_t s) {
+    if ((reinterpret_cast<uintptr_t>(a) & 15) || (n & 3)) return hipMemsetAsync(a, 0, sizeof(float) * n, s);
+    const int64_t n4 = n / 4;
+    int64_t nb = (n4 + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_zero4, dim3((unsigned)nb), dim3(256), 0, s, reinterpret_cast<float4 *>(a), n4);
     return hipGetLastError();
 }
 
@@ -613,10 +666,12 @@ __global__ void k_box_init(Slot *slot) {
     else if (threadIdx.x < 6) mm[threadIdx.x] = 0u;
 }
 
-__global__ void k_box_reduce(const float *x, const float *y, const float *z, int64_t n, Slot *slot) {
+__global__ void __launch_bounds__(256) k_box_reduce(const float *x, const float *y, const float *z,
+                                                    int64_t n, Slot *slot) {
+    __shared__ uint32_t red[4][6];
     uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t k[3] = {fkey(x[i]), fkey(y[i]), fkey(z[i])};
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -631,13 +686,22 @@ __global__ void k_box_reduce(const float *x, const float *y, const float *z, int
             mx[a] = max(mx[a], (uint32_t)__shfl_xor((int)mx[a], o, 64));
         }
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        uint32_t *mm = reinterpret_cast<uint32_t *>(slot->minmax);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            atomicMin(&mm[a], mn[a]);
-            atomicMax(&mm[3 + a], mx[a]);
+            red[w][a] = mn[a];
+            red[w][3 + a] = mx[a];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        uint32_t v = red[0][a];
+        for (int q = 1; q < 4; q++) v = a < 3 ? min(v, red[q][a]) : max(v, red[q][a]);
+        uint32_t *mm = reinterpret_cast<uint32_t *>(slot->minmax);
+        if (a < 3) atomicMin(&mm[a], v);
+        else atomicMax(&mm[a], v);
     }
 }
 
@@ -662,8 +726,8 @@ __global__ void k_box_final(Slot *slot, float pad) {
 hipError_t launch_setup_box(const float *x, const float *y, const float *z, int64_t n, Slot *slot,
                             float pad, hipStream_t s) {
     hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, s, slot);
-    int64_t nb = (n + 1023) / 1024;
-    if (nb > 65536) nb = 65536;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 1024) nb = 1024;
     if (nb < 1) nb = 1;
     hipLaunchKernelGGL(k_box_reduce, dim3((unsigned)nb), dim3(256), 0, s, x, y, z, n, slot);
     hipLaunchKernelGGL(k_box_final, dim3(1), dim3(64), 0, s, slot, pad);
