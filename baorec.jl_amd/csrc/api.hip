@@ -283,6 +283,8 @@ void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, f
     a.err = &slot->err;
     a.rho = rho;
     a.rstride = G.R();
+    a.wmax = bufT<uint32_t>(c, "sc_wmax", 64);
+    hipchk(c, hipMemsetAsync(a.wmax, 0, sizeof(uint32_t) * 64, c->stream), "memset");
     (void)tag;
     if (sort_fits(a)) {
         const int nblk = sort_nblk(n);

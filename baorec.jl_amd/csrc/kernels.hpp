@@ -125,6 +125,7 @@ struct ScatterArgs {
     int32_t *err;               // set to 1 on an out-of-grid index
     float *rho;                 // output grids, catalog c at c*rstride
     int64_t rstride;
+    uint32_t *wmax;             // per catalog: max |w| as float bits (fixed-point scale)
 };
 
 hipError_t launch_scatter(ScatterArgs &a, uint32_t *scan_tmp, hipStream_t s);

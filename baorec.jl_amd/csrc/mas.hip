@@ -149,6 +149,7 @@ __global__ void k_scatter_count(ScatterArgs a) {
         atomicOr(a.err, 1);
         return;
     }
+    if (p.w != 0.f) atomicMax(&a.wmax[p.cat], __float_as_uint(fabsf(p.w)));
     const int64_t tbase = (int64_t)p.cat * a.ntx * a.nty * a.ntz;
     for (int k = 0; k < cz; k++)
         for (int j = 0; j < cy; j++)
@@ -201,14 +202,25 @@ __global__ void __launch_bounds__(1024) k_scan(const uint32_t *count, uint32_t *
     }
 }
 
-constexpr int kAccumThreads = 256;
+constexpr int kAccumThreads = 512;
 constexpr int kTileMax = 32 * 16 * 16;
 
+// Fixed-point contribution: c * 2^e rounded to an integer (exact scaling).
+__device__ __forceinline__ unsigned long long fixq(float c, double S) {
+    return (unsigned long long)(long long)rint((double)c * S);
+}
+
+// One workgroup per (catalog, tile).  Contributions (the reference's FP32
+// weight products) are summed exactly in 64-bit fixed point with integer LDS
+// atomics (float LDS atomics are several times slower on gfx950, and integer
+// sums do not depend on the order); the scale 2^e is a per-tile power of two
+// chosen from the tile's placement count and the catalog's max |w| so no cell
+// can overflow.  Each cell is written once as the FP32 rounding of its sum.
 // accumulate: 0 overwrite every tile, 1 add into rho, 2 rho is zero already
 // (empty tiles are skipped).
 template <int TSC>
 __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, int accumulate) {
-    __shared__ __align__(16) float tile[kTileMax];
+    __shared__ unsigned long long tile[kTileMax];
     const int64_t tid_global = blockIdx.x;  // (catalog, tile)
     const int64_t ntiles = (int64_t)a.ntx * a.nty * a.ntz;
     const int cat = (int)(tid_global / ntiles);
@@ -217,91 +229,84 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
     const int x0 = tix * a.tx, y0 = tiy * a.ty, z0 = tiz * a.tz;
     const int tcells = a.tx * a.ty * a.tz;
     const uint32_t end = a.cursor[tid_global];
-    const uint32_t beg = end - a.count[tid_global];
-    // accumulate == 2: the grid was zero-filled beforehand, empty tiles stay
-    if (accumulate == 2 && beg == end) return;
-    for (int c = threadIdx.x; c < tcells; c += kAccumThreads) tile[c] = 0.f;
+    const uint32_t cnt = a.count[tid_global];
+    const uint32_t beg = end - cnt;
+    if (accumulate == 2 && cnt == 0) return;
+    const float bound = (float)cnt * __uint_as_float(a.wmax[cat]);
+    const int e = bound > 0.f ? 61 - ilogbf(bound) : 0;
+    const double S = ldexp(1.0, e);
+    for (int c = threadIdx.x; c < tcells; c += kAccumThreads) tile[c] = 0ull;
     __syncthreads();
     constexpr int U = 4;  // placements in flight per thread
     for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += U * kAccumThreads) {
-      float4 rbuf[U];
+        float4 rbuf[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-          const uint32_t s = s0 + u * kAccumThreads;
-          rbuf[u] = s < end ? a.sorted[s] : make_float4(1.f, 1.f, 1.f, 0.f);
-      }
+        for (int u = 0; u < U; u++) {
+            const uint32_t s = s0 + u * kAccumThreads;
+            rbuf[u] = s < end ? a.sorted[s] : make_float4(1.f, 1.f, 1.f, 0.f);
+        }
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (s0 + u * kAccumThreads >= end) break;
-        const float4 r = rbuf[u];
-        if constexpr (TSC == 0) {
-            int ix0, ix1, iy0, iy1, iz0, iz1;
-            float wx0, wx1, wy0, wy1, wz0, wz1;
-            cic_axis(r.x, a.nx, a.wrap, ix0, ix1, wx0, wx1);
-            cic_axis(r.y, a.ny, a.wrap, iy0, iy1, wy0, wy1);
-            cic_axis(r.z, a.nz, a.wrap, iz0, iz1, wz0, wz1);
-            wx0 *= r.w;  // src/mas.jl:86-87
-            wx1 *= r.w;
-            const int lx0 = ix0 - x0, lx1 = ix1 - x0, ly0 = iy0 - y0, ly1 = iy1 - y0;
-            const int lz0 = iz0 - z0, lz1 = iz1 - z0;
-            const bool bx0 = (unsigned)lx0 < (unsigned)a.tx, bx1 = (unsigned)lx1 < (unsigned)a.tx;
-            const bool by0 = (unsigned)ly0 < (unsigned)a.ty, by1 = (unsigned)ly1 < (unsigned)a.ty;
-            const bool bz0 = (unsigned)lz0 < (unsigned)a.tz, bz1 = (unsigned)lz1 < (unsigned)a.tz;
+        for (int u = 0; u < U; u++) {
+            if (s0 + u * kAccumThreads >= end) break;
+            const float4 r = rbuf[u];
 #define TI(lx, ly, lz) ((lx) + a.tx * ((ly) + a.ty * (lz)))
-            // corner order and products of src/mas.jl:90-97
-            if (bx0 && by0 && bz0) atomicAdd(&tile[TI(lx0, ly0, lz0)], wx0 * wy0 * wz0);
-            if (bx1 && by0 && bz0) atomicAdd(&tile[TI(lx1, ly0, lz0)], wx1 * wy0 * wz0);
-            if (bx0 && by1 && bz0) atomicAdd(&tile[TI(lx0, ly1, lz0)], wx0 * wy1 * wz0);
-            if (bx0 && by0 && bz1) atomicAdd(&tile[TI(lx0, ly0, lz1)], wx0 * wy0 * wz1);
-            if (bx1 && by1 && bz0) atomicAdd(&tile[TI(lx1, ly1, lz0)], wx1 * wy1 * wz0);
-            if (bx1 && by0 && bz1) atomicAdd(&tile[TI(lx1, ly0, lz1)], wx1 * wy0 * wz1);
-            if (bx0 && by1 && bz1) atomicAdd(&tile[TI(lx0, ly1, lz1)], wx0 * wy1 * wz1);
-            if (bx1 && by1 && bz1) atomicAdd(&tile[TI(lx1, ly1, lz1)], wx1 * wy1 * wz1);
-        } else {
-            int ix[3], iy[3], iz[3];
-            float wx[3], wy[3], wz[3];
-            tsc_axis(r.x, a.nx, a.wrap, ix, wx);
-            tsc_axis(r.y, a.ny, a.wrap, iy, wy);
-            tsc_axis(r.z, a.nz, a.wrap, iz, wz);
+            if constexpr (TSC == 0) {
+                int ix0, ix1, iy0, iy1, iz0, iz1;
+                float wx0, wx1, wy0, wy1, wz0, wz1;
+                cic_axis(r.x, a.nx, a.wrap, ix0, ix1, wx0, wx1);
+                cic_axis(r.y, a.ny, a.wrap, iy0, iy1, wy0, wy1);
+                cic_axis(r.z, a.nz, a.wrap, iz0, iz1, wz0, wz1);
+                wx0 *= r.w;  // src/mas.jl:86-87
+                wx1 *= r.w;
+                const int lx0 = ix0 - x0, lx1 = ix1 - x0, ly0 = iy0 - y0, ly1 = iy1 - y0;
+                const int lz0 = iz0 - z0, lz1 = iz1 - z0;
+                const bool bx0 = (unsigned)lx0 < (unsigned)a.tx, bx1 = (unsigned)lx1 < (unsigned)a.tx;
+                const bool by0 = (unsigned)ly0 < (unsigned)a.ty, by1 = (unsigned)ly1 < (unsigned)a.ty;
+                const bool bz0 = (unsigned)lz0 < (unsigned)a.tz, bz1 = (unsigned)lz1 < (unsigned)a.tz;
+                // corner order and products of src/mas.jl:90-97
+                if (bx0 && by0 && bz0) atomicAdd(&tile[TI(lx0, ly0, lz0)], fixq(wx0 * wy0 * wz0, S));
+                if (bx1 && by0 && bz0) atomicAdd(&tile[TI(lx1, ly0, lz0)], fixq(wx1 * wy0 * wz0, S));
+                if (bx0 && by1 && bz0) atomicAdd(&tile[TI(lx0, ly1, lz0)], fixq(wx0 * wy1 * wz0, S));
+                if (bx0 && by0 && bz1) atomicAdd(&tile[TI(lx0, ly0, lz1)], fixq(wx0 * wy0 * wz1, S));
+                if (bx1 && by1 && bz0) atomicAdd(&tile[TI(lx1, ly1, lz0)], fixq(wx1 * wy1 * wz0, S));
+                if (bx1 && by0 && bz1) atomicAdd(&tile[TI(lx1, ly0, lz1)], fixq(wx1 * wy0 * wz1, S));
+                if (bx0 && by1 && bz1) atomicAdd(&tile[TI(lx0, ly1, lz1)], fixq(wx0 * wy1 * wz1, S));
+                if (bx1 && by1 && bz1) atomicAdd(&tile[TI(lx1, ly1, lz1)], fixq(wx1 * wy1 * wz1, S));
+            } else {
+                int ix[3], iy[3], iz[3];
+                float wx[3], wy[3], wz[3];
+                tsc_axis(r.x, a.nx, a.wrap, ix, wx);
+                tsc_axis(r.y, a.ny, a.wrap, iy, wy);
+                tsc_axis(r.z, a.nz, a.wrap, iz, wz);
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int lz = iz[k] - z0;
-                if ((unsigned)lz >= (unsigned)a.tz) continue;
+                for (int k = 0; k < 3; k++) {
+                    const int lz = iz[k] - z0;
+                    if ((unsigned)lz >= (unsigned)a.tz) continue;
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const int ly = iy[j] - y0;
-                    if ((unsigned)ly >= (unsigned)a.ty) continue;
+                    for (int j = 0; j < 3; j++) {
+                        const int ly = iy[j] - y0;
+                        if ((unsigned)ly >= (unsigned)a.ty) continue;
 #pragma unroll
-                    for (int l = 0; l < 3; l++) {
-                        const int lx = ix[l] - x0;
-                        if ((unsigned)lx >= (unsigned)a.tx) continue;
-                        atomicAdd(&tile[TI(lx, ly, lz)], ((r.w * wx[l]) * wy[j]) * wz[k]);
+                        for (int l = 0; l < 3; l++) {
+                            const int lx = ix[l] - x0;
+                            if ((unsigned)lx >= (unsigned)a.tx) continue;
+                            atomicAdd(&tile[TI(lx, ly, lz)], fixq(((r.w * wx[l]) * wy[j]) * wz[k], S));
+                        }
                     }
                 }
             }
+#undef TI
         }
-      }
     }
     __syncthreads();
     float *rho = a.rho + (int64_t)cat * a.rstride;
-    if (a.tx == 32 && a.nx % 32 == 0 && accumulate != 1 && (reinterpret_cast<uintptr_t>(a.rho) & 15) == 0) {
-        // full-width rows: 8 float4 per tile row, no edge masking in x
-        for (int c4 = threadIdx.x; c4 < tcells / 4; c4 += kAccumThreads) {
-            const int c = 4 * c4;
-            const int lx = c % 32, ly = (c / 32) % a.ty, lz = c / (32 * a.ty);
-            if (y0 + ly >= a.ny || z0 + lz >= a.nz) continue;
-            const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
-            *reinterpret_cast<float4 *>(rho + g) = *reinterpret_cast<const float4 *>(&tile[c]);
-        }
-        return;
-    }
     for (int c = threadIdx.x; c < tcells; c += kAccumThreads) {
         const int lx = c % a.tx, ly = (c / a.tx) % a.ty, lz = c / (a.tx * a.ty);
         if (x0 + lx >= a.nx || y0 + ly >= a.ny || z0 + lz >= a.nz) continue;  // partial edge tile
         const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
-        rho[g] = accumulate == 1 ? rho[g] + tile[c] : tile[c];
+        const float v = (float)ldexp((double)(long long)tile[c], -e);
+        rho[g] = accumulate == 1 ? rho[g] + v : v;
     }
-#undef TI
 }
 
 // ------------------------------------------------ blocked counting sort
@@ -326,6 +331,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
     const int64_t end = blk_beg ? blk_beg[blk + 1] : min(beg + (int64_t)kSortBlock, a.ntotal);
     const int T = a.ntx * a.nty * a.ntz;
     for (int i = threadIdx.x; i < T; i += kSortThreads) hist[i] = 0u;
+    float wm = 0.f;
     __syncthreads();
     for (int64_t i = beg + threadIdx.x; i < end; i += kSortThreads) {
         PartU p = load_particle(a, i, a.wrap != 0, a.wrap && a.mutate);
@@ -338,11 +344,13 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
             atomicOr(a.err, 1);
             continue;
         }
+        wm = fmaxf(wm, fabsf(p.w));
         for (int k = 0; k < cz; k++)
             for (int j = 0; j < cy; j++)
                 for (int l = 0; l < cx; l++) atomicAdd(&hist[tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
     }
-    (void)cat;
+    for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o, 64));
+    if ((threadIdx.x & 63) == 0 && wm > 0.f) atomicMax(&a.wmax[cat], __float_as_uint(wm));
     __syncthreads();
     uint32_t *row = H + (int64_t)blk * T;
     for (int i = threadIdx.x; i < T; i += kSortThreads) row[i] = hist[i];
