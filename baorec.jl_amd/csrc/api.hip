@@ -464,34 +464,39 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             }
             z.job[0].nout = 3;
             strided(c, G, z, 1, 1, "z iterate");
-            // y inverse: G0 -> H00,H10,H20 ; G1 -> H01,H11 ; G2 -> H02
-            // H slots: 0:H00 1:H10 2:H01 3:H20 4:H11 5:H02 (x pass order)
-            StridedArgs y = sargs(G, f, 1, 1, MUL_NONE);
-            y.job[0].in = Gb;
-            y.job[0].nout = 3;
-            y.job[0].out[0] = Hb + 0 * C; y.job[0].pw[0] = 0;
-            y.job[0].out[1] = Hb + 1 * C; y.job[0].pw[1] = 1;
-            y.job[0].out[2] = Hb + 3 * C; y.job[0].pw[2] = 2;
-            y.job[1].in = Gb + C;
-            y.job[1].nout = 2;
-            y.job[1].out[0] = Hb + 2 * C; y.job[1].pw[0] = 0;
-            y.job[1].out[1] = Hb + 4 * C; y.job[1].pw[1] = 1;
-            y.job[2].in = Gb + 2 * C;
-            y.job[2].nout = 1;
-            y.job[2].out[0] = Hb + 5 * C; y.job[2].pw[0] = 0;
-            strided(c, G, y, 3, 1, "y iterate");
-            XInvArgs a = xargs(G, f, slot);
-            const int kxp[6] = {2, 1, 1, 0, 0, 0};
+            // y inverse: G0,G1,G2 -> A, B, C (the six Hessian terms grouped by
+            // their power of x along an x line)
+            YIterArgs y{};
+            for (int m = 0; m < 3; m++) {
+                y.g[m] = Gb + m * C;
+                y.out[m] = Hb + m * C;
+            }
+            y.cstride = C;
+            y.ktab = f.ktab;
+            y.xtab = f.xtab;
+            y.P = G.PP();
+            y.Pv = G.P;
+            y.nx = G.nx;
+            y.ny = G.ny;
+            y.nz = G.nz;
             const int pi[6] = {0, 0, 0, 1, 1, 2}, pj[6] = {0, 1, 2, 1, 2, 2};
             for (int q = 0; q < 6; q++) {
-                a.in[q] = Hb + q * C;
-                a.kxpow[q] = kxp[q];
                 // factor::T = (1. + T(i != j)) * beta  (src/iterative.jl:23-24)
                 float fac = (float)((1.0 + (double)(float)(pi[q] != pj[q])) * (double)beta);
                 if (it == 1) fac = fac / (1.0f + beta);
-                a.fac[q] = fac;
+                y.fac[q] = fac;
             }
-            a.nin = 6;
+            {
+                ProfScope ps(c, "fft_y", 8.0 * G.C() * 6);
+                hipchk(c, launch_y_iter(y, 1, c->stream), "y iterate");
+            }
+            XInvArgs a = xargs(G, f, slot);
+            const int kxp[3] = {2, 1, 0};
+            for (int q = 0; q < 3; q++) {
+                a.in[q] = Hb + q * C;
+                a.kxpow[q] = kxp[q];
+            }
+            a.nin = 3;
             a.aux = ds;
             a.out = dr;
             x_inv(c, G, a, XE_ITER_LOCAL, 1, "x iterate");

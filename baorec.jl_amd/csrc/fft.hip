@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_r2c(XFwdArgs a) {
 
 template <int EPI>
 struct XIn {
-    static constexpr int N = EPI == XE_ITER_LOCAL ? 6 : EPI == XE_DISP ? 3 : EPI == XE_OD_RAND ? 2 : 1;
+    static constexpr int N = EPI == XE_ITER_LOCAL ? 3 : EPI == XE_DISP ? 3 : EPI == XE_OD_RAND ? 2 : 1;
 };
 
 // Loads X[k], k = t + i*M/8, of one spectrum line (times kx^pw) and, for
@@ -182,6 +182,7 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
     }
     float4 disp[8][2];
     float rr[8][2];
+    float qs[8][2];
 
     float2 cur[8], cn;
     xline_load<M>(a.in[0] + b * a.cstride + line * a.P, valid, t, cur, cn);
@@ -229,18 +230,21 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
                 acc[i].x = acc[i].x - fac * r0;
                 acc[i].y = acc[i].y - fac * r1;
             } else if constexpr (EPI == XE_ITER_LOCAL) {
+                // q = 0: a (times x^2), 1: b (times x), 2: c; summed as
+                // (x^2 a + x b) + c, then delta_s - sum/x^2 (src/iterative.jl:20-36)
                 const int m = PI::out_pos(t, i);
-                const float fac = a.fac[q];
                 const float dv[2] = {r0, r1};
-                float *ac = &acc[i].x;
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const float xx = xv[2 * m + h];
-                    // pair order of src/iterative.jl:20-21: xx, xy, xz, yy, yz, zz
-                    const float xi = q < 3 ? xx : (q < 5 ? xy : xz);
-                    const float xj = q == 0 ? xx : ((q == 1 || q == 3) ? xy : xz);
-                    const float c = ix2[i][h];
-                    ac[h] = c > 0.f ? ac[h] - (((fac * dv[h]) * xi) * xj) * c : 0.f;
+                    if (q == 0) qs[i][h] = (xx * xx) * dv[h];
+                    else if (q == 1) qs[i][h] += xx * dv[h];
+                    else {
+                        const float u = qs[i][h] + dv[h];
+                        const float c = ix2[i][h];
+                        float *ac = &acc[i].x;
+                        ac[h] = c > 0.f ? ac[h] - u * c : 0.f;
+                    }
                 }
             } else if constexpr (EPI == XE_DISP) {
                 if (q == 0) { disp[i][0].x = r0; disp[i][1].x = r1; }
@@ -409,7 +413,110 @@ __global__ void __launch_bounds__(SShape<M>::NT) k_strided(StridedArgs a) {
     }
 }
 
+// ------------------------------------------------------------ y: local-LOS iteration
+
+template <int M>
+__global__ void __launch_bounds__(SShape<M>::NT) k_y_iter(YIterArgs a) {
+    constexpr int B = SShape<M>::B;
+    using PI = Plan<M, true>;
+    __shared__ float2 lds[B * M];
+    const int tid = threadIdx.x;
+    const int col = tid % B, t = tid / B;
+    const int ncg = (a.Pv + B - 1) / B;
+    const int cg = blockIdx.x % ncg, iz = blockIdx.x / ncg;
+    const int kxi = cg * B + col;
+    const bool valid = kxi < a.Pv;
+    const int64_t b = blockIdx.z;
+    const int64_t base = b * a.cstride + (int64_t)iz * a.P * a.ny + kxi;
+    const int64_t rs = a.P;
+    const float *ky = a.ktab + b * a.ktabStride + a.Pv;
+    const float *xv = a.xtab + b * a.xtabStride;
+    const float zc = xv[a.nx + a.ny + iz];
+    LayStrided lay{col, B};
+    TwCache<M, true> tc;
+    tc.load(t, g_tw);
+    float kyv[8], yc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        kyv[i] = ky[PI::in_pos(t, i)];
+        yc[i] = xv[a.nx + PI::out_pos(t, i)];
+    }
+    float2 g0[8], g1[8], g2[8], v[8], acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int64_t o = base + PI::in_pos(t, i) * rs;
+        g0[i] = valid ? a.g[0][o] : make_float2(0.f, 0.f);
+        g1[i] = valid ? a.g[1][o] : make_float2(0.f, 0.f);
+        g2[i] = valid ? a.g[2][o] : make_float2(0.f, 0.f);
+    }
+    const float fxx = a.fac[0], fxy = a.fac[1], fxz = a.fac[2];
+    const float fyy = a.fac[3], fyz = a.fac[4], fzz = a.fac[5];
+    // A = fxx IFFT(G0)
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = g0[i];
+    fft_line_c<M, true>(v, lds, lay, t, tc);
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) a.out[0][base + PI::out_pos(t, i) * rs] = cscale(v[i], fxx);
+    }
+    // B = fxy y IFFT(ky G0) + fxz z IFFT(G1)
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = cscale(g0[i], kyv[i]);
+    fft_line_c<M, true>(v, lds, lay, t, tc);
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = cscale(v[i], fxy * yc[i]);
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = g1[i];
+    fft_line_c<M, true>(v, lds, lay, t, tc);
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            a.out[1][base + PI::out_pos(t, i) * rs] = cadd(acc[i], cscale(v[i], fxz * zc));
+    }
+    // C = fyy y^2 IFFT(ky^2 G0) + fyz y z IFFT(ky G1) + fzz z^2 IFFT(G2)
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = cscale(g0[i], kyv[i] * kyv[i]);
+    fft_line_c<M, true>(v, lds, lay, t, tc);
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = cscale(v[i], fyy * (yc[i] * yc[i]));
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = cscale(g1[i], kyv[i]);
+    fft_line_c<M, true>(v, lds, lay, t, tc);
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = cadd(acc[i], cscale(v[i], fyz * (yc[i] * zc)));
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = g2[i];
+    fft_line_c<M, true>(v, lds, lay, t, tc);
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            a.out[2][base + PI::out_pos(t, i) * rs] = cadd(acc[i], cscale(v[i], fzz * (zc * zc)));
+    }
+}
+
 // ------------------------------------------------------------ launchers
+
+template <int M>
+static hipError_t y_iter_m(const YIterArgs &a, int nbatch, hipStream_t s) {
+    using S = SShape<M>;
+    const int ncg = (a.Pv + S::B - 1) / S::B;
+    dim3 grid((unsigned)(ncg * a.nz), 1, nbatch);
+    hipLaunchKernelGGL(k_y_iter<M>, grid, dim3(S::NT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_y_iter(const YIterArgs &a, int nbatch, hipStream_t s) {
+    switch (a.ny) {
+        case 16: return y_iter_m<16>(a, nbatch, s);
+        case 32: return y_iter_m<32>(a, nbatch, s);
+        case 64: return y_iter_m<64>(a, nbatch, s);
+        case 128: return y_iter_m<128>(a, nbatch, s);
+        case 256: return y_iter_m<256>(a, nbatch, s);
+        case 512: return y_iter_m<512>(a, nbatch, s);
+        case 1024: return y_iter_m<1024>(a, nbatch, s);
+        default: return hipErrorInvalidValue;
+    }
+}
 
 template <int M>
 static hipError_t x_r2c_m(const XFwdArgs &a, int nbatch, hipStream_t s) {

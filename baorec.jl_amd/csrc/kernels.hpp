@@ -52,7 +52,7 @@ enum XEpi : int {
     XE_PLAIN = 0,     // out = scale * c2r(in0)
     XE_OD_BOX = 1,    // delta = ((rho/mean) - 1)/bias (src/recon.jl:54-55)
     XE_OD_RAND = 2,   // data - alpha*randoms normalisation (src/recon.jl:78-85)
-    XE_ITER_LOCAL = 3,// delta_r = delta_s - sum_ij fac_ij D_ij x_i x_j / x^2
+    XE_ITER_LOCAL = 3,// delta_r = delta_s - (x^2 a + x b + c) / x^2 (inputs A, B, C)
     XE_ITER_FIXED = 4,// delta_r = delta_s - fac * D
     XE_DISP = 5,      // float4 (Psi_x, Psi_y, Psi_z, 0)
 };
@@ -89,6 +89,26 @@ struct XFwdArgs {
     double *sums;         // optional FP64 total of each input (per batch item)
     int64_t sumStride;
 };
+
+// Local-LOS iteration, y inverse pass: from G_m = IFFT_z(k_z^m F) (m = 0,1,2)
+// it forms, per (kx, y, z), the three spectra the x pass needs
+//   A = fxx IFFT_y(G0)
+//   B = fxy y IFFT_y(ky G0) + fxz z IFFT_y(G1)
+//   C = fyy y^2 IFFT_y(ky^2 G0) + fyz y z IFFT_y(ky G1) + fzz z^2 IFFT_y(G2)
+// (y, z: cell centres), so that delta_r = delta_s - (x^2 a + x b + c)/x^2
+// with a, b, c the C2R of kx^2 A, kx B and C.
+struct YIterArgs {
+    const float2 *g[3];
+    float2 *out[3];
+    int64_t cstride;
+    const float *ktab;     // kx | ky | kz (per batch item)
+    int64_t ktabStride;
+    const float *xtab;     // x | y | z cell centres (per batch item)
+    int64_t xtabStride;
+    int P, Pv, nx, ny, nz;
+    float fac[6];          // xx, xy, xz, yy, yz, zz
+};
+hipError_t launch_y_iter(const YIterArgs &a, int nbatch, hipStream_t s);
 
 hipError_t launch_x_r2c(const XFwdArgs &a, int nbatch, hipStream_t s);
 hipError_t launch_x_c2r(const XInvArgs &a, int epi, int nbatch, hipStream_t s);
