@@ -31,6 +31,8 @@ struct baorec_ctx {
     uint64_t result_gen = 0;
     // displacement grid cached for the result (Psi_x, Psi_y, Psi_z, 0)
     uint64_t disp_gen = 0;
+    // generation of the result whose R2C spectrum is still in cwork[0]
+    uint64_t spec_gen = 0;
     float disp_box[6] = {0, 0, 0, 0, 0, 0};
     int disp_valid = 0;
     // optional per-launch timing (HIP events on `stream`)
@@ -314,6 +316,7 @@ struct FFTBufs {
 };
 
 FFTBufs fft_bufs(baorec_ctx *c, const Grid &G) {
+    c->spec_gen = 0;  // any new use of the work spectra invalidates the kept one
     FFTBufs f;
     f.cw = bufT<float2>(c, "cwork", (size_t)11 * G.C());
     f.ktab = bufT<float>(c, "ktab", (size_t)4 * (G.P + G.ny + G.nz));
@@ -333,9 +336,10 @@ void strided(baorec_ctx *c, const Grid &G, const StridedArgs &a, int njobs, int 
 }
 
 void x_inv(baorec_ctx *c, const Grid &G, const XInvArgs &a, int epi, int nbatch, const char *what) {
-    double out = epi == XE_DISP ? 16.0 : 4.0;
+    double out = a.out ? (epi == XE_DISP ? 16.0 : 4.0) : 0.0;
     double aux = (epi == XE_ITER_LOCAL || epi == XE_ITER_FIXED) ? 4.0 : 0.0;
-    ProfScope ps(c, "fft_x_c2r", (double)nbatch * (8.0 * G.C() * a.nin + (out + aux) * G.R()));
+    double fwd = a.fwd_out ? 8.0 * G.C() : 0.0;
+    ProfScope ps(c, "fft_x_c2r", (double)nbatch * (8.0 * G.C() * a.nin + (out + aux) * G.R() + fwd));
     hipchk(c, launch_x_c2r(a, epi, nbatch, c->stream), what);
 }
 
@@ -367,6 +371,8 @@ void x_fwd(baorec_ctx *c, const Grid &G, const float *in, float2 *out, int nbatc
     a.nz = G.nz;
     a.sums = sums;
     a.sumStride = sumStride;
+    a.partials = sums ? bufT<double>(c, "sum_partials", (size_t)nbatch * x_r2c_partials(G.nx, G.ny, G.nz))
+                      : nullptr;
     ProfScope ps(c, "fft_x_r2c", (double)nbatch * (4.0 * G.R() + 8.0 * G.C()));
     hipchk(c, launch_x_r2c(a, nbatch, c->stream), "x r2c");
 }
@@ -417,7 +423,7 @@ void smooth_to_half(baorec_ctx *c, const Grid &G, FFTBufs &f, const float *rin, 
 // setup_overdensity! (src/recon.jl:42-91) -> delta (device).
 void overdensity(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, float *delta,
                  float *x, float *y, float *z, const float *w, int64_t n, float *rx, float *ry,
-                 float *rz, const float *rw, int64_t nr) {
+                 float *rz, const float *rw, int64_t nr, bool fwd_spectrum = false) {
     FFTBufs f = fft_bufs(c, G);
     tables(c, G, slot, f);
     float *rho = bufT<float>(c, "rho2", (size_t)2 * G.R());
@@ -432,6 +438,7 @@ void overdensity(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
     a.bias = p->bias;
     a.ran_min = p->ran_min;
     a.out = delta;
+    if (fwd_spectrum) a.fwd_out = f.cw;  // line-local in place: each line is read before it is written
     if (rand) {
         a.in[1] = f.cw + G.C();
         a.kxpow[1] = 0;
@@ -441,8 +448,10 @@ void overdensity(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
 }
 
 // n_iter x iterate! (src/iterative.jl:151-211): delta_s -> delta_r
+// c0_ready: cwork[0] already holds R2C(delta_s); keep_spectrum: leave
+// R2C(delta_r) in cwork[0] at the end (for the displacement read-out).
 void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const float *ds,
-                 float *dr) {
+                 float *dr, bool c0_ready = false, bool keep_spectrum = false) {
     FFTBufs f = fft_bufs(c, G);
     const int64_t C = G.C();
     float2 *C0 = f.cw, *Gb = f.cw + 2 * C, *Hb = f.cw + 5 * C;
@@ -453,7 +462,9 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
     }
     const float beta = p->beta;
     for (int it = 1; it <= p->n_iter; it++) {
-        x_fwd(c, G, it == 1 ? ds : dr, C0, 1, nullptr, 0);
+        const bool last = it == p->n_iter;
+        // later iterations get R2C(delta_r) from the previous x pass (fwd_out)
+        if (it == 1 && !c0_ready) x_fwd(c, G, ds, C0, 1, nullptr, 0);
         strided_plain(c, G, f, 1, false, C0, 1);
         if (!p->has_los) {
             StridedArgs z = sargs(G, f, 2, 2, MUL_INVK2);
@@ -498,7 +509,8 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             }
             a.nin = 3;
             a.aux = ds;
-            a.out = dr;
+            a.out = last ? dr : nullptr;
+            if (!last || keep_spectrum) a.fwd_out = C0;
             x_inv(c, G, a, XE_ITER_LOCAL, 1, "x iterate");
         } else {
             StridedArgs z = sargs(G, f, 2, 2, MUL_FIXEDLOS);
@@ -519,7 +531,8 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             if (it == 1) fac = fac / (1.0f + beta);
             a.fac[0] = fac;
             a.aux = ds;
-            a.out = dr;
+            a.out = last ? dr : nullptr;
+            if (!last || keep_spectrum) a.fwd_out = C0;
             x_inv(c, G, a, XE_ITER_FIXED, 1, "x iterate fixed");
         }
     }
@@ -527,12 +540,12 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
 
 // compute_displacements grids: float4 (Psi_x, Psi_y, Psi_z, 0)
 void displacement_grid(baorec_ctx *c, const Grid &G, int kind, Slot *slot, const float *field,
-                       float4 *disp) {
+                       float4 *disp, bool c0_ready = false) {
     FFTBufs f = fft_bufs(c, G);
     tables(c, G, slot, f);
     const int64_t C = G.C();
     float2 *C0 = f.cw, *Gb = f.cw + 2 * C, *Hb = f.cw + 5 * C;
-    x_fwd(c, G, field, C0, 1, nullptr, 0);
+    if (!c0_ready) x_fwd(c, G, field, C0, 1, nullptr, 0);
     strided_plain(c, G, f, 1, false, C0, 1);
     StridedArgs z = sargs(G, f, 2, 2, kind == BAOREC_MULTIGRID ? MUL_I : MUL_I_INVK2);
     z.job[0].in = C0;
@@ -841,7 +854,8 @@ void mutate_back(baorec_ctx *c, int mem, const baorec_params *p, int wrap, float
 // run! pieces shared by both recon kinds: box + overdensity into delta
 void run_common(baorec_ctx *c, baorec_params *p, const Grid &G, int mem, float *x, float *y, float *z,
                 const float *w, int64_t n, const float *rx, const float *ry, const float *rz,
-                const float *rw, int64_t nr, float *delta, bool do_setup_box, Inputs &in, Slot *slot) {
+                const float *rw, int64_t nr, float *delta, bool do_setup_box, Inputs &in, Slot *slot,
+                bool fwd_spectrum = false) {
     if (n < 0 || nr < 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "negative particle count");
     in = stage_catalogs(c, mem, x, y, z, w, n, rx, ry, rz, rw, nr);
     if (nr > 0 && do_setup_box) {
@@ -857,7 +871,8 @@ void run_common(baorec_ctx *c, baorec_params *p, const Grid &G, int mem, float *
         check_box(c, p);
         slot_from_params(c, slot, p, (double)nr);
     }
-    overdensity(c, G, p, slot, delta, in.x, in.y, in.z, in.w, n, in.rx, in.ry, in.rz, in.rw, nr);
+    overdensity(c, G, p, slot, delta, in.x, in.y, in.z, in.w, n, in.rx, in.ry, in.rz, in.rw, nr,
+                fwd_spectrum);
 }
 
 template <class F>
@@ -985,9 +1000,11 @@ static int run_impl(baorec_ctx *c, int kind, baorec_params *p, const int32_t gri
         Inputs in;
         c->has_result = 0;
         c->disp_valid = 0;
-        run_common(c, p, G, mem, x, y, z, w, n, rx, ry, rz, rw, nr, ds, true, in, slot);
-        if (kind == BAOREC_ITERATIVE) {
-            iterate_all(c, G, p, slot, ds, result);
+        const bool iter = kind == BAOREC_ITERATIVE;
+        run_common(c, p, G, mem, x, y, z, w, n, rx, ry, rz, rw, nr, ds, true, in, slot,
+                   iter && p->n_iter > 0);
+        if (iter) {
+            iterate_all(c, G, p, slot, ds, result, p->n_iter > 0, true);
         } else {
             multigrid_solve(c, G, p, slot, result, ds);
         }
@@ -998,6 +1015,8 @@ static int run_impl(baorec_ctx *c, int kind, baorec_params *p, const int32_t gri
         c->result_kind = kind;
         for (int a = 0; a < 3; a++) c->result_grid[a] = grid[a];
         c->result_gen++;
+        // the spectrum of the iterative result is left in cwork[0]
+        if (iter && p->n_iter > 0) c->spec_gen = c->result_gen;
     });
 }
 
@@ -1115,7 +1134,8 @@ static int shifts_impl(baorec_ctx *c, const baorec_params *p, int mem, const flo
                                   p->box_min[0], p->box_min[1], p->box_min[2]};
             if (!c->disp_valid || c->disp_gen != c->result_gen || memcmp(key, c->disp_box, sizeof key)) {
                 slot_from_params(c, slot, p, 0.0);
-                displacement_grid(c, G, kind, slot, bufT<float>(c, "result", G.R()), disp);
+                const bool spec = c->spec_gen == c->result_gen;  // R2C(result) kept by run!
+                displacement_grid(c, G, kind, slot, bufT<float>(c, "result", G.R()), disp, spec);
                 c->disp_valid = 1;
                 c->disp_gen = c->result_gen;
                 memcpy(c->disp_box, key, sizeof key);

@@ -17,6 +17,9 @@ constexpr int kMaxSide = 1024;
 __host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    // FFT arithmetic only: fused multiply-adds are allowed here (the
+    // reference's FFT library rounding is not part of its semantics)
+#pragma clang fp contract(fast)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __host__ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }

@@ -34,8 +34,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Workgroup total written to dst (one slot per workgroup; reduced later by
+// k_sum_partials in a fixed order, so the FP64 totals are deterministic).
 template <int NT>
-__device__ void block_sum_atomic(double v, double *dst) {
+__device__ void block_sum_store(double v, double *dst) {
     __shared__ double red[NT / 64];
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -44,8 +46,20 @@ __device__ void block_sum_atomic(double v, double *dst) {
     if (threadIdx.x == 0) {
         double s = 0;
         for (int i = 0; i < NT / 64; i++) s += red[i];
-        atomicAdd(dst, s);
+        *dst = s;
     }
+}
+
+__global__ void __launch_bounds__(256) k_sum_partials(const double *part, int64_t n, double *out,
+                                                      int64_t outStride) {
+    __shared__ double red[4];
+    const double *p = part + (int64_t)blockIdx.x * n;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += p[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x * outStride] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 // ------------------------------------------------------------ x: R2C
@@ -100,7 +114,7 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_r2c(XFwdArgs a) {
             dst[M] = make_float2(z0.x - z0.y, 0.f);
         }
     }
-    if (a.sums) block_sum_atomic<S::NT>(valid ? s : 0.0, a.sums + b * a.sumStride);
+    if (a.partials) block_sum_store<S::NT>(valid ? s : 0.0, a.partials + b * (int64_t)gridDim.x + blockIdx.x);
 }
 
 // ------------------------------------------------------------ x: C2R
@@ -258,9 +272,8 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
             cn = nn;
         }
     }
-    if (!valid) return;
-
     if constexpr (EPI == XE_DISP) {
+        if (!valid) return;
         float4 *o4 = reinterpret_cast<float4 *>(a.out) + rbase;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -296,9 +309,40 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
             }
         }
     }
-    float2 *o2 = reinterpret_cast<float2 *>(a.out + rbase);
+    if (valid && a.out) {
+        float2 *o2 = reinterpret_cast<float2 *>(a.out + rbase);
 #pragma unroll
-    for (int i = 0; i < 8; i++) o2[PI::out_pos(t, i)] = acc[i];
+        for (int i = 0; i < 8; i++) o2[PI::out_pos(t, i)] = acc[i];
+    }
+    if constexpr (EPI != XE_DISP) {
+        if (a.fwd_out) {
+            // forward R2C of the line just produced: acc is in the forward
+            // plan's input order (inverse plans end with the forward's first radix)
+            using PF = Plan<M, false>;
+            fft_line<M, false>(acc, lds, lay, t, g_tw);
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 8; i++) lds[lay(PF::out_pos(t, i))] = acc[i];
+            __syncthreads();
+            if (valid) {
+                float2 *dst = a.fwd_out + b * a.cstride + line * a.P;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const int k = t + i * TPL;
+                    float2 zk = lds[lay(k)];
+                    float2 zc = cconj(lds[lay((M - k) & (M - 1))]);
+                    float2 e = cscale(cadd(zk, zc), 0.5f);
+                    float2 d = csub(zk, zc);
+                    float2 o = make_float2(0.5f * d.y, -0.5f * d.x);
+                    dst[k] = cadd(e, cmul(g_tw[2 * M + k], o));
+                }
+                if (t == 0) {
+                    float2 z0 = lds[lay(0)];
+                    dst[M] = make_float2(z0.x - z0.y, 0.f);
+                }
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------ strided y / z
@@ -522,9 +566,18 @@ template <int M>
 static hipError_t x_r2c_m(const XFwdArgs &a, int nbatch, hipStream_t s) {
     using S = XShape<M>;
     const int64_t nlines = (int64_t)a.ny * a.nz;
-    dim3 grid((unsigned)((nlines + S::LPW - 1) / S::LPW), nbatch);
+    const unsigned nwg = (unsigned)((nlines + S::LPW - 1) / S::LPW);
+    dim3 grid(nwg, nbatch);
     hipLaunchKernelGGL(k_x_r2c<M>, grid, dim3(S::NT), 0, s, a);
+    if (a.partials)
+        hipLaunchKernelGGL(k_sum_partials, dim3(nbatch), dim3(256), 0, s, a.partials, (int64_t)nwg, a.sums,
+                           a.sumStride);
     return hipGetLastError();
+}
+
+int64_t x_r2c_partials(int nx, int ny, int nz) {
+    int M = nx / 2, TPL = M / 8, LPW = TPL >= 256 ? 1 : 256 / TPL;
+    return ((int64_t)ny * nz + LPW - 1) / LPW;
 }
 
 hipError_t launch_x_r2c(const XFwdArgs &a, int nbatch, hipStream_t s) {

@@ -78,6 +78,8 @@ struct XInvArgs {
     float bias;
     float fac[kMaxXIn];   // update factors (iterative)
     float ran_min;
+    float2 *fwd_out;      // optional: R2C of the result line written here (pitch P),
+                          // so the next transform skips reading the real grid
 };
 
 struct XFwdArgs {
@@ -88,7 +90,10 @@ struct XFwdArgs {
     int nx, ny, nz;
     double *sums;         // optional FP64 total of each input (per batch item)
     int64_t sumStride;
+    double *partials;     // workgroup partial sums (x_r2c_partials(..) per item), set with sums
 };
+// Number of workgroup partial sums per batch item of launch_x_r2c.
+int64_t x_r2c_partials(int nx, int ny, int nz);
 
 // Local-LOS iteration, y inverse pass: from G_m = IFFT_z(k_z^m F) (m = 0,1,2)
 // it forms, per (kx, y, z), the three spectra the x pass needs
