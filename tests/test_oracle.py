@@ -1,0 +1,322 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+Each oracle function is checked against something its own code does not
+restate: exact-arithmetic golden vectors (tests/golden/), closed forms
+(plane waves), invariants (mass conservation, adjointness), a brute-force DFT
+on a tiny grid, and, for the multigrid operator, its exact Fourier symbol.
+A planted-mistake test shows the checks would catch a plausible slip.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import baorec_oracle as orc
+
+F32, F64 = np.float32, np.float64
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as fh:
+        return json.load(fh)
+
+
+# ------------------------------------------------------------------ CIC
+
+@pytest.mark.parametrize("dt", [F32, F64])
+def test_cic_golden(dt):
+    for case in load("cic.json"):
+        n, L, b = case["n"], case["L"], case["b"]
+        rho = np.zeros((n[2], n[1], n[0]), dt)
+        p = [np.array([v], F32) for v in case["p"]]
+        orc.cic(rho, p[0], p[1], p[2], np.array([case["w"]], F32), L, b, wrap=case["wrap"])
+        ref = np.zeros_like(rho)
+        for i, j, k, v in case["cells"]:
+            ref[k, j, i] += v
+        np.testing.assert_array_equal(rho, ref)
+
+
+def test_cic_mass_conservation_and_wrap_mutation():
+    rng = np.random.default_rng(0)
+    n = 5000
+    x, y, z = (rng.random(n).astype(F32) * F32(100.0) for _ in range(3))
+    x[:10] = F32(100.0) + F32(7.0)  # beyond the box: wrapped once
+    w = rng.random(n).astype(F32)
+    rho = np.zeros((16, 8, 12), F64)
+    xx = x.copy()
+    orc.cic(rho, xx, y.copy(), z.copy(), w, [100.0] * 3, [0.0] * 3, wrap=True)
+    assert abs(rho.sum() - w.astype(F64).sum()) < 1e-9 * w.sum()
+    np.testing.assert_array_equal(xx[:10], np.full(10, F32(7.0)))
+    np.testing.assert_array_equal(xx[10:], x[10:])
+
+
+def test_cic_nowrap_bounds_error():
+    rho = np.zeros((8, 8, 8), F32)
+    with pytest.raises(IndexError):
+        orc.cic(rho, np.array([7.9], F32), np.array([1.0], F32), np.array([1.0], F32), np.ones(1, F32),
+                [8.0] * 3, [0.0] * 3, wrap=False)
+
+
+# ------------------------------------------------------------------ gather
+
+def test_read_cic_golden():
+    g = load("read_cic.json")
+    field = np.array(g["field"], F32)
+    for pt in g["points"]:
+        p = [np.array([v], F32) for v in pt["p"]]
+        for fml in (True, False):
+            got = orc.read_cic(field, *p, g["L"], g["b"], gpu_formula=fml)
+            assert got[0] == pytest.approx(pt["value"], abs=1e-6)
+
+
+def test_read_cic_linear_and_constant():
+    n = 16
+    L = 32.0
+    ix = np.arange(n, dtype=F64)
+    field = (2.0 * ix[None, None, :] - 1.0 * ix[None, :, None] + 0.5 * ix[:, None, None] + 3.0)
+    rng = np.random.default_rng(1)
+    # interior points (no wrap of the upper corner): linear reproduced exactly
+    p = [rng.uniform(0, L * (n - 1) / n, 200).astype(F32) for _ in range(3)]
+    got = orc.read_cic(field, *p, [L] * 3, [0.0] * 3)
+    u = [q.astype(F64) * n / L for q in p]
+    np.testing.assert_allclose(got, 2 * u[0] - u[1] + 0.5 * u[2] + 3.0, atol=1e-9)
+    np.testing.assert_allclose(orc.read_cic(np.full((n, n, n), 4.25), *p, [L] * 3, [0.0] * 3), 4.25)
+
+
+def test_scatter_gather_adjoint():
+    """<gather(F), w> == <F, scatter(w)> (same nodes, same weights)."""
+    rng = np.random.default_rng(2)
+    n, L = 12, 24.0
+    x, y, z = (rng.random(3000).astype(F32) * F32(L) for _ in range(3))
+    w = rng.random(3000).astype(F32)
+    F = rng.standard_normal((n, n, n))
+    rho = np.zeros((n, n, n), F64)
+    orc.cic(rho, x.copy(), y.copy(), z.copy(), w, [L] * 3, [0.0] * 3, wrap=True)
+    lhs = np.dot(orc.read_cic(F, x, y, z, [L] * 3, [0.0] * 3), w.astype(F64))
+    rhs = np.sum(F * rho)
+    assert lhs == pytest.approx(rhs, rel=1e-5)
+
+
+# ------------------------------------------------------------------ grid vectors
+
+def test_k_vec_and_x_vec_conventions():
+    kx, ky, kz = orc.k_vec((8, 8, 6), [1000.0, 1000.0, 600.0])
+    dk = F32(F32(2 * np.pi * 8 / 1000.0) / F32(8))
+    np.testing.assert_array_equal(kx, np.arange(5, dtype=F32) * dk)          # rfftfreq: 0..n/2
+    np.testing.assert_array_equal(ky, np.array([0, 1, 2, 3, -4, -3, -2, -1], F32) * dk)  # Nyquist negative
+    assert kz.shape == (6,)
+    xs = orc.x_vec((4, 4, 4), [8.0] * 3, [-4.0, 0.0, 1.0])
+    np.testing.assert_array_equal(xs[0], np.array([-3, -1, 1, 3], F32))
+    np.testing.assert_array_equal(xs[2], np.array([2, 4, 6, 8], F32))
+
+
+def test_setup_box():
+    x = np.array([1.0, 5.0], F32)
+    y = np.array([-2.0, 0.0], F32)
+    z = np.array([3.0, 3.5], F32)
+    bs, bm = orc.setup_box(x, y, z, 10.0)
+    np.testing.assert_array_equal(bm, np.array([-4.0, -7.0, -2.0], F32))
+    np.testing.assert_array_equal(bs, np.full(3, 14.0, F32))
+
+
+# ------------------------------------------------------------------ smoothing / FFT steps
+
+def plane_wave(n, L, m, phase=0.0):
+    x = np.arange(n) * (L / n)
+    kv = [2 * np.pi * mi / L for mi in m]
+    arg = kv[0] * x[None, None, :] + kv[1] * x[None, :, None] + kv[2] * x[:, None, None] + phase
+    return np.cos(arg), np.array(kv)
+
+
+def test_smoothing_plane_wave():
+    n, L, R = 16, 100.0, 7.0
+    f, k = plane_wave(n, L, (2, -3, 1))
+    got = orc.smooth(f.copy(), R, [L] * 3)
+    np.testing.assert_allclose(got, np.exp(-0.5 * R * R * (k @ k)) * f, atol=1e-12)
+
+
+def test_iterate_beta_zero_is_identity():
+    rng = np.random.default_rng(3)
+    ds = rng.standard_normal((8, 8, 8))
+    kv = orc.k_vec((8, 8, 8), [50.0] * 3, F64)
+    xv = orc.x_vec((8, 8, 8), [50.0] * 3, [10.0] * 3, F64)
+    for los, x in ((None, xv), ((0.0, 0.0, 1.0), None)):
+        dr = ds.copy()
+        orc.iterate(dr, ds, kv, 1, 0.0, los=los, xv=x)
+        np.testing.assert_allclose(dr, ds, atol=1e-14)
+
+
+def test_iterate_fixed_los_plane_wave_recursion():
+    """delta^(n) = delta_s - fac_n mu^2 delta^(n-1), mu = k_z/k, fac_1 = b/(1+b)."""
+    n, L, beta = 16, 200.0, 0.4
+    ds, k = plane_wave(n, L, (1, 2, 3))
+    mu2 = k[2] ** 2 / (k @ k)
+    kv = orc.k_vec((n, n, n), [L] * 3, F64)
+    dr = ds.copy()
+    expect = ds.copy()
+    for it in (1, 2, 3):
+        orc.iterate(dr, ds, kv, it, beta, los=(0.0, 0.0, 1.0))
+        fac = beta / (1 + beta) if it == 1 else beta
+        expect = ds - fac * mu2 * expect
+        np.testing.assert_allclose(dr, expect, atol=1e-12)
+
+
+def dft_matrix(n, inverse=False):
+    j = np.arange(n)
+    s = 1 if inverse else -1
+    return np.exp(s * 2j * np.pi * np.outer(j, j) / n)
+
+
+def test_iterate_local_los_brute_force():
+    """Local-LOS update against explicit DFT matrices (no FFT library).  All
+    sides are odd: with no self-paired Nyquist bins the k_i k_j spectra stay
+    Hermitian, so the full-spectrum brute force equals the reference's
+    R2C/C2R exactly (at a Nyquist bin the two conventions legitimately differ)."""
+    n = (7, 5, 9)
+    L = (70.0, 50.0, 45.0)
+    beta = 0.35
+    bmin = [5.0, -20.0, 13.0]
+    rng = np.random.default_rng(4)
+    ds = rng.standard_normal((n[2], n[1], n[0]))
+    W = [dft_matrix(m) for m in n]
+    Wi = [dft_matrix(m, True) for m in n]
+    dk = np.einsum("ai,bj,ck,kji->cba", W[0], W[1], W[2], ds)  # axes (z,y,x), full spectrum
+    m = [np.fft.fftfreq(n[a], 1.0 / n[a]) * 2 * np.pi / L[a] for a in range(3)]
+    K = [m[0][None, None, :], m[1][None, :, None], m[2][:, None, None]]
+    k2 = K[0] ** 2 + K[1] ** 2 + K[2] ** 2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        phi = np.where(k2 > 0, dk / k2, 0)
+    xv = orc.x_vec(n, L, bmin, F64)
+    X = [xv[0][None, None, :], xv[1][None, :, None], xv[2][:, None, None]]
+    x2 = X[0] ** 2 + X[1] ** 2 + X[2] ** 2
+    expect = ds.copy()
+    for i in range(3):
+        for j in range(i, 3):
+            D = np.einsum("ai,bj,ck,kji->cba", Wi[0], Wi[1], Wi[2], K[i] * K[j] * phi).real / np.prod(n)
+            fac = (2.0 if i != j else 1.0) * beta / (1 + beta)
+            expect = expect - fac * D * X[i] * X[j] / x2
+    dr = ds.copy()
+    orc.iterate(dr, ds, orc.k_vec(n, L, F64), 1, beta, los=None, xv=xv)
+    np.testing.assert_allclose(dr, expect, atol=1e-12)
+
+
+def test_displacement_plane_wave():
+    """delta = A cos(k.x)  ->  Psi = -A k sin(k.x)/k^2 (iterative: Psi_k = i k delta_k / k^2)."""
+    n, L, A = 16, 160.0, 0.7
+    d, k = plane_wave(n, L, (1, -2, 2))
+    rec = orc.IterativeRecon(bias=1.0, f=0.5, smoothing_radius=0.0, box_size=[L] * 3, box_min=[0.0] * 3)
+    grids = orc.displacement_grids(A * d, rec)
+    s, _ = plane_wave(n, L, (1, -2, 2), phase=-np.pi / 2)  # sin(k.x)
+    for i in range(3):
+        np.testing.assert_allclose(grids[i], -A * k[i] / (k @ k) * s, atol=1e-12)
+    # multigrid read-out: Psi_k = i k phi_k  ->  Psi = -A k sin(k.x) for phi = A cos(k.x)
+    mg = orc.MultigridRecon(bias=1.0, f=0.5, smoothing_radius=0.0, box_size=[L] * 3, box_min=[0.0] * 3)
+    grids = orc.displacement_grids(A * d, mg)
+    for i in range(3):
+        np.testing.assert_allclose(grids[i], -A * k[i] * s, atol=1e-12)
+
+
+def test_rsd_shift_properties():
+    n, L = 8, 80.0
+    rng = np.random.default_rng(5)
+    field = rng.standard_normal((n, n, n))
+    x, y, z = (rng.uniform(1.0, L - 1.0, 300).astype(F32) for _ in range(3))
+    base = dict(bias=2.0, smoothing_radius=10.0, box_size=[L] * 3, box_min=[0.0] * 3)
+    r0 = orc.IterativeRecon(f=0.0, **base)
+    np.testing.assert_array_equal(np.array(orc.read_shifts(r0, x, y, z, field, "sum")),
+                                  np.array(orc.read_shifts(r0, x, y, z, field, "disp")))
+    r1 = orc.IterativeRecon(f=0.7, los=(0.0, 0.0, 1.0), **base)
+    disp = orc.read_shifts(r1, x, y, z, field, "disp")
+    rsd = orc.read_shifts(r1, x, y, z, field, "rsd")
+    np.testing.assert_allclose(rsd[2], 0.7 * disp[2], rtol=1e-12)   # along LOS: f * Psi
+    np.testing.assert_array_equal(rsd[0], 0.0)                       # perpendicular: 0
+    r2 = orc.IterativeRecon(f=0.7, los=None, **base)
+    disp = orc.read_shifts(r2, x, y, z, field, "disp")
+    rsd = orc.read_shifts(r2, x, y, z, field, "rsd")
+    rhat = np.stack([x, y, z]).astype(F64)
+    rhat /= np.linalg.norm(rhat, axis=0)
+    dot = np.sum(np.stack(disp) * rhat, axis=0)
+    np.testing.assert_allclose(np.stack(rsd), 0.7 * dot * rhat, atol=1e-12)
+
+
+# ------------------------------------------------------------------ multigrid
+
+def test_restrict_prolong_golden():
+    g = load("restrict.json")
+    np.testing.assert_array_equal(orc.mg_reduce(np.array(g["fine"], F64)), np.array(g["restricted"]))
+    np.testing.assert_array_equal(orc.mg_reduce(np.array(g["fine"], F32)), np.array(g["restricted"], F32))
+    np.testing.assert_array_equal(orc.mg_prolong(np.array(g["coarse"], F64)), np.array(g["prolonged"]))
+
+
+def test_restrict_prolong_constants():
+    np.testing.assert_array_equal(orc.mg_reduce(np.full((8, 8, 8), 2.5)), np.full((4, 4, 4), 2.5))
+    np.testing.assert_array_equal(orc.mg_prolong(np.full((4, 2, 4), -1.25)), np.full((8, 4, 8), -1.25))
+
+
+def mg_symbol(n, L, beta, los):
+    """Fourier symbol of A v = D v - N - X for a fixed LOS (src/multigrid.jl:213-326)."""
+    cell = np.array(L) / np.array(n)
+    p = np.array(los, dtype=F32).astype(F64) / cell   # the oracle takes the LOS in FP32
+    g = beta / np.sum(cell ** 2 * p ** 2)
+    gp = 1 / cell ** 2 + g * p ** 2
+    th = [2 * np.pi * np.fft.fftfreq(n[a]) for a in range(3)]
+    T = [th[0][None, None, :], th[1][None, :, None], th[2][:, None, None]]
+    A = sum(gp[a] * (2 - 2 * np.cos(T[a])) for a in range(3))
+    for i, j in ((0, 1), (0, 2), (1, 2)):
+        A = A + 2 * g * p[i] * p[j] * np.sin(T[i]) * np.sin(T[j])
+    return A
+
+
+@pytest.mark.parametrize("los", [(0.0, 0.0, 1.0), (0.6, 0.0, 0.8)])
+def test_mg_residual_matches_operator_symbol(los):
+    n, L, beta = (16, 8, 12), (32.0, 16.0, 24.0), 0.4
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal((n[2], n[1], n[0]))
+    r = orc.mg_residual(v, np.zeros_like(v), L, [0.0] * 3, beta, los=los)   # r = -A v
+    Av = np.fft.ifftn(np.fft.fftn(v) * mg_symbol(n, L, beta, los)).real
+    np.testing.assert_allclose(-r, Av, atol=1e-11)
+
+
+def test_mg_vcycles_converge_to_fft_solution():
+    n, L, beta, los = (16, 16, 16), (64.0,) * 3, 0.4, (0.0, 0.0, 1.0)
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal((16, 16, 16))
+    f -= f.mean()
+    A = mg_symbol(n, L, beta, los)
+    fk = np.fft.fftn(f)
+    phik = np.where(A > 1e-14, fk / np.where(A > 1e-14, A, 1), 0)
+    phi = np.fft.ifftn(phik).real
+    v = np.zeros_like(f)
+    res = []
+    for _ in range(25):
+        orc.mg_vcycle(v, f, L, [0.0] * 3, beta, 0.4, 5, los=los)
+        res.append(np.linalg.norm(orc.mg_residual(v, f, L, [0.0] * 3, beta, los=los)))
+    assert res[1] / res[0] < 0.5 and res[5] / res[4] < 0.5           # contraction per cycle
+    np.testing.assert_allclose(v - v.mean(), phi, atol=1e-8 * np.abs(phi).max())
+
+
+def test_mg_fmg_reduces_residual_local_los():
+    n, L = 32, 500.0
+    rng = np.random.default_rng(8)
+    f = rng.standard_normal((n, n, n))
+    f -= f.mean()
+    r0 = np.linalg.norm(f)
+    v = orc.fmg(f, [L] * 3, [200.0, 300.0, 100.0], 0.375, 0.4, 5, 6, los=None)
+    r = np.linalg.norm(orc.mg_residual(v, f, [L] * 3, [200.0, 300.0, 100.0], 0.375, los=None))
+    assert r / r0 < 1e-2
+
+
+# ------------------------------------------------------------------ planted mistake
+
+def test_planted_sign_error_is_caught(monkeypatch):
+    """A sign flip in the displacement multiplier must fail the plane-wave check."""
+    real = orc.displacement_grids
+
+    def flipped(field, recon):
+        return tuple(-g for g in real(field, recon))
+
+    monkeypatch.setattr(orc, "displacement_grids", flipped)
+    with pytest.raises(AssertionError):
+        test_displacement_plane_wave()
