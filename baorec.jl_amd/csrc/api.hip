@@ -330,7 +330,7 @@ void tables(baorec_ctx *c, const Grid &G, const Slot *slot, FFTBufs &f) {
 
 void strided(baorec_ctx *c, const Grid &G, const StridedArgs &a, int njobs, int nbatch, const char *what) {
     double bytes = 0.0;
-    for (int j = 0; j < njobs; j++) bytes += 8.0 * G.C() * (1 + a.job[j].nout);
+    for (int j = 0; j < njobs; j++) bytes += 8.0 * G.Cdense() * (1 + a.job[j].nout);
     ProfScope ps(c, a.axis == 1 ? "fft_y" : "fft_z", bytes * nbatch);
     hipchk(c, launch_strided(a, njobs, nbatch, c->stream), what);
 }
@@ -338,8 +338,8 @@ void strided(baorec_ctx *c, const Grid &G, const StridedArgs &a, int njobs, int 
 void x_inv(baorec_ctx *c, const Grid &G, const XInvArgs &a, int epi, int nbatch, const char *what) {
     double out = a.out ? (epi == XE_DISP ? 16.0 : 4.0) : 0.0;
     double aux = (epi == XE_ITER_LOCAL || epi == XE_ITER_FIXED) ? 4.0 : 0.0;
-    double fwd = a.fwd_out ? 8.0 * G.C() : 0.0;
-    ProfScope ps(c, "fft_x_c2r", (double)nbatch * (8.0 * G.C() * a.nin + (out + aux) * G.R() + fwd));
+    double fwd = a.fwd_out ? 8.0 * G.Cdense() : 0.0;
+    ProfScope ps(c, "fft_x_c2r", (double)nbatch * (8.0 * G.Cdense() * a.nin + (out + aux) * G.R() + fwd));
     hipchk(c, launch_x_c2r(a, epi, nbatch, c->stream), what);
 }
 
@@ -373,7 +373,7 @@ void x_fwd(baorec_ctx *c, const Grid &G, const float *in, float2 *out, int nbatc
     a.sumStride = sumStride;
     a.partials = sums ? bufT<double>(c, "sum_partials", (size_t)nbatch * x_r2c_partials(G.nx, G.ny, G.nz))
                       : nullptr;
-    ProfScope ps(c, "fft_x_r2c", (double)nbatch * (4.0 * G.R() + 8.0 * G.C()));
+    ProfScope ps(c, "fft_x_r2c", (double)nbatch * (4.0 * G.R() + 8.0 * G.Cdense()));
     hipchk(c, launch_x_r2c(a, nbatch, c->stream), "x r2c");
 }
 
@@ -498,7 +498,7 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
                 y.fac[q] = fac;
             }
             {
-                ProfScope ps(c, "fft_y", 8.0 * G.C() * 6);
+                ProfScope ps(c, "fft_y_iter", 8.0 * G.Cdense() * 6);
                 hipchk(c, launch_y_iter(y, 1, c->stream), "y iterate");
             }
             XInvArgs a = xargs(G, f, slot);

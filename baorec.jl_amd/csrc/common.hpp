@@ -247,10 +247,15 @@ __device__ __forceinline__ void fft_line_c(float2 (&v)[8], float2 *lds, const La
 }
 
 // LDS layouts
-struct LayStrided {  // [pos][col], B columns per row
+// [pos][col], B columns per row, one pad row after every 8: the first stage
+// exchange writes rows 8 apart from consecutive threads, which would
+// otherwise land on the same LDS banks.
+struct LayStrided {
     int col, B;
-    __device__ __forceinline__ int operator()(int pos) const { return pos * B + col; }
+    __device__ __forceinline__ int operator()(int pos) const { return (pos + (pos >> 3)) * B + col; }
 };
+// LDS entries a LayStrided exchange of M positions x B columns needs
+constexpr int lay_strided_size(int M, int B) { return (M + M / 8) * B; }
 struct LayLine {     // contiguous lines with one pad slot every 8 entries
     int base;
     __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 3); }

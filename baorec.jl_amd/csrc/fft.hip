@@ -347,11 +347,22 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
 
 // ------------------------------------------------------------ strided y / z
 
+// Columns per workgroup: 8 (64-byte row segments, 512 threads at n = 512)
+// lets two workgroups share a CU (<= 128 VGPRs), so one workgroup's loads
+// overlap the other's LDS exchanges; at n = 512 that measured faster than 16
+// columns with one 1024-thread workgroup per CU.
 template <int M>
 struct SShape {
-    static constexpr int B = M >= 1024 ? 8 : 16;  // kx columns per workgroup
+    static constexpr int B = M >= 512 ? 8 : 16;  // kx columns per workgroup
     static constexpr int NT = B * M / 8;
 };
+
+// Workgroup i is dispatched to XCD i % 8: renumber so that consecutive
+// logical tiles share an XCD (and its L2).
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+    const int q = nb >> 3, r = nb & 7, x = bid & 7;
+    return x * q + (x < r ? x : r) + (bid >> 3);
+}
 
 __device__ __forceinline__ float2 apply_mul(float2 F, int mul, float kx, float ky, float kz,
                                             float R2, float l0, float l1, float l2) {
@@ -378,15 +389,17 @@ __device__ __forceinline__ float2 apply_mul(float2 F, int mul, float kx, float k
 }
 
 template <int M>
-__global__ void __launch_bounds__(SShape<M>::NT) k_strided(StridedArgs a) {
+__global__ void __launch_bounds__(SShape<M>::NT) __attribute__((amdgpu_waves_per_eu(4)))
+k_strided(StridedArgs a) {
     constexpr int B = SShape<M>::B;
     using PF = Plan<M, false>;
     using PI = Plan<M, true>;
-    __shared__ float2 lds[B * M];
+    __shared__ float2 lds[lay_strided_size(M, B)];
     const int tid = threadIdx.x;
     const int col = tid % B, t = tid / B;
     const int ncg = (a.Pv + B - 1) / B;
-    const int cg = blockIdx.x % ncg, other = blockIdx.x / ncg;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int cg = tile % ncg, other = tile / ncg;
     const int kxi = cg * B + col;
     const bool valid = kxi < a.Pv;
     const StridedJob &J = a.job[blockIdx.y];
@@ -463,11 +476,12 @@ template <int M>
 __global__ void __launch_bounds__(SShape<M>::NT) k_y_iter(YIterArgs a) {
     constexpr int B = SShape<M>::B;
     using PI = Plan<M, true>;
-    __shared__ float2 lds[B * M];
+    __shared__ float2 lds[lay_strided_size(M, B)];
     const int tid = threadIdx.x;
     const int col = tid % B, t = tid / B;
     const int ncg = (a.Pv + B - 1) / B;
-    const int cg = blockIdx.x % ncg, iz = blockIdx.x / ncg;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int cg = tile % ncg, iz = tile / ncg;
     const int kxi = cg * B + col;
     const bool valid = kxi < a.Pv;
     const int64_t b = blockIdx.z;

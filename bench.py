@@ -155,6 +155,21 @@ def cpu_baseline(args, data, rand, cfg):
                         f"read-out {t_read:.1f} s"))
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """Plain device-to-device copies of a known size: the reference point for
+    the HBM byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
+    from baorec_hip.devmem import DeviceArray, hip, D2D
+    h = hip()
+    a = DeviceArray((nbytes // 4,))
+    b = DeviceArray((nbytes // 4,))
+    for _ in range(reps):
+        rc = h.hipMemcpy(ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), nbytes, D2D)
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed: {rc}")
+    h.hipDeviceSynchronize()
+    print(json.dumps({"calibration_copy_bytes": nbytes, "copies": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,7 +179,12 @@ def main():
     ap.add_argument("--ndata", type=int, default=2_000_000)
     ap.add_argument("--nrand", type=int, default=20_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run 1 GiB device-to-device copies (byte-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
