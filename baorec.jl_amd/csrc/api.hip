@@ -339,7 +339,9 @@ void x_inv(baorec_ctx *c, const Grid &G, const XInvArgs &a, int epi, int nbatch,
     double out = a.out ? (epi == XE_DISP ? 16.0 : 4.0) : 0.0;
     double aux = (epi == XE_ITER_LOCAL || epi == XE_ITER_FIXED) ? 4.0 : 0.0;
     double fwd = a.fwd_out ? 8.0 * G.Cdense() : 0.0;
-    ProfScope ps(c, "fft_x_c2r", (double)nbatch * (8.0 * G.Cdense() * a.nin + (out + aux) * G.R() + fwd));
+    static const char *const label[] = {"fft_x_c2r", "fft_x_c2r_od", "fft_x_c2r_od", "fft_x_c2r_iter",
+                                        "fft_x_c2r_iter", "fft_x_c2r_disp"};
+    ProfScope ps(c, label[epi], (double)nbatch * (8.0 * G.Cdense() * a.nin + (out + aux) * G.R() + fwd));
     hipchk(c, launch_x_c2r(a, epi, nbatch, c->stream), what);
 }
 
