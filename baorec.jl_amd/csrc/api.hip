@@ -781,14 +781,19 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
         return;
     }
     if (p->mg_schedule != BAOREC_MG_RBGS) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mg_schedule");
-    // RBGS: zero-mean right-hand side (the periodic operator annihilates
-    // constants), V-cycles from v = 0 until ||f - A v|| <= tol ||f||.
+    // RBGS: V-cycles from v = 0.  The periodic operator annihilates
+    // constants; with a fixed LOS it is symmetric, so its range is exactly the
+    // zero-mean fields and f is demeaned (the consistent projection).  With a
+    // local LOS the first-derivative term makes it non-symmetric and the
+    // range is not orthogonal to constants: f is used as given.
     float *fz = bufT<float>(c, "mg_fz", G.R());
     double *red = bufT<double>(c, "mg_red", 8);
     hipchk(c, hipMemsetAsync(red, 0, sizeof(double) * 8, c->stream), "memset");
-    hipchk(c, launch_sum<float>(f0, G.R(), red, c->stream), "sum f");
     hipchk(c, hipMemcpyAsync(fz, f0, sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream), "copy");
-    hipchk(c, launch_add_const<float>(fz, G.R(), red, (double)G.R(), -1.0, c->stream), "demean");
+    if (p->has_los) {
+        hipchk(c, launch_sum<float>(f0, G.R(), red, c->stream), "sum f");
+        hipchk(c, launch_add_const<float>(fz, G.R(), red, (double)G.R(), -1.0, c->stream), "demean");
+    }
     hipchk(c, launch_sumsq<float>(fz, G.R(), red + 1, c->stream), "norm f");
     MGPyr P = mg_build(c, G, p, slot, v0, fz, p->fp64_coarse_max);
     zero_level(c, P, 0);
@@ -796,22 +801,43 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     hipchk(c, hipMemcpyAsync(h, red, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
     sync(c);
     const double fn = sqrt(h[1]);
-    int cyc = 0;
-    double rel = fn > 0 ? 1.0 : 0.0;
-    while (rel > p->tolerance && cyc < p->max_vcycles) {
+    // Stop at ||f - A v|| <= tol ||f||; or once a V-cycle changes the
+    // residual by <= tol ||f||, or two V-cycles in a row reduce it by less
+    // than 10 %: with a local LOS the operator is not symmetric, its range
+    // misses a direction and an arbitrary f has a residual floor above tol
+    // (the iteration has then converged to the attainable solution; FP32
+    // rounding keeps the residual moving at about 1e-6 of ||f||).
+    float *rb[2] = {bufT<float>(c, "mg_r0", G.R()), bufT<float>(c, "mg_r1", G.R())};
+    const float *rprev = P.F[0];   // residual of v = 0
+    int cyc = 0, stall = 0;
+    double rel = fn > 0 ? 1.0 : 0.0, chg = rel;
+    while (rel > p->tolerance && chg > p->tolerance && stall < 2 && cyc < p->max_vcycles) {
+        const double rel_prev = rel;
         mg_vcycle_rb(c, P, 0, p);
+        // A annihilates constants: keep v zero-mean, so that an inconsistent
+        // f (residual floor) does not make v drift by a constant every cycle
+        // until FP32 rounding of v disturbs the residual.
+        hipchk(c, hipMemsetAsync(red + 4, 0, sizeof(double), c->stream), "memset");
+        hipchk(c, launch_sum<float>(vcur(P, 0), G.R(), red + 4, c->stream), "sum v");
+        hipchk(c, launch_add_const<float>(vcur(P, 0), G.R(), red + 4, (double)G.R(), -1.0, c->stream),
+               "demean v");
+        float *r = rb[cyc & 1];
         cyc++;
-        float *r = vother(P, 0);
         hipchk(c, launch_mg_residual(r, vcur(P, 0), P.F[0], P.L[0], p->beta, p->has_los, c->stream),
                "residual");
-        hipchk(c, hipMemsetAsync(red + 2, 0, sizeof(double), c->stream), "memset");
+        hipchk(c, hipMemsetAsync(red + 2, 0, 2 * sizeof(double), c->stream), "memset");
         hipchk(c, launch_sumsq<float>(r, G.R(), red + 2, c->stream), "norm r");
-        hipchk(c, hipMemcpyAsync(h + 2, red + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hipchk(c, launch_sumsq_diff<float>(r, rprev, G.R(), red + 3, c->stream), "norm dr");
+        hipchk(c, hipMemcpyAsync(h + 2, red + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
         sync(c);
         rel = fn > 0 ? sqrt(h[2]) / fn : 0.0;
+        chg = fn > 0 ? sqrt(h[3]) / fn : 0.0;
+        stall = rel > 0.9 * rel_prev ? stall + 1 : 0;
+        rprev = r;
     }
     p->vcycles_done = cyc;
     p->final_residual = rel;
+    p->final_change = chg;
     if (P.cur[0])
         hipchk(c, hipMemcpyAsync(v0, P.B[0], sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream),
                "copy");

@@ -226,6 +226,8 @@ hipError_t launch_mg_prolong_t(T *fine, const U *coarse, const int coarse_n[3], 
 template <typename T>
 hipError_t launch_sumsq(const T *a, int64_t n, double *out, hipStream_t s);
 template <typename T>
+hipError_t launch_sumsq_diff(const T *a, const T *b, int64_t n, double *out, hipStream_t s);
+template <typename T>
 hipError_t launch_sum(const T *a, int64_t n, double *out, hipStream_t s);
 template <typename T>
 hipError_t launch_add_const(T *a, int64_t n, const double *num, double den, double sign,

@@ -204,6 +204,19 @@ __global__ void k_sumsq(const T *a, int64_t n, double *out) {
     if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
 }
 
+// sum of (a - b)^2 in FP64
+template <typename T>
+__global__ void k_sumsq_diff(const T *a, const T *b, int64_t n, double *out) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)a[i] - (double)b[i];
+        s += v * v;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
 template <typename T>
 __global__ void k_sum(const T *a, int64_t n, double *out) {
     double s = 0.0;
@@ -317,6 +330,11 @@ hipError_t launch_sumsq(const T *a, int64_t n, double *out, hipStream_t s) {
     return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_sumsq_diff(const T *a, const T *b, int64_t n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sumsq_diff<T>, dim3(red_blocks(n)), dim3(256), 0, s, a, b, n, out);
+    return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_sum(const T *a, int64_t n, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_sum<T>, dim3(red_blocks(n)), dim3(256), 0, s, a, n, out);
     return hipGetLastError();
@@ -349,6 +367,7 @@ template hipError_t launch_mg_prolong_t<float, double>(float *, const double *, 
 template hipError_t launch_mg_prolong_t<double, double>(double *, const double *, const int *, int, hipStream_t);
 template hipError_t launch_sumsq<float>(const float *, int64_t, double *, hipStream_t);
 template hipError_t launch_sumsq<double>(const double *, int64_t, double *, hipStream_t);
+template hipError_t launch_sumsq_diff<float>(const float *, const float *, int64_t, double *, hipStream_t);
 template hipError_t launch_sum<float>(const float *, int64_t, double *, hipStream_t);
 template hipError_t launch_sum<double>(const double *, int64_t, double *, hipStream_t);
 template hipError_t launch_add_const<float>(float *, int64_t, const double *, double, double, hipStream_t);

@@ -56,6 +56,7 @@ class Params(ctypes.Structure):
         ("mutate_inputs", ctypes.c_int32),
         ("vcycles_done", ctypes.c_int32),
         ("final_residual", ctypes.c_double),
+        ("final_change", ctypes.c_double),
     ]
 
 

@@ -162,6 +162,7 @@ class MultigridRecon(_Recon):
     fp64_coarse_max: int = 8
     vcycles_done: int = 0
     final_residual: float = float("nan")
+    final_change: float = float("nan")
 
     kind = L.MULTIGRID
 
@@ -290,6 +291,7 @@ def run_(recon: _Recon, grid_size, x, y, z, w, rx=None, ry=None, rz=None, rw=Non
     if isinstance(recon, MultigridRecon):
         recon.vcycles_done = p.vcycles_done
         recon.final_residual = p.final_residual
+        recon.final_change = p.final_change
     recon.result_cache = ResultCache(ctx, tuple(int(v) for v in grid_size), recon.kind)
     return recon.result_cache.numpy() if copy else recon.result_cache
 
@@ -305,6 +307,7 @@ def _grid_call(name, recon, delta, x, y, z, w, rx, ry, rz, rw, pass_p_mut=False)
     if isinstance(recon, MultigridRecon):
         recon.vcycles_done = p.vcycles_done
         recon.final_residual = p.final_residual
+        recon.final_change = p.final_change
     return delta
 
 
@@ -500,4 +503,5 @@ def fmg(f, recon: MultigridRecon, v=None):
                                     _ptr(_f32(f))))
     recon.vcycles_done = p.vcycles_done
     recon.final_residual = p.final_residual
+    recon.final_change = p.final_change
     return v

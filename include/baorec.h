@@ -60,8 +60,15 @@ typedef enum { BAOREC_MAS_CIC = 0, BAOREC_MAS_TSC = 1 } baorec_mas;
 
 /* Multigrid schedule.  REFERENCE is fmg with damped Jacobi, a fixed number of
  * V-cycles per level (src/multigrid.jl:654-752).  RBGS is an extension:
- * red-black Gauss-Seidel V-cycles until the relative residual (FP64 norm)
- * drops below `tolerance`, with levels of side <= fp64_coarse_max in FP64. */
+ * Gauss-Seidel V-cycles (rb_sweeps pre- and post-smoothing sweeps) in the
+ * four-colour ordering that generalises red-black to the operator's 19-point
+ * stencil (its cross-derivative terms couple diagonal neighbours), with levels
+ * of side <= fp64_coarse_max in FP64.  It stops when the relative residual
+ * (FP64 norm) is <= `tolerance`, when one V-cycle changes the residual by
+ * <= tolerance * ||f||, or when two V-cycles in a row reduce it by less than
+ * 10 %: with a local LOS the operator is not symmetric and an arbitrary f has
+ * a residual floor above zero (final_residual reports where it stopped).  The
+ * solution is returned with zero mean (A annihilates constants). */
 typedef enum { BAOREC_MG_REFERENCE = 0, BAOREC_MG_RBGS = 1 } baorec_mg_schedule;
 
 /*
@@ -94,7 +101,8 @@ typedef struct baorec_params {
     int32_t mutate_inputs;    /* write wrapped positions back (wrap=true), 1 */
     /* ---- outputs of the last run ---- */
     int32_t vcycles_done;     /* RBGS: V-cycles used on the finest level */
-    double final_residual;    /* RBGS: last relative residual */
+    double final_residual;    /* RBGS: last relative residual ||f - A v|| / ||f|| */
+    double final_change;      /* RBGS: last V-cycle's residual change / ||f|| */
 } baorec_params;
 
 /* Defaults of the Julia constructors (src/recon.jl:2-33); beta = f/bias in

@@ -364,3 +364,143 @@ def test_no_result_error(ctx):
     with pytest.raises(bh.BaorecError):
         bh.reconstructed_positions(rec, np.ones(3, F32), np.ones(3, F32), np.ones(3, F32))
     c.close()
+
+
+# ------------------------------------------------------------------ RBGS schedule (extension)
+
+def fixed_los_symbol(n, L, beta, los):
+    """Fourier symbol of the discrete multigrid operator for a fixed LOS
+    (constant coefficients, so the FFT diagonalises it exactly)."""
+    cell = np.array(L, np.float64) / np.array(n)
+    p = np.array(los, dtype=F32).astype(np.float64) / cell
+    g = beta / np.sum(cell ** 2 * p ** 2)
+    gp = 1 / cell ** 2 + g * p ** 2
+    th = [2 * np.pi * np.fft.fftfreq(n[a]) for a in range(3)]
+    T = [th[0][None, None, :], th[1][None, :, None], th[2][:, None, None]]
+    A = sum(gp[a] * (2 - 2 * np.cos(T[a])) for a in range(3))
+    for i, j in ((0, 1), (0, 2), (1, 2)):
+        A = A + 2 * g * p[i] * p[j] * np.sin(T[i]) * np.sin(T[j])
+    return A
+
+
+@pytest.mark.parametrize("fp64_coarse_max", [8, 0])
+def test_mg_rbgs_fixed_los_matches_exact_inverse(ctx, fp64_coarse_max):
+    """Multicolour Gauss-Seidel V-cycles (2+2 sweeps) to relative residual
+    1e-6 reproduce the exact inverse of the discrete operator.  Tolerance:
+    rel L2 3e-5 on the zero-mean potential (a 1e-6 residual bounds the error
+    by the operator's condition number over the modes it leaves)."""
+    n = (64, 64, 64)
+    kw = mg_rec((0.0, 0.6, 0.8))
+    rng = np.random.default_rng(91)
+    f = rng.standard_normal(n[::-1]).astype(F32)
+    rec = bh.MultigridRecon(**kw, schedule="rbgs", tolerance=1e-6, fp64_coarse_max=fp64_coarse_max)
+    phi = bh.fmg(f, rec)
+    assert rec.final_residual <= 1e-6 and 0 < rec.vcycles_done < rec.max_vcycles
+    o = orc.MultigridRecon(**kw)
+    A = fixed_los_symbol(n, kw["box_size"], o.beta, kw["los"])
+    fk = np.fft.fftn(f.astype(np.float64) - f.astype(np.float64).mean())
+    exact = np.fft.ifftn(np.where(A > 1e-12, fk / np.where(A > 1e-12, A, 1), 0)).real
+    got = phi.astype(np.float64) - phi.mean(dtype=np.float64)
+    err = rel_l2(got, exact)
+    print(f"rbgs fixed LOS: {rec.vcycles_done} V-cycles, residual {rec.final_residual:.2e}, rel L2 {err:.2e}")
+    assert err < 3e-5
+
+
+@pytest.mark.parametrize("box_min", [(200.0, 300.0, 100.0), (-500.0, -500.0, -500.0)])
+def test_mg_rbgs_local_los_manufactured_solution(ctx, box_min):
+    """Local (radial) LOS, observer outside and inside the box: f = A v_true
+    with A the oracle's operator (FP64), so f lies in A's range; the RBGS
+    V-cycles must reach relative residual 1e-6 and return v_true up to a
+    constant."""
+    from scipy.ndimage import gaussian_filter
+    n = 32
+    kw = dict(bias=2.0, f=0.8, smoothing_radius=15.0, box_size=[1000.0] * 3, box_min=list(box_min), los=None)
+    o = orc.MultigridRecon(**kw)
+    rng = np.random.default_rng(93)
+    vt = gaussian_filter(rng.standard_normal((n, n, n)), 1.5, mode="wrap")
+    f = -orc.mg_residual(vt, np.zeros_like(vt), o.box_size, o.box_min, o.beta, los=None)
+    rec = bh.MultigridRecon(**kw, schedule="rbgs", tolerance=1e-6)
+    phi = bh.fmg(f.astype(F32), rec)
+    assert rec.final_residual <= 1e-6, (rec.final_residual, rec.vcycles_done)
+    err = rel_l2(phi.astype(np.float64) - phi.mean(dtype=np.float64), vt - vt.mean())
+    print(f"rbgs local LOS: {rec.vcycles_done} V-cycles, residual {rec.final_residual:.2e}, rel L2 {err:.2e}")
+    assert err < 3e-5
+
+
+def test_mg_rbgs_local_los_stops_at_attainable_residual(ctx):
+    """Random f with a local LOS has a residual floor (the non-symmetric
+    operator's range misses a direction): the solve stops by stagnation
+    instead of running to max_vcycles, and the floor it reports is the
+    one the reference's own damped-Jacobi V-cycles reach (FP64 oracle) within
+    a factor 2 (the two iterations' fixed points differ in the unreachable
+    direction)."""
+    n = 32
+    kw = mg_rec(None)
+    rng = np.random.default_rng(92)
+    f = rng.standard_normal((n, n, n)).astype(F32)
+    rec = bh.MultigridRecon(**kw, schedule="rbgs", tolerance=1e-6)
+    bh.fmg(f, rec)
+    assert rec.vcycles_done < 60 and rec.final_residual > rec.tolerance
+    o = orc.MultigridRecon(**kw)
+    f64 = f.astype(np.float64)
+    v = np.zeros_like(f64)
+    for _ in range(40):
+        orc.mg_vcycle(v, f64, o.box_size, o.box_min, o.beta, 0.4, 5, los=None)
+    floor = np.linalg.norm(orc.mg_residual(v, f64, o.box_size, o.box_min, o.beta, los=None)) / np.linalg.norm(f64)
+    print(f"rbgs floor {rec.final_residual:.2e} after {rec.vcycles_done} V-cycles; Jacobi V-cycles {floor:.2e}")
+    assert 1e-5 < floor and 0.5 * floor < rec.final_residual < 2.0 * floor
+
+
+# ------------------------------------------------------------------ edge cases
+
+def test_empty_catalog_gives_nan_overdensity(ctx):
+    """No tracers: mean density 0, so ((rho/mean) - 1)/b is NaN everywhere,
+    as in the reference (src/recon.jl:54-55)."""
+    n = 16
+    e = np.zeros(0, F32)
+    rec = bh.IterativeRecon(**box_kw())
+    delta = bh.setup_overdensity_(np.zeros((n, n, n), F32), rec, e, e.copy(), e.copy(), e.copy())
+    o = orc.IterativeRecon(**box_kw())
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ref = orc.setup_overdensity(np.zeros((n, n, n), F32), o, e, e.copy(), e.copy(), e.copy())
+    assert np.isnan(ref).all() and np.isnan(delta).all()
+
+
+# ------------------------------------------------------------------ full size (512^3)
+
+def test_full_size_fft_roundtrip_and_parseval(ctx):
+    """512^3: irfft(rfft(x)) == x to FP32 rounding, and Parseval."""
+    n = 512
+    rng = np.random.default_rng(93)
+    x = rng.standard_normal((n, n, n), dtype=F32)
+    X = bh.rfft3d(x, ctx=ctx)
+    back = bh.irfft3d(X, (n, n, n), ctx=ctx)
+    assert rel_l2(back, x) < 2e-6
+    w = np.full(X.shape[-1], 2.0)
+    w[0] = w[-1] = 1.0                                    # DC and Nyquist planes counted once
+    p_spec = float(np.sum(w * np.sum(np.abs(X.astype(np.complex128)) ** 2, axis=(0, 1))))
+    p_real = float(np.sum(x.astype(np.float64) ** 2)) * n ** 3
+    assert abs(p_spec / p_real - 1) < 1e-5
+
+
+def test_full_size_lightcone_mass_and_determinism(ctx):
+    """The 512^3 lightcone workload (2M tracers, 20M randoms): CIC conserves
+    mass on the full grid, and the whole reconstruction is bitwise
+    reproducible (fixed-point scatter sums, ordered FP64 reductions)."""
+    n = 512
+    data = synth.lightcone_catalog(2_000_000, r_min=1000.0, r_max=1800.0, mesh=256, seed=2)
+    rand = synth.lightcone_randoms(20_000_000, r_min=1000.0, r_max=1800.0, seed=3)
+    ext = max(float(np.max(a) - np.min(a)) for a in rand[:3])
+    kw = dict(bias=2.2, f=0.75, smoothing_radius=15.0, los=None)
+    outs = []
+    for _ in range(2):
+        rec = bh.IterativeRecon(**kw, box_pad=0.1 * ext, ctx=ctx)
+        bh.run_(rec, (n, n, n), *data, *rand, copy=False)
+        outs.append(bh.reconstructed_positions(rec, *data[:3], field="sum"))
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+    shift = np.sqrt(np.mean([(np.asarray(p, np.float64) - q) ** 2 for p, q in zip(outs[0], data[:3])]))
+    assert np.all(np.isfinite(outs[0])) and 0.5 < shift < 50.0
+    rho = np.zeros((n, n, n), F32)
+    bh.cic_(rho, *(a.copy() for a in data), rec, wrap=False)
+    assert abs(float(np.sum(rho, dtype=np.float64)) / float(np.sum(data[3], dtype=np.float64)) - 1) < 1e-6

@@ -137,6 +137,52 @@ def test_smoothing_plane_wave():
     np.testing.assert_allclose(got, np.exp(-0.5 * R * R * (k @ k)) * f, atol=1e-12)
 
 
+def nodal_catalog(n, L, w, m=None):
+    """One FP32 particle exactly on every grid node (CIC then deposits all of
+    its weight into that node's cell), weights w[z, y, x] (exact in FP32)."""
+    c = L / n
+    g = np.arange(n if m is None else m, dtype=F32) * F32(c)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    assert np.all(w.astype(F32) == w) and w.shape == x.shape
+    return x.ravel().copy(), y.ravel().copy(), z.ravel().copy(), w.ravel().astype(F32)
+
+
+def test_setup_overdensity_box_plane_wave():
+    """rho = 1 + eps cos(k.x) on the nodes: mean 1, so delta must be the
+    smoothed mode eps exp(-R^2 k^2 / 2) cos(k.x) / b (src/recon.jl:42-57).
+    k.x = pi (i + j) / 2, so the weights are exact in FP32."""
+    n, L, R, b, eps = 16, 160.0, 12.0, 1.7, 0.25
+    f, k = plane_wave(n, L, (4, 4, 0))
+    f = np.round(f)
+    x, y, z, w = nodal_catalog(n, L, 1 + eps * f)
+    rec = orc.IterativeRecon(bias=b, f=0.5, smoothing_radius=R, box_size=[L] * 3, box_min=[0.0] * 3)
+    delta = orc.setup_overdensity(np.zeros((n, n, n)), rec, x, y, z, w)
+    np.testing.assert_allclose(delta, eps * np.exp(-0.5 * R * R * (k @ k)) * f / b, atol=1e-12)
+
+
+def test_setup_overdensity_randoms_exact():
+    """R = 0 (smoothing is the identity): alpha = S_d/S_r, cells whose randoms
+    density is at most 0.01 S_r/N_r give 0, others (d - alpha r)/(b alpha r)
+    (src/recon.jl:60-91); expectations in exact rational arithmetic.  The
+    particles sit on the nodes 0..n-2 (no wrap: a particle on the last node
+    would index past the grid), so the last plane of every axis stays empty."""
+    from fractions import Fraction as Fr
+    n, L, b = 8, 80.0, 2.0
+    m = n - 1
+    N = m ** 3
+    wr = np.ones((m, m, m)); wr[1, 2, 3] = 1 / 256         # below the threshold
+    wd = np.ones((m, m, m)); wd[4, 5, 6] = 3.0
+    rec = orc.IterativeRecon(bias=b, f=0.5, smoothing_radius=0.0, box_size=[L] * 3, box_min=[0.0] * 3)
+    delta = orc.setup_overdensity_randoms(np.zeros((n, n, n)), rec, *nodal_catalog(n, L, wd, m),
+                                          *nodal_catalog(n, L, wr, m))
+    alpha = Fr(N + 2) / (Fr(N) - 1 + Fr(1, 256))
+    expect = np.zeros((n, n, n))
+    expect[:m, :m, :m] = float((1 - alpha) / (b * alpha))
+    expect[4, 5, 6] = float((3 - alpha) / (b * alpha))
+    expect[1, 2, 3] = 0.0
+    np.testing.assert_allclose(delta, expect, atol=1e-12)
+
+
 def test_iterate_beta_zero_is_identity():
     rng = np.random.default_rng(3)
     ds = rng.standard_normal((8, 8, 8))
@@ -277,6 +323,24 @@ def test_mg_residual_matches_operator_symbol(los):
     r = orc.mg_residual(v, np.zeros_like(v), L, [0.0] * 3, beta, los=los)   # r = -A v
     Av = np.fft.ifftn(np.fft.fftn(v) * mg_symbol(n, L, beta, los)).real
     np.testing.assert_allclose(-r, Av, atol=1e-11)
+
+
+@pytest.mark.parametrize("los", [(0.0, 0.0, 1.0), (0.6, 0.0, 0.8)])
+def test_mg_jacobi_plane_wave_amplification(los):
+    """With f = 0 a damped Jacobi sweep multiplies each Fourier mode by
+    1 - omega lambda(k) / D, D the stencil's diagonal 2 sum_a (1/h_a^2 + g p_a^2)
+    (src/multigrid.jl:14-135)."""
+    n, L, beta, omega, niter = (16, 8, 12), (32.0, 16.0, 24.0), 0.4, 0.4, 3
+    cell = np.array(L) / np.array(n)
+    p = np.array(los, dtype=F32).astype(F64) / cell
+    g = beta / np.sum(cell ** 2 * p ** 2)
+    D = 2 * np.sum(1 / cell ** 2 + g * p ** 2)
+    rng = np.random.default_rng(9)
+    v = rng.standard_normal((n[2], n[1], n[0]))
+    amp = (1 - omega * mg_symbol(n, L, beta, los) / D) ** niter
+    expect = np.fft.ifftn(np.fft.fftn(v) * amp).real
+    got = orc.mg_jacobi(v.copy(), np.zeros_like(v), L, [0.0] * 3, beta, omega, niter, los=los)
+    np.testing.assert_allclose(got, expect, atol=1e-11)
 
 
 def test_mg_vcycles_converge_to_fft_solution():
