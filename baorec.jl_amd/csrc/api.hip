@@ -584,7 +584,12 @@ struct MGPyr {
     std::vector<MGLevel> L;
     std::vector<int> cur;              // 0: A holds v, 1: B holds v
     std::vector<int> dbl;
+    int lc = 0;                        // levels >= lc run in one k_mg_coarse launch
 };
+
+// Levels of at most this many cells are latency-bound: they run inside one
+// workgroup (k_mg_coarse) instead of one launch per sweep.
+constexpr int64_t kMGCoarseCells = 512;
 
 bool mg_recurse(const Grid &G) {
     return G.nx > 4 && G.nx % 2 == 0 && G.ny > 4 && G.ny % 2 == 0 && G.nz > 4 && G.nz % 2 == 0;
@@ -651,6 +656,12 @@ MGPyr mg_build(baorec_ctx *c, const Grid &G0, const baorec_params *p, Slot *slot
         P.L.push_back(L);
         P.cur.push_back(0);
     }
+    // first level from which every level is small and of one precision
+    P.lc = nl;
+    for (int i = nl - 1; i >= 0; i--) {
+        if (P.g[i].R() > kMGCoarseCells || P.dbl[i] != P.dbl[nl - 1] || nl - i > kMGCoarseLevels) break;
+        P.lc = i;
+    }
     return P;
 }
 
@@ -678,8 +689,46 @@ void zero_level(baorec_ctx *c, MGPyr &P, int l) {
     }
 }
 
+// Levels [P.lc, nl) as one k_mg_coarse launch: a V-cycle from level lc, or
+// the whole fmg from lc.
+template <typename T>
+void mg_coarse(baorec_ctx *c, MGPyr &P, const baorec_params *p, int fmg, int gs) {
+    MGCoarse<T> a{};
+    a.nl = (int)P.g.size() - P.lc;
+    for (int i = 0; i < a.nl; i++) {
+        const int l = P.lc + i;
+        a.L[i] = P.L[l];
+        if constexpr (sizeof(T) == sizeof(double)) {
+            a.A[i] = P.Ad[l];
+            a.B[i] = P.Bd[l];
+            a.F[i] = P.Fd[l];
+        } else {
+            a.A[i] = P.A[l];
+            a.B[i] = P.B[l];
+            a.F[i] = P.F[l];
+        }
+        a.cur[i] = P.cur[l];
+    }
+    a.fmg = fmg;
+    a.gs = gs;
+    a.beta = p->beta;
+    a.omega = p->jacobi_damping_factor;
+    a.has_los = p->has_los;
+    a.jacobi_n = p->jacobi_niterations;
+    a.rb_sweeps = p->rb_sweeps;
+    a.vcycles = p->vcycle_niterations;
+    ProfScope ps(c, sizeof(T) == sizeof(double) ? "mg_coarse64" : "mg_coarse", 0.0);
+    hipchk(c, launch_mg_coarse<T>(a, c->stream), "mg coarse levels");
+    // damped Jacobi flips A/B once per sweep; every level visit does an even
+    // number (pre + post), so P.cur is unchanged
+}
+
 // vcycle! (src/multigrid.jl:689-719), reference schedule, FP32
 void mg_vcycle_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
+    if (l == P.lc) {
+        mg_coarse<float>(c, P, p, 0, 0);
+        return;
+    }
     mg_jacobi_n(c, P, l, p, p->jacobi_niterations);
     if (l + 1 < (int)P.g.size()) {
         float *r = vother(P, l);
@@ -706,6 +755,10 @@ void mg_vcycle_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
 
 // fmg (src/multigrid.jl:722-752), reference schedule
 void mg_fmg_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
+    if (l == P.lc) {
+        mg_coarse<float>(c, P, p, 1, 0);
+        return;
+    }
     if (l + 1 < (int)P.g.size()) {
         const int fn[3] = {P.g[l].nx, P.g[l].ny, P.g[l].nz};
         hipchk(c, launch_mg_reduce(P.F[l + 1], P.F[l], fn, c->stream), "reduce");
@@ -735,21 +788,37 @@ void mg_smooth_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int sw
 }
 
 void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
+    if (l == P.lc) {
+        if (P.dbl[l])
+            mg_coarse<double>(c, P, p, 0, 1);
+        else
+            mg_coarse<float>(c, P, p, 0, 1);
+        return;
+    }
     const int nl = (int)P.g.size();
     const bool coarsest = l + 1 >= nl;
     mg_smooth_rb(c, P, l, p, coarsest ? 16 * p->rb_sweeps : p->rb_sweeps);
     if (!coarsest) {
         const int fn[3] = {P.g[l].nx, P.g[l].ny, P.g[l].nz};
         const int cn[3] = {P.g[l + 1].nx, P.g[l + 1].ny, P.g[l + 1].nz};
+        const double R = (double)P.g[l].R(), w = P.dbl[l] ? 8.0 : 4.0;
         if (P.dbl[l]) {
             double *r = votherd(P, l);
-            hipchk(c, launch_mg_residual_t<double>(r, vcurd(P, l), P.Fd[l], P.L[l], p->beta, p->has_los,
-                                                   c->stream), "residual64");
+            {
+                ProfScope ps(c, "mg_residual64", 3.0 * w * R);
+                hipchk(c, launch_mg_residual_t<double>(r, vcurd(P, l), P.Fd[l], P.L[l], p->beta, p->has_los,
+                                                       c->stream), "residual64");
+            }
+            ProfScope ps(c, "mg_reduce64", 1.125 * w * R);
             hipchk(c, launch_mg_reduce_t<double, double>(P.Fd[l + 1], r, fn, c->stream), "reduce64");
         } else {
             float *r = vother(P, l);
-            hipchk(c, launch_mg_residual(r, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
-                   "residual");
+            {
+                ProfScope ps(c, "mg_residual", 12.0 * R);
+                hipchk(c, launch_mg_residual(r, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
+                       "residual");
+            }
+            ProfScope ps(c, "mg_reduce", 4.5 * R);
             if (P.dbl[l + 1])
                 hipchk(c, launch_mg_reduce_t<double, float>(P.Fd[l + 1], r, fn, c->stream), "reduce");
             else
@@ -757,6 +826,7 @@ void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
         }
         zero_level(c, P, l + 1);
         mg_vcycle_rb(c, P, l + 1, p);
+        ProfScope ps(c, P.dbl[l] ? "mg_prolong64" : "mg_prolong", (2.0 + 0.125) * w * R);
         if (P.dbl[l])
             hipchk(c, launch_mg_prolong_t<double, double>(vcurd(P, l), vcurd(P, l + 1), cn, 1, c->stream),
                    "prolong64");

@@ -201,6 +201,26 @@ struct MGLevel {
     const float *xv;   // x[n0] | y[n1] | z[n2] cell centres of this level
 };
 
+// Levels [0, nl) of a V-cycle (fmg = 0) or of the whole FMG (fmg = 1) run by
+// one workgroup (the launch-latency-bound coarse levels).  cur[l] says which
+// of A[l] / B[l] holds v (damped Jacobi ping-pongs; an even number of sweeps
+// per visit leaves it unchanged).  gs = 0: damped Jacobi, jacobi_n sweeps
+// (float only); gs = 1: four-colour Gauss-Seidel, rb_sweeps (16x on the
+// coarsest level).
+constexpr int kMGCoarseLevels = 4;
+template <typename T>
+struct MGCoarse {
+    int nl;
+    MGLevel L[kMGCoarseLevels];
+    T *A[kMGCoarseLevels], *B[kMGCoarseLevels], *F[kMGCoarseLevels];
+    int cur[kMGCoarseLevels];
+    int fmg, gs;
+    float beta, omega;
+    int has_los, jacobi_n, rb_sweeps, vcycles;
+};
+template <typename T>
+hipError_t launch_mg_coarse(const MGCoarse<T> &a, hipStream_t s);
+
 hipError_t launch_mg_jacobi(float *vout, const float *vin, const float *f, const MGLevel &L,
                             float beta, float omega, int has_los, hipStream_t s);
 hipError_t launch_mg_residual(float *r, const float *v, const float *f, const MGLevel &L,

@@ -66,14 +66,14 @@ __device__ __forceinline__ Nbr<T> nbr(const T *__restrict__ v, const MGLevel &L,
     return r;
 }
 
-// One damped Jacobi sweep (src/multigrid.jl:137-191 and the broadcast at :207),
-// out of place: vout = (1 - w) vin + w jac(vin).
-__global__ void k_mg_jacobi(float *__restrict__ vout, const float *__restrict__ vin,
-                            const float *__restrict__ f, MGLevel L, float beta, float omega,
-                            int has_los) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y, iz = blockIdx.z;
-    if (ix >= L.n[0]) return;
+// Per-cell operations, shared by the grid-wide kernels below and the
+// single-workgroup coarse-level kernel (same arithmetic in both).
+
+// One damped Jacobi update (src/multigrid.jl:137-191 and the broadcast at
+// :207), out of place: vout = (1 - w) vin + w jac(vin).
+__device__ __forceinline__ void cell_jacobi(float *__restrict__ vout, const float *__restrict__ vin,
+                                            const float *__restrict__ f, const MGLevel &L, int ix, int iy,
+                                            int iz, float beta, float omega, int has_los) {
     const Coef<float> c = coef<float>(L, ix, iy, iz, beta, has_los);
     const Nbr<float> s = nbr<float>(vin, L, ix, iy, iz, c);
     const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
@@ -85,11 +85,9 @@ __global__ void k_mg_jacobi(float *__restrict__ vout, const float *__restrict__ 
 
 // r = f - A v (src/multigrid.jl:328-376 and the broadcast at :393).
 template <typename T>
-__global__ void k_mg_residual(T *__restrict__ r, const T *__restrict__ v, const T *__restrict__ f,
-                              MGLevel L, float beta, int has_los) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y, iz = blockIdx.z;
-    if (ix >= L.n[0]) return;
+__device__ __forceinline__ void cell_residual(T *__restrict__ r, const T *__restrict__ v, const T *__restrict__ f,
+                                              const MGLevel &L, int ix, int iy, int iz, float beta,
+                                              int has_los) {
     const Coef<T> c = coef<T>(L, ix, iy, iz, (T)beta, has_los);
     const Nbr<T> s = nbr<T>(v, L, ix, iy, iz, c);
     const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
@@ -101,13 +99,11 @@ __global__ void k_mg_residual(T *__restrict__ r, const T *__restrict__ v, const 
 // Multicolour Gauss-Seidel (RBGS schedule): colour = ((ix+iy)&1) + 2((iy+iz)&1)
 // separates every face and edge neighbour of the 19-point stencil, so one
 // colour updates in place without races.
+__device__ __forceinline__ int mg_color(int ix, int iy, int iz) { return ((ix + iy) & 1) + 2 * ((iy + iz) & 1); }
+
 template <typename T>
-__global__ void k_mg_mcgs(T *__restrict__ v, const T *__restrict__ f, MGLevel L, float beta,
-                          int has_los, int color) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y, iz = blockIdx.z;
-    if (ix >= L.n[0]) return;
-    if ((((ix + iy) & 1) + 2 * ((iy + iz) & 1)) != color) return;
+__device__ __forceinline__ void cell_mcgs(T *v, const T *__restrict__ f, const MGLevel &L, int ix, int iy, int iz,
+                                          float beta, int has_los) {
     const Coef<T> c = coef<T>(L, ix, iy, iz, (T)beta, has_los);
     const Nbr<T> s = nbr<T>(v, L, ix, iy, iz, c);
     const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
@@ -116,13 +112,36 @@ __global__ void k_mg_mcgs(T *__restrict__ v, const T *__restrict__ f, MGLevel L,
     v[I] = j / ((T)2 * (c.gpx2 + c.gpy2 + c.gpz2));
 }
 
+__global__ void k_mg_jacobi(float *__restrict__ vout, const float *__restrict__ vin,
+                            const float *__restrict__ f, MGLevel L, float beta, float omega,
+                            int has_los) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= L.n[0]) return;
+    cell_jacobi(vout, vin, f, L, ix, blockIdx.y, blockIdx.z, beta, omega, has_los);
+}
+
+template <typename T>
+__global__ void k_mg_residual(T *__restrict__ r, const T *__restrict__ v, const T *__restrict__ f,
+                              MGLevel L, float beta, int has_los) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= L.n[0]) return;
+    cell_residual<T>(r, v, f, L, ix, blockIdx.y, blockIdx.z, beta, has_los);
+}
+
+template <typename T>
+__global__ void k_mg_mcgs(T *__restrict__ v, const T *__restrict__ f, MGLevel L, float beta,
+                          int has_los, int color) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= L.n[0] || mg_color(ix, iy, iz) != color) return;
+    cell_mcgs<T>(v, f, L, ix, iy, iz, beta, has_los);
+}
+
 // 27-point full weighting around fine cell 2c+1 (src/multigrid.jl:586-640).
 template <typename T, typename U>
-__global__ void k_mg_reduce(T *__restrict__ coarse, const U *__restrict__ w, int fx, int fy, int fz) {
+__device__ __forceinline__ void cell_reduce(T *__restrict__ coarse, const U *__restrict__ w, int fx, int fy, int fz,
+                                            int kx, int ky, int kz) {
     const int cx = fx / 2, cy = fy / 2;
-    const int kx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ky = blockIdx.y, kz = blockIdx.z;
-    if (kx >= cx) return;
     const int ix = 2 * kx + 1, iy = 2 * ky + 1, iz = 2 * kz + 1;
     const int ixp = ix + 1 >= fx ? ix + 1 - fx : ix + 1, ixm = ix - 1;
     const int iyp = iy + 1 >= fy ? iy + 1 - fy : iy + 1, iym = iy - 1;
@@ -141,16 +160,20 @@ __global__ void k_mg_reduce(T *__restrict__ coarse, const U *__restrict__ w, int
     coarse[kx + (int64_t)cx * (ky + (int64_t)cy * kz)] = (T)((double)s / 64.0);
 }
 
+template <typename T, typename U>
+__global__ void k_mg_reduce(T *__restrict__ coarse, const U *__restrict__ w, int fx, int fy, int fz) {
+    const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kx >= fx / 2) return;
+    cell_reduce<T, U>(coarse, w, fx, fy, fz, kx, blockIdx.y, blockIdx.z);
+}
+
 // Trilinear prolongation (src/multigrid.jl:456-509) evaluated per fine cell:
 // odd fine index 2c+1 copies coarse c, even index 2c+2 averages c and c+1;
 // the sums keep the reference's term order for each of the 8 cases.
 template <typename T, typename U>
-__global__ void k_mg_prolong(T *__restrict__ fine, const U *__restrict__ v, int cx, int cy, int cz,
-                             int add) {
+__device__ __forceinline__ void cell_prolong(T *__restrict__ fine, const U *__restrict__ v, int cx, int cy, int cz,
+                                             int add, int ix, int iy, int iz) {
     const int fx = 2 * cx, fy = 2 * cy;
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y, iz = blockIdx.z;
-    if (ix >= fx) return;
     const int tx = (ix & 1) == 0, ty = (iy & 1) == 0, tz = (iz & 1) == 0;
     const int ax = tx ? (ix == 0 ? cx - 1 : ix / 2 - 1) : (ix - 1) / 2;
     const int ay = ty ? (iy == 0 ? cy - 1 : iy / 2 - 1) : (iy - 1) / 2;
@@ -178,6 +201,109 @@ __global__ void k_mg_prolong(T *__restrict__ fine, const U *__restrict__ v, int 
 #undef C
     const int64_t I = ix + (int64_t)fx * (iy + (int64_t)fy * iz);
     fine[I] = add ? (T)(fine[I] + (T)val) : (T)val;
+}
+
+template <typename T, typename U>
+__global__ void k_mg_prolong(T *__restrict__ fine, const U *__restrict__ v, int cx, int cy, int cz,
+                             int add) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= 2 * cx) return;
+    cell_prolong<T, U>(fine, v, cx, cy, cz, add, ix, blockIdx.y, blockIdx.z);
+}
+
+// ------------------------------------------------------------ coarse levels
+//
+// The levels below a few thousand cells are latency-bound: one launch per
+// sweep costs more than the sweep.  k_mg_coarse runs a whole V-cycle (or the
+// whole FMG) over levels [0, nl) of an MGCoarse<T> in one 1024-thread
+// workgroup, with a barrier between stages, applying the same per-cell
+// operations in the same order as the per-level launches.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_mg_coarse(MGCoarse<T> a) {
+    int cur[kMGCoarseLevels];
+    for (int l = 0; l < a.nl; l++) cur[l] = a.cur[l];
+    auto vc = [&](int l) { return cur[l] ? a.B[l] : a.A[l]; };
+    auto vo = [&](int l) { return cur[l] ? a.A[l] : a.B[l]; };
+    // every cell (ix, iy, iz) of level l, then a barrier
+    auto forall = [&](int l, auto &&fn) {
+        const int nx = a.L[l].n[0], ny = a.L[l].n[1], n = nx * ny * a.L[l].n[2];
+        for (int c = threadIdx.x; c < n; c += blockDim.x) fn(c % nx, (c / nx) % ny, c / (nx * ny));
+        __syncthreads();
+    };
+    auto smooth = [&](int l, int n) {
+        if constexpr (sizeof(T) == sizeof(float)) {
+            if (!a.gs) {
+                for (int i = 0; i < n; i++) {
+                    T *out = vo(l);
+                    const T *in = vc(l);
+                    forall(l, [&](int x, int y, int z) {
+                        cell_jacobi(out, in, a.F[l], a.L[l], x, y, z, a.beta, a.omega, a.has_los);
+                    });
+                    cur[l] ^= 1;
+                }
+                return;
+            }
+        }
+        for (int i = 0; i < n; i++)
+            for (int col = 0; col < 4; col++) {
+                T *v = vc(l);
+                forall(l, [&](int x, int y, int z) {
+                    if (mg_color(x, y, z) == col) cell_mcgs<T>(v, a.F[l], a.L[l], x, y, z, a.beta, a.has_los);
+                });
+            }
+    };
+    auto sweeps = [&](int l) {
+        if (!a.gs) return a.jacobi_n;
+        return l + 1 == a.nl ? 16 * a.rb_sweeps : a.rb_sweeps;
+    };
+    auto vcycle = [&](int top) {
+        int l = top;
+        for (;; l++) {   // down: pre-smooth, residual, restrict, zero the coarse correction
+            smooth(l, sweeps(l));
+            if (l + 1 == a.nl) break;
+            T *r = vo(l);
+            const T *v = vc(l);
+            forall(l, [&](int x, int y, int z) { cell_residual<T>(r, v, a.F[l], a.L[l], x, y, z, a.beta, a.has_los); });
+            const int *fn = a.L[l].n;
+            forall(l + 1, [&](int x, int y, int z) { cell_reduce<T, T>(a.F[l + 1], r, fn[0], fn[1], fn[2], x, y, z); });
+            T *vz = vc(l + 1);
+            const int nx1 = a.L[l + 1].n[0], ny1 = a.L[l + 1].n[1];
+            forall(l + 1, [&](int x, int y, int z) { vz[x + nx1 * (y + ny1 * z)] = (T)0; });
+        }
+        smooth(l, sweeps(l));   // coarsest: post-smoothing right away
+        for (l = l - 1; l >= top; l--) {   // up: prolong and add, post-smooth
+            T *v = vc(l);
+            const T *vcoarse = vc(l + 1);
+            const int *cn = a.L[l + 1].n;
+            forall(l, [&](int x, int y, int z) { cell_prolong<T, T>(v, vcoarse, cn[0], cn[1], cn[2], 1, x, y, z); });
+            smooth(l, sweeps(l));
+        }
+    };
+    if (!a.fmg) {
+        vcycle(0);
+        return;
+    }
+    // fmg: restrict f down the levels, zero the coarsest, then on the way up
+    // prolong (overwrite) and run vcycles V-cycles per level
+    for (int l = 0; l + 1 < a.nl; l++) {
+        const int *fn = a.L[l].n;
+        const T *fl = a.F[l];
+        forall(l + 1, [&](int x, int y, int z) { cell_reduce<T, T>(a.F[l + 1], fl, fn[0], fn[1], fn[2], x, y, z); });
+    }
+    {
+        T *vz = vc(a.nl - 1);
+        const int nx1 = a.L[a.nl - 1].n[0], ny1 = a.L[a.nl - 1].n[1];
+        forall(a.nl - 1, [&](int x, int y, int z) { vz[x + nx1 * (y + ny1 * z)] = (T)0; });
+    }
+    for (int l = a.nl - 1; l >= 0; l--) {
+        if (l + 1 < a.nl) {
+            T *v = vc(l);
+            const T *vcoarse = vc(l + 1);
+            const int *cn = a.L[l + 1].n;
+            forall(l, [&](int x, int y, int z) { cell_prolong<T, T>(v, vcoarse, cn[0], cn[1], cn[2], 0, x, y, z); });
+        }
+        for (int i = 0; i < a.vcycles; i++) vcycle(l);
+    }
 }
 
 // Cell centres of every level (x_vec per level, src/multigrid.jl:656).
@@ -290,6 +416,14 @@ hipError_t launch_mg_rbgs(T *v, const T *f, const MGLevel &L, float beta, int ha
                        beta, has_los, color);
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_mg_coarse(const MGCoarse<T> &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(1024), 0, s, a);
+    return hipGetLastError();
+}
+template hipError_t launch_mg_coarse<float>(const MGCoarse<float> &, hipStream_t);
+template hipError_t launch_mg_coarse<double>(const MGCoarse<double> &, hipStream_t);
 
 hipError_t launch_mg_reduce(float *coarse, const float *fine, const int fine_n[3], hipStream_t s) {
     return launch_mg_reduce_t<float, float>(coarse, fine, fine_n, s);

@@ -48,6 +48,14 @@ def make_workload(args):
         data = synth.box_catalog(1_000_000, L=1000.0, mesh=256, seed=1)
         rand = None
         cfg = dict(grid=(128, 128, 128), bias=2.0, f=0.8, R=15.0, los=(0.0, 0.0, 1.0), pad=0.0)
+    elif args.workload in ("mg512", "mg512_ref"):
+        # multigrid on the 1M-tracer periodic-box catalog at 512^3, local LOS
+        # from the box corner; RBGS to relative residual 1e-6 with FP64 levels
+        # <= 8^3, or the reference's FMG schedule
+        data = synth.box_catalog(1_000_000, L=1000.0, mesh=256, seed=1)
+        rand = None
+        cfg = dict(grid=(512, 512, 512), bias=2.0, f=0.8, R=15.0, los=None, pad=0.0, recon="multigrid",
+                   schedule="reference" if args.workload == "mg512_ref" else "rbgs")
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     log(f"synthetic catalogs in {time.perf_counter() - t0:.1f} s")
@@ -64,12 +72,19 @@ def gpu_bench(args, data, rand, cfg):
     p = L.Params()
     lib.baorec_params_default(ctypes.byref(p), cfg["bias"], cfg["f"], cfg["R"])
     p.box_pad = cfg["pad"]
-    if cfg["los"] is None:
-        p.has_los = 0
-    else:
+    multigrid = cfg.get("recon") == "multigrid"
+    if multigrid:
+        p.mg_schedule = L.MG_SCHEDULE[cfg["schedule"]]
+        p.tolerance = 1e-6
+        p.rb_sweeps = 2
+        p.fp64_coarse_max = 8
+    if cfg["los"] is not None:
         p.has_los = 1
         for a in range(3):
             p.los[a] = cfg["los"][a]
+    else:
+        p.has_los = 0
+    if rand is None:
         for a in range(3):
             p.box_size[a] = 1000.0
             p.box_min[a] = 0.0
@@ -83,12 +98,15 @@ def gpu_bench(args, data, rand, cfg):
     # so repeated steps see the same inputs
     vp = lambda a: None if a is None else ctypes.c_void_p(a.ptr)  # noqa: E731
 
+    run = lib.baorec_run_multigrid if multigrid else lib.baorec_run_iterative
+    kind = L.MULTIGRID if multigrid else L.ITERATIVE
+
     def step():
-        rc = lib.baorec_run_iterative(ctx.handle, ctypes.byref(p), grid, L.MEM_DEVICE, vp(d[0]), vp(d[1]),
-                                      vp(d[2]), vp(d[3]), n, vp(r[0]), vp(r[1]), vp(r[2]), vp(r[3]), nr)
+        rc = run(ctx.handle, ctypes.byref(p), grid, L.MEM_DEVICE, vp(d[0]), vp(d[1]),
+                 vp(d[2]), vp(d[3]), n, vp(r[0]), vp(r[1]), vp(r[2]), vp(r[3]), nr)
         ctx.check(rc)
         rc = lib.baorec_reconstructed_positions(ctx.handle, ctypes.byref(p), L.MEM_DEVICE, vp(d[0]), vp(d[1]),
-                                                vp(d[2]), n, None, grid, L.ITERATIVE, L.FIELD["sum"],
+                                                vp(d[2]), n, None, grid, kind, L.FIELD["sum"],
                                                 vp(out[0]), vp(out[1]), vp(out[2]))
         ctx.check(rc)
 
@@ -112,11 +130,11 @@ def gpu_bench(args, data, rand, cfg):
     hout = [np.empty(n, np.float32) for _ in range(3)]
 
     def step_host():
-        rc = lib.baorec_run_iterative(ctx.handle, ctypes.byref(p), grid, L.MEM_HOST, hp(hd[0]), hp(hd[1]),
-                                      hp(hd[2]), hp(hd[3]), n, hp(hr[0]), hp(hr[1]), hp(hr[2]), hp(hr[3]), nr)
+        rc = run(ctx.handle, ctypes.byref(p), grid, L.MEM_HOST, hp(hd[0]), hp(hd[1]),
+                 hp(hd[2]), hp(hd[3]), n, hp(hr[0]), hp(hr[1]), hp(hr[2]), hp(hr[3]), nr)
         ctx.check(rc)
         rc = lib.baorec_reconstructed_positions(ctx.handle, ctypes.byref(p), L.MEM_HOST, hp(hd[0]), hp(hd[1]),
-                                                hp(hd[2]), n, None, grid, L.ITERATIVE, L.FIELD["sum"],
+                                                hp(hd[2]), n, None, grid, kind, L.FIELD["sum"],
                                                 hp(hout[0]), hp(hout[1]), hp(hout[2]))
         ctx.check(rc)
 
@@ -137,7 +155,10 @@ def gpu_bench(args, data, rand, cfg):
     prof = json.loads(buf.value.decode())
     pos = [o.numpy() for o in out]
     finite = all(np.isfinite(a).all() for a in pos)
-    return dict(dt=dt, prof=prof, finite=finite, box=(list(p.box_size), list(p.box_min)), dt_host=dt_host)
+    mg = {"schedule": cfg["schedule"], "vcycles": p.vcycles_done, "final_residual": p.final_residual,
+          "final_change": p.final_change} if multigrid else None
+    return dict(dt=dt, prof=prof, finite=finite, box=(list(p.box_size), list(p.box_min)), dt_host=dt_host,
+                mg=mg)
 
 
 def kernel_symbol(label, grid, rand):
@@ -149,7 +170,10 @@ def kernel_symbol(label, grid, rand):
             "fft_x_r2c": f"k_x_r2c<{h}>", "fft_x_c2r": f"k_x_c2r<{h}, 0>",
             "fft_x_c2r_od": f"k_x_c2r<{h}, {2 if rand else 1}>", "fft_x_c2r_iter": f"k_x_c2r<{h}, 3>",
             "fft_x_c2r_disp": f"k_x_c2r<{h}, 5>", "scatter_accum": "k_scatter_accum<0>", "zero": "k_zero4",
-            "gather": "k_gather"}.get(label)
+            "gather": "k_gather", "mg_jacobi": "k_mg_jacobi", "mg_residual": "k_mg_residual<float>",
+            "mg_reduce": "k_mg_reduce<float, float>", "mg_prolong": "k_mg_prolong<float, float>",
+            "mg_mcgs": "k_mg_mcgs<float>", "mg_mcgs64": "k_mg_mcgs<double>",
+            "mg_coarse": "k_mg_coarse<float>", "mg_coarse64": "k_mg_coarse<double>"}.get(label)
 
 
 def measured_traffic(workload):
@@ -275,8 +299,11 @@ def main():
         cpu = cpu_baseline(args, data, rand, cfg)
     ms = g["dt"] / args.steps * 1e3
     nx, ny, nz = cfg["grid"]
+    multigrid = cfg.get("recon") == "multigrid"
+    metric = ("recons/s at 512\u00b3 grid, 2M particles: learned warm-start vs plain at equal residual"
+              if args.workload == "lightcone512" else f"recons/s, {args.workload}")
     line = {
-        "metric": "recons/s at 512\u00b3 grid, 2M particles: learned warm-start vs plain at equal residual",
+        "metric": metric,
         "value": round(args.steps / g["dt"], 4),
         "unit": "recons/s",
         "n_gpus": 1,
@@ -290,9 +317,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "form": "plain (cold start)", "grid": [nx, ny, nz],
                    "n_data": len(data[0]),
-                   "n_randoms": 0 if rand is None else len(rand[0]), "recon": "iterative",
-                   "n_iter": 3, "los": "local" if cfg["los"] is None else "fixed",
-                   "box_pad": round(cfg["pad"], 2)},
+                   "n_randoms": 0 if rand is None else len(rand[0]),
+                   "recon": "multigrid" if multigrid else "iterative",
+                   "los": "local" if cfg["los"] is None else "fixed", "box_pad": round(cfg["pad"], 2),
+                   **(g["mg"] if multigrid else {"n_iter": 3})},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": {k: {"count": v["count"], "ms": round(v["ms"], 3),

@@ -265,14 +265,15 @@ def mg_rec(los=None, L=1000.0):
                 los=None if los is None else list(los))
 
 
-@pytest.mark.parametrize("los", [None, (0.0, 0.0, 1.0)])
-def test_mg_primitives(ctx, los):
+@pytest.mark.parametrize("shape", [(16, 32, 32), (32, 16, 128)])   # (nz, ny, nx); the second is
+@pytest.mark.parametrize("los", [None, (0.0, 0.0, 1.0)])            # a multiple of the plane-marching tile
+def test_mg_primitives(ctx, los, shape):
     rng = np.random.default_rng(51)
     kw = mg_rec(los)
     rec = bh.MultigridRecon(**kw)
     o = orc.MultigridRecon(**kw)
-    v = rng.standard_normal((16, 32, 32)).astype(F32)
-    f = rng.standard_normal((16, 32, 32)).astype(F32)
+    v = rng.standard_normal(shape).astype(F32)
+    f = rng.standard_normal(shape).astype(F32)
     got = bh.jacobi_(v.copy(), f, rec)
     exp = orc.mg_jacobi(v.copy(), f, o.box_size, o.box_min, o.beta, F32(0.4), 5, los=o.los)
     assert np.max(np.abs(got - exp)) < 1e-6 * np.abs(exp).max()
@@ -280,10 +281,10 @@ def test_mg_primitives(ctx, los):
     bh.residual_(r, v, f, rec)
     exp = orc.mg_residual(v, f, o.box_size, o.box_min, o.beta, los=o.los)
     assert np.max(np.abs(r - exp)) < 1e-6 * np.abs(exp).max()
-    c = np.zeros((8, 16, 16), F32)
+    c = np.zeros(tuple(a // 2 for a in shape), F32)
     bh.reduce_(c, v, ctx)
     np.testing.assert_allclose(c, orc.mg_reduce(v), rtol=0, atol=1e-6)
-    fine = np.zeros((16, 32, 32), F32)
+    fine = np.zeros(shape, F32)
     bh.prolong_(fine, c, add=False, ctx=ctx)
     np.testing.assert_array_equal(fine, orc.mg_prolong(c))
     fine2 = v.copy()
@@ -406,14 +407,14 @@ def test_mg_rbgs_fixed_los_matches_exact_inverse(ctx, fp64_coarse_max):
     assert err < 3e-5
 
 
-@pytest.mark.parametrize("box_min", [(200.0, 300.0, 100.0), (-500.0, -500.0, -500.0)])
-def test_mg_rbgs_local_los_manufactured_solution(ctx, box_min):
+@pytest.mark.parametrize("n,box_min", [(32, (200.0, 300.0, 100.0)), (32, (-500.0, -500.0, -500.0)),
+                                        (64, (200.0, 300.0, 100.0))])
+def test_mg_rbgs_local_los_manufactured_solution(ctx, n, box_min):
     """Local (radial) LOS, observer outside and inside the box: f = A v_true
     with A the oracle's operator (FP64), so f lies in A's range; the RBGS
     V-cycles must reach relative residual 1e-6 and return v_true up to a
-    constant."""
+    constant.  At 64^3 the finest level runs the plane-marching kernels."""
     from scipy.ndimage import gaussian_filter
-    n = 32
     kw = dict(bias=2.0, f=0.8, smoothing_radius=15.0, box_size=[1000.0] * 3, box_min=list(box_min), los=None)
     o = orc.MultigridRecon(**kw)
     rng = np.random.default_rng(93)
