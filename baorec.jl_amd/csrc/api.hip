@@ -49,6 +49,10 @@ struct baorec_ctx {
         double ms = 0.0, bytes = 0.0;
     };
     std::map<std::string, Agg> prof;
+    // comoving-distance table of the last cosmology used
+    int cosmo_valid = 0;
+    baorec_cosmology cosmo{};
+    double cosmo_r0 = 0.0, cosmo_dr_inv = 0.0;
 };
 
 static thread_local std::string g_create_err;
@@ -994,6 +998,89 @@ int guarded(baorec_ctx *c, F &&fn) {
     }
 }
 
+// ------------------------------------------------------------- cosmology
+
+// Derived densities of BAOrec.Cosmology (src/cosmo.jl:22-58) in the Julia
+// constructor's precision: Float32 fields, with the photon and neutrino
+// densities computed in Float64 from physical constants and rounded to
+// Float32, and Omega_Lambda closing the budget in Float32.
+CosmoArgs cosmo_args(const baorec_cosmology *c) {
+    const double G = 6.67408e-11, hbar = 6.582119514e-16, mpc = 3.085677581491367e22;
+    const double evc2 = 1.782661907e-36, kb = 8.6173303e-5, ckms = kSpeedOfLightKms;
+    const double pi = 3.14159265358979323846;
+    const float h2 = c->h * c->h;
+    const double hbar3 = hbar * hbar * hbar, impc = 1.0 / (1e-3 * mpc), c3 = (ckms * 1e3) * (ckms * 1e3) * (ckms * 1e3);
+    const float rho_g0 = (float)(30000.0 / (8.0 * pi * G) * hbar3 * (impc * impc) * (1.0 / evc2) * c3);
+    const double tk = 2.725 * kb, tr = (double)c->T_cmb / 2.725;
+    const float omega_g0 = (float)((pi * pi / 15.0) * (tk * tk * tk * tk) / (double)rho_g0 * (tr * tr * tr * tr) /
+                                   (double)h2);
+    const float omega_nu0 = (float)((double)c->Neff * (7.0 / 8.0 * pow(4.0 / 11.0, 4.0 / 3.0)) * (double)omega_g0);
+    const float omega_b0 = c->omega_b / h2, omega_c0 = c->omega_c / h2;
+    const float omega_l0 = 1.0f - c->Omega_k0 - omega_b0 - omega_c0 - omega_nu0 - omega_g0;
+    CosmoArgs a;
+    a.omega_nu0 = omega_nu0;
+    a.omega_g0 = omega_g0;
+    a.omega_b0 = omega_b0;
+    a.omega_c0 = omega_c0;
+    a.omega_k0 = c->Omega_k0;
+    a.omega_l0 = omega_l0;
+    a.w0 = c->w0;
+    a.wa = c->wa;
+    a.h100 = (double)(c->h * 100.0f);
+    return a;
+}
+
+// The table for cosmology c on the device (rebuilt when the parameters change).
+CosmoTable cosmo_table(baorec_ctx *c, const baorec_cosmology *cp) {
+    if (!cp) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "cosmology is NULL");
+    const int num = cp->z_tab_num;
+    if (num < 2 || !(cp->z_tab_max > cp->z_tab_min) || cp->z_tab_min < 0.0 || !(cp->h > 0.f))
+        fail(c, BAOREC_ERR_INVALID_ARGUMENT, "bad cosmology table (need z_tab_num >= 2, 0 <= z_tab_min < z_tab_max, h > 0)");
+    double *zk = bufT<double>(c, "cosmo_z", num);
+    double *rk = bufT<double>(c, "cosmo_r", num);
+    int *guide = bufT<int>(c, "cosmo_guide", num);
+    CosmoTable t{zk, rk, num, cp->z_tab_min, (double)(num - 1) / (cp->z_tab_max - cp->z_tab_min), guide, num,
+                 0.0, 0.0};
+    if (!c->cosmo_valid || memcmp(&c->cosmo, cp, sizeof(*cp)) != 0) {
+        // knots of range(z_tab_min, z_tab_max, length = z_tab_num)
+        std::vector<double> h(num);
+        const long double a = cp->z_tab_min, b = cp->z_tab_max;
+        for (int i = 0; i < num; i++) h[i] = (double)(a + (b - a) * (long double)i / (long double)(num - 1));
+        h[num - 1] = cp->z_tab_max;
+        hipchk(c, hipMemcpyAsync(zk, h.data(), sizeof(double) * num, hipMemcpyHostToDevice, c->stream), "H2D");
+        double *seg = bufT<double>(c, "cosmo_seg", num);
+        hipchk(c, launch_cosmo_table(cosmo_args(cp), zk, seg, rk, num, c->stream), "cosmology table");
+        double ends[2];
+        hipchk(c, hipMemcpyAsync(&ends[0], rk, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hipchk(c, hipMemcpyAsync(&ends[1], rk + num - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hipchk(c, hipStreamSynchronize(c->stream), "cosmology table");
+        c->cosmo_r0 = ends[0];
+        c->cosmo_dr_inv = (double)num / (ends[1] - ends[0]);
+        t.r0 = c->cosmo_r0;
+        t.dr_inv = c->cosmo_dr_inv;
+        hipchk(c, launch_cosmo_guide(t, guide, c->stream), "cosmology guide");
+        c->cosmo = *cp;
+        c->cosmo_valid = 1;
+    }
+    t.r0 = c->cosmo_r0;
+    t.dr_inv = c->cosmo_dr_inv;
+    return t;
+}
+
+int *cosmo_err(baorec_ctx *c) {
+    int *e = bufT<int>(c, "cosmo_err", 1);
+    hipchk(c, hipMemsetAsync(e, 0, sizeof(int), c->stream), "memset");
+    return e;
+}
+
+void cosmo_check(baorec_ctx *c, int *e, const char *what) {
+    int h = 0;
+    hipchk(c, hipMemcpyAsync(&h, e, sizeof(int), hipMemcpyDeviceToHost, c->stream), "D2H");
+    sync(c);
+    if (h) fail(c, BAOREC_ERR_OUT_OF_BOX, std::string(what) + ": value outside the cosmology table "
+                                                          "(the reference raises BoundsError)");
+}
+
 }  // namespace
 
 // ====================================================================== ABI
@@ -1580,6 +1667,98 @@ int baorec_mg_fmg(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int me
         multigrid_solve(c, G, p, slot, dv, df);
         copy_out(c, mem, v, dv, G.R());
         sync(c);
+    });
+}
+
+void baorec_cosmology_default(baorec_cosmology *c) {
+    if (!c) return;
+    memset(c, 0, sizeof(*c));
+    c->omega_b = 0.0225f;
+    c->omega_c = 0.12f;
+    c->h = 0.67f;
+    c->Neff = 3.044f;
+    c->Omega_k0 = 0.f;
+    c->T_cmb = 2.725f;
+    c->w0 = -1.f;
+    c->wa = 0.f;
+    c->z_tab_min = 0.0;
+    c->z_tab_max = 3.0;
+    c->z_tab_num = 100000;
+}
+
+void baorec_cosmology_desi(baorec_cosmology *c) {
+    if (!c) return;
+    baorec_cosmology_default(c);
+    c->omega_b = 0.02237f;
+    c->omega_c = 0.1200f;
+    c->h = 0.6736f;
+    c->Neff = 2.0328f + 1.f;
+}
+
+int baorec_comoving_distance(baorec_ctx *c, const baorec_cosmology *cp, int mem, const double *z, int64_t n,
+                             double *r) {
+    return guarded(c, [&] {
+        if (n < 0 || (n > 0 && (!z || !r))) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "bad arrays");
+        CosmoTable t = cosmo_table(c, cp);
+        if (n == 0) return;
+        const double *dz = z;
+        double *dr = r;
+        if (mem != BAOREC_MEM_DEVICE) {
+            double *bz = bufT<double>(c, "cosmo_in", n);
+            hipchk(c, hipMemcpyAsync(bz, z, sizeof(double) * n, hipMemcpyHostToDevice, c->stream), "H2D");
+            dz = bz;
+            dr = bufT<double>(c, "cosmo_out", n);
+        }
+        int *e = cosmo_err(c);
+        hipchk(c, launch_comoving_distance(t, dz, dr, n, e, c->stream), "comoving distance");
+        if (mem != BAOREC_MEM_DEVICE)
+            hipchk(c, hipMemcpyAsync(r, dr, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream), "D2H");
+        cosmo_check(c, e, "comoving_distance");
+    });
+}
+
+int baorec_sky_to_cartesian(baorec_ctx *c, const baorec_cosmology *cp, int mem, const float *ra, const float *dec,
+                            const float *red, int64_t n, float *x, float *y, float *z) {
+    return guarded(c, [&] {
+        if (n < 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "negative count");
+        CosmoTable t = cosmo_table(c, cp);
+        if (n == 0) return;
+        const float *a = stage_in(c, mem, ra, n, "sky_a"), *b = stage_in(c, mem, dec, n, "sky_b");
+        const float *r = stage_in(c, mem, red, n, "sky_c");
+        float *ox = stage_out(c, mem, x, n, "sky_x"), *oy = stage_out(c, mem, y, n, "sky_y");
+        float *oz = stage_out(c, mem, z, n, "sky_z");
+        const float hh = (cp->h * 100.0f) / 100.0f;   // cosmo.H0 / 100 in Float32, as the driver uses
+        int *e = cosmo_err(c);
+        {
+            ProfScope ps(c, "sky_to_cartesian", 24.0 * n);
+            hipchk(c, launch_sky_to_cart(t, hh, a, b, r, ox, oy, oz, n, e, c->stream), "sky_to_cartesian");
+        }
+        copy_out(c, mem, x, ox, n);
+        copy_out(c, mem, y, oy, n);
+        copy_out(c, mem, z, oz, n);
+        cosmo_check(c, e, "sky_to_cartesian");
+    });
+}
+
+int baorec_cartesian_to_sky(baorec_ctx *c, const baorec_cosmology *cp, int mem, const float *x, const float *y,
+                            const float *z, int64_t n, float *ra, float *dec, float *red) {
+    return guarded(c, [&] {
+        if (n < 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "negative count");
+        CosmoTable t = cosmo_table(c, cp);
+        if (n == 0) return;
+        const float *a = stage_in(c, mem, x, n, "sky_a"), *b = stage_in(c, mem, y, n, "sky_b");
+        const float *d = stage_in(c, mem, z, n, "sky_c");
+        float *o1 = stage_out(c, mem, ra, n, "sky_x"), *o2 = stage_out(c, mem, dec, n, "sky_y");
+        float *o3 = stage_out(c, mem, red, n, "sky_z");
+        int *e = cosmo_err(c);
+        {
+            ProfScope ps(c, "cartesian_to_sky", 24.0 * n);
+            hipchk(c, launch_cart_to_sky(t, cp->h, a, b, d, o1, o2, o3, n, e, c->stream), "cartesian_to_sky");
+        }
+        copy_out(c, mem, ra, o1, n);
+        copy_out(c, mem, dec, o2, n);
+        copy_out(c, mem, red, o3, n);
+        cosmo_check(c, e, "cartesian_to_sky");
     });
 }
 

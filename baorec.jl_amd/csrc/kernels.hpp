@@ -201,6 +201,37 @@ struct MGLevel {
     const float *xv;   // x[n0] | y[n1] | z[n2] cell centres of this level
 };
 
+// ------------------------------------------------------------- cosmology
+
+constexpr double kSpeedOfLightKms = 299792.458;   // src/cosmo.jl:12
+
+// Derived densities of BAOrec.Cosmology (computed on the host in the Julia
+// constructor's precision) and h*100 (Float32 product, as H() forms it).
+struct CosmoArgs {
+    double omega_nu0, omega_g0, omega_b0, omega_c0, omega_k0, omega_l0;
+    double w0, wa, h100;
+};
+// Comoving-distance table: num knots zk (increasing, uniform) and distances
+// rk (Mpc, increasing).  Interval lookup: direct index on the uniform z knots;
+// for r, guide[j] = last knot with rk <= r0 + j / dr_inv (nguide entries).
+struct CosmoTable {
+    const double *zk, *rk;
+    int num;
+    double z0, dz_inv;
+    const int *guide;
+    int nguide;
+    double r0, dr_inv;
+};
+hipError_t launch_cosmo_guide(const CosmoTable &t, int *guide, hipStream_t s);
+hipError_t launch_cosmo_table(const CosmoArgs &c, const double *zk, double *seg, double *rk, int num,
+                              hipStream_t s);
+hipError_t launch_comoving_distance(const CosmoTable &t, const double *z, double *r, int64_t n, int *err,
+                                    hipStream_t s);
+hipError_t launch_sky_to_cart(const CosmoTable &t, float hh, const float *ra, const float *dec, const float *red,
+                              float *x, float *y, float *z, int64_t n, int *err, hipStream_t s);
+hipError_t launch_cart_to_sky(const CosmoTable &t, float h, const float *x, const float *y, const float *z,
+                              float *ra, float *dec, float *red, int64_t n, int *err, hipStream_t s);
+
 // Levels [0, nl) of a V-cycle (fmg = 0) or of the whole FMG (fmg = 1) run by
 // one workgroup (the launch-latency-bound coarse levels).  cur[l] says which
 // of A[l] / B[l] holds v (damped Jacobi ping-pongs; an even number of sweeps

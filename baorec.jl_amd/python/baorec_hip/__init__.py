@@ -8,10 +8,13 @@ from .api import (  # noqa: F401
     BaorecError,
     BoundsError,
     Context,
+    Cosmology,
     IterativeRecon,
     MultigridRecon,
     ResultCache,
+    cartesian_to_sky,
     cic_,
+    comoving_distance,
     default_context,
     fmg,
     irfft3d,
@@ -29,6 +32,7 @@ from .api import (  # noqa: F401
     run_,
     setup_box,
     setup_overdensity_,
+    sky_to_cartesian,
     smooth_,
     vcycle_,
     x_vec,

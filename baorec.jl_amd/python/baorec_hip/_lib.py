@@ -60,6 +60,23 @@ class Params(ctypes.Structure):
     ]
 
 
+class Cosmology(ctypes.Structure):
+    """struct baorec_cosmology (include/baorec.h)."""
+    _fields_ = [
+        ("omega_b", ctypes.c_float),
+        ("omega_c", ctypes.c_float),
+        ("h", ctypes.c_float),
+        ("Neff", ctypes.c_float),
+        ("Omega_k0", ctypes.c_float),
+        ("T_cmb", ctypes.c_float),
+        ("w0", ctypes.c_float),
+        ("wa", ctypes.c_float),
+        ("z_tab_min", ctypes.c_double),
+        ("z_tab_max", ctypes.c_double),
+        ("z_tab_num", ctypes.c_int32),
+    ]
+
+
 P = ctypes.c_void_p
 I32P = ctypes.POINTER(ctypes.c_int32)
 FP = ctypes.c_void_p  # float* (host or device address)
@@ -97,6 +114,11 @@ SIGNATURES = {
     "baorec_mg_prolong": (INT, [P, I32P, INT, FP, FP, INT]),
     "baorec_mg_vcycle": (INT, [P, PP, I32P, INT, FP, FP]),
     "baorec_mg_fmg": (INT, [P, PP, I32P, INT, FP, FP]),
+    "baorec_cosmology_default": (None, [ctypes.POINTER(Cosmology)]),
+    "baorec_cosmology_desi": (None, [ctypes.POINTER(Cosmology)]),
+    "baorec_comoving_distance": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, I64, FP]),
+    "baorec_sky_to_cartesian": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
+    "baorec_cartesian_to_sky": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
     "baorec_run_batch": (INT, [P, PP, I32P, INT, INT, FP, FP, FP, FP, FP, FP, FP, FP]),
     "baorec_profile": (INT, [P, INT]),
     "baorec_profile_report": (I64, [P, ctypes.c_char_p, I64]),

@@ -505,3 +505,73 @@ def fmg(f, recon: MultigridRecon, v=None):
     recon.final_residual = p.final_residual
     recon.final_change = p.final_change
     return v
+
+
+# ------------------------------------------------------------------ sky <-> Cartesian
+
+@dataclass
+class Cosmology:
+    """BAOrec.Cosmology (src/cosmo.jl:22-72): the Float32 parameters; the
+    library derives the densities as the Julia constructor does.  The
+    comoving-distance table spans [z_tab_min, z_tab_max] with z_tab_num knots."""
+    omega_b: float = 0.0225
+    omega_c: float = 0.12
+    h: float = 0.67
+    Neff: float = 3.044
+    Omega_k0: float = 0.0
+    T_cmb: float = 2.725
+    w0: float = -1.0
+    wa: float = 0.0
+    z_tab_min: float = 0.0
+    z_tab_max: float = 3.0
+    z_tab_num: int = 100000
+
+    @classmethod
+    def desi(cls, **kw):
+        """DESICosmology() (src/cosmo.jl:60-62)."""
+        c = cls(omega_b=0.02237, omega_c=0.1200, h=0.6736, Neff=float(F32(F32(2.0328) + F32(1.0))))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+    def struct(self) -> L.Cosmology:
+        s = L.Cosmology()
+        for name, _ in L.Cosmology._fields_:
+            setattr(s, name, getattr(self, name))
+        return s
+
+
+def comoving_distance(z, cosmo: Cosmology, ctx: Optional[Context] = None):
+    """comoving_distance_interp(cosmo)(z) (src/cosmo.jl:86-93), Mpc, FP64."""
+    ctx = ctx or default_context()
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    r = np.empty_like(z)
+    c = cosmo.struct()
+    ctx.check(ctx.lib.baorec_comoving_distance(ctx.handle, ctypes.byref(c), L.MEM_HOST, _ptr(z), z.size, _ptr(r)))
+    return r
+
+
+def _sky(name, a, b, c3, cosmo, ctx):
+    ctx = ctx or default_context()
+    a, b, c3 = (_f32(v) for v in (a, b, c3))
+    n = len(a)
+    if not (len(b) == n and len(c3) == n):
+        raise ValueError("array lengths differ")
+    mem = _mem(a, b, c3)
+    out = [np.empty(n, F32) if mem == L.MEM_HOST else _dev_empty((n,), a) for _ in range(3)]
+    s = cosmo.struct()
+    ctx.check(getattr(ctx.lib, name)(ctx.handle, ctypes.byref(s), mem, _ptr(a), _ptr(b), _ptr(c3), n,
+                                     _ptr(out[0]), _ptr(out[1]), _ptr(out[2])))
+    return tuple(out)
+
+
+def sky_to_cartesian(ra, dec, red, cosmo: Cosmology, ctx: Optional[Context] = None):
+    """sky_to_cartesian of the reference's drivers (examples/lightcone.jl:30-48):
+    RA, Dec in degrees and redshift -> (x, y, z) in Mpc/h."""
+    return _sky("baorec_sky_to_cartesian", ra, dec, red, cosmo, ctx)
+
+
+def cartesian_to_sky(x, y, z, cosmo: Cosmology, ctx: Optional[Context] = None):
+    """cartesian_to_sky (examples/lightcone.jl:50-80): (RA, Dec, redshift), with
+    RA = rem(lon - 360, 360) as the driver computes it."""
+    return _sky("baorec_cartesian_to_sky", x, y, z, cosmo, ctx)

@@ -254,6 +254,80 @@ def cpu_baseline(args, data, rand, cfg):
                         f"read-out {t_read:.1f} s"))
 
 
+def sky_bench(args):
+    """Sky <-> Cartesian conversion of the 22M galaxies of the lightcone
+    workload (2M tracers + 20M randoms): one step = sky_to_cartesian then
+    cartesian_to_sky (the drivers' step before and after the reconstruction),
+    device-resident arrays."""
+    import baorec_hip as bh
+    from baorec_hip import _lib as L
+    from baorec_hip.devmem import DeviceArray, hip
+    n = args.ndata + args.nrand
+    rng = np.random.default_rng(4)
+    ra = rng.uniform(0, 90, n).astype(np.float32)
+    dec = np.degrees(np.arcsin(rng.uniform(0, 1, n))).astype(np.float32)
+    red = rng.uniform(0.3, 0.8, n).astype(np.float32)
+    ctx = bh.Context(0)
+    lib = ctx.lib
+    cos = bh.Cosmology().struct()
+    din = [DeviceArray.from_numpy(a) for a in (ra, dec, red)]
+    mid = [DeviceArray((n,)) for _ in range(3)]
+    out = [DeviceArray((n,)) for _ in range(3)]
+    vp = lambda a: ctypes.c_void_p(a.ptr)  # noqa: E731
+
+    def step():
+        ctx.check(lib.baorec_sky_to_cartesian(ctx.handle, ctypes.byref(cos), L.MEM_DEVICE, *(vp(a) for a in din),
+                                              n, *(vp(a) for a in mid)))
+        ctx.check(lib.baorec_cartesian_to_sky(ctx.handle, ctypes.byref(cos), L.MEM_DEVICE, *(vp(a) for a in mid),
+                                              n, *(vp(a) for a in out)))
+
+    h = hip()
+    for _ in range(args.warmup):
+        step()
+    h.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    h.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    lib.baorec_profile(ctx.handle, 1)
+    step()
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib.baorec_profile_report(ctx.handle, buf, len(buf))
+    prof = json.loads(buf.value.decode())
+    name, st = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    achieved = st["bytes"] / st["count"] / (st["ms"] * 1e-3 / st["count"]) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import baorec_oracle as orc
+        m = 2_000_000
+        o = orc.Cosmology()
+        t0 = time.perf_counter()
+        table = orc.distance_table(orc.Cosmology(z_tab_num=2001))
+        t1 = time.perf_counter()
+        xyz = orc.sky_to_cartesian(ra[:m], dec[:m], red[:m], o, table)
+        orc.cartesian_to_sky(*xyz, o, table)
+        t2 = time.perf_counter()
+        cpu = {"value": round(m / (t2 - t1), 1), "unit": "galaxies/s", "cores": 1, "kind": "port",
+               "sample": f"numpy oracle on {m} galaxies ({t2 - t1:.1f} s; table of 2001 knots by quadrature "
+                         f"in {t1 - t0:.1f} s not counted)"}
+    line = {
+        "metric": "galaxies/s, sky->Cartesian->sky (22M galaxies)", "value": round(n * args.steps / dt, 1),
+        "unit": "galaxies/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32/f64 table", "data": "synthetic",
+        "config": {"workload": "sky22m", "n": n, "z_tab_num": 100000},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                     "avg_launch_ms": round(st["ms"] / st["count"], 4)},
+        "cpu_baseline": cpu,
+        "kernels": {k: {"count": v["count"], "ms": round(v["ms"], 3),
+                        "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """Plain device-to-device copies of a known size: the reference point for
     the HBM byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
@@ -288,6 +362,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
+    if args.workload == "sky22m":
+        sky_bench(args)
+        return
     data, rand, cfg = make_workload(args)
     g = gpu_bench(args, data, rand, cfg)
     prof = g["prof"]

@@ -261,6 +261,45 @@ int baorec_mg_vcycle(baorec_ctx *ctx, const baorec_params *p, const int32_t grid
 int baorec_mg_fmg(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3], int mem,
                   float *v, const float *f);
 
+/* ------------------------------------------------ sky <-> Cartesian */
+
+/*
+ * Parameters of BAOrec.Cosmology (src/cosmo.jl:22-72): the Float32 fields of
+ * the Julia struct; the library derives the densities exactly as the Julia
+ * constructor does (photons from T_cmb, neutrinos from Neff, Omega_Lambda
+ * closing the budget).  The comoving-distance table of
+ * comoving_distance_interp / redshift_interp (src/cosmo.jl:86-103) has
+ * z_tab_num knots spanning [z_tab_min, z_tab_max] and is built on the GPU
+ * (cached per context and parameter set).
+ */
+typedef struct baorec_cosmology {
+    float omega_b, omega_c, h, Neff, Omega_k0, T_cmb, w0, wa;
+    double z_tab_min, z_tab_max;
+    int32_t z_tab_num;
+} baorec_cosmology;
+
+/* Cosmology() and DESICosmology() defaults (src/cosmo.jl:22-62), z_tab_max 3. */
+void baorec_cosmology_default(baorec_cosmology *c);
+void baorec_cosmology_desi(baorec_cosmology *c);
+
+/* comoving_distance_interp(c)(z) (src/cosmo.jl:86-93): linear interpolation
+ * in the table, Mpc, FP64 in and out.  A z outside the table returns
+ * BAOREC_ERR_OUT_OF_BOX (the reference's BoundsError). */
+int baorec_comoving_distance(baorec_ctx *ctx, const baorec_cosmology *c, int mem,
+                             const double *z, int64_t n, double *r);
+
+/* sky_to_cartesian(ra, dec, z, cosmo) of the reference's drivers
+ * (examples/lightcone.jl:30-48): degrees and redshift in, Mpc/h out. */
+int baorec_sky_to_cartesian(baorec_ctx *ctx, const baorec_cosmology *c, int mem,
+                            const float *ra, const float *dec, const float *red,
+                            int64_t n, float *x, float *y, float *z);
+
+/* cartesian_to_sky(x, y, z, cosmo) (examples/lightcone.jl:50-80): RA is
+ * rem(lon - 360, 360) as the driver computes it (values in (-360, 0]). */
+int baorec_cartesian_to_sky(baorec_ctx *ctx, const baorec_cosmology *c, int mem,
+                            const float *x, const float *y, const float *z,
+                            int64_t n, float *ra, float *dec, float *red);
+
 /* --------------------------------------------------------------- batching */
 
 /*

@@ -498,3 +498,117 @@ def reconstructed_positions(recon, x, y, z, field=None, kind="disp", gpu_formula
     sh = read_shifts(recon, x, y, z, field, kind, gpu_formula)
     dt = sh[0].dtype.type
     return tuple((np.asarray(p, dtype=dt) - s).astype(dt) for p, s in zip((x, y, z), sh))
+
+
+# ------------------------------------------------------------- cosmology
+
+@dataclass
+class Cosmology:
+    """Mirror of ``BAOrec.Cosmology`` (src/cosmo.jl:22-72), Float32 fields."""
+    omega_b: float = 0.0225
+    omega_c: float = 0.12
+    h: float = 0.67
+    Neff: float = 3.044
+    Omega_k0: float = 0.0
+    T_cmb: float = 2.725
+    w0: float = -1.0
+    wa: float = 0.0
+    z_tab_min: float = 0.0
+    z_tab_max: float = 3.0
+    z_tab_num: int = 100000
+
+
+C_KMS = 299792.458   # src/cosmo.jl:12
+
+
+def cosmo_derived(c: Cosmology) -> dict:
+    """Derived densities as the Julia constructor computes them
+    (src/cosmo.jl:36-58): Float32 fields, photon and neutrino densities in
+    Float64 rounded to Float32, Omega_Lambda closing the budget in Float32."""
+    f = np.float32
+    G, hbar, mpc = 6.67408e-11, 6.582119514e-16, 3.085677581491367e22
+    evc2, kb = 1.782661907e-36, 8.6173303e-5
+    h = f(c.h)
+    h2 = f(h * h)
+    rho_g0 = f(3 * 100 ** 2 / (8 * np.pi * G) * hbar ** 3 * (1.0 / (1e-3 * mpc)) ** 2 * (1.0 / evc2)
+               * (C_KMS * 1e3) ** 3)
+    omega_g0 = f((np.pi ** 2 / 15) * (2.725 * kb) ** 4 / float(rho_g0) * (float(f(c.T_cmb)) / 2.725) ** 4
+                 / float(h2))
+    omega_nu0 = f(float(f(c.Neff)) * (7 / 8 * (4 / 11) ** (4 / 3)) * float(omega_g0))
+    omega_b0 = f(f(c.omega_b) / h2)
+    omega_c0 = f(f(c.omega_c) / h2)
+    omega_k0 = f(c.Omega_k0)
+    omega_l0 = f(f(f(f(f(f(1) - omega_k0) - omega_b0) - omega_c0) - omega_nu0) - omega_g0)
+    return dict(omega_nu0=omega_nu0, omega_g0=omega_g0, omega_b0=omega_b0, omega_c0=omega_c0,
+                omega_k0=omega_k0, omega_l0=omega_l0, w0=f(c.w0), wa=f(c.wa), h=h, h100=f(h * f(100)))
+
+
+def inv_hubble(c: Cosmology, z):
+    """1/H(z) (src/cosmo.jl:74-85), FP64 with the Float32 parameters."""
+    d = cosmo_derived(c)
+    z = np.asarray(z, dtype=np.float64)
+    zp = 1.0 + z
+    a = 1.0 / zp
+    with np.errstate(divide="ignore", invalid="ignore"):
+        fde = -3 * (1 + float(d["w0"])) + 3 * float(d["wa"]) * ((a - 1) / np.log(a) - 1)
+        ol = float(d["omega_l0"]) * zp ** (-fde)
+    ol = np.where(zp == 1.0, float(d["omega_l0"]), ol)
+    e = np.sqrt(float(d["omega_nu0"]) * zp ** 4 + float(d["omega_g0"]) * zp ** 4 + float(d["omega_b0"]) * zp ** 3
+                + float(d["omega_c0"]) * zp ** 3 + float(d["omega_k0"]) * zp ** 2 + ol)
+    return 1.0 / (float(d["h100"]) * e)
+
+
+def comoving_distance(c: Cosmology, z) -> np.ndarray:
+    """comoving_distance(c, z) (src/cosmo.jl:85-87): c * quadgk(1/H, 0, z,
+    rtol = 1e-8); scipy's adaptive quadrature at the same relative tolerance."""
+    from scipy.integrate import quad
+    z = np.atleast_1d(np.asarray(z, dtype=np.float64))
+    return np.array([C_KMS * quad(lambda t: float(inv_hubble(c, t)), 0.0, float(zi), epsabs=0.0,
+                                  epsrel=1e-10, limit=200)[0] for zi in z])
+
+
+def distance_table(c: Cosmology):
+    """Knots and distances of comoving_distance_interp (src/cosmo.jl:88-93)."""
+    zk = np.linspace(c.z_tab_min, c.z_tab_max, c.z_tab_num)
+    return zk, comoving_distance(c, zk)
+
+
+def _interp(xk, yk, x):
+    x = np.asarray(x, dtype=np.float64)
+    if np.any(~(x >= xk[0]) | ~(x <= xk[-1])):
+        raise IndexError("outside the interpolation table (reference BoundsError)")
+    return np.interp(x, xk, yk)
+
+
+def sky_to_cartesian(ra, dec, red, c: Cosmology, table=None):
+    """sky_to_cartesian of examples/lightcone.jl:30-48."""
+    f = np.float32
+    zk, rk = table if table is not None else distance_table(c)
+    dist = _interp(zk, rk, np.asarray(red, f).astype(np.float64))
+    hh = f(f(f(c.h) * f(100)) / f(100))
+    pif = f(np.pi)
+    rar = (np.asarray(ra, f) * pif) / f(180)
+    der = (np.asarray(dec, f) * pif) / f(180)
+    cd = np.cos(der.astype(np.float64)).astype(f).astype(np.float64)
+    sd = np.sin(der.astype(np.float64)).astype(f).astype(np.float64)
+    cr = np.cos(rar.astype(np.float64)).astype(f).astype(np.float64)
+    sr = np.sin(rar.astype(np.float64)).astype(f).astype(np.float64)
+    hd = float(hh)
+    return (((dist * cd) * cr * hd).astype(f), ((dist * cd) * sr * hd).astype(f), ((dist * sd) * hd).astype(f))
+
+
+def cartesian_to_sky(x, y, z, c: Cosmology, table=None):
+    """cartesian_to_sky of examples/lightcone.jl:50-80 (RA = rem(lon - 360, 360))."""
+    f = np.float32
+    zk, rk = table if table is not None else distance_table(c)
+    x, y, z = (np.asarray(a, f) for a in (x, y, z))
+    r = np.sqrt((x * x + y * y) + z * z) / f(c.h)
+    red = _interp(rk, zk, r.astype(np.float64)).astype(f)
+    s = np.sqrt(x * x + y * y)
+    pif = f(np.pi)
+    lon = np.arctan2(y.astype(np.float64), x.astype(np.float64)).astype(f)
+    lat = np.arctan2(z.astype(np.float64), s.astype(np.float64)).astype(f)
+    lon = (lon * f(180)) / pif
+    lat = (lat * f(180)) / pif
+    lon = np.fmod(lon - f(360), f(360))
+    return lon, lat, red

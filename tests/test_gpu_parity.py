@@ -505,3 +505,74 @@ def test_full_size_lightcone_mass_and_determinism(ctx):
     rho = np.zeros((n, n, n), F32)
     bh.cic_(rho, *(a.copy() for a in data), rec, wrap=False)
     assert abs(float(np.sum(rho, dtype=np.float64)) / float(np.sum(data[3], dtype=np.float64)) - 1) < 1e-6
+
+
+# ------------------------------------------------------------------ sky <-> Cartesian
+
+def ulp_err(got, ref):
+    """Largest |got - ref| in units of the FP32 spacing at ref."""
+    ref = np.asarray(ref, F32)
+    return float(np.max(np.abs(np.asarray(got, np.float64) - ref.astype(np.float64)) /
+                        np.spacing(np.abs(ref)).astype(np.float64)))
+
+
+@pytest.mark.parametrize("num", [301, 100000])
+def test_comoving_distance_table_matches_quadrature(ctx, num):
+    """The GPU table (FP64 Gauss-Legendre per interval + ordered prefix sum)
+    against adaptive quadrature at the reference's knots, incl. the default
+    100000-knot table."""
+    c = bh.Cosmology(z_tab_num=num)
+    o = orc.Cosmology(z_tab_num=num)
+    zk = np.linspace(0.0, 3.0, num)
+    idx = np.unique(np.concatenate([[0, 1, num - 1], np.random.default_rng(94).integers(0, num, 60)]))
+    got = bh.comoving_distance(zk[idx], c, ctx=ctx)
+    ref = orc.comoving_distance(o, zk[idx])
+    assert got[0] == 0.0
+    np.testing.assert_allclose(got[1:], ref[1:], rtol=1e-9)
+
+
+def test_comoving_distance_einstein_de_sitter_gpu(ctx):
+    h = 0.5
+    c = bh.Cosmology(omega_b=0.0625, omega_c=0.1875, h=h, T_cmb=0.0, z_tab_num=3001)
+    z = np.linspace(0.0, 3.0, 3001)[::100]
+    exact = 2 * orc.C_KMS / (100 * h) * (1 - 1 / np.sqrt(1 + z))
+    np.testing.assert_allclose(bh.comoving_distance(z, c, ctx=ctx), exact, rtol=1e-9, atol=1e-9)
+
+
+def sky_case(n=200000, seed=95):
+    rng = np.random.default_rng(seed)
+    return (rng.uniform(0, 360, n).astype(F32), rng.uniform(-90, 90, n).astype(F32),
+            rng.uniform(0.0, 3.0, n).astype(F32))
+
+
+def test_sky_to_cartesian_matches_oracle(ctx):
+    """Within 1 FP32 ulp of the oracle (same Float32 rounding points; the
+    tables differ by quadrature error ~1e-12)."""
+    c, o = bh.Cosmology(z_tab_num=2001), orc.Cosmology(z_tab_num=2001)
+    table = orc.distance_table(o)
+    ra, dec, red = sky_case()
+    got = bh.sky_to_cartesian(ra, dec, red, c, ctx=ctx)
+    ref = orc.sky_to_cartesian(ra, dec, red, o, table)
+    for g, r in zip(got, ref):
+        assert ulp_err(g, r) <= 1.0
+    assert rms3(got, ref) < 1e-4
+
+
+def test_cartesian_to_sky_matches_oracle(ctx):
+    c, o = bh.Cosmology(z_tab_num=2001), orc.Cosmology(z_tab_num=2001)
+    table = orc.distance_table(o)
+    ra, dec, red = sky_case(seed=96)
+    x, y, z = orc.sky_to_cartesian(ra, dec, red, o, table)
+    got = bh.cartesian_to_sky(x, y, z, c, ctx=ctx)
+    ref = orc.cartesian_to_sky(x, y, z, o, table)
+    for g, r in zip(got, ref):
+        assert np.max(np.abs(g.astype(np.float64) - r)) <= np.max(np.spacing(np.abs(r))) + 1e-12
+    assert np.all(got[0] <= 0) and np.all(got[0] > -360)
+
+
+def test_sky_outside_table_raises(ctx):
+    c = bh.Cosmology(z_tab_num=1001)
+    with pytest.raises(bh.BoundsError):
+        bh.sky_to_cartesian(np.zeros(3, F32), np.zeros(3, F32), np.array([0.5, 3.5, 1.0], F32), c, ctx=ctx)
+    with pytest.raises(bh.BoundsError):
+        bh.cartesian_to_sky(np.array([1e5], F32), np.zeros(1, F32), np.zeros(1, F32), c, ctx=ctx)

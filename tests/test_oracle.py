@@ -384,3 +384,51 @@ def test_planted_sign_error_is_caught(monkeypatch):
     monkeypatch.setattr(orc, "displacement_grids", flipped)
     with pytest.raises(AssertionError):
         test_displacement_plane_wave()
+
+
+# ------------------------------------------------------------------ cosmology / sky conversion
+
+def test_cosmology_radiation_and_neutrino_densities():
+    """Photon density of a T = 2.725 K CMB: omega_gamma = 2.47e-5; neutrinos
+    Neff (7/8)(4/11)^(4/3) times that; the budget closes (src/cosmo.jl:36-58)."""
+    c = orc.Cosmology()
+    d = orc.cosmo_derived(c)
+    h2 = float(np.float32(c.h) ** 2)
+    assert abs(float(d["omega_g0"]) * h2 / 2.4730e-5 - 1) < 2e-3
+    assert abs(float(d["omega_nu0"]) / float(d["omega_g0"]) - 3.044 * 7 / 8 * (4 / 11) ** (4 / 3)) < 1e-6
+    tot = sum(float(d[k]) for k in ("omega_nu0", "omega_g0", "omega_b0", "omega_c0", "omega_k0", "omega_l0"))
+    assert abs(tot - 1) < 1e-6
+
+
+def test_comoving_distance_einstein_de_sitter():
+    """No radiation (T_cmb = 0), omega_b + omega_c = h^2: E = (1+z)^1.5 and
+    chi = 2 c / H0 (1 - 1 / sqrt(1 + z))."""
+    h = 0.5
+    c = orc.Cosmology(omega_b=0.0625, omega_c=0.1875, h=h, T_cmb=0.0)
+    d = orc.cosmo_derived(c)
+    assert abs(float(d["omega_l0"])) < 1e-6
+    z = np.array([0.1, 0.5, 1.0, 2.0, 3.0])
+    exact = 2 * orc.C_KMS / (100 * h) * (1 - 1 / np.sqrt(1 + z))
+    np.testing.assert_allclose(orc.comoving_distance(c, z), exact, rtol=2e-6)
+
+
+def test_sky_cartesian_round_trip_and_geometry():
+    """|pos| = chi(z) h; cartesian_to_sky(sky_to_cartesian(.)) returns Dec,
+    z and RA as rem(RA - 360, 360) (examples/lightcone.jl:30-80)."""
+    c = orc.Cosmology(z_tab_num=401)
+    table = orc.distance_table(c)
+    rng = np.random.default_rng(31)
+    ra = rng.uniform(0, 360, 2000).astype(F32)
+    dec = rng.uniform(-89, 89, 2000).astype(F32)
+    red = rng.uniform(0.05, 2.9, 2000).astype(F32)
+    x, y, z = orc.sky_to_cartesian(ra, dec, red, c, table)
+    r = np.sqrt(x.astype(F64) ** 2 + y.astype(F64) ** 2 + z.astype(F64) ** 2)
+    np.testing.assert_allclose(r, np.interp(red.astype(F64), *table) * float(np.float32(c.h)), rtol=1e-6)
+    ra2, dec2, red2 = orc.cartesian_to_sky(x, y, z, c, table)
+    assert np.all(ra2 <= 0) and np.all(ra2 > -360)
+    lon = np.where(ra > 180, ra - 360, ra)          # atan2 range
+    np.testing.assert_allclose(ra2, np.fmod(lon - 360, 360), atol=2e-3)
+    np.testing.assert_allclose(dec2, dec, atol=2e-3)
+    np.testing.assert_allclose(red2, red, atol=2e-5)
+    with pytest.raises(IndexError):
+        orc.sky_to_cartesian(ra[:1], dec[:1], np.array([3.5], F32), c, table)
