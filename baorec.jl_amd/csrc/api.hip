@@ -603,12 +603,10 @@ MGPyr mg_build(baorec_ctx *c, const Grid &G0, const baorec_params *p, Slot *slot
                const float *f0, int fp64_max) {
     MGPyr P;
     Grid g = G0;
-    int l = 0;
     while (true) {
         P.g.push_back(g);
         if (!mg_recurse(g)) break;
         g = Grid{g.nx / 2, g.ny / 2, g.nz / 2, 0};
-        l++;
     }
     const int nl = (int)P.g.size();
     size_t xvtot = 0;

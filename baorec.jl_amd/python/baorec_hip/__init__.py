@@ -21,6 +21,7 @@ from .api import (  # noqa: F401
     jacobi_,
     k_vec,
     prolong_,
+    read_catalog,
     read_cic_,
     read_shifts,
     reconstructed_overdensity_,
@@ -35,6 +36,7 @@ from .api import (  # noqa: F401
     sky_to_cartesian,
     smooth_,
     vcycle_,
+    write_npy,
     x_vec,
 )
 

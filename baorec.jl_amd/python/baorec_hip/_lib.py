@@ -119,6 +119,11 @@ SIGNATURES = {
     "baorec_comoving_distance": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, I64, FP]),
     "baorec_sky_to_cartesian": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
     "baorec_cartesian_to_sky": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
+    "baorec_catalog_rows": (INT, [ctypes.c_char_p, ctypes.POINTER(I64), I32P]),
+    "baorec_catalog_read": (INT, [ctypes.c_char_p, ctypes.c_int32, I32P, ctypes.POINTER(FP), I64,
+                                  ctypes.POINTER(I64)]),
+    "baorec_write_npy": (INT, [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(FP), I64]),
+    "baorec_io_last_error": (ctypes.c_char_p, []),
     "baorec_run_batch": (INT, [P, PP, I32P, INT, INT, FP, FP, FP, FP, FP, FP, FP, FP]),
     "baorec_profile": (INT, [P, INT]),
     "baorec_profile_report": (I64, [P, ctypes.c_char_p, I64]),

@@ -19,6 +19,7 @@ Every call runs on the GPU; errors raise BaorecError.
 from __future__ import annotations
 
 import ctypes
+import os
 import threading
 from dataclasses import dataclass, field as dfield
 from typing import Optional, Sequence
@@ -575,3 +576,39 @@ def cartesian_to_sky(x, y, z, cosmo: Cosmology, ctx: Optional[Context] = None):
     """cartesian_to_sky (examples/lightcone.jl:50-80): (RA, Dec, redshift), with
     RA = rem(lon - 360, 360) as the driver computes it."""
     return _sky("baorec_cartesian_to_sky", x, y, z, cosmo, ctx)
+
+
+# ------------------------------------------------------------------ catalog I/O
+
+def _io_check(lib, rc):
+    if rc != L.OK:
+        raise BaorecError(rc, lib.baorec_io_last_error().decode())
+
+
+def read_catalog(path, columns=(0, 1, 2, 3, 4)):
+    """An ASCII catalog as the reference's drivers read it (CSV.File(path;
+    delim=" ", types=Float32), test/benchmarks_cpu.jl:22-23): the selected
+    0-based columns as float32 arrays."""
+    lib = L.load()
+    bpath = os.fsencode(path)
+    n = ctypes.c_int64()
+    _io_check(lib, lib.baorec_catalog_rows(bpath, ctypes.byref(n), None))
+    cols = [int(c) for c in columns]
+    out = [np.empty(n.value, F32) for _ in cols]
+    ptrs = (ctypes.c_void_p * len(cols))(*[o.ctypes.data for o in out])
+    sel = (ctypes.c_int32 * len(cols))(*cols)
+    got = ctypes.c_int64()
+    _io_check(lib, lib.baorec_catalog_read(bpath, len(cols), sel, ptrs, n.value, ctypes.byref(got)))
+    return tuple(o[:got.value] for o in out)
+
+
+def write_npy(path, *columns):
+    """npzwrite(path, hcat(columns...)) of the drivers (examples/lightcone.jl:
+    125-127): an (n, ncols) float32 .npy in Fortran order."""
+    lib = L.load()
+    cols = [np.ascontiguousarray(c, dtype=F32) for c in columns]
+    n = len(cols[0])
+    if any(len(c) != n for c in cols):
+        raise ValueError("column lengths differ")
+    ptrs = (ctypes.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+    _io_check(lib, lib.baorec_write_npy(os.fsencode(path), len(cols), ptrs, n))

@@ -328,6 +328,56 @@ def sky_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def catalog_bench(args):
+    """Catalog I/O of the drivers (host code): parse a 2M-row ASCII catalog
+    in the reference's format (ra dec z w nz) and write the (n, 4) .npy
+    output.  The file repeats a 200k-row block of distinct galaxies ten times."""
+    import tempfile
+    import baorec_hip as bh
+    rng = np.random.default_rng(5)
+    m, rep = 200_000, 10
+    cols = [rng.uniform(0, 360, m), np.degrees(np.arcsin(rng.uniform(-1, 1, m))), rng.uniform(0.6, 1.1, m),
+            np.ones(m), 10 ** rng.uniform(-5, -3, m)]
+    cols = [c.astype(np.float32) for c in cols]
+    block = "\n".join(" ".join(f"{c[i]:.8g}" for c in cols) for i in range(m)) + "\n"
+    d = tempfile.mkdtemp(dir=os.environ.get("TMPDIR", "/tmp"))
+    path = os.path.join(d, "catalog.txt")
+    with open(path, "w") as fh:
+        for _ in range(rep):
+            fh.write(block)
+    size = os.path.getsize(path)
+    bh.read_catalog(path)   # warm the page cache
+    times = []
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        got = bh.read_catalog(path)
+        times.append(time.perf_counter() - t0)
+    t_read = min(times)
+    t0 = time.perf_counter()
+    bh.write_npy(os.path.join(d, "out.npy"), *got[:4])
+    t_write = time.perf_counter() - t0
+    cpu = None
+    if not args.no_cpu:
+        sample = os.path.join(d, "sample.txt")
+        with open(sample, "w") as fh:
+            fh.write(block)
+        t0 = time.perf_counter()
+        np.loadtxt(sample, dtype=np.float32)
+        t1 = time.perf_counter()
+        cpu = {"value": round(m / (t1 - t0), 1), "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"numpy.loadtxt (the oracle's text reader) on the {m}-row block: {t1 - t0:.2f} s"}
+    n = m * rep
+    line = {"metric": "rows/s, ASCII catalog parse (2M galaxies, 5 columns)", "value": round(n / t_read, 1),
+            "unit": "rows/s", "n_gpus": 0, "steps": len(times), "warmup": 1, "ms_per_step": round(t_read * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "catalog2m", "rows": n, "bytes": size, "threads": "min(16, cores)"},
+            "roofline": None, "parse_MBps": round(size / t_read / 1e6, 1),
+            "npy_write": {"rows": n, "ms": round(t_write * 1e3, 2)}, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    import shutil
+    shutil.rmtree(d, ignore_errors=True)
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """Plain device-to-device copies of a known size: the reference point for
     the HBM byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
@@ -364,6 +414,9 @@ def main():
 
     if args.workload == "sky22m":
         sky_bench(args)
+        return
+    if args.workload == "catalog2m":
+        catalog_bench(args)
         return
     data, rand, cfg = make_workload(args)
     g = gpu_bench(args, data, rand, cfg)

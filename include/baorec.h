@@ -300,6 +300,34 @@ int baorec_cartesian_to_sky(baorec_ctx *ctx, const baorec_cosmology *c, int mem,
                             const float *x, const float *y, const float *z,
                             int64_t n, float *ra, float *dec, float *red);
 
+/* ---------------------------------------------------------- catalog I/O */
+/* Host-only calls (no context, no GPU); a failure's message is in
+ * baorec_io_last_error() (per thread). */
+
+/*
+ * ASCII catalogs as the reference's drivers read them with CSV.File(path;
+ * delim = " ", types = Float32) (test/benchmarks_cpu.jl:22-23,
+ * examples/lightcone.jl:20-21): one galaxy per line, whitespace-separated
+ * numeric columns, every value parsed to the nearest Float32.  Blank lines
+ * and lines starting with '#' are skipped.
+ * baorec_catalog_rows counts the data lines (and the columns of the first).
+ * baorec_catalog_read parses columns cols[0..nsel-1] (0-based) into the host
+ * arrays out[0..nsel-1] (capacity rows each); a short or malformed line is
+ * BAOREC_ERR_INVALID_ARGUMENT naming the line.
+ */
+int baorec_catalog_rows(const char *path, int64_t *nrows, int32_t *ncols);
+int baorec_catalog_read(const char *path, int32_t nsel, const int32_t *cols,
+                        float *const *out, int64_t capacity, int64_t *nrows);
+
+/* npzwrite(path, hcat(cols...)) of the drivers (examples/lightcone.jl:
+ * 125-127): ncols float32 columns of n rows as an .npy array of shape
+ * (n, ncols), Fortran order (columns contiguous, as NPZ.jl writes a Julia
+ * Matrix{Float32}). */
+int baorec_write_npy(const char *path, int32_t ncols, const float *const *cols, int64_t n);
+
+/* Message of the last failed catalog call on this thread. */
+const char *baorec_io_last_error(void);
+
 /* --------------------------------------------------------------- batching */
 
 /*

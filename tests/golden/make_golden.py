@@ -14,6 +14,12 @@ intermediate is exactly representable in FP32:
 * restrict.json 27-point full weighting (src/multigrid.jl:520-584) and
                 trilinear prolongation (src/multigrid.jl:398-453) of small
                 integer arrays, evaluated exactly.
+* catalog.txt / catalog.json
+                an ASCII catalog in the drivers' format (5 whitespace-separated
+                columns, test/benchmarks_cpu.jl:22-23) with awkward decimal
+                spellings, and the Float32 each value must parse to: the
+                nearest float32 of the exact decimal (ties to even), found by
+                rational arithmetic, not by a float parser.
 
 Run:  python tests/golden/make_golden.py
 """
@@ -113,6 +119,42 @@ def prolong_exact(v, nc):
     return out
 
 
+def nearest_f32(text):
+    """Bits of the float32 nearest to the decimal `text` (ties to even)."""
+    import struct
+    import numpy as np
+    t = text.lstrip("+")
+    q = Fr(t)
+    c = np.float32(float(q))
+    best = None
+    with np.errstate(over="ignore"):
+        cands = (np.nextafter(c, np.float32(-np.inf)), c, np.nextafter(c, np.float32(np.inf)))
+    for cand in cands:
+        if not np.isfinite(cand):
+            continue
+        d = abs(Fr(float(cand)) - q)
+        bits = struct.unpack("<I", struct.pack("<f", float(cand)))[0]
+        key = (d, bits & 1)
+        if best is None or key < best[0]:
+            best = (key, bits)
+    if q == 0 and t.startswith("-"):
+        return 0x80000000          # negative zero keeps its sign
+    return best[1]
+
+
+CATALOG_LINES = [
+    "# ra dec z w nz  (comment lines and blank lines are skipped)",
+    "10.5 -3.25 0.85 1 0.0004",
+    "359.99997 -89.999997 0.9999999 1.0000001 3.0517578125e-05",
+    "",
+    "+0.1 0.2 0.3 .5 5.",
+    "1e-3 2E2 7.006492321624085e-46 1.17549435e-38 3.4028234e38",
+    "123.456789012345678 -0.0 0.099999994 0.100000001 0.10000000149011612",
+    "   42\t43   44\t 45 46   ",
+    "0.0000001 16777217 16777219 8388607.5 2.5e-45",
+]
+
+
 def main():
     cases = []
     n, L, b = (8, 8, 8), (8.0, 8.0, 8.0), (0.0, 0.0, 0.0)
@@ -155,5 +197,15 @@ def main():
         }, fh)
 
 
+def write_catalog():
+    lines = [ln.replace("\\t", "\t") for ln in CATALOG_LINES]
+    with open(os.path.join(HERE, "catalog.txt"), "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    rows = [[nearest_f32(tok) for tok in ln.split()] for ln in lines if ln.strip() and not ln.lstrip().startswith("#")]
+    with open(os.path.join(HERE, "catalog.json"), "w") as fh:
+        json.dump({"rows_f32_bits": rows}, fh)
+
+
 if __name__ == "__main__":
     main()
+    write_catalog()
