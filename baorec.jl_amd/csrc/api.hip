@@ -1760,6 +1760,146 @@ int baorec_cartesian_to_sky(baorec_ctx *c, const baorec_cosmology *cp, int mem, 
     });
 }
 
+void baorec_pk_params_default(baorec_pk_params *p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    for (int a = 0; a < 3; a++) p->box_size[a] = 1000.f;
+    p->los[2] = 1.f;
+    p->mas = BAOREC_MAS_TSC;
+    p->interlace = 1;
+    p->kmin = 0.0;
+    p->dk = 0.005;
+    p->nbins = 300;
+}
+
+int baorec_power_multipoles(baorec_ctx *c, const baorec_pk_params *pk, const int32_t grid[3], int mem,
+                            const float *x, const float *y, const float *z, const float *w, int64_t n,
+                            double *k_mean, double *nmodes, double *p0, double *p2, double *p4) {
+    return guarded(c, [&] {
+        if (!pk || !k_mean || !nmodes || !p0 || !p2 || !p4) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "NULL argument");
+        if (pk->nbins < 1 || pk->nbins > 1600 || !(pk->dk > 0.0) || !(pk->kmin >= 0.0))
+            fail(c, BAOREC_ERR_INVALID_ARGUMENT, "need 1 <= nbins <= 1600, dk > 0, kmin >= 0");
+        if (pk->mas != BAOREC_MAS_CIC && pk->mas != BAOREC_MAS_TSC)
+            fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mass-assignment scheme");
+        if (n <= 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "the power spectrum needs at least one particle");
+        const double ln = sqrt((double)pk->los[0] * pk->los[0] + (double)pk->los[1] * pk->los[1] +
+                               (double)pk->los[2] * pk->los[2]);
+        if (!(ln > 0.0)) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "los is zero");
+        Grid G = check_grid(c, grid, true);
+        baorec_params p;
+        baorec_params_default(&p, 1.f, 0.f, 0.f);
+        for (int a = 0; a < 3; a++) {
+            if (!(pk->box_size[a] > 0.f)) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "box_size must be positive");
+            p.box_size[a] = pk->box_size[a];
+            p.box_min[a] = pk->box_min[a];
+        }
+        p.mas = pk->mas;
+        p.mutate_inputs = 0;
+        // positions are only read (no wrap write-back: mutate_inputs = 0)
+        Inputs in{};
+        in.x = const_cast<float *>(stage_in(c, mem, x, n, "in_x"));
+        in.y = const_cast<float *>(stage_in(c, mem, y, n, "in_y"));
+        in.z = const_cast<float *>(stage_in(c, mem, z, n, "in_z"));
+        const float *wd;
+        if (w) {
+            wd = stage_in(c, mem, w, n, "in_w");
+        } else {
+            float *ones = bufT<float>(c, "pk_ones", n);
+            hipchk(c, launch_fill(ones, n, 1.f, c->stream), "fill");
+            wd = ones;
+        }
+        const int ng = pk->interlace ? 2 : 1;
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, &p, 0.0);
+        float *rho = bufT<float>(c, "pk_rho", (size_t)2 * G.R());
+        scatter(c, G, &p, slot, rho, in.x, in.y, in.z, wd, n, 1, 0, "pk grid");
+        if (ng == 2) {
+            // the interlaced grid: box origin moved back by half a cell puts
+            // every particle half a cell further along each axis
+            baorec_params q = p;
+            for (int a = 0; a < 3; a++) {
+                const int na = a == 0 ? G.nx : (a == 1 ? G.ny : G.nz);
+                q.box_min[a] = p.box_min[a] - 0.5f * (p.box_size[a] / (float)na);
+            }
+            Slot *slot2 = slot_buf(c) + 2;
+            slot_from_params(c, slot2, &q, 0.0);
+            scatter(c, G, &q, slot2, rho + G.R(), in.x, in.y, in.z, wd, n, 1, 0, "pk grid 2");
+            check_err_flag(c, slot2);
+        }
+        check_err_flag(c, slot);
+        FFTBufs f = fft_bufs(c, G);
+        x_fwd(c, G, rho, f.cw, ng, nullptr, 0);
+        strided_plain(c, G, f, 1, false, f.cw, ng);
+        strided_plain(c, G, f, 2, false, f.cw, ng);
+        // per-axis phase of half a cell and window factor
+        const int pw = pk->mas == BAOREC_MAS_TSC ? 3 : 2;
+        std::vector<float4> tab((size_t)G.P + G.ny + G.nz);
+        const int n3[3] = {G.nx, G.ny, G.nz};
+        double kf[3];
+        size_t o = 0;
+        for (int a = 0; a < 3; a++) {
+            kf[a] = 2.0 * M_PI / (double)pk->box_size[a];
+            const int cnt = a == 0 ? G.P : n3[a];
+            const double hcell = (double)pk->box_size[a] / n3[a];
+            for (int i = 0; i < cnt; i++) {
+                const int sidx = (a == 0 || i < n3[a] / 2) ? i : i - n3[a];
+                const double arg = 0.5 * sidx * kf[a] * hcell;
+                const double sc = arg == 0.0 ? 1.0 : sin(arg) / arg;
+                tab[o++] = make_float4((float)cos(arg), (float)sin(arg), (float)pow(sc, pw), 0.f);
+            }
+        }
+        float4 *dtab = bufT<float4>(c, "pk_tab", tab.size());
+        hipchk(c, hipMemcpyAsync(dtab, tab.data(), sizeof(float4) * tab.size(), hipMemcpyHostToDevice, c->stream),
+               "H2D");
+        const int nb = pk->nbins;
+        PkArgs a{};
+        a.F1 = f.cw;
+        a.F2 = f.cw + G.C();
+        a.tab = dtab;
+        a.nx = G.nx;
+        a.ny = G.ny;
+        a.nz = G.nz;
+        a.P = G.PP();
+        a.Pv = G.P;
+        for (int i = 0; i < 3; i++) {
+            a.kf[i] = kf[i];
+            a.los[i] = pk->los[i] / ln;
+        }
+        a.kmin = pk->kmin;
+        a.dk = pk->dk;
+        a.nbins = nb;
+        a.interlace = ng == 2;
+        a.part = bufT<double>(c, "pk_part", (size_t)pk_blocks() * 5 * nb);
+        double *dout = bufT<double>(c, "pk_out", (size_t)5 * nb + 2);
+        {
+            ProfScope ps(c, "pk_modes", 8.0 * G.Cdense() * ng);
+            hipchk(c, launch_pk(a, dout, c->stream), "power multipoles");
+        }
+        // FP64 totals sum(w), sum(w^2) for the normalisation and shot noise
+        hipchk(c, hipMemsetAsync(dout + 5 * nb, 0, 2 * sizeof(double), c->stream), "memset");
+        hipchk(c, launch_sum<float>(wd, n, dout + 5 * nb, c->stream), "sum w");
+        hipchk(c, launch_sumsq<float>(wd, n, dout + 5 * nb + 1, c->stream), "sum w2");
+        std::vector<double> h((size_t)5 * nb + 2);
+        hipchk(c, hipMemcpyAsync(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "D2H");
+        sync(c);
+        const double V = (double)pk->box_size[0] * pk->box_size[1] * pk->box_size[2];
+        const double W = h[5 * nb], W2 = h[5 * nb + 1];
+        const double norm = V / (W * W), shot = V * W2 / (W * W);
+        for (int b = 0; b < nb; b++) {
+            const double *q = &h[5 * (size_t)b];
+            nmodes[b] = q[0];
+            if (q[0] > 0.0) {
+                k_mean[b] = q[1] / q[0];
+                p0[b] = norm * q[2] / q[0] - shot;
+                p2[b] = 5.0 * norm * q[3] / q[0];
+                p4[b] = 9.0 * norm * q[4] / q[0];
+            } else {
+                k_mean[b] = p0[b] = p2[b] = p4[b] = NAN;
+            }
+        }
+    });
+}
+
 int baorec_profile(baorec_ctx *c, int enable) {
     if (!c) return BAOREC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(c->device);

@@ -232,6 +232,24 @@ hipError_t launch_sky_to_cart(const CosmoTable &t, float hh, const float *ra, co
 hipError_t launch_cart_to_sky(const CosmoTable &t, float h, const float *x, const float *y, const float *z,
                               float *ra, float *dec, float *red, int64_t n, int *err, hipStream_t s);
 
+// ------------------------------------------------------------- power spectrum
+
+// Multipole estimator over the half spectra F1 (and F2, interlaced grid) of
+// an nx x ny x nz grid (pitch P, Pv = nx/2 + 1 valid columns).  tab holds per
+// axis (x: Pv entries, y: ny, z: nz) cos and sin of k_a H_a / 2 and the
+// window factor sinc(k_a H_a / 2)^p.  part: pk_blocks() * 5 * nbins doubles.
+struct PkArgs {
+    const float2 *F1, *F2;
+    const float4 *tab;
+    int nx, ny, nz, P, Pv;
+    double kf[3], los[3];
+    double kmin, dk;
+    int nbins, interlace;
+    double *part;
+};
+int pk_blocks();
+hipError_t launch_pk(const PkArgs &a, double *out, hipStream_t s);
+
 // Levels [0, nl) of a V-cycle (fmg = 0) or of the whole FMG (fmg = 1) run by
 // one workgroup (the launch-latency-bound coarse levels).  cur[l] says which
 // of A[l] / B[l] holds v (damped Jacobi ping-pongs; an even number of sweeps

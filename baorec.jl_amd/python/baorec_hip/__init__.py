@@ -20,6 +20,7 @@ from .api import (  # noqa: F401
     irfft3d,
     jacobi_,
     k_vec,
+    power_multipoles,
     prolong_,
     read_catalog,
     read_cic_,

@@ -77,6 +77,20 @@ class Cosmology(ctypes.Structure):
     ]
 
 
+class PkParams(ctypes.Structure):
+    """struct baorec_pk_params (include/baorec.h)."""
+    _fields_ = [
+        ("box_size", ctypes.c_float * 3),
+        ("box_min", ctypes.c_float * 3),
+        ("los", ctypes.c_float * 3),
+        ("mas", ctypes.c_int32),
+        ("interlace", ctypes.c_int32),
+        ("kmin", ctypes.c_double),
+        ("dk", ctypes.c_double),
+        ("nbins", ctypes.c_int32),
+    ]
+
+
 P = ctypes.c_void_p
 I32P = ctypes.POINTER(ctypes.c_int32)
 FP = ctypes.c_void_p  # float* (host or device address)
@@ -119,6 +133,9 @@ SIGNATURES = {
     "baorec_comoving_distance": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, I64, FP]),
     "baorec_sky_to_cartesian": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
     "baorec_cartesian_to_sky": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
+    "baorec_pk_params_default": (None, [ctypes.POINTER(PkParams)]),
+    "baorec_power_multipoles": (INT, [P, ctypes.POINTER(PkParams), I32P, INT, FP, FP, FP, FP, I64,
+                                      FP, FP, FP, FP, FP]),
     "baorec_catalog_rows": (INT, [ctypes.c_char_p, ctypes.POINTER(I64), I32P]),
     "baorec_catalog_read": (INT, [ctypes.c_char_p, ctypes.c_int32, I32P, ctypes.POINTER(FP), I64,
                                   ctypes.POINTER(I64)]),

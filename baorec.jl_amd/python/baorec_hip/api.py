@@ -612,3 +612,35 @@ def write_npy(path, *columns):
         raise ValueError("column lengths differ")
     ptrs = (ctypes.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
     _io_check(lib, lib.baorec_write_npy(os.fsencode(path), len(cols), ptrs, n))
+
+
+# ------------------------------------------------------------------ power spectrum
+
+def power_multipoles(x, y, z, w=None, grid=(512, 512, 512), box_size=1000.0, box_min=0.0, los=(0.0, 0.0, 1.0),
+                     mas="tsc", interlace=True, kmin=0.0, dk=0.005, nbins=300, ctx: Optional[Context] = None):
+    """P_0, P_2, P_4 of a periodic box catalog (TSC, interlacing; the
+    configuration of the reference's validation, test_helpers/
+    powspec_auto.conf).  Returns a dict of per-bin arrays k (mean |k| of the
+    modes), nmodes, p0 (shot noise subtracted), p2, p4."""
+    ctx = ctx or default_context()
+    x, y, z = (_f32(a) for a in (x, y, z))
+    w = _f32(w)
+    n = len(x)
+    if not (len(y) == n and len(z) == n and (w is None or len(w) == n)):
+        raise ValueError("array lengths differ")
+    p = L.PkParams()
+    ctx.lib.baorec_pk_params_default(ctypes.byref(p))
+    bs = np.broadcast_to(np.asarray(box_size, np.float64), (3,))
+    bm = np.broadcast_to(np.asarray(box_min, np.float64), (3,))
+    for a in range(3):
+        p.box_size[a] = bs[a]
+        p.box_min[a] = bm[a]
+        p.los[a] = los[a]
+    p.mas = L.MAS[mas]
+    p.interlace = int(bool(interlace))
+    p.kmin, p.dk, p.nbins = float(kmin), float(dk), int(nbins)
+    out = {k: np.empty(nbins) for k in ("k", "nmodes", "p0", "p2", "p4")}
+    ctx.check(ctx.lib.baorec_power_multipoles(ctx.handle, ctypes.byref(p), _grid(grid), _mem(x, y, z, w), _ptr(x),
+                                              _ptr(y), _ptr(z), _ptr(w), n, *(_ptr(out[k]) for k in
+                                                                             ("k", "nmodes", "p0", "p2", "p4"))))
+    return out

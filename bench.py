@@ -378,6 +378,71 @@ def catalog_bench(args):
     shutil.rmtree(d, ignore_errors=True)
 
 
+def pk_bench(args):
+    """Power-spectrum multipoles of a 1M-tracer periodic box at 512^3 with
+    TSC and interlacing, dk = 0.005 (the configuration of the reference's
+    validation runs, test_helpers/powspec_auto.conf); one step = one call
+    with device-resident positions."""
+    import baorec_hip as bh
+    from baorec_hip import _lib as L
+    from baorec_hip import synth
+    from baorec_hip.devmem import DeviceArray, hip
+    data = synth.box_catalog(1_000_000, L=1000.0, mesh=256, seed=1)
+    ctx = bh.Context(0)
+    lib = ctx.lib
+    n = len(data[0])
+    d = [DeviceArray.from_numpy(a) for a in data]
+    p = L.PkParams()
+    lib.baorec_pk_params_default(ctypes.byref(p))
+    nb = p.nbins
+    outs = [np.empty(nb) for _ in range(5)]
+    grid = (ctypes.c_int32 * 3)(512, 512, 512)
+    vp = lambda a: ctypes.c_void_p(a.ptr)  # noqa: E731
+    hp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+
+    def step():
+        ctx.check(lib.baorec_power_multipoles(ctx.handle, ctypes.byref(p), grid, L.MEM_DEVICE, *(vp(a) for a in d),
+                                              n, *(hp(o) for o in outs)))
+
+    h = hip()
+    for _ in range(args.warmup):
+        step()
+    h.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = time.perf_counter() - t0
+    lib.baorec_profile(ctx.handle, 1)
+    step()
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib.baorec_profile_report(ctx.handle, buf, len(buf))
+    prof = json.loads(buf.value.decode())
+    name, st = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    achieved = st["bytes"] / (st["ms"] * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import baorec_oracle as orc
+        t0 = time.perf_counter()
+        orc.power_multipoles(*data, (512, 512, 512), (1000.0,) * 3, dk=0.005, nbins=nb)
+        t1 = time.perf_counter()
+        cpu = {"value": round(1.0 / (t1 - t0), 4), "unit": "spectra/s", "cores": 1, "kind": "port",
+               "sample": f"numpy oracle (TSC by bincount, FP64 rfftn) at the full size: {t1 - t0:.1f} s"}
+    line = {"metric": "power spectra/s (P0, P2, P4; 512^3, 1M tracers, TSC + interlacing)",
+            "value": round(args.steps / dt, 3), "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32 grids, f64 sums", "data": "synthetic",
+            "config": {"workload": "pk512", "grid": [512, 512, 512], "n": n, "dk": 0.005, "nbins": nb},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel_family": name,
+                         "ms": round(st["ms"], 4)},
+            "cpu_baseline": cpu,
+            "kernels": {k: {"count": v["count"], "ms": round(v["ms"], 3),
+                            "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()},
+            "p0_at_0p1": float(np.interp(0.1, outs[0], outs[2]))}
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """Plain device-to-device copies of a known size: the reference point for
     the HBM byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
@@ -417,6 +482,9 @@ def main():
         return
     if args.workload == "catalog2m":
         catalog_bench(args)
+        return
+    if args.workload == "pk512":
+        pk_bench(args)
         return
     data, rand, cfg = make_workload(args)
     g = gpu_bench(args, data, rand, cfg)

@@ -300,6 +300,42 @@ int baorec_cartesian_to_sky(baorec_ctx *ctx, const baorec_cosmology *c, int mem,
                             const float *x, const float *y, const float *z,
                             int64_t n, float *ra, float *dec, float *red);
 
+/* ------------------------------------------------------- power spectrum */
+
+/*
+ * Multipoles P_0, P_2, P_4 of a periodic box catalog, the estimator of the
+ * reference's validation runs (test_helpers/powspec_auto.conf: cubic box,
+ * TSC, grid interlacing, multipoles 0 2 4, linear bins of width 0.005 from
+ * k = 0).  The weights are assigned to the grid (and, with interlacing, to a
+ * second grid holding the particles shifted by half a cell), both are
+ * Fourier transformed, combined with the half-cell phase, divided by the
+ * assignment window prod_a sinc(k_a H_a / 2)^p (p = 2 CIC, 3 TSC), and
+ * |delta_k|^2 L_l(mu) is averaged per |k| bin over the modes of the full
+ * k-space; mu = k.los / |k|.  P_0 has the shot noise V sum(w^2) / sum(w)^2
+ * subtracted.  This is an extension: its reference is a third-party code
+ * (pypowspec) that is not part of BAOrec.jl.
+ */
+typedef struct baorec_pk_params {
+    float box_size[3];   /* periodic box, positions in [box_min, box_min + box_size) */
+    float box_min[3];
+    float los[3];        /* line of sight (normalised by the call), default z */
+    int32_t mas;         /* BAOREC_MAS_CIC or BAOREC_MAS_TSC (default) */
+    int32_t interlace;   /* 1: interlaced grids (default) */
+    double kmin, dk;     /* bin b covers [kmin + b dk, kmin + (b+1) dk) */
+    int32_t nbins;       /* 1 .. 1600 */
+} baorec_pk_params;
+
+/* box 1000^3 at the origin, los z, TSC, interlacing, kmin 0, dk 0.005, 300 bins */
+void baorec_pk_params_default(baorec_pk_params *p);
+
+/* x, y, z, w (w may be NULL: unit weights) in `mem`; the bin outputs (nbins
+ * doubles each: mean |k| of the modes, number of modes, P_0, P_2, P_4 in
+ * (Mpc/h)^3) are host arrays.  Empty bins give 0 modes and NaN. */
+int baorec_power_multipoles(baorec_ctx *ctx, const baorec_pk_params *pk, const int32_t grid[3],
+                            int mem, const float *x, const float *y, const float *z,
+                            const float *w, int64_t n, double *k_mean, double *nmodes,
+                            double *p0, double *p2, double *p4);
+
 /* ---------------------------------------------------------- catalog I/O */
 /* Host-only calls (no context, no GPU); a failure's message is in
  * baorec_io_last_error() (per thread). */

@@ -432,3 +432,68 @@ def test_sky_cartesian_round_trip_and_geometry():
     np.testing.assert_allclose(red2, red, atol=2e-5)
     with pytest.raises(IndexError):
         orc.sky_to_cartesian(ra[:1], dec[:1], np.array([3.5], F32), c, table)
+
+
+# ------------------------------------------------------------------ power spectrum
+
+def test_tsc_weights_closed_form():
+    """One particle: three weights per axis summing to 1; on a node
+    (1/8, 3/4, 1/8) around it; mass is conserved with periodic wrap."""
+    g = orc.tsc_grid(np.array([2.0], F32), np.array([0.0], F32), np.array([7.75], F32), np.array([2.0], F32),
+                     (8, 8, 8), (8.0, 8.0, 8.0), (0.0, 0.0, 0.0))
+    assert abs(float(g.sum(dtype=F64)) - 2.0) < 1e-6
+    wx = g.sum(axis=(0, 1)) / 2.0
+    np.testing.assert_allclose(wx[[1, 2, 3]], [0.125, 0.75, 0.125], atol=1e-7)
+    wz = g.sum(axis=(1, 2)) / 2.0      # z = 7.75: d = -0.25 around node 8 == 0 (wrapped)
+    np.testing.assert_allclose(wz[[7, 0, 1]], [0.5 * 0.75 ** 2, 0.75 - 0.0625, 0.5 * 0.25 ** 2], atol=1e-7)
+
+
+def test_power_multipoles_plane_wave_closed_form():
+    """A particle lattice (one per cell, on the nodes) with weights
+    1 + A cos(k0 x), k0 = 2 pi / L along x: the mode amplitude through TSC
+    is T1 = prod_a (3/4 + cos(k_a H)/4) on the grid and T2 = prod_a cos(k_a H/2)
+    on the interlaced grid, so P(k0) = V (A/2)^2 ((T1 + T2)/2)^2 / W(k0)^2 with
+    W = prod sinc^3(k_a H/2).  The bin [0.8, 1.1) k_f holds exactly the six
+    unit modes, two of them with power, all with mu = 0 for a z line of sight:
+    P0 = P/3 (plus the shot noise the estimator subtracts), P2 = -5P/6,
+    P4 = 9P/8."""
+    n, L, A = 16, 160.0, 0.25
+    H = L / n
+    g = np.arange(n, dtype=F32) * F32(H)
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    x, y, z = X.ravel(), Y.ravel(), Z.ravel()
+    k0 = 2 * np.pi / L
+    w = (1 + A * np.cos(k0 * x.astype(F64))).astype(F32)
+    kf = 2 * np.pi / L
+    out = orc.power_multipoles(x, y, z, w, (n, n, n), (L, L, L), kmin=0.8 * kf, dk=0.3 * kf, nbins=1)
+    assert out["nmodes"][0] == 6
+    a = k0 * H
+    T1 = 0.75 + 0.25 * np.cos(a)          # y, z factors are 1 at k_y = k_z = 0
+    T2 = np.cos(a / 2)
+    Wk = (np.sin(a / 2) / (a / 2)) ** 3
+    V = L ** 3
+    Wt = float(np.sum(w, dtype=F64))
+    amp = float(np.sum(w.astype(F64) * np.cos(k0 * x)))  # = N A / 2 for the lattice
+    P = V * (amp / Wt) ** 2 * ((T1 + T2) / 2) ** 2 / Wk ** 2
+    shot = V * float(np.sum(w.astype(F64) ** 2)) / Wt ** 2
+    np.testing.assert_allclose(out["p0"][0] + shot, P / 3, rtol=1e-6)
+    np.testing.assert_allclose(out["p2"][0], -5 * P / 6, rtol=1e-6)
+    np.testing.assert_allclose(out["p4"][0], 9 * P / 8, rtol=1e-6)
+
+
+def test_power_multipoles_poisson_is_shot_noise():
+    """Unclustered points: P0 - shot noise consistent with zero, P2 and P4
+    consistent with zero (5 sigma of the Gaussian mode scatter), below
+    0.7 k_Nyquist (beyond it the window correction of aliased shot noise is
+    not flat)."""
+    rng = np.random.default_rng(43)
+    n, L, N = 32, 500.0, 40000
+    x, y, z = (rng.uniform(0, L, N).astype(F32) for _ in range(3))
+    out = orc.power_multipoles(x, y, z, None, (n, n, n), (L, L, L), dk=0.05, nbins=6)
+    shot = L ** 3 / N
+    sig = shot * np.sqrt(1.0 / out["nmodes"])
+    ok = (out["nmodes"] > 20) & (out["k"] < 0.7 * np.pi * n / L)
+    assert ok.sum() >= 3
+    assert np.all(np.abs(out["p0"][ok]) < 5 * sig[ok] + 0.03 * shot)
+    assert np.all(np.abs(out["p2"][ok]) < 5 * np.sqrt(5) * sig[ok] + 0.05 * shot)
+    assert np.all(np.abs(out["p4"][ok]) < 5 * 3 * sig[ok] + 0.05 * shot)
