@@ -234,6 +234,23 @@ __global__ void k_slot_reset(Slot *s, double count_r) {
     s->err = 0;
 }
 
+// n slots with the same box (the catalogs of a batch)
+__global__ void k_slots_set(Slot *s, int n, float b0, float b1, float b2, float m0, float m1, float m2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Slot &t = s[i];
+    t.box_size[0] = b0;
+    t.box_size[1] = b1;
+    t.box_size[2] = b2;
+    t.box_min[0] = m0;
+    t.box_min[1] = m1;
+    t.box_min[2] = m2;
+    t.sum[0] = 0.0;
+    t.sum[1] = 0.0;
+    t.count_r = 0.0;
+    t.err = 0;
+}
+
 Slot *slot_buf(baorec_ctx *c) { return bufT<Slot>(c, "slot", 4); }
 
 void slot_from_params(baorec_ctx *c, Slot *s, const baorec_params *p, double count_r) {
@@ -453,42 +470,62 @@ void overdensity(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
     x_inv(c, G, a, rand ? XE_OD_RAND : XE_OD_BOX, 1, "x overdensity");
 }
 
-// n_iter x iterate! (src/iterative.jl:151-211): delta_s -> delta_r
-// c0_ready: cwork[0] already holds R2C(delta_s); keep_spectrum: leave
-// R2C(delta_r) in cwork[0] at the end (for the displacement read-out).
-void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const float *ds,
-                 float *dr, bool c0_ready = false, bool keep_spectrum = false) {
-    FFTBufs f = fft_bufs(c, G);
+// Work spectra of the iterative solve and the displacement read-out for
+// nb catalogs: quantity m of catalog b at base_m + b * C, quantities q
+// complex elements apart.
+struct Work {
+    float2 *C0;   // R2C of the field being iterated / read out
+    float2 *Gb;   // three quantities (z-pass outputs)
+    float2 *Hb;   // three quantities (y-pass outputs)
+    int64_t q;
+};
+
+// Layout of the single-catalog path inside the shared work spectra.
+Work work_single(const FFTBufs &f, const Grid &G) {
     const int64_t C = G.C();
-    float2 *C0 = f.cw, *Gb = f.cw + 2 * C, *Hb = f.cw + 5 * C;
+    return Work{f.cw, f.cw + 2 * C, f.cw + 5 * C, C};
+}
+
+// n_iter x iterate! (src/iterative.jl:151-211): delta_s -> delta_r for nb
+// catalogs (grids R apart, spectra C apart, per-catalog slots).
+// c0_ready: W.C0 already holds R2C(delta_s); keep_spectrum: leave
+// R2C(delta_r) in W.C0 at the end (for the displacement read-out); dr may be
+// NULL when only that spectrum is wanted.
+void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const float *ds,
+                 float *dr, bool c0_ready = false, bool keep_spectrum = false, int nb = 1,
+                 const Work *Wp = nullptr) {
+    FFTBufs f = fft_bufs(c, G);
+    const Work W = Wp ? *Wp : work_single(f, G);
+    float2 *C0 = W.C0;
     if (p->n_iter <= 0) {
-        hipchk(c, hipMemcpyAsync(dr, ds, sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream),
-               "copy");
+        if (dr)
+            hipchk(c, hipMemcpyAsync(dr, ds, sizeof(float) * G.R() * nb, hipMemcpyDeviceToDevice, c->stream),
+                   "copy");
         return;
     }
     const float beta = p->beta;
     for (int it = 1; it <= p->n_iter; it++) {
         const bool last = it == p->n_iter;
         // later iterations get R2C(delta_r) from the previous x pass (fwd_out)
-        if (it == 1 && !c0_ready) x_fwd(c, G, ds, C0, 1, nullptr, 0);
-        strided_plain(c, G, f, 1, false, C0, 1);
+        if (it == 1 && !c0_ready) x_fwd(c, G, ds, C0, nb, nullptr, 0);
+        strided_plain(c, G, f, 1, false, C0, nb);
         if (!p->has_los) {
             StridedArgs z = sargs(G, f, 2, 2, MUL_INVK2);
             z.job[0].in = C0;
             for (int m = 0; m < 3; m++) {
-                z.job[0].out[m] = Gb + m * C;
+                z.job[0].out[m] = W.Gb + m * W.q;
                 z.job[0].pw[m] = m;
             }
             z.job[0].nout = 3;
-            strided(c, G, z, 1, 1, "z iterate");
+            strided(c, G, z, 1, nb, "z iterate");
             // y inverse: G0,G1,G2 -> A, B, C (the six Hessian terms grouped by
             // their power of x along an x line)
             YIterArgs y{};
             for (int m = 0; m < 3; m++) {
-                y.g[m] = Gb + m * C;
-                y.out[m] = Hb + m * C;
+                y.g[m] = W.Gb + m * W.q;
+                y.out[m] = W.Hb + m * W.q;
             }
-            y.cstride = C;
+            y.cstride = G.C();
             y.ktab = f.ktab;
             y.xtab = f.xtab;
             y.P = G.PP();
@@ -504,20 +541,20 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
                 y.fac[q] = fac;
             }
             {
-                ProfScope ps(c, "fft_y_iter", 8.0 * G.Cdense() * 6);
-                hipchk(c, launch_y_iter(y, 1, c->stream), "y iterate");
+                ProfScope ps(c, "fft_y_iter", 8.0 * G.Cdense() * 6 * nb);
+                hipchk(c, launch_y_iter(y, nb, c->stream), "y iterate");
             }
             XInvArgs a = xargs(G, f, slot);
             const int kxp[3] = {2, 1, 0};
             for (int q = 0; q < 3; q++) {
-                a.in[q] = Hb + q * C;
+                a.in[q] = W.Hb + q * W.q;
                 a.kxpow[q] = kxp[q];
             }
             a.nin = 3;
             a.aux = ds;
             a.out = last ? dr : nullptr;
             if (!last || keep_spectrum) a.fwd_out = C0;
-            x_inv(c, G, a, XE_ITER_LOCAL, 1, "x iterate");
+            x_inv(c, G, a, XE_ITER_LOCAL, nb, "x iterate");
         } else {
             StridedArgs z = sargs(G, f, 2, 2, MUL_FIXEDLOS);
             z.los[0] = p->los[0];
@@ -527,8 +564,8 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             z.job[0].out[0] = C0;
             z.job[0].pw[0] = 0;
             z.job[0].nout = 1;
-            strided(c, G, z, 1, 1, "z iterate fixed");
-            strided_plain(c, G, f, 1, true, C0, 1);
+            strided(c, G, z, 1, nb, "z iterate fixed");
+            strided_plain(c, G, f, 1, true, C0, nb);
             XInvArgs a = xargs(G, f, slot);
             a.in[0] = C0;
             a.kxpow[0] = 0;
@@ -539,44 +576,46 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             a.aux = ds;
             a.out = last ? dr : nullptr;
             if (!last || keep_spectrum) a.fwd_out = C0;
-            x_inv(c, G, a, XE_ITER_FIXED, 1, "x iterate fixed");
+            x_inv(c, G, a, XE_ITER_FIXED, nb, "x iterate fixed");
         }
     }
 }
 
-// compute_displacements grids: float4 (Psi_x, Psi_y, Psi_z, 0)
+// compute_displacements grids of nb catalogs: float4 (Psi_x, Psi_y, Psi_z, 0),
+// catalog b at disp + b * R.  The k and x tables come from slot[0] (the
+// catalogs share one box).
 void displacement_grid(baorec_ctx *c, const Grid &G, int kind, Slot *slot, const float *field,
-                       float4 *disp, bool c0_ready = false) {
+                       float4 *disp, bool c0_ready = false, int nb = 1, const Work *Wp = nullptr) {
     FFTBufs f = fft_bufs(c, G);
     tables(c, G, slot, f);
-    const int64_t C = G.C();
-    float2 *C0 = f.cw, *Gb = f.cw + 2 * C, *Hb = f.cw + 5 * C;
-    if (!c0_ready) x_fwd(c, G, field, C0, 1, nullptr, 0);
-    strided_plain(c, G, f, 1, false, C0, 1);
+    const Work W = Wp ? *Wp : work_single(f, G);
+    float2 *C0 = W.C0, *Gb = W.Gb, *Hb = W.Hb;
+    if (!c0_ready) x_fwd(c, G, field, C0, nb, nullptr, 0);
+    strided_plain(c, G, f, 1, false, C0, nb);
     StridedArgs z = sargs(G, f, 2, 2, kind == BAOREC_MULTIGRID ? MUL_I : MUL_I_INVK2);
     z.job[0].in = C0;
     z.job[0].out[0] = Gb;
     z.job[0].pw[0] = 0;
-    z.job[0].out[1] = Gb + C;
+    z.job[0].out[1] = Gb + W.q;
     z.job[0].pw[1] = 1;
     z.job[0].nout = 2;
-    strided(c, G, z, 1, 1, "z displacement");
+    strided(c, G, z, 1, nb, "z displacement");
     StridedArgs y = sargs(G, f, 1, 1, MUL_NONE);
     y.job[0].in = Gb;
     y.job[0].nout = 2;
     y.job[0].out[0] = Hb; y.job[0].pw[0] = 0;
-    y.job[0].out[1] = Hb + C; y.job[0].pw[1] = 1;
-    y.job[1].in = Gb + C;
+    y.job[0].out[1] = Hb + W.q; y.job[0].pw[1] = 1;
+    y.job[1].in = Gb + W.q;
     y.job[1].nout = 1;
-    y.job[1].out[0] = Hb + 2 * C; y.job[1].pw[0] = 0;
-    strided(c, G, y, 2, 1, "y displacement");
+    y.job[1].out[0] = Hb + 2 * W.q; y.job[1].pw[0] = 0;
+    strided(c, G, y, 2, nb, "y displacement");
     XInvArgs a = xargs(G, f, slot);
     a.in[0] = Hb; a.kxpow[0] = 1;
-    a.in[1] = Hb + C; a.kxpow[1] = 0;
-    a.in[2] = Hb + 2 * C; a.kxpow[2] = 0;
+    a.in[1] = Hb + W.q; a.kxpow[1] = 0;
+    a.in[2] = Hb + 2 * W.q; a.kxpow[2] = 0;
     a.nin = 3;
     a.out = reinterpret_cast<float *>(disp);
-    x_inv(c, G, a, XE_DISP, 1, "x displacement");
+    x_inv(c, G, a, XE_DISP, nb, "x displacement");
 }
 
 // ---------------------------------------------------------------- multigrid
@@ -993,6 +1032,181 @@ int guarded(baorec_ctx *c, F &&fn) {
     } catch (const std::bad_alloc &) {
         c->err = "host allocation failed";
         return BAOREC_ERR_OUT_OF_MEMORY;
+    }
+}
+
+// ------------------------------------------------------------- batching
+
+// A batch runs in groups of consecutive catalogs; each group is one launch
+// sequence over [quantity][catalog] buffers.  Grid and spectrum buffers of a
+// group stay within kBatchGroupBytes and its sorted placements within
+// kBatchPlacements (16 B each, uint32 cursors).
+constexpr double kBatchGroupBytes = 16.0 * (1ull << 30);
+constexpr int64_t kBatchPlacements = (int64_t)1 << 29;
+
+struct BatchGroup {
+    int c0, c1;          // catalogs [c0, c1)
+    int nblk;            // sort blocks
+    size_t o64, o32;     // offsets of the group's tables in the packed arrays
+};
+
+// Per catalog: rho, delta_s (FP32), displacement (float4) and seven spectra.
+double batch_bytes_per_catalog(const Grid &G) { return 24.0 * G.R() + 7.0 * 8.0 * G.C(); }
+
+std::vector<BatchGroup> batch_groups(const Grid &G, const baorec_params *p, int ncat, const int64_t *off,
+                                     std::vector<int64_t> &t64, std::vector<int> &t32) {
+    const int per = p->mas == BAOREC_MAS_TSC ? 27 : 8;
+    int maxnb = (int)std::max(1.0, std::floor(kBatchGroupBytes / batch_bytes_per_catalog(G)));
+    maxnb = std::min(maxnb, ncat);
+    const int ng = (ncat + maxnb - 1) / maxnb;
+    maxnb = (ncat + ng - 1) / ng;   // balanced group sizes
+    std::vector<BatchGroup> gs;
+    int c = 0;
+    while (c < ncat) {
+        int e = c + 1;
+        while (e < ncat && e - c < maxnb && (off[e + 1] - off[c]) * per <= kBatchPlacements) e++;
+        BatchGroup g{c, e, 0, t64.size(), t32.size()};
+        std::vector<int64_t> rel(e - c + 1);
+        for (int k = c; k <= e; k++) rel[k - c] = off[k] - off[c];
+        std::vector<int> blk_cat, cat_blk0;
+        std::vector<int64_t> blk_beg;
+        g.nblk = sort_blocks(rel[e - c], rel.data(), e - c, blk_cat, blk_beg, cat_blk0);
+        // int64: relative offsets (nb + 1) | block starts (nblk + 1)
+        t64.insert(t64.end(), rel.begin(), rel.end());
+        t64.insert(t64.end(), blk_beg.begin(), blk_beg.end());
+        // int32: block catalogs (nblk, at least one entry) | first block per catalog (nb + 1)
+        if (blk_cat.empty()) blk_cat.push_back(0);
+        t32.insert(t32.end(), blk_cat.begin(), blk_cat.end());
+        t32.insert(t32.end(), cat_blk0.begin(), cat_blk0.end());
+        gs.push_back(g);
+        c = e;
+    }
+    return gs;
+}
+
+// One group: run!(IterativeRecon, grid, x, y, z, w) (periodic box, no randoms)
+// followed by reconstructed_positions(recon, x, y, z; field = :sum) for each
+// catalog, every step batched over the group's catalogs.
+void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int64_t *off,
+                 const BatchGroup &g, const int64_t *d64, const int *d32, float *x, float *y, float *z,
+                 const float *w, float *ox, float *oy, float *oz, int32_t *err, int maxnb) {
+    const int nb = g.c1 - g.c0;
+    const int64_t base = off[g.c0], ntot = off[g.c1] - base;
+    const int64_t R = G.R(), C = G.C();
+    Slot *slots = bufT<Slot>(c, "bt_slots", maxnb);
+    hipLaunchKernelGGL(k_slots_set, dim3((nb + 63) / 64), dim3(64), 0, c->stream, slots, nb, p->box_size[0],
+                       p->box_size[1], p->box_size[2], p->box_min[0], p->box_min[1], p->box_min[2]);
+    hipchk(c, hipGetLastError(), "slots_set");
+    float *rho = bufT<float>(c, "bt_rho", (size_t)maxnb * R);
+    float *ds = bufT<float>(c, "bt_delta_s", (size_t)maxnb * R);
+    float4 *disp = bufT<float4>(c, "bt_disp", (size_t)maxnb * R);
+    float2 *cw = bufT<float2>(c, "bt_cwork", (size_t)7 * maxnb * C);
+    const Work W{cw, cw + (int64_t)nb * C, cw + (int64_t)4 * nb * C, (int64_t)nb * C};
+    const int64_t *d_off = d64 + g.o64, *d_beg = d_off + nb + 1;
+    const int *d_cat = d32 + g.o32, *d_blk0 = d_cat + std::max(g.nblk, 1);
+
+    // cic! of every catalog (one-sided wrap into the box, src/recon.jl:48-50)
+    ScatterArgs a{};
+    a.x = x + base;
+    a.y = y + base;
+    a.z = z + base;
+    a.w = w + base;
+    a.offsets = d_off;
+    a.ncat = nb;
+    a.ntotal = ntot;
+    a.slot = slots;
+    a.slotStride = 1;
+    a.nx = G.nx;
+    a.ny = G.ny;
+    a.nz = G.nz;
+    a.wrap = 1;
+    a.mutate = p->mutate_inputs;
+    a.tsc = p->mas == BAOREC_MAS_TSC;
+    tiles_for(G, a);
+    const int64_t T = (int64_t)a.ntx * a.nty * a.ntz;
+    a.count = bufT<uint32_t>(c, "bt_count", (size_t)maxnb * T);
+    a.cursor = bufT<uint32_t>(c, "bt_cursor", (size_t)maxnb * T);
+    a.sorted = bufT<float4>(c, "bt_sorted", scatter_capacity(ntot > 0 ? ntot : 1, a.tsc));
+    a.err = err;
+    a.rho = rho;
+    a.rstride = R;
+    a.wmax = bufT<uint32_t>(c, "bt_wmax", maxnb);
+    hipchk(c, hipMemsetAsync(a.wmax, 0, sizeof(uint32_t) * nb, c->stream), "memset");
+    const double pb = (double)ntot * (a.mutate ? 40.0 : 28.0) + 16.0 * ntot * 1.2;
+    if (sort_fits(a)) {
+        uint32_t *H = bufT<uint32_t>(c, "bt_hist", (size_t)std::max(g.nblk, 1) * T);
+        ProfScope ps(c, "scatter_bin", pb);
+        hipchk(c, launch_sort(a, g.nblk, H, d_cat, d_beg, d_blk0, c->stream), "batch sort");
+    } else {
+        ProfScope ps(c, "scatter_bin", pb);
+        hipchk(c, launch_scatter(a, nullptr, c->stream), "batch scatter");
+    }
+    {
+        ProfScope ps(c, "zero", 4.0 * R * nb);
+        hipchk(c, launch_zero(rho, R * nb, c->stream), "zero grids");
+    }
+    {
+        ProfScope ps(c, "scatter_accum", 16.0 * ntot * 1.2 + 4.0 * R * nb);
+        hipchk(c, launch_scatter_accum(a, 2, c->stream), "batch accumulate");
+    }
+
+    // smooth! and the overdensity (src/recon.jl:42-58); per-catalog FP64 sums
+    FFTBufs f = fft_bufs(c, G);
+    tables(c, G, slots, f);
+    x_fwd(c, G, rho, W.C0, nb, &slots[0].sum[0], (int64_t)(sizeof(Slot) / sizeof(double)));
+    strided_plain(c, G, f, 1, false, W.C0, nb);
+    {
+        StridedArgs s = sargs(G, f, 2, 2, MUL_GAUSS);
+        s.R2 = p->smoothing_radius * p->smoothing_radius;
+        s.job[0].in = W.C0;
+        s.job[0].out[0] = W.C0;
+        s.job[0].nout = 1;
+        s.job[0].pw[0] = 0;
+        strided(c, G, s, 1, nb, "z smooth");
+    }
+    strided_plain(c, G, f, 1, true, W.C0, nb);
+    {
+        XInvArgs xa = xargs(G, f, slots);
+        xa.in[0] = W.C0;
+        xa.kxpow[0] = 0;
+        xa.nin = 1;
+        xa.bias = p->bias;
+        xa.ran_min = p->ran_min;
+        xa.out = ds;
+        xa.fwd_out = W.C0;   // R2C(delta_s) for the first iteration
+        x_inv(c, G, xa, XE_OD_BOX, nb, "x overdensity");
+    }
+    // iterations; only the spectrum of delta_r is kept
+    iterate_all(c, G, p, slots, ds, nullptr, true, true, nb, &W);
+    displacement_grid(c, G, BAOREC_ITERATIVE, slots, nullptr, disp, true, nb, &W);
+
+    // reconstructed_positions(...; field = :sum), src/recon.jl:366-381
+    for (int b = 0; b < nb; b++) {
+        const int64_t o = off[g.c0 + b], n = off[g.c0 + b + 1] - o;
+        if (n == 0) continue;
+        GatherArgs ga{};
+        ga.x = x + o;
+        ga.y = y + o;
+        ga.z = z + o;
+        ga.n = n;
+        ga.disp = disp + (int64_t)b * R;
+        ga.slot = slots + b;
+        ga.nx = G.nx;
+        ga.ny = G.ny;
+        ga.nz = G.nz;
+        ga.field = BAOREC_FIELD_SUM;
+        ga.positions = 1;
+        ga.has_los = p->has_los;
+        ga.los[0] = p->los[0];
+        ga.los[1] = p->los[1];
+        ga.los[2] = p->los[2];
+        ga.f = p->f;
+        ga.ox = ox + o;
+        ga.oy = oy + o;
+        ga.oz = oz + o;
+        ga.err = err;
+        ProfScope ps(c, "gather", (double)n * (24.0 + 128.0));
+        hipchk(c, launch_gather(ga, c->stream), "batch gather");
     }
 }
 
@@ -1937,9 +2151,50 @@ int baorec_run_batch(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int
                      const int64_t *offsets, float *x, float *y, float *z, const float *w, float *ox,
                      float *oy, float *oz) {
     return guarded(c, [&] {
-        (void)p; (void)grid; (void)mem; (void)ncat; (void)offsets; (void)x; (void)y; (void)z;
-        (void)w; (void)ox; (void)oy; (void)oz;
-        fail(c, BAOREC_ERR_UNSUPPORTED, "batched reconstruction is not built yet");
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, true);
+        if (mem != BAOREC_MEM_HOST && mem != BAOREC_MEM_DEVICE) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "mem");
+        if (ncat < 0 || (ncat > 0 && !offsets)) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "ncat / offsets");
+        if (ncat == 0) return;
+        if (offsets[0] != 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "offsets[0] must be 0");
+        for (int k = 0; k < ncat; k++)
+            if (offsets[k + 1] < offsets[k]) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "offsets must be non-decreasing");
+        const int64_t n = offsets[ncat];
+        const int per = p->mas == BAOREC_MAS_TSC ? 27 : 8;
+        for (int k = 0; k < ncat; k++)
+            if ((offsets[k + 1] - offsets[k]) * per >= ((int64_t)1 << 32))
+                fail(c, BAOREC_ERR_UNSUPPORTED, "a catalog of the batch is too large for one launch sequence");
+        std::vector<int64_t> t64;
+        std::vector<int> t32;
+        std::vector<BatchGroup> gs = batch_groups(G, p, ncat, offsets, t64, t32);
+        int maxnb = 0;
+        for (const BatchGroup &g : gs) maxnb = std::max(maxnb, g.c1 - g.c0);
+        int64_t *d64 = bufT<int64_t>(c, "bt_t64", t64.size());
+        int *d32 = bufT<int>(c, "bt_t32", t32.size());
+        hipchk(c, hipMemcpyAsync(d64, t64.data(), sizeof(int64_t) * t64.size(), hipMemcpyHostToDevice, c->stream),
+               "H2D");
+        hipchk(c, hipMemcpyAsync(d32, t32.data(), sizeof(int) * t32.size(), hipMemcpyHostToDevice, c->stream),
+               "H2D");
+        int32_t *err = bufT<int32_t>(c, "bt_err", 1);
+        hipchk(c, hipMemsetAsync(err, 0, sizeof(int32_t), c->stream), "memset");
+        Inputs in = stage_catalogs(c, mem, x, y, z, w, n, nullptr, nullptr, nullptr, nullptr, 0);
+        float *o1 = stage_out(c, mem, ox, n, "bt_ox");
+        float *o2 = stage_out(c, mem, oy, n, "bt_oy");
+        float *o3 = stage_out(c, mem, oz, n, "bt_oz");
+        // the host tables are read by the copies above before they go out of scope
+        hipchk(c, hipStreamSynchronize(c->stream), "sync");
+        for (const BatchGroup &g : gs)
+            batch_group(c, G, p, offsets, g, d64, d32, in.x, in.y, in.z, in.w, o1, o2, o3, err, maxnb);
+        int32_t herr = 0;
+        hipchk(c, hipMemcpyAsync(&herr, err, sizeof herr, hipMemcpyDeviceToHost, c->stream), "D2H");
+        sync(c);
+        if (herr) fail(c, BAOREC_ERR_OUT_OF_BOX, "a particle indexes outside the grid (BoundsError)");
+        mutate_back(c, mem, p, 1, x, y, z, in, n);
+        copy_out(c, mem, ox, o1, n);
+        copy_out(c, mem, oy, o2, n);
+        copy_out(c, mem, oz, o3, n);
+        sync(c);
     });
 }
 

@@ -27,6 +27,7 @@ from .api import (  # noqa: F401
     read_shifts,
     reconstructed_overdensity_,
     reconstructed_positions,
+    run_batch,
     reconstructed_potential_,
     reduce_,
     residual_,

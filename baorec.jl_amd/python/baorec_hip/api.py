@@ -359,6 +359,32 @@ def reconstructed_positions(recon, x, y, z, field3d=None, field="disp"):
     return _shifts("baorec_reconstructed_positions", recon, x, y, z, field3d, field)
 
 
+def run_batch(recon: IterativeRecon, grid_size, x, y, z, w, offsets):
+    """Extension (no reference counterpart): for every catalog c of a batch,
+    run!(recon, grid_size, x,y,z,w) on particles [offsets[c], offsets[c+1])
+    followed by reconstructed_positions(recon, x,y,z; field=:sum), all
+    catalogs in one launch sequence per group.  Periodic box (recon.box_size
+    and recon.box_min set), shared grid and parameters.  Positions are wrapped
+    in place as run! does.  Returns the reconstructed positions."""
+    if recon.kind != L.ITERATIVE:
+        raise ValueError("run_batch reconstructs with IterativeRecon")
+    ctx = recon.context()
+    x, y, z, w, _, _, _, _, n, _ = _cat(x, y, z, w)
+    mem = _mem(x, y, z, w)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    if off.ndim != 1 or len(off) < 1 or off[0] != 0 or off[-1] != n:
+        raise ValueError("offsets must run from 0 to the number of particles")
+    if _is_device(x):
+        outs = [_dev_empty((n,), x) for _ in range(3)]
+    else:
+        outs = [np.empty(n, dtype=F32) for _ in range(3)]
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_run_batch(ctx.handle, ctypes.byref(p), _grid(grid_size), mem, len(off) - 1,
+                                       off.ctypes.data_as(ctypes.c_void_p), _ptr(x), _ptr(y), _ptr(z), _ptr(w),
+                                       *(_ptr(o) for o in outs)))
+    return tuple(outs)
+
+
 # ------------------------------------------------------------------ utils
 
 def k_vec(grid, box_size):

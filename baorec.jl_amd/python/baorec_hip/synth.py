@@ -106,3 +106,25 @@ def lightcone_catalog(n_part: int, r_min=1000.0, r_max=1800.0, mesh: int = 256, 
     perm = rng.permutation(n_part)
     x, y, z = (p[perm] for p in pos)
     return x, y, z, np.ones(n_part, dtype=F32)
+
+
+def batch_catalogs(ncat: int, n_min: float = 2e5, n_max: float = 5e6, L: float = 1000.0, mesh: int = 256,
+                   seed0: int = 100, sizes=None):
+    """A batch of periodic-box catalogs laid end to end: catalog c has its own
+    seed (seed0 + c), a power-law slope drawn uniformly in [-2, -1] and a size
+    drawn log-uniformly in [n_min, n_max] (or sizes[c]).  Returns
+    (x, y, z, w, offsets) with catalog c at [offsets[c], offsets[c+1])."""
+    rng = np.random.default_rng(seed0)
+    if sizes is None:
+        sizes = np.exp(rng.uniform(np.log(n_min), np.log(n_max), ncat)).astype(np.int64)
+    slopes = rng.uniform(-2.0, -1.0, ncat)
+    cols = [[], [], [], []]
+    for c in range(ncat):
+        if sizes[c] == 0:
+            continue
+        for k, a in enumerate(box_catalog(int(sizes[c]), L=L, mesh=mesh, seed=seed0 + c, slope=float(slopes[c]))):
+            cols[k].append(a)
+    offsets = np.zeros(ncat + 1, np.int64)
+    offsets[1:] = np.cumsum(sizes)
+    out = [np.concatenate(a) if a else np.zeros(0, F32) for a in cols]
+    return (*out, offsets)

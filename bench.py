@@ -443,6 +443,98 @@ def pk_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def _box_catalog_job(job):
+    from baorec_hip import synth
+    n, seed, slope = job
+    return synth.box_catalog(n, L=1000.0, mesh=128, seed=seed, slope=slope)
+
+
+def batch_bench(args):
+    """Batched plain reconstruction: 64 periodic-box catalogs (L = 1000 Mpc/h,
+    256^3 grids) of sizes drawn log-uniformly in [2e5, 5e6], each a lognormal
+    field with its own seed (100..163) and power-law slope drawn uniformly in
+    [-2, -1]; iterative FFT, LOS along z, CIC, FP32, 3 iterations.  One step =
+    baorec_run_batch over all 64 catalogs (run! + reconstructed_positions with
+    field = :sum per catalog), inputs resident in HBM."""
+    from concurrent.futures import ProcessPoolExecutor
+    ncat = args.ncat
+    rng = np.random.default_rng(100)
+    sizes = np.exp(rng.uniform(np.log(2e5), np.log(5e6), ncat)).astype(np.int64)
+    slopes = rng.uniform(-2.0, -1.0, ncat)
+    t0 = time.perf_counter()
+    # catalogs are generated before anything touches the GPU (forked workers)
+    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        cats = list(ex.map(_box_catalog_job, [(int(sizes[c]), 100 + c, float(slopes[c])) for c in range(ncat)]))
+    data = [np.concatenate([c[k] for c in cats]) for k in range(4)]
+    off = np.zeros(ncat + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    n = int(off[-1])
+    log(f"{ncat} catalogs, {n} particles, generated in {time.perf_counter() - t0:.1f} s")
+
+    import baorec_hip as bh
+    from baorec_hip import _lib as L
+    from baorec_hip.devmem import DeviceArray, hip
+    ctx = bh.Context(0)
+    lib = ctx.lib
+    p = L.Params()
+    cfg = dict(grid=(256, 256, 256), bias=2.0, f=0.8, R=15.0, los=(0.0, 0.0, 1.0), pad=0.0)
+    lib.baorec_params_default(ctypes.byref(p), cfg["bias"], cfg["f"], cfg["R"])
+    p.has_los = 1
+    for a in range(3):
+        p.los[a] = cfg["los"][a]
+        p.box_size[a] = 1000.0
+        p.box_min[a] = 0.0
+    grid = (ctypes.c_int32 * 3)(*cfg["grid"])
+    d = [DeviceArray.from_numpy(a) for a in data]
+    out = [DeviceArray((n,)) for _ in range(3)]
+    vp = lambda a: ctypes.c_void_p(a.ptr)  # noqa: E731
+    offp = off.ctypes.data_as(ctypes.c_void_p)
+
+    def step():
+        ctx.check(lib.baorec_run_batch(ctx.handle, ctypes.byref(p), grid, L.MEM_DEVICE, ncat, offp,
+                                       *(vp(a) for a in d), *(vp(o) for o in out)))
+
+    h = hip()
+    for i in range(args.warmup):
+        step()
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    h.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    h.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    log(f"timed {args.steps} steps in {dt:.3f} s")
+    lib.baorec_profile(ctx.handle, 1)
+    step()
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib.baorec_profile_report(ctx.handle, buf, len(buf))
+    lib.baorec_profile(ctx.handle, 0)
+    prof = json.loads(buf.value.decode())
+    roof = roofline(prof, cfg["grid"], "batch64", rand=False)
+    finite = all(np.isfinite(o.numpy()).all() for o in out)
+    total_ms = sum(v["ms"] for v in prof.values())
+    total_bytes = sum(v["bytes"] for v in prof.values())
+    cpu = None
+    if not args.no_cpu:
+        c0 = [a[off[0]:off[1]] for a in data]
+        cpu = cpu_baseline(args, c0, None, cfg)
+        cpu["unit"] = "catalogs/s"
+        cpu["sample"] = f"catalog 0 of the batch (N = {int(sizes[0])}): " + cpu["sample"]
+    line = {"metric": "catalogs/s, batched plain recon (64 periodic boxes, 256\u00b3, N ~ logU[2e5, 5e6])",
+            "value": round(ncat * args.steps / dt, 3), "unit": "catalogs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "batch64", "grid": [256, 256, 256], "ncat": ncat, "n_total": n,
+                       "n_min": int(sizes.min()), "n_max": int(sizes.max()), "los": "fixed (z)", "n_iter": 3},
+            "roofline": roof, "cpu_baseline": cpu,
+            "kernels": {k: {"count": v["count"], "ms": round(v["ms"], 3),
+                            "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()},
+            "profiled_step": {"kernel_ms": round(total_ms, 3), "algorithmic_GB": round(total_bytes / 1e9, 3)},
+            "outputs_finite": finite}
+    print(json.dumps(line), flush=True)
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """Plain device-to-device copies of a known size: the reference point for
     the HBM byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
@@ -466,6 +558,7 @@ def main():
     ap.add_argument("--workload", default="lightcone512")
     ap.add_argument("--ndata", type=int, default=2_000_000)
     ap.add_argument("--nrand", type=int, default=20_000_000)
+    ap.add_argument("--ncat", type=int, default=64, help="catalogs of the batch64 workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run 1 GiB device-to-device copies (byte-counter calibration)")
@@ -485,6 +578,9 @@ def main():
         return
     if args.workload == "pk512":
         pk_bench(args)
+        return
+    if args.workload == "batch64":
+        batch_bench(args)
         return
     data, rand, cfg = make_workload(args)
     g = gpu_bench(args, data, rand, cfg)
