@@ -319,13 +319,10 @@ void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, f
         ProfScope ps(c, "scatter_bin", (double)n * (a.mutate && wrap ? 40.0 : 28.0) + 16.0 * n * 1.2);
         hipchk(c, launch_scatter(a, nullptr, c->stream), "scatter");
     }
-    if (!accumulate) {
-        ProfScope ps(c, "zero", 4.0 * G.R());
-        hipchk(c, launch_zero(rho, G.R(), c->stream), "zero grid");
-    }
     {
+        // overwrite mode writes every cell once (zeros in empty tiles)
         ProfScope ps(c, "scatter_accum", 16.0 * n * 1.2 + 4.0 * G.R() * (accumulate ? 2 : 1));
-        hipchk(c, launch_scatter_accum(a, accumulate ? 1 : 2, c->stream), "scatter accumulate");
+        hipchk(c, launch_scatter_accum(a, accumulate ? 1 : 0, c->stream), "scatter accumulate");
     }
 }
 
@@ -1142,12 +1139,8 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
         hipchk(c, launch_scatter(a, nullptr, c->stream), "batch scatter");
     }
     {
-        ProfScope ps(c, "zero", 4.0 * R * nb);
-        hipchk(c, launch_zero(rho, R * nb, c->stream), "zero grids");
-    }
-    {
         ProfScope ps(c, "scatter_accum", 16.0 * ntot * 1.2 + 4.0 * R * nb);
-        hipchk(c, launch_scatter_accum(a, 2, c->stream), "batch accumulate");
+        hipchk(c, launch_scatter_accum(a, 0, c->stream), "batch accumulate");
     }
 
     // smooth! and the overdensity (src/recon.jl:42-58); per-catalog FP64 sums

@@ -88,13 +88,14 @@ __device__ __forceinline__ PartU load_particle(const ScatterArgs &a, int64_t i, 
     float x = a.x[i], y = a.y[i], z = a.z[i];
     if (apply_wrap) {  // src/mas.jl:56-60: one-sided, box_min[1]/box_size[1] for all axes
         const float b0 = sl.box_min[0], L0 = sl.box_size[0];
-        x = (x - b0) > L0 ? x - L0 : x;
-        y = (y - b0) > L0 ? y - L0 : y;
-        z = (z - b0) > L0 ? z - L0 : z;
-        if (write_back) {
-            a.x[i] = x;
-            a.y[i] = y;
-            a.z[i] = z;
+        const bool wx = (x - b0) > L0, wy = (y - b0) > L0, wz = (z - b0) > L0;
+        x = wx ? x - L0 : x;
+        y = wy ? y - L0 : y;
+        z = wz ? z - L0 : z;
+        if (write_back) {  // only the coordinates that moved are stored
+            if (wx) a.x[i] = x;
+            if (wy) a.y[i] = y;
+            if (wz) a.z[i] = z;
         }
     }
     const float pp[3] = {x, y, z};

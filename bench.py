@@ -462,9 +462,13 @@ def batch_bench(args):
     sizes = np.exp(rng.uniform(np.log(2e5), np.log(5e6), ncat)).astype(np.int64)
     slopes = rng.uniform(-2.0, -1.0, ncat)
     t0 = time.perf_counter()
-    # catalogs are generated before anything touches the GPU (forked workers)
-    with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-        cats = list(ex.map(_box_catalog_job, [(int(sizes[c]), 100 + c, float(slopes[c])) for c in range(ncat)]))
+    jobs = [(int(sizes[c]), 100 + c, float(slopes[c])) for c in range(ncat)]
+    if args.gen_workers > 1:
+        # forked workers, started before anything touches the GPU
+        with ProcessPoolExecutor(max_workers=args.gen_workers) as ex:
+            cats = list(ex.map(_box_catalog_job, jobs))
+    else:
+        cats = [_box_catalog_job(j) for j in jobs]
     data = [np.concatenate([c[k] for c in cats]) for k in range(4)]
     off = np.zeros(ncat + 1, np.int64)
     off[1:] = np.cumsum(sizes)
@@ -559,6 +563,8 @@ def main():
     ap.add_argument("--ndata", type=int, default=2_000_000)
     ap.add_argument("--nrand", type=int, default=20_000_000)
     ap.add_argument("--ncat", type=int, default=64, help="catalogs of the batch64 workload")
+    ap.add_argument("--gen-workers", type=int, default=8,
+                    help="processes generating the batch64 catalogs (1 under a profiler)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run 1 GiB device-to-device copies (byte-counter calibration)")

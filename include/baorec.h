@@ -367,12 +367,20 @@ const char *baorec_io_last_error(void);
 /* --------------------------------------------------------------- batching */
 
 /*
- * Extension (no reference counterpart): reconstruct `ncat` periodic-box
- * catalogs of uneven sizes with one captured launch sequence (hipGraph per
- * grid/catalog-size class).  Catalog c occupies particles
- * [offsets[c], offsets[c+1]) of x,y,z,w (device or host per `mem`); all share
- * the grid and params.  Outputs: reconstructed positions (field = SUM) for
- * every particle into ox, oy, oz.
+ * Extension (no reference counterpart; the reference's drivers loop over
+ * catalogs): for each of `ncat` periodic-box catalogs, the equivalent of
+ *   run!(IterativeRecon, grid, x, y, z, w)                     src/recon.jl:134
+ *   reconstructed_positions(recon, x, y, z; field = :sum)      src/recon.jl:366
+ * with every launch covering a group of catalogs.  Catalog c occupies
+ * particles [offsets[c], offsets[c+1]) of x, y, z, w (offsets is a host
+ * array of ncat + 1 entries, offsets[0] = 0, non-decreasing); all catalogs
+ * share the grid and the params (box_size / box_min must be set; no randoms).
+ * Outputs: the reconstructed positions of every particle in ox, oy, oz.
+ * Inputs are wrapped in place when mutate_inputs, as run! does.  Results are
+ * bitwise those of the single-catalog calls.  Catalogs run in groups whose
+ * grid and spectrum buffers stay within 16 GiB (0.93 GB per catalog at 256^3),
+ * one launch sequence per group on the context's stream.  A particle outside
+ * the grid gives BAOREC_ERR_OUT_OF_BOX.
  */
 int baorec_run_batch(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3],
                      int mem, int ncat, const int64_t *offsets, float *x, float *y,
