@@ -104,8 +104,11 @@ T *bufT(baorec_ctx *c, const std::string &name, size_t count) {
 struct Grid {
     int nx, ny, nz, P;
     int PP() const { return (P + 15) / 16 * 16; }
+    // kz planes are one row longer than ny rows, so that the z passes'
+    // column loads are not all a multiple of 64 KB apart
+    int64_t PZ() const { return (int64_t)PP() * (ny + 1); }
     int64_t R() const { return (int64_t)nx * ny * nz; }
-    int64_t C() const { return (int64_t)PP() * ny * nz; }
+    int64_t C() const { return PZ() * nz; }
     int64_t Cdense() const { return (int64_t)P * ny * nz; }
 };
 
@@ -369,6 +372,7 @@ StridedArgs sargs(const Grid &G, const FFTBufs &f, int axis, int mode, int mul) 
     a.ktab = f.ktab;
     a.ktabStride = 0;
     a.P = G.PP();
+    a.PZ = G.PZ();
     a.Pv = G.P;
     a.ny = G.ny;
     a.nz = G.nz;
@@ -386,6 +390,7 @@ void x_fwd(baorec_ctx *c, const Grid &G, const float *in, float2 *out, int nbatc
     a.rstride = G.R();
     a.cstride = G.C();
     a.P = G.PP();
+    a.PZ = G.PZ();
     a.nx = G.nx;
     a.ny = G.ny;
     a.nz = G.nz;
@@ -412,6 +417,7 @@ XInvArgs xargs(const Grid &G, const FFTBufs &f, const Slot *slot) {
     a.cstride = G.C();
     a.rstride = G.R();
     a.P = G.PP();
+    a.PZ = G.PZ();
     a.nx = G.nx;
     a.ny = G.ny;
     a.nz = G.nz;
@@ -526,6 +532,7 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
             y.ktab = f.ktab;
             y.xtab = f.xtab;
             y.P = G.PP();
+            y.PZ = G.PZ();
             y.Pv = G.P;
             y.nx = G.nx;
             y.ny = G.ny;
@@ -1685,10 +1692,12 @@ int baorec_rfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *in
         x_fwd(c, G, d, o, 1, nullptr, 0);
         strided_plain(c, G, f, 1, false, o, 1);
         strided_plain(c, G, f, 2, false, o, 1);
-        hipchk(c, hipMemcpy2DAsync(out_complex, sizeof(float2) * G.P, o, sizeof(float2) * G.PP(),
-                                   sizeof(float2) * G.P, (size_t)G.ny * G.nz,
-                                   mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                                   c->stream), "copy spectrum");
+        for (int kz = 0; kz < G.nz; kz++)   // one plane of ny rows per copy
+            hipchk(c, hipMemcpy2DAsync(out_complex + (int64_t)2 * G.P * G.ny * kz, sizeof(float2) * G.P,
+                                       o + G.PZ() * kz, sizeof(float2) * G.PP(), sizeof(float2) * G.P,
+                                       (size_t)G.ny,
+                                       mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                       c->stream), "copy spectrum");
         sync(c);
     });
 }
@@ -1699,10 +1708,12 @@ int baorec_irfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *i
         if (!in_complex || !out) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "NULL argument");
         FFTBufs f = fft_bufs(c, G);
         float2 *w = f.cw;  // work copy: the caller's spectrum is not modified
-        hipchk(c, hipMemcpy2DAsync(w, sizeof(float2) * G.PP(), in_complex, sizeof(float2) * G.P,
-                                   sizeof(float2) * G.P, (size_t)G.ny * G.nz,
-                                   mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                                   c->stream), "copy spectrum");
+        for (int kz = 0; kz < G.nz; kz++)
+            hipchk(c, hipMemcpy2DAsync(w + G.PZ() * kz, sizeof(float2) * G.PP(),
+                                       in_complex + (int64_t)2 * G.P * G.ny * kz, sizeof(float2) * G.P,
+                                       sizeof(float2) * G.P, (size_t)G.ny,
+                                       mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                       c->stream), "copy spectrum");
         Slot *slot = slot_buf(c) + 1;
         strided_plain(c, G, f, 2, true, w, 1);
         strided_plain(c, G, f, 1, true, w, 1);
@@ -2067,6 +2078,7 @@ int baorec_power_multipoles(baorec_ctx *c, const baorec_pk_params *pk, const int
         a.ny = G.ny;
         a.nz = G.nz;
         a.P = G.PP();
+    a.PZ = G.PZ();
         a.Pv = G.P;
         for (int i = 0; i < 3; i++) {
             a.kf[i] = kf[i];

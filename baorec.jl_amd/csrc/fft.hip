@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_r2c(XFwdArgs a) {
 #pragma unroll
     for (int i = 0; i < 8; i++) lds[lay(PF::out_pos(t, i))] = v[i];
     __syncthreads();
-    float2 *dst = a.out + b * a.cstride + line * a.P;
+    float2 *dst = a.out + b * a.cstride + (line % a.ny) * a.P + (line / a.ny) * a.PZ;
     if (valid) {
 #pragma unroll
         for (int i = 0; i < 8; i++) {
@@ -199,11 +199,12 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
     float qs[8][2];
 
     float2 cur[8], cn;
-    xline_load<M>(a.in[0] + b * a.cstride + line * a.P, valid, t, cur, cn);
+    const int64_t crow = (int64_t)iy * a.P + (int64_t)iz * a.PZ;
+    xline_load<M>(a.in[0] + b * a.cstride + crow, valid, t, cur, cn);
 #pragma unroll
     for (int q = 0; q < NIN; q++) {
         float2 nxt[8], nn;
-        if (q + 1 < NIN) xline_load<M>(a.in[q + 1] + b * a.cstride + line * a.P, valid, t, nxt, nn);
+        if (q + 1 < NIN) xline_load<M>(a.in[q + 1] + b * a.cstride + crow, valid, t, nxt, nn);
         const int pw = a.kxpow[q];
         float2 v[8];
         float2 xk[8];
@@ -325,7 +326,7 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
             for (int i = 0; i < 8; i++) lds[lay(PF::out_pos(t, i))] = acc[i];
             __syncthreads();
             if (valid) {
-                float2 *dst = a.fwd_out + b * a.cstride + line * a.P;
+                float2 *dst = a.fwd_out + b * a.cstride + crow;
 #pragma unroll
                 for (int i = 0; i < 8; i++) {
                     const int k = t + i * TPL;
@@ -406,11 +407,11 @@ k_strided(StridedArgs a) {
     const int64_t b = blockIdx.z;
     int64_t base, rs;
     if (a.axis == 1) {
-        base = (int64_t)other * a.P * a.ny + kxi;
+        base = (int64_t)other * a.PZ + kxi;
         rs = a.P;
     } else {
         base = (int64_t)other * a.P + kxi;
-        rs = (int64_t)a.P * a.ny;
+        rs = a.PZ;
     }
     base += b * a.cstride;
     const float *kt = a.ktab + b * a.ktabStride;
@@ -472,8 +473,17 @@ k_strided(StridedArgs a) {
 
 // ------------------------------------------------------------ y: local-LOS iteration
 
+// The pointer as an opaque value: table loads through it are not hoisted
+// across the transforms that use it (which would keep them all live).
+template <class T>
+__device__ __forceinline__ T *opaque(T *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 template <int M>
-__global__ void __launch_bounds__(SShape<M>::NT) k_y_iter(YIterArgs a) {
+__global__ void __launch_bounds__(SShape<M>::NT) __attribute__((amdgpu_waves_per_eu(4)))
+k_y_iter(YIterArgs a) {
     constexpr int B = SShape<M>::B;
     using PI = Plan<M, true>;
     __shared__ float2 lds[lay_strided_size(M, B)];
@@ -485,7 +495,7 @@ __global__ void __launch_bounds__(SShape<M>::NT) k_y_iter(YIterArgs a) {
     const int kxi = cg * B + col;
     const bool valid = kxi < a.Pv;
     const int64_t b = blockIdx.z;
-    const int64_t base = b * a.cstride + (int64_t)iz * a.P * a.ny + kxi;
+    const int64_t base = b * a.cstride + (int64_t)iz * a.PZ + kxi;
     const int64_t rs = a.P;
     const float *ky = a.ktab + b * a.ktabStride + a.Pv;
     const float *xv = a.xtab + b * a.xtabStride;
@@ -493,62 +503,69 @@ __global__ void __launch_bounds__(SShape<M>::NT) k_y_iter(YIterArgs a) {
     LayStrided lay{col, B};
     TwCache<M, true> tc;
     tc.load(t, g_tw);
-    float kyv[8], yc[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        kyv[i] = ky[PI::in_pos(t, i)];
-        yc[i] = xv[a.nx + PI::out_pos(t, i)];
-    }
-    float2 g0[8], g1[8], g2[8], v[8], acc[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const int64_t o = base + PI::in_pos(t, i) * rs;
-        g0[i] = valid ? a.g[0][o] : make_float2(0.f, 0.f);
-        g1[i] = valid ? a.g[1][o] : make_float2(0.f, 0.f);
-        g2[i] = valid ? a.g[2][o] : make_float2(0.f, 0.f);
-    }
     const float fxx = a.fac[0], fxy = a.fac[1], fxz = a.fac[2];
     const float fyy = a.fac[3], fyz = a.fac[4], fzz = a.fac[5];
-    // A = fxx IFFT(G0)
+    // in: v = G[i] * ky^p (in the inverse plan's input order)
+    auto load = [&](const float2 *g0, int p, float2 (&v)[8]) {
+        // opaque: each transform loads its own input (no reuse across
+        // transforms that would keep a whole line live)
+        const float2 *g = opaque(g0);
+        const float *kyo = opaque(ky);
 #pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = g0[i];
+        for (int i = 0; i < 8; i++) {
+            const int r = PI::in_pos(t, i);
+            const float2 x = valid ? g[base + r * rs] : make_float2(0.f, 0.f);
+            const float k = kyo[r];
+            v[i] = p == 0 ? x : cscale(x, p == 1 ? k : k * k);
+        }
+    };
+    // The six transforms in an order that keeps at most two accumulators and
+    // one line live: G0 terms of A, B and C; then G1 (read twice: for B and
+    // for C); then G2.
+    float2 v[8], accb[8], accc[8];
+    // A = fxx IFFT(G0)
+    load(a.g[0], 0, v);
     fft_line_c<M, true>(v, lds, lay, t, tc);
     if (valid) {
 #pragma unroll
         for (int i = 0; i < 8; i++) a.out[0][base + PI::out_pos(t, i) * rs] = cscale(v[i], fxx);
     }
-    // B = fxy y IFFT(ky G0) + fxz z IFFT(G1)
-#pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = cscale(g0[i], kyv[i]);
+    // B = fxy y IFFT(ky G0) + ...
+    load(a.g[0], 1, v);
     fft_line_c<M, true>(v, lds, lay, t, tc);
 #pragma unroll
-    for (int i = 0; i < 8; i++) acc[i] = cscale(v[i], fxy * yc[i]);
-#pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = g1[i];
+    for (int i = 0; i < 8; i++) accb[i] = cscale(v[i], fxy * opaque(xv)[a.nx + PI::out_pos(t, i)]);
+    // C = fyy y^2 IFFT(ky^2 G0) + ...
+    load(a.g[0], 2, v);
     fft_line_c<M, true>(v, lds, lay, t, tc);
-    if (valid) {
 #pragma unroll
-        for (int i = 0; i < 8; i++)
-            a.out[1][base + PI::out_pos(t, i) * rs] = cadd(acc[i], cscale(v[i], fxz * zc));
+    for (int i = 0; i < 8; i++) {
+        const float yc = opaque(xv)[a.nx + PI::out_pos(t, i)];
+        accc[i] = cscale(v[i], fyy * (yc * yc));
     }
-    // C = fyy y^2 IFFT(ky^2 G0) + fyz y z IFFT(ky G1) + fzz z^2 IFFT(G2)
-#pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = cscale(g0[i], kyv[i] * kyv[i]);
-    fft_line_c<M, true>(v, lds, lay, t, tc);
-#pragma unroll
-    for (int i = 0; i < 8; i++) acc[i] = cscale(v[i], fyy * (yc[i] * yc[i]));
-#pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = cscale(g1[i], kyv[i]);
-    fft_line_c<M, true>(v, lds, lay, t, tc);
-#pragma unroll
-    for (int i = 0; i < 8; i++) acc[i] = cadd(acc[i], cscale(v[i], fyz * (yc[i] * zc)));
-#pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = g2[i];
+    // B += fxz z IFFT(G1)
+    load(a.g[1], 0, v);
     fft_line_c<M, true>(v, lds, lay, t, tc);
     if (valid) {
 #pragma unroll
         for (int i = 0; i < 8; i++)
-            a.out[2][base + PI::out_pos(t, i) * rs] = cadd(acc[i], cscale(v[i], fzz * (zc * zc)));
+            a.out[1][base + PI::out_pos(t, i) * rs] = cadd(accb[i], cscale(v[i], fxz * zc));
+    }
+    // C += fyz y z IFFT(ky G1)
+    load(a.g[1], 1, v);
+    fft_line_c<M, true>(v, lds, lay, t, tc);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const float yc = opaque(xv)[a.nx + PI::out_pos(t, i)];
+        accc[i] = cadd(accc[i], cscale(v[i], fyz * (yc * zc)));
+    }
+    // C += fzz z^2 IFFT(G2)
+    load(a.g[2], 0, v);
+    fft_line_c<M, true>(v, lds, lay, t, tc);
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            a.out[2][base + PI::out_pos(t, i) * rs] = cadd(accc[i], cscale(v[i], fzz * (zc * zc)));
     }
 }
 

@@ -39,6 +39,7 @@ struct StridedArgs {
     const float *ktab;         // per batch item: kx[P] | ky[ny] | kz[nz]
     int64_t ktabStride;        // floats between batch items (0: shared)
     int P;                     // complex pitch of a kx row (>= Pv, multiple of 16)
+    int64_t PZ;                // complex pitch of a kz plane (P * (ny + 1))
     int Pv;                    // valid kx columns, nx/2 + 1
     int ny, nz;
     int axis;                  // 1: y, 2: z
@@ -66,6 +67,7 @@ struct XInvArgs {
     int64_t cstride;      // complex elements between batch items
     int64_t rstride;      // real elements between batch items
     int P;                // complex pitch of a kx row
+    int64_t PZ;           // complex pitch of a kz plane
     float *out;           // real output (float4 for XE_DISP)
     const float *aux;     // delta_s for the iterative updates
     int nx, ny, nz;
@@ -87,6 +89,7 @@ struct XFwdArgs {
     float2 *out;
     int64_t rstride, cstride;
     int P;                // complex pitch of a kx row
+    int64_t PZ;           // complex pitch of a kz plane
     int nx, ny, nz;
     double *sums;         // optional FP64 total of each input (per batch item)
     int64_t sumStride;
@@ -111,6 +114,7 @@ struct YIterArgs {
     const float *xtab;     // x | y | z cell centres (per batch item)
     int64_t xtabStride;
     int P, Pv, nx, ny, nz;
+    int64_t PZ;
     float fac[6];          // xx, xy, xz, yy, yz, zz
 };
 hipError_t launch_y_iter(const YIterArgs &a, int nbatch, hipStream_t s);
@@ -242,6 +246,7 @@ struct PkArgs {
     const float2 *F1, *F2;
     const float4 *tab;
     int nx, ny, nz, P, Pv;
+    int64_t PZ;
     double kf[3], los[3];
     double kmin, dk;
     int nbins, interlace;

@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(kPkThreads) k_pk_modes(PkArgs a) {
         const double bf = floor((k - a.kmin) / a.dk);
         if (!(bf >= 0.0 && bf < (double)a.nbins)) continue;
         const int bin = (int)bf;
-        const int64_t I = kx + (int64_t)a.P * t;
+        const int64_t I = kx + (int64_t)a.P * ky + a.PZ * kz;
         float2 d = a.F1[I];
         if (a.interlace) {
             // e^{i k.H/2} = product of the per-axis factors
