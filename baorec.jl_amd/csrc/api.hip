@@ -352,7 +352,8 @@ void tables(baorec_ctx *c, const Grid &G, const Slot *slot, FFTBufs &f) {
 void strided(baorec_ctx *c, const Grid &G, const StridedArgs &a, int njobs, int nbatch, const char *what) {
     double bytes = 0.0;
     for (int j = 0; j < njobs; j++) bytes += 8.0 * G.Cdense() * (1 + a.job[j].nout);
-    ProfScope ps(c, a.axis == 1 ? "fft_y" : "fft_z", bytes * nbatch);
+    const bool pair = strided_uses_pair(a, njobs);
+    ProfScope ps(c, a.axis == 1 ? (pair ? "fft_y_pair" : "fft_y") : (pair ? "fft_z_pair" : "fft_z"), bytes * nbatch);
     hipchk(c, launch_strided(a, njobs, nbatch, c->stream), what);
 }
 

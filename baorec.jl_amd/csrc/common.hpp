@@ -246,6 +246,87 @@ __device__ __forceinline__ void fft_line_c(float2 (&v)[8], float2 *lds, const La
     fft_stages_c<M, INV, 0>(v, lds, lay, t, tc);
 }
 
+// Two lines per thread: the same stages on v[0] and v[1] (two adjacent
+// columns 2c, 2c+1 of a strided layout), sharing twiddles and barriers; each
+// LDS access moves both columns as one float4.
+struct LayPair {
+    int c2, B;   // first column (even) and columns per LDS row
+    __device__ __forceinline__ int operator()(int pos) const { return ((pos + (pos >> 3)) * B + c2) >> 1; }
+};
+
+template <int M, bool INV, int ST, bool CACHED>
+__device__ __forceinline__ void fft_stages2(float2 (&v)[2][8], float4 *lds, const LayPair &lay, int t,
+                                            const float2 *__restrict__ tw, const TwCache<M, INV> &tc) {
+    using P = Plan<M, INV>;
+    constexpr int R = P::radix(ST);
+    constexpr int NS = P::ns(ST);
+    constexpr int NB = 8 / R;
+    if constexpr (ST > 0) {
+        constexpr int RP = P::radix(ST - 1);
+        constexpr int NSP = P::ns(ST - 1);
+        constexpr int NBP = 8 / RP;
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NBP; s++) {
+            int j = t + s * (M / 8);
+            int k = j & (NSP - 1);
+            int base = (j - k) * RP + k;
+#pragma unroll
+            for (int q = 0; q < RP; q++) {
+                const float2 a = v[0][s * RP + q], b = v[1][s * RP + q];
+                lds[lay(base + q * NSP)] = make_float4(a.x, a.y, b.x, b.y);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NB; s++) {
+            int j = t + s * (M / 8);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const float4 c = lds[lay(j + r * (M / R))];
+                v[0][s * R + r] = make_float2(c.x, c.y);
+                v[1][s * R + r] = make_float2(c.z, c.w);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NB; s++) {
+        if constexpr (NS > 1) {
+            const int k = (t + s * (M / 8)) & (NS - 1);
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                float2 w;
+                if constexpr (CACHED) {
+                    w = tc.w[ST - 1][s * (R - 1) + r - 1];
+                } else {
+                    w = tw[NS * R + r * k];
+                    if (INV) w.y = -w.y;
+                }
+                v[0][s * R + r] = cmul(v[0][s * R + r], w);
+                v[1][s * R + r] = cmul(v[1][s * R + r], w);
+            }
+        }
+        dft<R, INV>(&v[0][s * R]);
+        dft<R, INV>(&v[1][s * R]);
+    }
+    if constexpr (ST + 1 < P::NST) fft_stages2<M, INV, ST + 1, CACHED>(v, lds, lay, t, tw, tc);
+}
+
+// twiddles from the table
+template <int M, bool INV>
+__device__ __forceinline__ void fft_line2(float2 (&v)[2][8], float4 *lds, const LayPair &lay, int t,
+                                          const float2 *__restrict__ tw) {
+    TwCache<M, INV> none;
+    fft_stages2<M, INV, 0, false>(v, lds, lay, t, tw, none);
+}
+
+// twiddles cached per thread
+template <int M, bool INV>
+__device__ __forceinline__ void fft_line2_c(float2 (&v)[2][8], float4 *lds, const LayPair &lay, int t,
+                                            const TwCache<M, INV> &tc) {
+    fft_stages2<M, INV, 0, true>(v, lds, lay, t, nullptr, tc);
+}
+
 // LDS layouts
 // [pos][col], B columns per row, one pad row after every 8: the first stage
 // exchange writes rows 8 apart from consecutive threads, which would

@@ -471,6 +471,98 @@ k_strided(StridedArgs a) {
     }
 }
 
+// The same pass for jobs with one output, with 16 columns per workgroup and
+// two adjacent columns per thread (M threads): every row access of a wave is
+// one float4 per thread, 128 contiguous bytes per row, and the two columns
+// share twiddles and barriers.  Columns past Pv (inside the row pitch) are
+// transformed and stored like the others; nothing reads them.
+template <int M>
+__global__ void __launch_bounds__(M) __attribute__((amdgpu_waves_per_eu(4)))
+k_strided2(StridedArgs a) {
+    constexpr int B = 16;
+    using PF = Plan<M, false>;
+    using PI = Plan<M, true>;
+    __shared__ float4 lds[lay_strided_size(M, B) / 2];
+    const int tid = threadIdx.x;
+    const int pc = tid % (B / 2), t = tid / (B / 2);
+    const int ncg = (a.Pv + B - 1) / B;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int cg = tile % ncg, other = tile / ncg;
+    const int kx0 = cg * B + 2 * pc;
+    const bool valid = kx0 < a.Pv;
+    const StridedJob &J = a.job[blockIdx.y];
+    const int64_t b = blockIdx.z;
+    int64_t base, rs;
+    if (a.axis == 1) {
+        base = (int64_t)other * a.PZ + kx0;
+        rs = a.P;
+    } else {
+        base = (int64_t)other * a.P + kx0;
+        rs = a.PZ;
+    }
+    base += b * a.cstride;
+    const float *kt = a.ktab + b * a.ktabStride;
+    const float *kax = a.axis == 1 ? kt + a.Pv : kt + a.Pv + a.ny;
+    const LayPair lay{2 * pc, B};
+    float2 v[2][8];
+    auto load = [&](const float2 *src, int pos, int i) {
+        const float4 c = valid ? *reinterpret_cast<const float4 *>(src + base + pos * rs)
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+        v[0][i] = make_float2(c.x, c.y);
+        v[1][i] = make_float2(c.z, c.w);
+    };
+    auto store = [&](float2 *dst, int pos, const float2 &x0, const float2 &x1) {
+        *reinterpret_cast<float4 *>(dst + base + pos * rs) = make_float4(x0.x, x0.y, x1.x, x1.y);
+    };
+
+    if (a.mode == 0) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) load(J.in, PF::in_pos(t, i), i);
+        fft_line2<M, false>(v, lds, lay, t, g_tw);
+        if (valid) {
+#pragma unroll
+            for (int i = 0; i < 8; i++) store(J.out[0], PF::out_pos(t, i), v[0][i], v[1][i]);
+        }
+        return;
+    }
+    if (a.mode == 2) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) load(J.in, PF::in_pos(t, i), i);
+        fft_line2<M, false>(v, lds, lay, t, g_tw);
+        const float kx0v = valid ? kt[kx0] : 0.f;
+        const float kx1v = kx0 + 1 < a.Pv ? kt[kx0 + 1] : 0.f;
+        const float kother = a.axis == 1 ? kt[a.Pv + a.ny + other] : kt[a.Pv + other];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float kal = kax[PF::out_pos(t, i)];
+            const float ky = a.axis == 1 ? kal : kother;
+            const float kz = a.axis == 1 ? kother : kal;
+            v[0][i] = apply_mul(v[0][i], a.mul, kx0v, ky, kz, a.R2, a.los[0], a.los[1], a.los[2]);
+            v[1][i] = apply_mul(v[1][i], a.mul, kx1v, ky, kz, a.R2, a.los[0], a.los[1], a.los[2]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) load(J.in, PI::in_pos(t, i), i);
+    }
+    // one output (the launcher uses this kernel only then), k-power pw
+    const int pw = J.pw[0];
+    if (pw) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float m = kpow(kax[PI::in_pos(t, i)], pw);
+            v[0][i] = cscale(v[0][i], m);
+            v[1][i] = cscale(v[1][i], m);
+        }
+    }
+    TwCache<M, true> tc;
+    tc.load(t, g_tw);
+    fft_line2_c<M, true>(v, lds, lay, t, tc);
+    if (valid) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) store(J.out[0], PI::out_pos(t, i), v[0][i], v[1][i]);
+    }
+}
+
 // ------------------------------------------------------------ y: local-LOS iteration
 
 // The pointer as an opaque value: table loads through it are not hoisted
@@ -659,11 +751,26 @@ hipError_t launch_x_c2r(const XInvArgs &a, int epi, int nbatch, hipStream_t s) {
     }
 }
 
+bool strided_uses_pair(const StridedArgs &a, int njobs) {
+    const int M = a.axis == 1 ? a.ny : a.nz;
+    bool single = true;
+    for (int j = 0; j < njobs; j++) single = single && a.job[j].nout == 1;
+    return M >= 256 && M <= 1024 && single;
+}
+
 template <int M>
 static hipError_t strided_m(const StridedArgs &a, int njobs, int nbatch, hipStream_t s) {
     using S = SShape<M>;
-    const int ncg = (a.Pv + S::B - 1) / S::B;
     const int nother = a.axis == 1 ? a.nz : a.ny;
+    if constexpr (M >= 256) {
+        if (strided_uses_pair(a, njobs)) {
+            const int ncg = (a.Pv + 15) / 16;
+            dim3 grid((unsigned)(ncg * nother), njobs, nbatch);
+            hipLaunchKernelGGL(k_strided2<M>, grid, dim3(M), 0, s, a);
+            return hipGetLastError();
+        }
+    }
+    const int ncg = (a.Pv + S::B - 1) / S::B;
     dim3 grid((unsigned)(ncg * nother), njobs, nbatch);
     hipLaunchKernelGGL(k_strided<M>, grid, dim3(S::NT), 0, s, a);
     return hipGetLastError();

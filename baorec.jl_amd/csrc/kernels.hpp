@@ -122,6 +122,9 @@ hipError_t launch_y_iter(const YIterArgs &a, int nbatch, hipStream_t s);
 hipError_t launch_x_r2c(const XFwdArgs &a, int nbatch, hipStream_t s);
 hipError_t launch_x_c2r(const XInvArgs &a, int epi, int nbatch, hipStream_t s);
 hipError_t launch_strided(const StridedArgs &a, int njobs, int nbatch, hipStream_t s);
+// True when launch_strided runs the pass with two columns per thread
+// (k_strided2: lines of 256..1024, every job with one output).
+bool strided_uses_pair(const StridedArgs &a, int njobs);
 bool fft_side_ok(int n);
 
 // ---------------------------------------------------------------- tables

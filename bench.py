@@ -166,7 +166,8 @@ def kernel_symbol(label, grid, rand):
     profiler labels; None for labels that cover several kernels."""
     nx, ny, nz = grid
     h = nx // 2
-    return {"fft_y": f"k_strided<{ny}>", "fft_z": f"k_strided<{nz}>", "fft_y_iter": f"k_y_iter<{ny}>",
+    return {"fft_y": f"k_strided<{ny}>", "fft_z": f"k_strided<{nz}>", "fft_y_pair": f"k_strided2<{ny}>",
+            "fft_z_pair": f"k_strided2<{nz}>", "fft_y_iter": f"k_y_iter<{ny}>",
             "fft_x_r2c": f"k_x_r2c<{h}>", "fft_x_c2r": f"k_x_c2r<{h}, 0>",
             "fft_x_c2r_od": f"k_x_c2r<{h}, {2 if rand else 1}>", "fft_x_c2r_iter": f"k_x_c2r<{h}, 3>",
             "fft_x_c2r_disp": f"k_x_c2r<{h}, 5>", "scatter_accum": "k_scatter_accum<0>", "zero": "k_zero4",

@@ -46,7 +46,8 @@ def box_kw(L=1000.0, los=(0.0, 0.0, 1.0)):
 
 # ------------------------------------------------------------------ FFT
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (16, 32, 64), (64, 16, 32), (128, 128, 128), (32, 256, 16)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (16, 32, 64), (64, 16, 32), (128, 128, 128), (32, 256, 16),
+                                   (16, 512, 32), (64, 32, 512), (32, 512, 512), (16, 1024, 16)])
 def test_rfft3d_matches_numpy(ctx, shape):
     nx, ny, nz = shape
     rng = np.random.default_rng(1)
@@ -148,9 +149,12 @@ def test_setup_box_exact(ctx):
 
 # ------------------------------------------------------------------ smoothing / overdensity
 
-def test_smooth_matches_oracle(ctx):
+@pytest.mark.parametrize("shape", [(32, 32, 32), (512, 16, 32), (16, 512, 64), (256, 32, 16)])
+def test_smooth_matches_oracle(ctx, shape):
+    """(nz, ny, nx): 512-long y and z axes take the two-columns-per-thread
+    strided pass (fwd * Gaussian * inv in z, plain y)."""
     rng = np.random.default_rng(7)
-    a = rng.standard_normal((32, 32, 32)).astype(F32)
+    a = rng.standard_normal(shape).astype(F32)
     rec = bh.IterativeRecon(**box_kw(300.0))
     got = bh.smooth_(a.copy(), rec)
     ref = orc.smooth(a.copy(), 15.0, rec.box_size)
@@ -204,6 +208,26 @@ def test_run_iterative_box(ctx, los):
         got = bh.read_shifts(rec, gx, gy, gz, field=kind)
         exp = orc.read_shifts(o, ox, oy, oz, kind=kind)
         assert rms3(got, exp) < 1e-4, kind
+    got = bh.reconstructed_positions(rec, gx, gy, gz, field="sum")
+    exp = orc.reconstructed_positions(o, ox, oy, oz, kind="sum")
+    assert rms3(got, exp) < 1e-4
+
+
+@pytest.mark.parametrize("los,g", [((0.0, 0.0, 1.0), (16, 512, 512)), (None, (16, 512, 512)),
+                                   ((0.0, 0.0, 1.0), (32, 256, 256))])
+def test_run_iterative_box_long_axes(ctx, los, g):
+    """y and z passes of length 256 and 512: the two-columns-per-thread
+    kernel for single-output passes, the one-column kernel for multi-output
+    ones."""
+    x, y, z, w = synth.box_catalog(100000, L=1000.0, mesh=64, seed=23)
+    kw = box_kw(los=los)
+    rec = bh.IterativeRecon(**kw)
+    gx, gy, gz = x.copy(), y.copy(), z.copy()
+    d = bh.run_(rec, g, gx, gy, gz, w)
+    o = orc.IterativeRecon(**kw)
+    ox, oy, oz = x.copy(), y.copy(), z.copy()
+    ref = orc.run_iterative(o, g, ox, oy, oz, w)
+    assert rel_l2(d, ref) < 1e-5
     got = bh.reconstructed_positions(rec, gx, gy, gz, field="sum")
     exp = orc.reconstructed_positions(o, ox, oy, oz, kind="sum")
     assert rms3(got, exp) < 1e-4
