@@ -232,7 +232,28 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
     const uint32_t end = a.cursor[tid_global];
     const uint32_t cnt = a.count[tid_global];
     const uint32_t beg = end - cnt;
-    if (accumulate == 2 && cnt == 0) return;
+    float *rho = a.rho + (int64_t)cat * a.rstride;
+    // whole tile inside the grid, rows of float4: vector stores
+    const bool vec = x0 + a.tx <= a.nx && y0 + a.ty <= a.ny && z0 + a.tz <= a.nz && (a.tx & 3) == 0 &&
+                     (a.nx & 3) == 0 && (reinterpret_cast<uintptr_t>(rho) & 15) == 0;
+    if (cnt == 0) {   // no particle touches this tile
+        if (accumulate != 0) return;
+        if (vec) {
+            const int tx4 = a.tx >> 2;
+            for (int c4 = threadIdx.x; c4 < tcells / 4; c4 += kAccumThreads) {
+                const int lx = (c4 % tx4) * 4, ly = (c4 / tx4) % a.ty, lz = c4 / (tx4 * a.ty);
+                const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
+                *reinterpret_cast<float4 *>(rho + g) = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        } else {
+            for (int c = threadIdx.x; c < tcells; c += kAccumThreads) {
+                const int lx = c % a.tx, ly = (c / a.tx) % a.ty, lz = c / (a.tx * a.ty);
+                if (x0 + lx >= a.nx || y0 + ly >= a.ny || z0 + lz >= a.nz) continue;
+                rho[(x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz))] = 0.f;
+            }
+        }
+        return;
+    }
     const float bound = (float)cnt * __uint_as_float(a.wmax[cat]);
     const int e = bound > 0.f ? 61 - ilogbf(bound) : 0;
     const double S = ldexp(1.0, e);
@@ -300,27 +321,52 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
         }
     }
     __syncthreads();
-    float *rho = a.rho + (int64_t)cat * a.rstride;
+    auto cell = [&](int c) { return (float)ldexp((double)(long long)tile[c], -e); };
+    if (vec) {
+        const int tx4 = a.tx >> 2;
+        for (int c4 = threadIdx.x; c4 < tcells / 4; c4 += kAccumThreads) {
+            const int lx = (c4 % tx4) * 4, ly = (c4 / tx4) % a.ty, lz = c4 / (tx4 * a.ty);
+            const int c = lx + a.tx * (ly + a.ty * lz);
+            const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
+            float4 v = make_float4(cell(c), cell(c + 1), cell(c + 2), cell(c + 3));
+            float4 *dst = reinterpret_cast<float4 *>(rho + g);
+            if (accumulate == 1) {
+                const float4 o = *dst;
+                v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
+            }
+            *dst = v;
+        }
+        return;
+    }
     for (int c = threadIdx.x; c < tcells; c += kAccumThreads) {
         const int lx = c % a.tx, ly = (c / a.tx) % a.ty, lz = c / (a.tx * a.ty);
         if (x0 + lx >= a.nx || y0 + ly >= a.ny || z0 + lz >= a.nz) continue;  // partial edge tile
         const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
-        const float v = (float)ldexp((double)(long long)tile[c], -e);
+        const float v = cell(c);
         rho[g] = accumulate == 1 ? rho[g] + v : v;
     }
 }
 
 // ------------------------------------------------ blocked counting sort
 //
-// Binning without global atomics.  Particles are cut into blocks of
-// kSortBlock; block b builds an LDS histogram of the tiles its particles
-// touch and stores it as row H[b][*].  A column scan turns H into per-block
-// offsets inside each tile, a scan over tile totals gives the tile starts,
-// and the place pass ranks each particle within its block with LDS atomics.
+// Binning without global atomics.  Particles are cut into blocks (at most
+// 512 equal ranges of each catalog); block b builds an LDS histogram of the
+// tiles its particles touch and stores it as row H[b][*].  A column scan
+// turns H into per-block offsets inside each tile, a scan over tile totals
+// gives the tile starts, and the place pass ranks each particle within its
+// block with LDS atomics.
 
 constexpr int kSortThreads = 1024;
-constexpr int kSortBlock = 32768;          // particles per block
+constexpr int kSortBlockMin = 8192;        // particles per block, at least
+constexpr int kSortMaxBlocks = 512;        // two 64 KB-histogram blocks per CU on 256 CUs
 constexpr int kSortMaxTiles = 16384;        // LDS histogram capacity (64 KB)
+
+// Particles [beg, end) of block blk of a single catalog of n particles cut
+// into nblk equal ranges.
+__device__ __forceinline__ void sort_range(int64_t n, int nblk, int blk, int64_t &beg, int64_t &end) {
+    beg = n * blk / nblk;
+    end = n * (blk + 1) / nblk;
+}
 
 __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint32_t *H,
                                                             const int *blk_cat,
@@ -328,8 +374,13 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
     __shared__ uint32_t hist[kSortMaxTiles];
     const int blk = blockIdx.x;
     const int cat = blk_cat ? blk_cat[blk] : 0;
-    const int64_t beg = blk_beg ? blk_beg[blk] : (int64_t)blk * kSortBlock;
-    const int64_t end = blk_beg ? blk_beg[blk + 1] : min(beg + (int64_t)kSortBlock, a.ntotal);
+    int64_t beg, end;
+    if (blk_beg) {
+        beg = blk_beg[blk];
+        end = blk_beg[blk + 1];
+    } else {
+        sort_range(a.ntotal, gridDim.x, blk, beg, end);
+    }
     const int T = a.ntx * a.nty * a.ntz;
     for (int i = threadIdx.x; i < T; i += kSortThreads) hist[i] = 0u;
     float wm = 0.f;
@@ -404,8 +455,13 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, cons
     __shared__ uint32_t cur[kSortMaxTiles];
     const int blk = blockIdx.x;
     const int cat = blk_cat ? blk_cat[blk] : 0;
-    const int64_t beg = blk_beg ? blk_beg[blk] : (int64_t)blk * kSortBlock;
-    const int64_t end = blk_beg ? blk_beg[blk + 1] : min(beg + (int64_t)kSortBlock, a.ntotal);
+    int64_t beg, end;
+    if (blk_beg) {
+        beg = blk_beg[blk];
+        end = blk_beg[blk + 1];
+    } else {
+        sort_range(a.ntotal, gridDim.x, blk, beg, end);
+    }
     const int T = a.ntx * a.nty * a.ntz;
     const uint32_t *row = H + (int64_t)blk * T;
     const uint32_t *start = a.cursor + (int64_t)cat * T;
@@ -435,20 +491,34 @@ __global__ void k_add_count(uint32_t *cursor, const uint32_t *count, int64_t n) 
     if (i < n) cursor[i] += count[i];
 }
 
-int sort_nblk(int64_t n) { return (int)((n + kSortBlock - 1) / kSortBlock); }
+// Blocks of a single catalog: equal ranges of at least kSortBlockMin
+// particles, at most kSortMaxBlocks (one wave of blocks, no ragged tail).
+int sort_nblk(int64_t n) {
+    const int64_t b = (n + kSortBlockMin - 1) / kSortBlockMin;
+    return (int)(b < kSortMaxBlocks ? b : kSortMaxBlocks);
+}
 
+// Blocks of several catalogs: each catalog is cut into equal ranges of about
+// max(kSortBlockMin, n / (kSortMaxBlocks - ncat)) particles, so that the
+// per-catalog rounding up still leaves at most kSortMaxBlocks blocks (a block
+// never spans two catalogs).
 int sort_blocks(int64_t n, const int64_t *offsets_host, int ncat, std::vector<int> &blk_cat,
                 std::vector<int64_t> &blk_beg, std::vector<int> &cat_blk0) {
     blk_cat.clear();
     blk_beg.clear();
     cat_blk0.assign(ncat + 1, 0);
+    const int64_t total = offsets_host ? offsets_host[ncat] - offsets_host[0] : n;
+    const int64_t slots = kSortMaxBlocks - ncat > 1 ? kSortMaxBlocks - ncat : 1;
+    int64_t chunk = (total + slots - 1) / slots;
+    if (chunk < kSortBlockMin) chunk = kSortBlockMin;
     for (int c = 0; c < ncat; c++) {
         const int64_t b0 = offsets_host ? offsets_host[c] : 0;
         const int64_t b1 = offsets_host ? offsets_host[c + 1] : n;
         cat_blk0[c] = (int)blk_cat.size();
-        for (int64_t s = b0; s < b1; s += kSortBlock) {
+        const int64_t nb = (b1 - b0 + chunk - 1) / chunk;
+        for (int64_t k = 0; k < nb; k++) {
             blk_cat.push_back(c);
-            blk_beg.push_back(s);
+            blk_beg.push_back(b0 + (b1 - b0) * k / nb);
         }
     }
     cat_blk0[ncat] = (int)blk_cat.size();
