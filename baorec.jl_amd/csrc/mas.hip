@@ -80,6 +80,35 @@ struct PartU {
     bool ok;
 };
 
+// Grid coordinates of particle i of catalog cat from its loaded position and
+// weight (wrapped in place first when apply_wrap; see load_particle).
+__device__ __forceinline__ PartU particle_at(const ScatterArgs &a, int cat, int64_t i, float x, float y, float z,
+                                             float w, bool apply_wrap, bool write_back) {
+    PartU p;
+    p.cat = cat;
+    const Slot &sl = a.slot[cat * a.slotStride];
+    if (apply_wrap) {  // src/mas.jl:56-60: one-sided, box_min[1]/box_size[1] for all axes
+        const float b0 = sl.box_min[0], L0 = sl.box_size[0];
+        const bool wx = (x - b0) > L0, wy = (y - b0) > L0, wz = (z - b0) > L0;
+        x = wx ? x - L0 : x;
+        y = wy ? y - L0 : y;
+        z = wz ? z - L0 : z;
+        if (write_back) {  // only the coordinates that moved are stored
+            if (wx) a.x[i] = x;
+            if (wy) a.y[i] = y;
+            if (wz) a.z[i] = z;
+        }
+    }
+    const float pp[3] = {x, y, z};
+    const int nn[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+    for (int k = 0; k < 3; k++)  // src/mas.jl:62-64
+        p.u[k] = (pp[k] - sl.box_min[k]) * (float)nn[k] / sl.box_size[k] + 1.0f;
+    p.w = w;
+    p.ok = true;
+    return p;
+}
+
 __device__ __forceinline__ PartU load_particle(const ScatterArgs &a, int64_t i, bool apply_wrap,
                                                bool write_back) {
     PartU p;
@@ -360,6 +389,7 @@ constexpr int kSortThreads = 1024;
 constexpr int kSortBlockMin = 8192;        // particles per block, at least
 constexpr int kSortMaxBlocks = 512;        // two 64 KB-histogram blocks per CU on 256 CUs
 constexpr int kSortMaxTiles = 16384;        // LDS histogram capacity (64 KB)
+constexpr int kSortUnroll = 4;             // particles in flight per thread
 
 // Particles [beg, end) of block blk of a single catalog of n particles cut
 // into nblk equal ranges.
@@ -385,21 +415,39 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
     for (int i = threadIdx.x; i < T; i += kSortThreads) hist[i] = 0u;
     float wm = 0.f;
     __syncthreads();
-    for (int64_t i = beg + threadIdx.x; i < end; i += kSortThreads) {
-        PartU p = load_particle(a, i, a.wrap != 0, a.wrap && a.mutate);
-        bool ok = true;
-        int tx[3], ty[3], tz[3];
-        const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
-        const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
-        const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
-        if (!ok) {
-            atomicOr(a.err, 1);
-            continue;
+    // kSortUnroll particles per thread per step: their loads are issued
+    // together, ahead of the LDS atomics
+    for (int64_t i0 = beg + threadIdx.x; i0 < end; i0 += kSortUnroll * kSortThreads) {
+        float px[kSortUnroll], py[kSortUnroll], pz[kSortUnroll], pw[kSortUnroll];
+#pragma unroll
+        for (int u = 0; u < kSortUnroll; u++) {
+            const int64_t i = i0 + (int64_t)u * kSortThreads;
+            if (i < end) {
+                px[u] = a.x[i];
+                py[u] = a.y[i];
+                pz[u] = a.z[i];
+                pw[u] = a.w[i];
+            }
         }
-        wm = fmaxf(wm, fabsf(p.w));
-        for (int k = 0; k < cz; k++)
-            for (int j = 0; j < cy; j++)
-                for (int l = 0; l < cx; l++) atomicAdd(&hist[tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
+#pragma unroll
+        for (int u = 0; u < kSortUnroll; u++) {
+            const int64_t i = i0 + (int64_t)u * kSortThreads;
+            if (i >= end) break;
+            PartU p = particle_at(a, cat, i, px[u], py[u], pz[u], pw[u], a.wrap != 0, a.wrap && a.mutate);
+            bool ok = true;
+            int tx[3], ty[3], tz[3];
+            const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
+            const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
+            const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
+            if (!ok) {
+                atomicOr(a.err, 1);
+                continue;
+            }
+            wm = fmaxf(wm, fabsf(p.w));
+            for (int k = 0; k < cz; k++)
+                for (int j = 0; j < cy; j++)
+                    for (int l = 0; l < cx; l++) atomicAdd(&hist[tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
+        }
     }
     for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o, 64));
     if ((threadIdx.x & 63) == 0 && wm > 0.f) atomicMax(&a.wmax[cat], __float_as_uint(wm));
@@ -467,21 +515,37 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, cons
     const uint32_t *start = a.cursor + (int64_t)cat * T;
     for (int i = threadIdx.x; i < T; i += kSortThreads) cur[i] = start[i] + row[i];
     __syncthreads();
-    for (int64_t i = beg + threadIdx.x; i < end; i += kSortThreads) {
-        PartU p = load_particle(a, i, a.wrap && !a.mutate, false);
-        bool ok = true;
-        int tx[3], ty[3], tz[3];
-        const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
-        const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
-        const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
-        if (!ok) continue;
-        const float4 rec = make_float4(p.u[0], p.u[1], p.u[2], p.w);
-        for (int k = 0; k < cz; k++)
-            for (int j = 0; j < cy; j++)
-                for (int l = 0; l < cx; l++) {
-                    const uint32_t slot = atomicAdd(&cur[tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
-                    a.sorted[slot] = rec;
-                }
+    for (int64_t i0 = beg + threadIdx.x; i0 < end; i0 += kSortUnroll * kSortThreads) {
+        float px[kSortUnroll], py[kSortUnroll], pz[kSortUnroll], pw[kSortUnroll];
+#pragma unroll
+        for (int u = 0; u < kSortUnroll; u++) {
+            const int64_t i = i0 + (int64_t)u * kSortThreads;
+            if (i < end) {
+                px[u] = a.x[i];
+                py[u] = a.y[i];
+                pz[u] = a.z[i];
+                pw[u] = a.w[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSortUnroll; u++) {
+            const int64_t i = i0 + (int64_t)u * kSortThreads;
+            if (i >= end) break;
+            PartU p = particle_at(a, cat, i, px[u], py[u], pz[u], pw[u], a.wrap && !a.mutate, false);
+            bool ok = true;
+            int tx[3], ty[3], tz[3];
+            const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
+            const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
+            const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
+            if (!ok) continue;
+            const float4 rec = make_float4(p.u[0], p.u[1], p.u[2], p.w);
+            for (int k = 0; k < cz; k++)
+                for (int j = 0; j < cy; j++)
+                    for (int l = 0; l < cx; l++) {
+                        const uint32_t slot = atomicAdd(&cur[tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
+                        a.sorted[slot] = rec;
+                    }
+        }
     }
 }
 
