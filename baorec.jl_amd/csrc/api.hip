@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <array>
 #include <map>
 #include <string>
 #include <vector>
@@ -49,6 +50,12 @@ struct baorec_ctx {
         double ms = 0.0, bytes = 0.0;
     };
     std::map<std::string, Agg> prof;
+    // autotuned choices (baorec_autotune, baorec_tune_load): scatter tile per
+    // (nx, ny, nz, tsc, floor(log2 n)); strided kernel per (axis, line length)
+    std::map<std::array<int64_t, 5>, std::array<int, 3>> tune_tile;
+    std::map<std::array<int, 2>, int> tune_pair;
+    int force_tile[3] = {0, 0, 0};   // set while tuning
+    int force_pair = -1;
     // comoving-distance table of the last cosmology used
     int cosmo_valid = 0;
     baorec_cosmology cosmo{};
@@ -272,10 +279,37 @@ Slot read_slot(baorec_ctx *c, const Slot *s) {
 
 // ----------------------------------------------------------------- scatter
 
-void tiles_for(const Grid &G, ScatterArgs &a) {
+int ilog2(int64_t n) {
+    int k = 0;
+    while (n > 1) {
+        n >>= 1;
+        k++;
+    }
+    return k;
+}
+
+std::array<int64_t, 5> tile_key(const Grid &G, int tsc, int64_t n) {
+    return {G.nx, G.ny, G.nz, tsc, ilog2(n)};
+}
+
+// Scatter tile shape: the one being timed while tuning, a tuned one for this
+// grid and catalog size, or 32 x 16 x 16.
+void tiles_for(baorec_ctx *c, const Grid &G, ScatterArgs &a, int64_t n) {
     a.tx = G.nx < 32 ? G.nx : 32;
     a.ty = G.ny < 16 ? G.ny : 16;
     a.tz = G.nz < 16 ? G.nz : 16;
+    if (c->force_tile[0]) {
+        a.tx = c->force_tile[0];
+        a.ty = c->force_tile[1];
+        a.tz = c->force_tile[2];
+    } else {
+        auto it = c->tune_tile.find(tile_key(G, a.tsc, n));
+        if (it != c->tune_tile.end()) {
+            a.tx = it->second[0];
+            a.ty = it->second[1];
+            a.tz = it->second[2];
+        }
+    }
     a.ntx = (G.nx + a.tx - 1) / a.tx;
     a.nty = (G.ny + a.ty - 1) / a.ty;
     a.ntz = (G.nz + a.tz - 1) / a.tz;
@@ -301,7 +335,7 @@ void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, f
     a.wrap = wrap;
     a.mutate = p->mutate_inputs;
     a.tsc = p->mas == BAOREC_MAS_TSC;
-    tiles_for(G, a);
+    tiles_for(c, G, a, n);
     const int64_t ntiles = (int64_t)a.ntx * a.nty * a.ntz;
     a.count = bufT<uint32_t>(c, "sc_count", ntiles);
     a.cursor = bufT<uint32_t>(c, "sc_cursor", ntiles);
@@ -349,7 +383,14 @@ void tables(baorec_ctx *c, const Grid &G, const Slot *slot, FFTBufs &f) {
     hipchk(c, launch_tables(slot, f.ktab, f.xtab, G.nx, G.ny, G.nz, 1, c->stream), "tables");
 }
 
-void strided(baorec_ctx *c, const Grid &G, const StridedArgs &a, int njobs, int nbatch, const char *what) {
+void strided(baorec_ctx *c, const Grid &G, const StridedArgs &a0, int njobs, int nbatch, const char *what) {
+    StridedArgs a = a0;
+    // kernel choice of single-output passes: being timed, tuned, or the rule
+    a.pair = c->force_pair;
+    if (a.pair < 0) {
+        auto it = c->tune_pair.find({a.axis, a.axis == 1 ? G.ny : G.nz});
+        if (it != c->tune_pair.end()) a.pair = it->second;
+    }
     double bytes = 0.0;
     for (int j = 0; j < njobs; j++) bytes += 8.0 * G.Cdense() * (1 + a.job[j].nout);
     const bool pair = strided_uses_pair(a, njobs);
@@ -1127,7 +1168,7 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
     a.wrap = 1;
     a.mutate = p->mutate_inputs;
     a.tsc = p->mas == BAOREC_MAS_TSC;
-    tiles_for(G, a);
+    tiles_for(c, G, a, nb > 0 ? ntot / nb : 0);
     const int64_t T = (int64_t)a.ntx * a.nty * a.ntz;
     a.count = bufT<uint32_t>(c, "bt_count", (size_t)maxnb * T);
     a.cursor = bufT<uint32_t>(c, "bt_cursor", (size_t)maxnb * T);
@@ -1297,6 +1338,27 @@ void cosmo_check(baorec_ctx *c, int *e, const char *what) {
 }  // namespace
 
 // ====================================================================== ABI
+
+// Milliseconds of the fastest of `reps` runs of fn on the context's stream
+// (after one untimed run).
+template <class F>
+double time_min(baorec_ctx *c, int reps, F &&fn) {
+    fn();
+    hipEvent_t a = take_event(c), b = take_event(c);
+    double best = 1e30;
+    for (int r = 0; r < reps; r++) {
+        hipchk(c, hipEventRecord(a, c->stream), "hipEventRecord");
+        fn();
+        hipchk(c, hipEventRecord(b, c->stream), "hipEventRecord");
+        hipchk(c, hipEventSynchronize(b), "hipEventSynchronize");
+        float ms = 0.f;
+        hipchk(c, hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        best = ms < best ? ms : best;
+    }
+    c->event_pool.push_back(a);
+    c->event_pool.push_back(b);
+    return best;
+}
 
 extern "C" {
 
@@ -2118,6 +2180,168 @@ int baorec_power_multipoles(baorec_ctx *c, const baorec_pk_params *pk, const int
             }
         }
     });
+}
+
+// ---------------------------------------------------------------- autotune
+
+int baorec_autotune(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, const float *x,
+                    const float *y, const float *z, const float *w, int64_t n) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, true);
+        if (mem != BAOREC_MEM_HOST && mem != BAOREC_MEM_DEVICE) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "mem");
+        if (n <= 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "the catalog is empty");
+        if (c->prof_on) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "profiling is on");
+        baorec_params q = *p;
+        q.mutate_inputs = 0;   // the timed scatters leave the inputs untouched
+        float *dx = const_cast<float *>(stage_in(c, mem, x, n, "tune_x"));
+        float *dy = const_cast<float *>(stage_in(c, mem, y, n, "tune_y"));
+        float *dz = const_cast<float *>(stage_in(c, mem, z, n, "tune_z"));
+        const float *dw = stage_in(c, mem, w, n, "tune_w");
+        Slot *slot = slot_buf(c) + 2;
+        float *rho = bufT<float>(c, "tune_rho", G.R());
+        const int tsc = q.mas == BAOREC_MAS_TSC;
+        // scatter tiles: every shape with sides in {16, 32, 64} x {8..64} x
+        // {8..64} that fits the LDS tile and the sort's histogram
+        std::array<int, 3> best{0, 0, 0};
+        double best_ms = 1e30;
+        for (int tx : {16, 32, 64})
+            for (int ty : {8, 16, 32, 64})
+                for (int tz : {8, 16, 32, 64}) {
+                    if (tx > G.nx || ty > G.ny || tz > G.nz || tx * ty * tz > scatter_tile_max()) continue;
+                    const int64_t T = (int64_t)((G.nx + tx - 1) / tx) * ((G.ny + ty - 1) / ty) * ((G.nz + tz - 1) / tz);
+                    if (T > 16384) continue;
+                    c->force_tile[0] = tx;
+                    c->force_tile[1] = ty;
+                    c->force_tile[2] = tz;
+                    const double ms = time_min(c, 3, [&] {
+                        slot_from_params(c, slot, &q, 0.0);
+                        scatter(c, G, &q, slot, rho, dx, dy, dz, dw, n, 1, 0, "tune");
+                    });
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = {tx, ty, tz};
+                    }
+                }
+        c->force_tile[0] = c->force_tile[1] = c->force_tile[2] = 0;
+        if (best[0]) c->tune_tile[tile_key(G, tsc, n)] = best;
+        // strided kernel of single-output passes, per axis of length >= 256
+        FFTBufs f = fft_bufs(c, G);
+        tables(c, G, slot, f);
+        hipchk(c, launch_zero(reinterpret_cast<float *>(f.cw), 2 * G.C(), c->stream), "zero");
+        for (int axis = 1; axis <= 2; axis++) {
+            const int M = axis == 1 ? G.ny : G.nz;
+            if (M < 256) continue;
+            double t[2];
+            for (int pr = 0; pr < 2; pr++) {
+                c->force_pair = pr;
+                t[pr] = time_min(c, 3, [&] { strided_plain(c, G, f, axis, false, f.cw, 1); });
+            }
+            c->force_pair = -1;
+            c->tune_pair[{axis, M}] = t[1] < t[0] ? 1 : 0;
+        }
+        sync(c);
+    });
+}
+
+int64_t baorec_tune_report(baorec_ctx *c, char *buf, int64_t len) {
+    if (!c) return BAOREC_ERR_INVALID_ARGUMENT;
+    std::string js = "{\"scatter\": [";
+    bool first = true;
+    for (const auto &e : c->tune_tile) {
+        char line[256];
+        snprintf(line, sizeof line,
+                 "%s\n  {\"grid\": [%lld, %lld, %lld], \"tsc\": %lld, \"log2n\": %lld, \"tile\": [%d, %d, %d]}",
+                 first ? "" : ",", (long long)e.first[0], (long long)e.first[1], (long long)e.first[2],
+                 (long long)e.first[3], (long long)e.first[4], e.second[0], e.second[1], e.second[2]);
+        js += line;
+        first = false;
+    }
+    js += "],\n \"strided\": [";
+    first = true;
+    for (const auto &e : c->tune_pair) {
+        char line[128];
+        snprintf(line, sizeof line, "%s\n  {\"axis\": %d, \"n\": %d, \"pair\": %d}", first ? "" : ",", e.first[0],
+                 e.first[1], e.second);
+        js += line;
+        first = false;
+    }
+    js += "]}\n";
+    if (buf && len > 0) {
+        strncpy(buf, js.c_str(), (size_t)len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int64_t)js.size() + 1;
+}
+
+int baorec_tune_save(baorec_ctx *c, const char *path) {
+    if (!c) return BAOREC_ERR_INVALID_ARGUMENT;
+    if (!path) {
+        c->err = "path is NULL";
+        return BAOREC_ERR_INVALID_ARGUMENT;
+    }
+    const int64_t len = baorec_tune_report(c, nullptr, 0);
+    std::string js((size_t)len, '\0');
+    baorec_tune_report(c, &js[0], len);
+    js.resize((size_t)len - 1);
+    FILE *fp = fopen(path, "w");
+    if (!fp) {
+        c->err = std::string("cannot create ") + path;
+        return BAOREC_ERR_INVALID_ARGUMENT;
+    }
+    const bool ok = fwrite(js.data(), 1, js.size(), fp) == js.size();
+    if (fclose(fp) != 0 || !ok) {
+        c->err = std::string("write failed: ") + path;
+        return BAOREC_ERR_INVALID_ARGUMENT;
+    }
+    return BAOREC_OK;
+}
+
+// Reads the entries baorec_tune_save writes (one object per line); entries
+// with sizes the library cannot use are rejected.
+int baorec_tune_load(baorec_ctx *c, const char *path) {
+    if (!c) return BAOREC_ERR_INVALID_ARGUMENT;
+    FILE *fp = path ? fopen(path, "r") : nullptr;
+    if (!fp) {
+        c->err = std::string("cannot open ") + (path ? path : "(NULL)");
+        return BAOREC_ERR_INVALID_ARGUMENT;
+    }
+    std::map<std::array<int64_t, 5>, std::array<int, 3>> tiles;
+    std::map<std::array<int, 2>, int> pairs;
+    char line[512];
+    bool bad = false;
+    while (fgets(line, sizeof line, fp)) {
+        long long g0, g1, g2, tsc, l2;
+        int t0, t1, t2, ax, nn, pr;
+        const char *s = strchr(line, '{');
+        if (!s) continue;
+        if (sscanf(s, " {\"grid\": [%lld, %lld, %lld], \"tsc\": %lld, \"log2n\": %lld, \"tile\": [%d, %d, %d]}", &g0,
+                   &g1, &g2, &tsc, &l2, &t0, &t1, &t2) == 8) {
+            const int64_t T = ((g0 + t0 - 1) / (t0 > 0 ? t0 : 1)) * ((g1 + t1 - 1) / (t1 > 0 ? t1 : 1)) *
+                              ((g2 + t2 - 1) / (t2 > 0 ? t2 : 1));
+            if (t0 < 1 || t1 < 1 || t2 < 1 || (int64_t)t0 * t1 * t2 > scatter_tile_max() || T > 16384 || t0 > g0 ||
+                t1 > g1 || t2 > g2) {
+                bad = true;
+                break;
+            }
+            tiles[{g0, g1, g2, tsc, l2}] = {t0, t1, t2};
+        } else if (sscanf(s, " {\"axis\": %d, \"n\": %d, \"pair\": %d}", &ax, &nn, &pr) == 3) {
+            if ((ax != 1 && ax != 2) || (pr != 0 && pr != 1)) {
+                bad = true;
+                break;
+            }
+            pairs[{ax, nn}] = pr;
+        }
+    }
+    fclose(fp);
+    if (bad) {
+        c->err = std::string(path) + ": entry outside what the library supports";
+        return BAOREC_ERR_INVALID_ARGUMENT;
+    }
+    for (const auto &e : tiles) c->tune_tile[e.first] = e.second;
+    for (const auto &e : pairs) c->tune_pair[e.first] = e.second;
+    return BAOREC_OK;
 }
 
 int baorec_profile(baorec_ctx *c, int enable) {

@@ -752,6 +752,7 @@ hipError_t launch_x_c2r(const XInvArgs &a, int epi, int nbatch, hipStream_t s) {
 }
 
 bool strided_uses_pair(const StridedArgs &a, int njobs) {
+    if (a.pair == 0) return false;
     const int M = a.axis == 1 ? a.ny : a.nz;
     bool single = true;
     for (int j = 0; j < njobs; j++) single = single && a.job[j].nout == 1;

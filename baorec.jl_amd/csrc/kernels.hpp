@@ -47,6 +47,8 @@ struct StridedArgs {
     int mul;                   // MulKind (mode 2)
     float R2;                  // smoothing radius squared (FP32)
     float los[3];
+    int pair;                  // kernel of single-output jobs: -1 the library's rule,
+                               // 0 one column per thread, 1 two columns per thread
 };
 
 enum XEpi : int {
@@ -174,6 +176,8 @@ hipError_t launch_sort(ScatterArgs &a, int nblk, uint32_t *H, const int *blk_cat
                        const int *cat_blk0, hipStream_t s);
 // Capacity (placements) the sorted buffer must hold for n particles.
 int64_t scatter_capacity(int64_t n, int tsc);
+// Cells of the largest tile k_scatter_accum holds in LDS.
+int scatter_tile_max();
 
 struct GatherArgs {
     const float *x, *y, *z;

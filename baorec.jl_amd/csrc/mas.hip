@@ -630,6 +630,8 @@ hipError_t launch_zero(float *a, int64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+int scatter_tile_max() { return kTileMax; }
+
 int64_t scatter_capacity(int64_t n, int tsc) {
     // worst case: every particle touches 8 (CIC) or 27 (TSC) tiles; in
     // practice ~1.2x for 32x16x16 tiles.  The bound used is exact-safe.

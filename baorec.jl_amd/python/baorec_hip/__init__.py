@@ -28,6 +28,10 @@ from .api import (  # noqa: F401
     reconstructed_overdensity_,
     reconstructed_positions,
     run_batch,
+    autotune,
+    tune_report,
+    tune_save,
+    tune_load,
     reconstructed_potential_,
     reduce_,
     residual_,

@@ -144,6 +144,10 @@ SIGNATURES = {
     "baorec_run_batch": (INT, [P, PP, I32P, INT, INT, FP, FP, FP, FP, FP, FP, FP, FP]),
     "baorec_profile": (INT, [P, INT]),
     "baorec_profile_report": (I64, [P, ctypes.c_char_p, I64]),
+    "baorec_autotune": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, I64]),
+    "baorec_tune_report": (I64, [P, ctypes.c_char_p, I64]),
+    "baorec_tune_save": (INT, [P, ctypes.c_char_p]),
+    "baorec_tune_load": (INT, [P, ctypes.c_char_p]),
 }
 
 _lib = None

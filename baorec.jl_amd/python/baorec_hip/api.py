@@ -385,6 +385,40 @@ def run_batch(recon: IterativeRecon, grid_size, x, y, z, w, offsets):
     return tuple(outs)
 
 
+def autotune(recon: _Recon, grid_size, x, y, z, w):
+    """Extension (no reference counterpart): times the library's scatter
+    tile shapes and strided-pass kernels on this catalog and keeps the
+    fastest in the recon's context (results are bitwise the same for every
+    choice).  Needs recon.box_size / box_min.  Returns the tuned choices."""
+    ctx = recon.context()
+    x, y, z, w = (_f32(a) for a in (x, y, z, w))
+    n = len(x)
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_autotune(ctx.handle, ctypes.byref(p), _grid(grid_size), _mem(x, y, z, w), _ptr(x),
+                                      _ptr(y), _ptr(z), _ptr(w), n))
+    return tune_report(ctx)
+
+
+def tune_report(ctx: Optional[Context] = None) -> dict:
+    """The context's tuned choices (as baorec_tune_save writes them)."""
+    import json
+    ctx = ctx or default_context()
+    need = ctx.lib.baorec_tune_report(ctx.handle, None, 0)
+    buf = ctypes.create_string_buffer(int(need))
+    ctx.lib.baorec_tune_report(ctx.handle, buf, need)
+    return json.loads(buf.value.decode())
+
+
+def tune_save(path, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    ctx.check(ctx.lib.baorec_tune_save(ctx.handle, os.fsencode(path)))
+
+
+def tune_load(path, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    ctx.check(ctx.lib.baorec_tune_load(ctx.handle, os.fsencode(path)))
+
+
 # ------------------------------------------------------------------ utils
 
 def k_vec(grid, box_size):

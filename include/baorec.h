@@ -386,6 +386,32 @@ int baorec_run_batch(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3],
                      int mem, int ncat, const int64_t *offsets, float *x, float *y,
                      float *z, const float *w, float *ox, float *oy, float *oz);
 
+/* ------------------------------------------------------------- autotuning */
+
+/*
+ * Extension, no reference counterpart (the reference has no launch shapes to
+ * tune).  Times the library's runtime choices on this device with the given
+ * catalog and keeps the fastest in the context, where later calls use them:
+ *  - the scatter tile shape (every tile of 16..64 x 8..64 x 8..64 cells that
+ *    fits the LDS tile and the sort's histogram) for this grid, mass
+ *    assignment and catalog size class (floor(log2 n));
+ *  - per FFT line length >= 256 and axis, the kernel of single-output strided
+ *    passes (one or two columns per thread).
+ * The transforms are identical for every choice; the density grids agree
+ * to the fixed-point quantum of the scatter (2^-61 of a tile's total weight,
+ * in practice bitwise).  Needs box_size and box_min in params; the inputs
+ * are not modified.
+ */
+int baorec_autotune(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3], int mem,
+                    const float *x, const float *y, const float *z, const float *w, int64_t n);
+/* The tuned choices as JSON into buf; returns the length needed including
+ * the terminating NUL. */
+int64_t baorec_tune_report(baorec_ctx *ctx, char *buf, int64_t len);
+/* Writes the tuned choices to a JSON file / adds those of a file written by
+ * baorec_tune_save (entries the library cannot use are rejected). */
+int baorec_tune_save(baorec_ctx *ctx, const char *path);
+int baorec_tune_load(baorec_ctx *ctx, const char *path);
+
 /* ------------------------------------------------------------- profiling */
 
 /* Extension, no reference counterpart (the reference times whole calls with
