@@ -2230,13 +2230,28 @@ int baorec_autotune(baorec_ctx *c, const baorec_params *p, const int32_t grid[3]
         FFTBufs f = fft_bufs(c, G);
         tables(c, G, slot, f);
         hipchk(c, launch_zero(reinterpret_cast<float *>(f.cw), 2 * G.C(), c->stream), "zero");
+        // the single-output passes as the reconstruction runs them: y forward
+        // and inverse; z forward * Gaussian * inverse
         for (int axis = 1; axis <= 2; axis++) {
             const int M = axis == 1 ? G.ny : G.nz;
             if (M < 256) continue;
             double t[2];
             for (int pr = 0; pr < 2; pr++) {
                 c->force_pair = pr;
-                t[pr] = time_min(c, 3, [&] { strided_plain(c, G, f, axis, false, f.cw, 1); });
+                t[pr] = time_min(c, 5, [&] {
+                    if (axis == 1) {
+                        strided_plain(c, G, f, 1, false, f.cw, 1);
+                        strided_plain(c, G, f, 1, true, f.cw, 1);
+                    } else {
+                        StridedArgs sa = sargs(G, f, 2, 2, MUL_GAUSS);
+                        sa.R2 = q.smoothing_radius * q.smoothing_radius;
+                        sa.job[0].in = f.cw;
+                        sa.job[0].out[0] = f.cw;
+                        sa.job[0].nout = 1;
+                        sa.job[0].pw[0] = 0;
+                        strided(c, G, sa, 1, 1, "tune z");
+                    }
+                });
             }
             c->force_pair = -1;
             c->tune_pair[{axis, M}] = t[1] < t[0] ? 1 : 0;

@@ -100,6 +100,26 @@ def gpu_bench(args, data, rand, cfg):
 
     run = lib.baorec_run_multigrid if multigrid else lib.baorec_run_iterative
     kind = L.MULTIGRID if multigrid else L.ITERATIVE
+    tuned = None
+    if args.autotune:
+        # tune the scatter of each catalog (its size class) with the box run!
+        # would set up from the randoms, and the FFT passes
+        q = L.Params()
+        ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(p))
+        if rand is not None:
+            bs, bm = np.zeros(3, np.float32), np.zeros(3, np.float32)
+            ctx.check(lib.baorec_setup_box(ctx.handle, L.MEM_DEVICE, vp(r[0]), vp(r[1]), vp(r[2]), nr,
+                                           float(cfg["pad"]), bs.ctypes.data_as(ctypes.c_void_p),
+                                           bm.ctypes.data_as(ctypes.c_void_p)))
+            for a in range(3):
+                q.box_size[a], q.box_min[a] = float(bs[a]), float(bm[a])
+        t0 = time.perf_counter()
+        for src, nsrc in ((d, n), (r, nr)):
+            if nsrc:
+                ctx.check(lib.baorec_autotune(ctx.handle, ctypes.byref(q), grid, L.MEM_DEVICE, vp(src[0]),
+                                              vp(src[1]), vp(src[2]), vp(src[3]), nsrc))
+        tuned = bh.tune_report(ctx)
+        log(f"autotuned in {time.perf_counter() - t0:.1f} s: {tuned}")
 
     def step():
         rc = run(ctx.handle, ctypes.byref(p), grid, L.MEM_DEVICE, vp(d[0]), vp(d[1]),
@@ -158,7 +178,7 @@ def gpu_bench(args, data, rand, cfg):
     mg = {"schedule": cfg["schedule"], "vcycles": p.vcycles_done, "final_residual": p.final_residual,
           "final_change": p.final_change} if multigrid else None
     return dict(dt=dt, prof=prof, finite=finite, box=(list(p.box_size), list(p.box_min)), dt_host=dt_host,
-                mg=mg)
+                mg=mg, tuned=tuned)
 
 
 def kernel_symbol(label, grid, rand):
@@ -499,6 +519,15 @@ def batch_bench(args):
         ctx.check(lib.baorec_run_batch(ctx.handle, ctypes.byref(p), grid, L.MEM_DEVICE, ncat, offp,
                                        *(vp(a) for a in d), *(vp(o) for o in out)))
 
+    tuned = None
+    if args.autotune:
+        # a batch looks its scatter tile up by the mean catalog size: tune on
+        # that many particles (the first catalogs' particles)
+        ctx.check(lib.baorec_autotune(ctx.handle, ctypes.byref(p), grid, L.MEM_DEVICE, *(vp(a) for a in d),
+                                      int(off[-1]) // ncat))
+        tuned = bh.tune_report(ctx)
+        log(f"autotuned: {tuned}")
+
     h = hip()
     for i in range(args.warmup):
         step()
@@ -536,7 +565,7 @@ def batch_bench(args):
             "kernels": {k: {"count": v["count"], "ms": round(v["ms"], 3),
                             "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()},
             "profiled_step": {"kernel_ms": round(total_ms, 3), "algorithmic_GB": round(total_bytes / 1e9, 3)},
-            "outputs_finite": finite}
+            "outputs_finite": finite, "autotuned": tuned}
     print(json.dumps(line), flush=True)
 
 
@@ -567,6 +596,8 @@ def main():
     ap.add_argument("--gen-workers", type=int, default=8,
                     help="processes generating the batch64 catalogs (1 under a profiler)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run baorec_autotune on the workload's catalog before timing")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run 1 GiB device-to-device copies (byte-counter calibration)")
     args = ap.parse_args()
@@ -628,6 +659,7 @@ def main():
                         "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()},
         "profiled_step": {"kernel_ms": round(total_ms, 3), "algorithmic_GB": round(total_bytes / 1e9, 3)},
         "outputs_finite": g["finite"],
+        "autotuned": g["tuned"],
         "pcie_inclusive": {"value": round(1.0 / g["dt_host"], 4), "unit": "recons/s",
                            "note": "catalogs in host memory, copied in and out by the library each step"},
     }
