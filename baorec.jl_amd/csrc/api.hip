@@ -2193,6 +2193,14 @@ int baorec_autotune(baorec_ctx *c, const baorec_params *p, const int32_t grid[3]
         if (mem != BAOREC_MEM_HOST && mem != BAOREC_MEM_DEVICE) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "mem");
         if (n <= 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "the catalog is empty");
         if (c->prof_on) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "profiling is on");
+        // the forced choices end with this call, also when it fails
+        struct Unforce {
+            baorec_ctx *c;
+            ~Unforce() {
+                c->force_tile[0] = c->force_tile[1] = c->force_tile[2] = 0;
+                c->force_pair = -1;
+            }
+        } unforce{c};
         baorec_params q = *p;
         q.mutate_inputs = 0;   // the timed scatters leave the inputs untouched
         float *dx = const_cast<float *>(stage_in(c, mem, x, n, "tune_x"));
@@ -2225,6 +2233,7 @@ int baorec_autotune(baorec_ctx *c, const baorec_params *p, const int32_t grid[3]
                     }
                 }
         c->force_tile[0] = c->force_tile[1] = c->force_tile[2] = 0;
+        check_err_flag(c, slot);   // a particle outside the grid: BoundsError, nothing stored
         if (best[0]) c->tune_tile[tile_key(G, tsc, n)] = best;
         // strided kernel of single-output passes, per axis of length >= 256
         FFTBufs f = fft_bufs(c, G);

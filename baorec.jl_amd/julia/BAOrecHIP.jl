@@ -205,4 +205,157 @@ reconstructed_positions(recon::AbstractRecon, x, y, z, field3d; field = :disp,
 reconstructed_positions(recon::AbstractRecon, x, y, z; field = :disp, ctx::Context = default_context()) =
     _shifts(:baorec_reconstructed_positions, recon, x, y, z, nothing, field, ctx)
 
+# ------------------------------------------------------------ batches
+
+# A driver loop of run!(recon, grid, x,y,z,w) + reconstructed_positions(...;
+# field = :sum) over periodic-box catalogs laid end to end: catalog c is
+# x[offsets[c]+1 : offsets[c+1]] (offsets has ncat + 1 entries, offsets[1] == 0).
+# Returns the reconstructed positions (bitwise those of the loop); x, y, z are
+# wrapped in place as run! wraps them.
+function run_batch!(recon::IterativeRecon, grid_size::Tuple{Int,Int,Int}, x, y, z, w,
+                    offsets::Vector{Int64}; ctx::Context = default_context())
+    out = (similar(x), similar(y), similar(z))
+    st = ccall((:baorec_run_batch, libbaorec), Cint,
+               (Ptr{Cvoid}, Ref{Params}, Ptr{Int32}, Cint, Cint, Ptr{Int64}, Ptr{Float32}, Ptr{Float32},
+                Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}),
+               ctx.ptr, params(recon), _grid(grid_size), _mem(x, y, z, w, out...), length(offsets) - 1,
+               offsets, _ptr(x), _ptr(y), _ptr(z), _ptr(w), _ptr(out[1]), _ptr(out[2]), _ptr(out[3]))
+    check(ctx, st)
+    out
+end
+
+# Times the scatter tiles and strided-pass kernels on this catalog and keeps
+# the fastest in the context (extension; results unchanged).
+function autotune!(recon::AbstractRecon, grid_size::Tuple{Int,Int,Int}, x, y, z, w;
+                   ctx::Context = default_context())
+    check(ctx, ccall((:baorec_autotune, libbaorec), Cint,
+                     (Ptr{Cvoid}, Ref{Params}, Ptr{Int32}, Cint, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
+                      Ptr{Float32}, Int64),
+                     ctx.ptr, params(recon), _grid(grid_size), _mem(x, y, z, w), _ptr(x), _ptr(y), _ptr(z),
+                     _ptr(w), length(x)))
+end
+tune_save(path::AbstractString; ctx::Context = default_context()) =
+    check(ctx, ccall((:baorec_tune_save, libbaorec), Cint, (Ptr{Cvoid}, Cstring), ctx.ptr, path))
+tune_load(path::AbstractString; ctx::Context = default_context()) =
+    check(ctx, ccall((:baorec_tune_load, libbaorec), Cint, (Ptr{Cvoid}, Cstring), ctx.ptr, path))
+
+# ------------------------------------------------------------ cosmology and sky
+
+# Mirrors struct baorec_cosmology; Cosmology(BAOrec.Cosmology) copies the
+# reference's fields (src/cosmo.jl:22-62).
+mutable struct CosmologyC
+    omega_b::Float32
+    omega_c::Float32
+    h::Float32
+    Neff::Float32
+    Omega_k0::Float32
+    T_cmb::Float32
+    w0::Float32
+    wa::Float32
+    z_tab_min::Float64
+    z_tab_max::Float64
+    z_tab_num::Int32
+    CosmologyC() = new()
+end
+
+# The parameters of a BAOrec.Cosmology (src/cosmo.jl:22-62; the derived
+# densities are recomputed by the library in the same precision)
+function cosmology(c::BAOrec.Cosmology)
+    k = CosmologyC()
+    ccall((:baorec_cosmology_default, libbaorec), Cvoid, (Ref{CosmologyC},), k)
+    k.omega_b, k.omega_c, k.h, k.Neff = c.ω_b, c.ω_c, c.h, c.Neff
+    k.Omega_k0, k.T_cmb, k.w0, k.wa = c.Ω_k₀, c.T_cmb, c.w0, c.wa
+    k.z_tab_min, k.z_tab_max, k.z_tab_num = c.z_tab_min, c.z_tab_max, c.z_tab_num
+    k
+end
+
+# comoving_distance_interp(c)(z), src/cosmo.jl:86-93 (Mpc)
+function comoving_distance(z::Vector{Float64}, c::CosmologyC; ctx::Context = default_context())
+    r = similar(z)
+    check(ctx, ccall((:baorec_comoving_distance, libbaorec), Cint,
+                     (Ptr{Cvoid}, Ref{CosmologyC}, Cint, Ptr{Float64}, Int64, Ptr{Float64}),
+                     ctx.ptr, c, MEM_HOST, z, length(z), r))
+    r
+end
+
+function _sky(cfun::Symbol, a, b, d, c::CosmologyC, ctx)
+    out = (similar(a), similar(b), similar(d))
+    st = ccall(Libdl.dlsym(Libdl.dlopen(libbaorec), cfun), Cint,
+               (Ptr{Cvoid}, Ref{CosmologyC}, Cint, Ptr{Float32}, Ptr{Float32}, Ptr{Float32}, Int64,
+                Ptr{Float32}, Ptr{Float32}, Ptr{Float32}),
+               ctx.ptr, c, _mem(a, b, d, out...), _ptr(a), _ptr(b), _ptr(d), length(a), _ptr(out[1]),
+               _ptr(out[2]), _ptr(out[3]))
+    check(ctx, st)
+    out
+end
+# the drivers' sky_to_cartesian / cartesian_to_sky (examples/lightcone.jl:30-80)
+sky_to_cartesian(ra, dec, z, c::CosmologyC; ctx::Context = default_context()) =
+    _sky(:baorec_sky_to_cartesian, ra, dec, z, c, ctx)
+cartesian_to_sky(x, y, z, c::CosmologyC; ctx::Context = default_context()) =
+    _sky(:baorec_cartesian_to_sky, x, y, z, c, ctx)
+
+# ------------------------------------------------------------ catalog I/O (host)
+
+function _io_check(st)
+    st == 0 && return nothing
+    throw(BaorecError(st, unsafe_string(ccall((:baorec_io_last_error, libbaorec), Cstring, ()))))
+end
+
+# CSV.File(path; delim = " ", types = Float32) of the drivers
+# (test/benchmarks_cpu.jl:22-23): the selected columns (1-based) as Vector{Float32}s
+function read_catalog(path::AbstractString, columns = 1:5)
+    nrows = Ref{Int64}(0)
+    _io_check(ccall((:baorec_catalog_rows, libbaorec), Cint, (Cstring, Ref{Int64}, Ptr{Int32}), path,
+                    nrows, C_NULL))
+    out = [Vector{Float32}(undef, nrows[]) for _ in columns]
+    ptrs = [pointer(o) for o in out]
+    got = Ref{Int64}(0)
+    GC.@preserve out _io_check(ccall((:baorec_catalog_read, libbaorec), Cint,
+                                     (Cstring, Int32, Ptr{Int32}, Ptr{Ptr{Float32}}, Int64, Ref{Int64}), path,
+                                     length(columns), Int32[c - 1 for c in columns], ptrs, nrows[], got))
+    out
+end
+
+# npzwrite(path, hcat(cols...)) of the drivers (examples/lightcone.jl:125-127)
+function write_npy(path::AbstractString, cols::Vector{Float32}...)
+    n = length(cols[1])
+    all(length(c) == n for c in cols) || throw(ArgumentError("columns differ in length"))
+    ptrs = [pointer(c) for c in cols]
+    GC.@preserve cols _io_check(ccall((:baorec_write_npy, libbaorec), Cint,
+                                      (Cstring, Int32, Ptr{Ptr{Float32}}, Int64), path, length(cols), ptrs, n))
+end
+
+# ------------------------------------------------------------ power spectrum
+
+mutable struct PkParams
+    box_size::NTuple{3,Float32}
+    box_min::NTuple{3,Float32}
+    los::NTuple{3,Float32}
+    mas::Int32
+    interlace::Int32
+    kmin::Float64
+    dk::Float64
+    nbins::Int32
+    PkParams() = new()
+end
+
+# P_0, P_2, P_4 of a periodic box catalog with the configuration of the
+# reference's validation (test_helpers/powspec_auto.conf); extension
+function power_multipoles(x, y, z, w, grid_size::Tuple{Int,Int,Int}; box_size = 1000.0, box_min = 0.0,
+                          dk = 0.005, nbins = 300, ctx::Context = default_context())
+    pk = PkParams()
+    ccall((:baorec_pk_params_default, libbaorec), Cvoid, (Ref{PkParams},), pk)
+    pk.box_size = _t3(fill(box_size, 3))
+    pk.box_min = _t3(fill(box_min, 3))
+    pk.dk = dk
+    pk.nbins = nbins
+    k, nm, p0, p2, p4 = (zeros(Float64, nbins) for _ in 1:5)
+    check(ctx, ccall((:baorec_power_multipoles, libbaorec), Cint,
+                     (Ptr{Cvoid}, Ref{PkParams}, Ptr{Int32}, Cint, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
+                      Ptr{Float32}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                     ctx.ptr, pk, _grid(grid_size), _mem(x, y, z, w), _ptr(x), _ptr(y), _ptr(z), _ptr(w),
+                     length(x), k, nm, p0, p2, p4))
+    (k = k, nmodes = nm, p0 = p0, p2 = p2, p4 = p4)
+end
+
 end # module

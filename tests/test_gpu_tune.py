@@ -60,3 +60,21 @@ def test_tune_load_rejects_unusable_entries(tmp_path):
     with pytest.raises(bh.BaorecError):
         bh.tune_load(bad, ctx)
     assert bh.tune_report(ctx) == {"scatter": [], "strided": []}
+
+
+def test_failed_tuning_leaves_no_forced_choice():
+    """A catalog outside the box: BoundsError, nothing tuned, and later calls
+    use the library's own choices."""
+    x, y, z, w = synth.box_catalog(50000, L=1000.0, mesh=64, seed=43)
+    ctx = bh.Context(0)
+    bad = x.copy()
+    bad[:100] = -50.0      # below box_min: outside the grid
+    with pytest.raises(bh.BoundsError):
+        bh.autotune(bh.IterativeRecon(**box_kw(ctx)), (64, 64, 64), bad, y, z, w)
+    assert bh.tune_report(ctx) == {"scatter": [], "strided": []}
+    plain = bh.Context(0)
+    out = []
+    for c in (plain, ctx):
+        rec = bh.IterativeRecon(**box_kw(c))
+        out.append(bh.run_(rec, (64, 64, 64), x.copy(), y.copy(), z.copy(), w))
+    np.testing.assert_array_equal(out[0], out[1])
