@@ -319,6 +319,13 @@ void tiles_for(baorec_ctx *c, const Grid &G, ScatterArgs &a, int64_t n) {
 void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, float *rho, float *x,
              float *y, float *z, const float *w, int64_t n, int wrap, int accumulate,
              const char *tag) {
+    // placements are indexed with 32 bits; the bound is the worst case of a
+    // particle touching 8 (CIC) or 27 (TSC) tiles
+    const int64_t per = p->mas == BAOREC_MAS_TSC ? 27 : 8;
+    if (n * per >= ((int64_t)1 << 32))
+        fail(c, BAOREC_ERR_UNSUPPORTED,
+             "catalog too large for one scatter (at most " + std::to_string((((int64_t)1 << 32) - 1) / per) +
+                 " particles)");
     ScatterArgs a{};
     a.x = x;
     a.y = y;

@@ -11,7 +11,10 @@
  * Conventions
  *  - Grids are FP32, x fastest: element (i, j, k) of an (nx, ny, nz) grid is at
  *    i + nx*(j + ny*k), which is Julia's column-major Array{Float32,3}.
- *  - Particles are FP32 structure-of-arrays (x, y, z, w).
+ *  - Particles are FP32 structure-of-arrays (x, y, z, w).  One catalog of a
+ *    mass assignment holds at most 536,870,911 particles with CIC and
+ *    159,072,862 with TSC (32-bit placement indices); larger catalogs return
+ *    BAOREC_ERR_UNSUPPORTED.
  *  - `mem` says where every pointer argument of the call lives:
  *    BAOREC_MEM_HOST (Julia Array) or BAOREC_MEM_DEVICE (a device pointer on
  *    the context's GPU, e.g. a ROCArray).  Host inputs are copied in, host
