@@ -945,66 +945,92 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
         return;
     }
     if (p->mg_schedule != BAOREC_MG_RBGS) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mg_schedule");
-    // RBGS: V-cycles from v = 0.  The periodic operator annihilates
-    // constants; with a fixed LOS it is symmetric, so its range is exactly the
-    // zero-mean fields and f is demeaned (the consistent projection).  With a
-    // local LOS the first-derivative term makes it non-symmetric and the
-    // range is not orthogonal to constants: f is used as given.
-    float *fz = bufT<float>(c, "mg_fz", G.R());
+    // RBGS in correction form: each V-cycle (FP32 fine levels, FP64 coarse
+    // ones) solves A e = r from e = 0 for the residual r = f - A v of the
+    // solution so far, and v += e.
+    // The periodic operator annihilates constants; with a fixed LOS it is
+    // symmetric, so its range is exactly the zero-mean fields and f is taken
+    // minus its mean (the consistent projection).  The system is then
+    // consistent and converges to the tolerance: v is kept in FP64 and its
+    // residual computed in FP64 (iterative refinement), since in FP32 the fine
+    // residual of a converged v is rounding in A v, about 1e-4 of ||f|| at
+    // 512^3.  With a local LOS the first-derivative term makes the operator
+    // non-symmetric and the range is not orthogonal to constants: f is used
+    // as given, the residual has a floor far above FP32 rounding, and v and r
+    // stay in FP32.
+    const bool refine = p->has_los != 0;
+    const int64_t N = G.R();
     double *red = bufT<double>(c, "mg_red", 8);
     hipchk(c, hipMemsetAsync(red, 0, sizeof(double) * 8, c->stream), "memset");
-    hipchk(c, hipMemcpyAsync(fz, f0, sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream), "copy");
-    if (p->has_los) {
-        hipchk(c, launch_sum<float>(f0, G.R(), red, c->stream), "sum f");
-        hipchk(c, launch_add_const<float>(fz, G.R(), red, (double)G.R(), -1.0, c->stream), "demean");
-    }
-    hipchk(c, launch_sumsq<float>(fz, G.R(), red + 1, c->stream), "norm f");
-    MGPyr P = mg_build(c, G, p, slot, v0, fz, p->fp64_coarse_max);
-    zero_level(c, P, 0);
+    if (p->has_los) hipchk(c, launch_sum<float>(f0, N, red, c->stream), "sum f");
+    hipchk(c, launch_sumsq<float>(f0, N, red + 1, c->stream), "norm f");
+    float *rb[2] = {bufT<float>(c, "mg_r0", N), bufT<float>(c, "mg_r1", N)};
+    double *phi = refine ? bufT<double>(c, "mg_phi64", N) : nullptr;   // FP64 solution
+    float *phi32 = refine ? nullptr : bufT<float>(c, "mg_phi32", N);      // FP32 solution
+    if (refine)
+        hipchk(c, hipMemsetAsync(phi, 0, sizeof(double) * N, c->stream), "memset");
+    else
+        hipchk(c, launch_fill(phi32, N, 0.f, c->stream), "fill");
+    MGPyr P = mg_build(c, G, p, slot, v0, rb[0], p->fp64_coarse_max);
+    const double *fsum = p->has_los ? red : nullptr;
+    auto residual = [&](float *r) {
+        if (refine) {
+            ProfScope ps(c, "mg_residual_fp64", 16.0 * N);
+            hipchk(c, launch_mg_residual_mixed(r, phi, f0, fsum, (double)N, P.L[0], p->beta, p->has_los,
+                                               c->stream), "residual");
+        } else {
+            ProfScope ps(c, "mg_residual", 12.0 * N);
+            hipchk(c, launch_mg_residual(r, phi32, f0, P.L[0], p->beta, p->has_los, c->stream), "residual");
+        }
+    };
+    residual(rb[0]);   // f (minus its mean with a fixed LOS): v = 0
     double h[8];
     hipchk(c, hipMemcpyAsync(h, red, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
     sync(c);
-    const double fn = sqrt(h[1]);
+    const double f2 = h[1] - (p->has_los ? h[0] * h[0] / (double)N : 0.0);
+    const double fn = f2 > 0 ? sqrt(f2) : 0.0;
     // Stop at ||f - A v|| <= tol ||f||; or once a V-cycle changes the
     // residual by <= tol ||f||, or two V-cycles in a row reduce it by less
     // than 10 %: with a local LOS the operator is not symmetric, its range
     // misses a direction and an arbitrary f has a residual floor above tol
-    // (the iteration has then converged to the attainable solution; FP32
-    // rounding keeps the residual moving at about 1e-6 of ||f||).
-    float *rb[2] = {bufT<float>(c, "mg_r0", G.R()), bufT<float>(c, "mg_r1", G.R())};
-    const float *rprev = P.F[0];   // residual of v = 0
+    // (the iteration has then converged to the attainable solution).
     int cyc = 0, stall = 0;
     double rel = fn > 0 ? 1.0 : 0.0, chg = rel;
     while (rel > p->tolerance && chg > p->tolerance && stall < 2 && cyc < p->max_vcycles) {
         const double rel_prev = rel;
+        float *r = rb[cyc & 1], *rn = rb[(cyc + 1) & 1];
+        P.F[0] = r;
+        zero_level(c, P, 0);
         mg_vcycle_rb(c, P, 0, p);
-        // A annihilates constants: keep v zero-mean, so that an inconsistent
-        // f (residual floor) does not make v drift by a constant every cycle
-        // until FP32 rounding of v disturbs the residual.
+        // v += e - mean(e): v stays zero-mean (A annihilates constants), so an
+        // inconsistent f does not make it drift by a constant every cycle
         hipchk(c, hipMemsetAsync(red + 4, 0, sizeof(double), c->stream), "memset");
-        hipchk(c, launch_sum<float>(vcur(P, 0), G.R(), red + 4, c->stream), "sum v");
-        hipchk(c, launch_add_const<float>(vcur(P, 0), G.R(), red + 4, (double)G.R(), -1.0, c->stream),
-               "demean v");
-        float *r = rb[cyc & 1];
+        hipchk(c, launch_sum<float>(vcur(P, 0), N, red + 4, c->stream), "sum e");
+        {
+            ProfScope ps(c, "mg_correct", (refine ? 20.0 : 12.0) * N);
+            if (refine)
+                hipchk(c, launch_add_correction<double>(phi, vcur(P, 0), N, red + 4, c->stream), "v += e");
+            else
+                hipchk(c, launch_add_correction<float>(phi32, vcur(P, 0), N, red + 4, c->stream), "v += e");
+        }
         cyc++;
-        hipchk(c, launch_mg_residual(r, vcur(P, 0), P.F[0], P.L[0], p->beta, p->has_los, c->stream),
-               "residual");
+        residual(rn);
         hipchk(c, hipMemsetAsync(red + 2, 0, 2 * sizeof(double), c->stream), "memset");
-        hipchk(c, launch_sumsq<float>(r, G.R(), red + 2, c->stream), "norm r");
-        hipchk(c, launch_sumsq_diff<float>(r, rprev, G.R(), red + 3, c->stream), "norm dr");
+        hipchk(c, launch_sumsq<float>(rn, N, red + 2, c->stream), "norm r");
+        hipchk(c, launch_sumsq_diff<float>(rn, r, N, red + 3, c->stream), "norm dr");
         hipchk(c, hipMemcpyAsync(h + 2, red + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
         sync(c);
         rel = fn > 0 ? sqrt(h[2]) / fn : 0.0;
         chg = fn > 0 ? sqrt(h[3]) / fn : 0.0;
         stall = rel > 0.9 * rel_prev ? stall + 1 : 0;
-        rprev = r;
     }
     p->vcycles_done = cyc;
     p->final_residual = rel;
     p->final_change = chg;
-    if (P.cur[0])
-        hipchk(c, hipMemcpyAsync(v0, P.B[0], sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream),
-               "copy");
+    if (refine)
+        hipchk(c, launch_convert<float, double>(v0, phi, N, c->stream), "phi to FP32");
+    else
+        hipchk(c, hipMemcpyAsync(v0, phi32, sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream), "copy");
 }
 
 // ------------------------------------------------------------ entry helpers

@@ -316,5 +316,11 @@ hipError_t launch_add_const(T *a, int64_t n, const double *num, double den, doub
 template <typename T, typename U>
 hipError_t launch_convert(T *dst, const U *src, int64_t n, hipStream_t s);
 hipError_t launch_fill(float *a, int64_t n, float v, hipStream_t s);
+// r = (f - fsum/count) - A v, FP64 arithmetic, FP32 result (fsum NULL: no shift)
+hipError_t launch_mg_residual_mixed(float *r, const double *v, const float *f, const double *fsum, double count,
+                                    const MGLevel &L, float beta, int has_los, hipStream_t s);
+// phi += e - esum/n (sum in FP64, stored in phi's type)
+template <typename T>
+hipError_t launch_add_correction(T *phi, const float *e, int64_t n, const double *esum, hipStream_t s);
 
 }  // namespace baorec

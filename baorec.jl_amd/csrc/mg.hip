@@ -96,6 +96,22 @@ __device__ __forceinline__ void cell_residual(T *__restrict__ r, const T *__rest
     r[I] = f[I] - t;
 }
 
+// r = (f - mean f) - A v for the refined RBGS solve: v in FP64 and every
+// operation in FP64, so that the residual of a converged v is not lost to
+// FP32 cancellation in A v; stored as FP32 (the right-hand side of the next
+// correction V-cycle).  fsum: device FP64 sum of f (NULL: no shift).
+__device__ __forceinline__ void cell_residual_mixed(float *__restrict__ r, const double *__restrict__ v,
+                                                    const float *__restrict__ f, double fshift,
+                                                    const MGLevel &L, int ix, int iy, int iz, float beta,
+                                                    int has_los) {
+    const Coef<double> c = coef<double>(L, ix, iy, iz, (double)beta, has_los);
+    const Nbr<double> s = nbr<double>(v, L, ix, iy, iz, c);
+    const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
+    double t = 2.0 * (c.gpx2 + c.gpy2 + c.gpz2) * v[I] - (s.A + s.B + s.C + s.X);
+    if (!has_los) t -= s.G;
+    r[I] = (float)(((double)f[I] - fshift) - t);
+}
+
 // Multicolour Gauss-Seidel (RBGS schedule): colour = ((ix+iy)&1) + 2((iy+iz)&1)
 // separates every face and edge neighbour of the 19-point stencil, so one
 // colour updates in place without races.
@@ -126,6 +142,25 @@ __global__ void k_mg_residual(T *__restrict__ r, const T *__restrict__ v, const 
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     if (ix >= L.n[0]) return;
     cell_residual<T>(r, v, f, L, ix, blockIdx.y, blockIdx.z, beta, has_los);
+}
+
+__global__ void k_mg_residual_mixed(float *__restrict__ r, const double *__restrict__ v,
+                                    const float *__restrict__ f, const double *fsum, double count, MGLevel L,
+                                    float beta, int has_los) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= L.n[0]) return;
+    const double fshift = fsum ? *fsum / count : 0.0;
+    cell_residual_mixed(r, v, f, fshift, L, ix, blockIdx.y, blockIdx.z, beta, has_los);
+}
+
+// phi += e - mean(e): the correction of a V-cycle added to the FP64 solution,
+// keeping it zero-mean (the operator annihilates constants).
+template <typename T>
+__global__ void k_add_correction(T *__restrict__ phi, const float *__restrict__ e, int64_t n, const double *esum) {
+    const double m = *esum / (double)n;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        phi[i] = (T)((double)phi[i] + ((double)e[i] - m));
 }
 
 template <typename T>
@@ -484,6 +519,22 @@ hipError_t launch_convert(T *dst, const U *src, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL((k_convert<T, U>), dim3(red_blocks(n)), dim3(256), 0, s, dst, src, n);
     return hipGetLastError();
 }
+hipError_t launch_mg_residual_mixed(float *r, const double *v, const float *f, const double *fsum, double count,
+                                    const MGLevel &L, float beta, int has_los, hipStream_t s) {
+    const int bx = bx_for(L.n[0]);
+    hipLaunchKernelGGL(k_mg_residual_mixed, cell_grid(L.n[0], L.n[1], L.n[2], bx), dim3(bx), 0, s, r, v, f, fsum,
+                       count, L, beta, has_los);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_add_correction(T *phi, const float *e, int64_t n, const double *esum, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_correction<T>, dim3(red_blocks(n)), dim3(256), 0, s, phi, e, n, esum);
+    return hipGetLastError();
+}
+template hipError_t launch_add_correction<float>(float *, const float *, int64_t, const double *, hipStream_t);
+template hipError_t launch_add_correction<double>(double *, const float *, int64_t, const double *, hipStream_t);
+
 hipError_t launch_fill(float *a, int64_t n, float v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill, dim3(red_blocks(n)), dim3(256), 0, s, a, n, v);
     return hipGetLastError();

@@ -48,13 +48,15 @@ def make_workload(args):
         data = synth.box_catalog(1_000_000, L=1000.0, mesh=256, seed=1)
         rand = None
         cfg = dict(grid=(128, 128, 128), bias=2.0, f=0.8, R=15.0, los=(0.0, 0.0, 1.0), pad=0.0)
-    elif args.workload in ("mg512", "mg512_ref"):
+    elif args.workload in ("mg512", "mg512_ref", "mg512_zlos"):
         # multigrid on the 1M-tracer periodic-box catalog at 512^3, local LOS
-        # from the box corner; RBGS to relative residual 1e-6 with FP64 levels
-        # <= 8^3, or the reference's FMG schedule
+        # from the box corner (LOS z for mg512_zlos); RBGS to relative
+        # residual 1e-6 with FP64 levels <= 8^3, or the reference's FMG
+        # schedule
         data = synth.box_catalog(1_000_000, L=1000.0, mesh=256, seed=1)
         rand = None
-        cfg = dict(grid=(512, 512, 512), bias=2.0, f=0.8, R=15.0, los=None, pad=0.0, recon="multigrid",
+        cfg = dict(grid=(512, 512, 512), bias=2.0, f=0.8, R=15.0,
+                   los=(0.0, 0.0, 1.0) if args.workload == "mg512_zlos" else None, pad=0.0, recon="multigrid",
                    schedule="reference" if args.workload == "mg512_ref" else "rbgs")
     else:
         raise SystemExit(f"unknown workload {args.workload}")

@@ -66,7 +66,11 @@ typedef enum { BAOREC_MAS_CIC = 0, BAOREC_MAS_TSC = 1 } baorec_mas;
  * Gauss-Seidel V-cycles (rb_sweeps pre- and post-smoothing sweeps) in the
  * four-colour ordering that generalises red-black to the operator's 19-point
  * stencil (its cross-derivative terms couple diagonal neighbours), with levels
- * of side <= fp64_coarse_max in FP64.  It stops when the relative residual
+ * of side <= fp64_coarse_max in FP64.  With a fixed LOS the V-cycles refine
+ * an FP64 solution: its fine residual is computed in FP64 and each V-cycle
+ * solves for the correction (FP32 fine levels), so the residual can fall
+ * below the FP32 rounding of A v (about 1e-4 of ||f|| at 512^3).  It stops
+ * when the relative residual
  * (FP64 norm) is <= `tolerance`, when one V-cycle changes the residual by
  * <= tolerance * ||f||, or when two V-cycles in a row reduce it by less than
  * 10 %: with a local LOS the operator is not symmetric and an arbitrary f has
