@@ -385,8 +385,9 @@ const char *baorec_io_last_error(void);
  * Outputs: the reconstructed positions of every particle in ox, oy, oz.
  * Inputs are wrapped in place when mutate_inputs, as run! does.  Results are
  * bitwise those of the single-catalog calls.  Catalogs run in groups whose
- * grid and spectrum buffers stay within 16 GiB (0.93 GB per catalog at 256^3),
- * one launch sequence per group on the context's stream.  A particle outside
+ * grid and spectrum buffers stay within 2 GiB (0.93 GB per catalog at 256^3,
+ * so pairs of catalogs; at least one catalog per group), one launch sequence
+ * per group on the context's stream.  A particle outside
  * the grid gives BAOREC_ERR_OUT_OF_BOX.
  */
 int baorec_run_batch(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3],
