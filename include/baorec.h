@@ -10,7 +10,9 @@
  *
  * Conventions
  *  - Grids are FP32, x fastest: element (i, j, k) of an (nx, ny, nz) grid is at
- *    i + nx*(j + ny*k), which is Julia's column-major Array{Float32,3}.
+ *    i + nx*(j + ny*k), which is Julia's column-major Array{Float32,3}.  Each
+ *    side is a power of two from 16 to 1024 (other sizes return
+ *    BAOREC_ERR_UNSUPPORTED); the reference's drivers and tests use 64 and 512.
  *  - Particles are FP32 structure-of-arrays (x, y, z, w).  One catalog of a
  *    mass assignment holds at most 536,870,911 particles with CIC and
  *    159,072,862 with TSC (32-bit placement indices); larger catalogs return
