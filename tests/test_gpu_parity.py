@@ -233,6 +233,48 @@ def test_run_iterative_box_long_axes(ctx, los, g):
     assert rms3(got, exp) < 1e-4
 
 
+def noncubic_case(seed):
+    """A catalog in a (1000, 800, 600) Mpc/h box on a (64, 32, 128) grid:
+    per-axis cell sizes, k and x tables."""
+    x, y, z, w = synth.box_catalog(100000, L=1000.0, mesh=64, seed=seed)
+    y = (y * np.float32(0.8)).astype(F32)
+    z = (z * np.float32(0.6)).astype(F32)
+    kw = dict(bias=2.0, f=0.8, smoothing_radius=15.0, box_size=[1000.0, 800.0, 600.0], box_min=[0.0] * 3)
+    return (x, y, z, w), kw, (64, 32, 128)
+
+
+@pytest.mark.parametrize("los", [(0.0, 0.6, 0.8), None])
+def test_run_iterative_noncubic(ctx, los):
+    (x, y, z, w), kw, g = noncubic_case(25)
+    kw["los"] = None if los is None else list(los)
+    rec = bh.IterativeRecon(**kw)
+    gx, gy, gz = x.copy(), y.copy(), z.copy()
+    d = bh.run_(rec, g, gx, gy, gz, w)
+    o = orc.IterativeRecon(**kw)
+    ox, oy, oz = x.copy(), y.copy(), z.copy()
+    ref = orc.run_iterative(o, g, ox, oy, oz, w)
+    assert d.shape == g[::-1]
+    assert rel_l2(d, ref) < 1e-5
+    got = bh.reconstructed_positions(rec, gx, gy, gz, field="sum")
+    exp = orc.reconstructed_positions(o, ox, oy, oz, kind="sum")
+    assert rms3(got, exp) < 1e-4
+
+
+def test_run_multigrid_noncubic(ctx):
+    (x, y, z, w), kw, g = noncubic_case(26)
+    kw["los"] = None
+    rec = bh.MultigridRecon(**kw)
+    gx, gy, gz = x.copy(), y.copy(), z.copy()
+    phi = bh.run_(rec, g, gx, gy, gz, w)
+    o = orc.MultigridRecon(**kw)
+    ox, oy, oz = x.copy(), y.copy(), z.copy()
+    ref = orc.run_multigrid(o, g, ox, oy, oz, w)
+    assert rel_l2(phi, ref) < 1e-5
+    got = bh.reconstructed_positions(rec, gx, gy, gz, field="sum")
+    exp = orc.reconstructed_positions(o, ox, oy, oz, kind="sum")
+    assert rms3(got, exp) < 1e-4
+
+
 def lightcone_case(seed_d, seed_r):
     """Lightcone test catalogs whose smoothed randoms stay >= 1e-4 (relative)
     away from the normalisation threshold in every cell, so that the
