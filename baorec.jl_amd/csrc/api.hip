@@ -933,6 +933,59 @@ void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
     mg_smooth_rb(c, P, l, p, coarsest ? 16 * p->rb_sweeps : p->rb_sweeps);
 }
 
+// Spectral seed of the RBGS solve: the exact inverse of the discrete
+// operator with the LOS fixed, by one forward and inverse FFT.  The LOS is
+// the given one, or for a local LOS the direction of the box centre seen
+// from the origin (the operator's first-derivative term, which falls off as
+// 1/|x|, is left out).  The k = 0 mode is zero.
+void mg_spectral_seed(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const MGLevel &L0,
+                      const float *f, float *out) {
+    double d[3];
+    if (p->has_los) {
+        for (int a = 0; a < 3; a++) d[a] = p->los[a];
+    } else {
+        for (int a = 0; a < 3; a++) d[a] = (double)p->box_min[a] + 0.5 * (double)p->box_size[a];
+        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (r > 0) {
+            for (int a = 0; a < 3; a++) d[a] /= r;
+        } else {
+            d[0] = d[1] = 0.0;
+            d[2] = 1.0;
+        }
+    }
+    // the coefficients of the per-cell operator (mg.hip coef()) for this LOS
+    double pp[3], s2 = 0.0;
+    for (int a = 0; a < 3; a++) {
+        pp[a] = d[a] / (double)L0.cell[a];
+        s2 += (double)L0.cell2[a] * pp[a] * pp[a];
+    }
+    const double g = s2 > 0 ? (double)p->beta / s2 : 0.0;
+    FFTBufs fb = fft_bufs(c, G);
+    tables(c, G, slot, fb);
+    x_fwd(c, G, f, fb.cw, 1, nullptr, 0);
+    strided_plain(c, G, fb, 1, false, fb.cw, 1);
+    StridedArgs z = sargs(G, fb, 2, 2, MUL_MG_INV);
+    for (int a = 0; a < 3; a++) {
+        z.mg_gp[a] = (float)((double)L0.icell2[a] + g * pp[a] * pp[a]);
+        z.mg_cell[a] = L0.cell[a];
+    }
+    z.mg_cross[0] = (float)(2.0 * g * pp[0] * pp[1]);
+    z.mg_cross[1] = (float)(2.0 * g * pp[0] * pp[2]);
+    z.mg_cross[2] = (float)(2.0 * g * pp[1] * pp[2]);
+    z.job[0].in = fb.cw;
+    z.job[0].out[0] = fb.cw;
+    z.job[0].nout = 1;
+    z.job[0].pw[0] = 0;
+    strided(c, G, z, 1, 1, "z multigrid inverse");
+    strided_plain(c, G, fb, 1, true, fb.cw, 1);
+    XInvArgs a = xargs(G, fb, slot);
+    a.in[0] = fb.cw;
+    a.kxpow[0] = 0;
+    a.nin = 1;
+    a.out = out;
+    x_inv(c, G, a, XE_PLAIN, 1, "x multigrid seed");
+}
+
 // Solves A phi = f on level 0.  v0 (device) receives the potential.
 void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot, float *v0,
                      const float *f0) {
@@ -983,19 +1036,32 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
             hipchk(c, launch_mg_residual(r, phi32, f0, P.L[0], p->beta, p->has_los, c->stream), "residual");
         }
     };
-    residual(rb[0]);   // f (minus its mean with a fixed LOS): v = 0
+    if (p->mg_warm_start) {
+        if (p->mg_warm_start != 1) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mg_warm_start");
+        if (!fft_side_ok(G.nx) || !fft_side_ok(G.ny) || !fft_side_ok(G.nz))
+            fail(c, BAOREC_ERR_UNSUPPORTED, "the spectral warm start needs grid sides that are powers of two in [16, 1024]");
+        float *seed = bufT<float>(c, "mg_seed", N);
+        mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed);
+        if (refine)
+            hipchk(c, launch_convert<double, float>(phi, seed, N, c->stream), "seed to FP64");
+        else
+            hipchk(c, hipMemcpyAsync(phi32, seed, sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream), "copy");
+    }
+    residual(rb[0]);   // of the start: f (minus its mean with a fixed LOS) for v = 0
+    hipchk(c, launch_sumsq<float>(rb[0], N, red + 5, c->stream), "norm r0");
     double h[8];
     hipchk(c, hipMemcpyAsync(h, red, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
     sync(c);
     const double f2 = h[1] - (p->has_los ? h[0] * h[0] / (double)N : 0.0);
     const double fn = f2 > 0 ? sqrt(f2) : 0.0;
+    p->seed_residual = fn > 0 ? sqrt(h[5]) / fn : 0.0;
     // Stop at ||f - A v|| <= tol ||f||; or once a V-cycle changes the
     // residual by <= tol ||f||, or two V-cycles in a row reduce it by less
     // than 10 %: with a local LOS the operator is not symmetric, its range
     // misses a direction and an arbitrary f has a residual floor above tol
     // (the iteration has then converged to the attainable solution).
     int cyc = 0, stall = 0;
-    double rel = fn > 0 ? 1.0 : 0.0, chg = rel;
+    double rel = p->seed_residual, chg = rel;
     while (rel > p->tolerance && chg > p->tolerance && stall < 2 && cyc < p->max_vcycles) {
         const double rel_prev = rel;
         float *r = rb[cyc & 1], *rn = rb[(cyc + 1) & 1];

@@ -365,10 +365,10 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
     return x * q + (x < r ? x : r) + (bid >> 3);
 }
 
-__device__ __forceinline__ float2 apply_mul(float2 F, int mul, float kx, float ky, float kz,
-                                            float R2, float l0, float l1, float l2) {
+__device__ __forceinline__ float2 apply_mul(float2 F, float kx, float ky, float kz, const StridedArgs &a) {
+    const float R2 = a.R2, l0 = a.los[0], l1 = a.los[1], l2 = a.los[2];
     const float k2 = (kx * kx + ky * ky) + kz * kz;
-    switch (mul) {
+    switch (a.mul) {
         case MUL_GAUSS: {
             const double g = exp(-0.5 * (double)R2 * (double)k2);
             return make_float2((float)((double)F.x * g), (float)((double)F.y * g));
@@ -384,6 +384,18 @@ __device__ __forceinline__ float2 apply_mul(float2 F, int mul, float kx, float k
             return k2 > 0.f ? make_float2(-F.y / k2, F.x / k2) : make_float2(0.f, 0.f);
         case MUL_I:
             return make_float2(-F.y, F.x);
+        case MUL_MG_INV: {
+            // exact inverse of the discrete operator for a fixed LOS (the
+            // multigrid stencil has constant coefficients then)
+            // t_a in [-pi, pi]: the hardware sine and cosine (a seed for the
+            // V-cycles, which remove its rounding)
+            const float tx = kx * a.mg_cell[0], ty = ky * a.mg_cell[1], tz = kz * a.mg_cell[2];
+            const float sx = __sinf(tx), sy = __sinf(ty), sz = __sinf(tz);
+            const float A = (a.mg_gp[0] * (2.0f - 2.0f * __cosf(tx)) + a.mg_gp[1] * (2.0f - 2.0f * __cosf(ty))) +
+                            a.mg_gp[2] * (2.0f - 2.0f * __cosf(tz)) +
+                            ((a.mg_cross[0] * sx * sy + a.mg_cross[1] * sx * sz) + a.mg_cross[2] * sy * sz);
+            return A > 0.f ? make_float2(F.x / A, F.y / A) : make_float2(0.f, 0.f);
+        }
         default:
             return F;
     }
@@ -442,7 +454,7 @@ k_strided(StridedArgs a) {
             const float kal = kax[PF::out_pos(t, i)];
             const float ky = a.axis == 1 ? kal : kother;
             const float kz = a.axis == 1 ? kother : kal;
-            v[i] = apply_mul(v[i], a.mul, kxv, ky, kz, a.R2, a.los[0], a.los[1], a.los[2]);
+            v[i] = apply_mul(v[i], kxv, ky, kz, a);
         }
     } else {
 #pragma unroll
@@ -537,8 +549,8 @@ k_strided2(StridedArgs a) {
             const float kal = kax[PF::out_pos(t, i)];
             const float ky = a.axis == 1 ? kal : kother;
             const float kz = a.axis == 1 ? kother : kal;
-            v[0][i] = apply_mul(v[0][i], a.mul, kx0v, ky, kz, a.R2, a.los[0], a.los[1], a.los[2]);
-            v[1][i] = apply_mul(v[1][i], a.mul, kx1v, ky, kz, a.R2, a.los[0], a.los[1], a.los[2]);
+            v[0][i] = apply_mul(v[0][i], kx0v, ky, kz, a);
+            v[1][i] = apply_mul(v[1][i], kx1v, ky, kz, a);
         }
     } else {
 #pragma unroll

@@ -21,6 +21,7 @@ enum MulKind : int {
     MUL_FIXEDLOS = 3,   // (sum_i k_i^2 los_i)/k^2 (src/iterative.jl:43-62)
     MUL_I_INVK2 = 4,    // i/k^2, 0 at k = 0 (src/iterative.jl:219-223)
     MUL_I = 5,          // i (src/multigrid.jl:776-779)
+    MUL_MG_INV = 6,     // 1 / symbol of the fixed-LOS multigrid operator, 0 at k = 0
 };
 
 constexpr int kMaxJobs = 3;
@@ -49,6 +50,9 @@ struct StridedArgs {
     float los[3];
     int pair;                  // kernel of single-output jobs: -1 the library's rule,
                                // 0 one column per thread, 1 two columns per thread
+    // MUL_MG_INV: symbol sum_a gp[a] (2 - 2 cos t_a) + sum_{a<b} cross[ab]
+    // sin t_a sin t_b with t_a = k_a cell[a] (cross: xy, xz, yz)
+    float mg_gp[3], mg_cross[3], mg_cell[3];
 };
 
 enum XEpi : int {

@@ -49,9 +49,11 @@ mutable struct Params
     box_pad::Float32
     ran_min::Float32
     mutate_inputs::Int32
+    mg_warm_start::Int32
     vcycles_done::Int32
     final_residual::Float64
     final_change::Float64
+    seed_residual::Float64
     Params() = new()
 end
 

@@ -161,9 +161,11 @@ class MultigridRecon(_Recon):
     max_vcycles: int = 200
     rb_sweeps: int = 2
     fp64_coarse_max: int = 8
+    warm_start: str = "none"          # RBGS start: "none" (v = 0) or "spectral"
     vcycles_done: int = 0
     final_residual: float = float("nan")
     final_change: float = float("nan")
+    seed_residual: float = float("nan")
 
     kind = L.MULTIGRID
 
@@ -177,6 +179,7 @@ class MultigridRecon(_Recon):
         p.max_vcycles = int(self.max_vcycles)
         p.rb_sweeps = int(self.rb_sweeps)
         p.fp64_coarse_max = int(self.fp64_coarse_max)
+        p.mg_warm_start = {"none": 0, "spectral": 1}[self.warm_start]
         return p
 
 
@@ -293,6 +296,7 @@ def run_(recon: _Recon, grid_size, x, y, z, w, rx=None, ry=None, rz=None, rw=Non
         recon.vcycles_done = p.vcycles_done
         recon.final_residual = p.final_residual
         recon.final_change = p.final_change
+        recon.seed_residual = p.seed_residual
     recon.result_cache = ResultCache(ctx, tuple(int(v) for v in grid_size), recon.kind)
     return recon.result_cache.numpy() if copy else recon.result_cache
 
@@ -309,6 +313,7 @@ def _grid_call(name, recon, delta, x, y, z, w, rx, ry, rz, rw, pass_p_mut=False)
         recon.vcycles_done = p.vcycles_done
         recon.final_residual = p.final_residual
         recon.final_change = p.final_change
+        recon.seed_residual = p.seed_residual
     return delta
 
 
@@ -565,6 +570,7 @@ def fmg(f, recon: MultigridRecon, v=None):
     recon.vcycles_done = p.vcycles_done
     recon.final_residual = p.final_residual
     recon.final_change = p.final_change
+    recon.seed_residual = p.seed_residual
     return v
 
 

@@ -80,6 +80,7 @@ def gpu_bench(args, data, rand, cfg):
         p.tolerance = 1e-6
         p.rb_sweeps = 2
         p.fp64_coarse_max = 8
+        p.mg_warm_start = 1 if args.warm_start else 0
     if cfg["los"] is not None:
         p.has_los = 1
         for a in range(3):
@@ -177,7 +178,8 @@ def gpu_bench(args, data, rand, cfg):
     prof = json.loads(buf.value.decode())
     pos = [o.numpy() for o in out]
     finite = all(np.isfinite(a).all() for a in pos)
-    mg = {"schedule": cfg["schedule"], "vcycles": p.vcycles_done, "final_residual": p.final_residual,
+    mg = {"schedule": cfg["schedule"], "warm_start": "spectral" if args.warm_start else "none",
+          "vcycles": p.vcycles_done, "seed_residual": p.seed_residual, "final_residual": p.final_residual,
           "final_change": p.final_change} if multigrid else None
     return dict(dt=dt, prof=prof, finite=finite, box=(list(p.box_size), list(p.box_min)), dt_host=dt_host,
                 mg=mg, tuned=tuned)
@@ -598,6 +600,8 @@ def main():
     ap.add_argument("--gen-workers", type=int, default=8,
                     help="processes generating the batch64 catalogs (1 under a profiler)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="multigrid workloads: start RBGS from the spectral seed")
     ap.add_argument("--autotune", action="store_true",
                     help="run baorec_autotune on the workload's catalog before timing")
     ap.add_argument("--calibrate", action="store_true",

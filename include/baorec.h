@@ -77,7 +77,13 @@ typedef enum { BAOREC_MAS_CIC = 0, BAOREC_MAS_TSC = 1 } baorec_mas;
  * <= tolerance * ||f||, or when two V-cycles in a row reduce it by less than
  * 10 %: with a local LOS the operator is not symmetric and an arbitrary f has
  * a residual floor above zero (final_residual reports where it stopped).  The
- * solution is returned with zero mean (A annihilates constants). */
+ * solution is returned with zero mean (A annihilates constants).
+ * mg_warm_start = 1 starts RBGS from a spectral seed instead of v = 0: the
+ * exact inverse of the discrete operator with the LOS fixed (the given LOS,
+ * or for a local LOS the direction of the box centre seen from the origin,
+ * without the operator's first-derivative term), one forward and inverse
+ * FFT.  With a fixed LOS that is the solution itself up to FP32 rounding; the
+ * V-cycles then continue to the same stopping rule. */
 typedef enum { BAOREC_MG_REFERENCE = 0, BAOREC_MG_RBGS = 1 } baorec_mg_schedule;
 
 /*
@@ -108,10 +114,12 @@ typedef struct baorec_params {
     float box_pad;            /* run with randoms: setup_box pad, 500 */
     float ran_min;            /* randoms threshold factor, 0.01 */
     int32_t mutate_inputs;    /* write wrapped positions back (wrap=true), 1 */
+    int32_t mg_warm_start;    /* RBGS: 0 from v = 0; 1 from the spectral seed, 0 */
     /* ---- outputs of the last run ---- */
     int32_t vcycles_done;     /* RBGS: V-cycles used on the finest level */
     double final_residual;    /* RBGS: last relative residual ||f - A v|| / ||f|| */
     double final_change;      /* RBGS: last V-cycle's residual change / ||f|| */
+    double seed_residual;     /* RBGS: relative residual of the starting v */
 } baorec_params;
 
 /* Defaults of the Julia constructors (src/recon.jl:2-33); beta = f/bias in

@@ -496,6 +496,47 @@ def test_mg_rbgs_fixed_los_refined_to_1e6_at_256(ctx):
     assert err < 1e-5
 
 
+def test_mg_rbgs_spectral_warm_start_fixed_los(ctx):
+    """With a fixed LOS the spectral seed is the exact inverse of the discrete
+    operator up to FP32 rounding: the solve starts near the tolerance, needs
+    far fewer V-cycles than from v = 0, and reaches the same solution."""
+    n = (256, 256, 256)
+    kw = mg_rec((0.0, 0.0, 1.0))
+    x, y, z, w = synth.box_catalog(400000, L=1000.0, box_min=-500.0, mesh=128, seed=95)
+    cold = bh.MultigridRecon(**kw, schedule="rbgs", tolerance=1e-6)
+    f = bh.setup_overdensity_(np.zeros(n[::-1], F32), cold, x.copy(), y.copy(), z.copy(), w)
+    phi_cold = bh.fmg(f, cold)
+    warm = bh.MultigridRecon(**kw, schedule="rbgs", tolerance=1e-6, warm_start="spectral")
+    phi_warm = bh.fmg(f, warm)
+    print(f"cold: {cold.vcycles_done} V-cycles to {cold.final_residual:.2e}; warm: seed "
+          f"{warm.seed_residual:.2e}, {warm.vcycles_done} V-cycles to {warm.final_residual:.2e}")
+    assert abs(cold.seed_residual - 1.0) < 1e-5
+    assert warm.seed_residual < 1e-3
+    assert warm.final_residual <= 1e-6 and warm.vcycles_done <= cold.vcycles_done // 2
+    assert rel_l2(phi_warm, phi_cold) < 1e-5
+
+
+@pytest.mark.parametrize("los", [None, (0.0, 0.6, 0.8)])
+def test_mg_rbgs_spectral_warm_start_matches_cold(ctx, los):
+    """Local LOS (the seed freezes the box-centre LOS) and a fixed oblique one:
+    the warm-started solve stops at the same solution, in no more V-cycles."""
+    n = 64
+    kw = mg_rec(los)
+    kw["box_min"] = [200.0, 300.0, 100.0]
+    rng = np.random.default_rng(97)
+    f = bh.smooth_(rng.standard_normal((n, n, n)).astype(F32), bh.IterativeRecon(**{**kw, "los": None}))
+    out = {}
+    for ws in ("none", "spectral"):
+        rec = bh.MultigridRecon(**kw, schedule="rbgs", tolerance=1e-6, warm_start=ws)
+        out[ws] = (bh.fmg(f, rec), rec)
+        print(f"{ws}: seed {rec.seed_residual:.2e}, {rec.vcycles_done} V-cycles to {rec.final_residual:.2e}")
+    (pc, rc), (pw, rw) = out["none"], out["spectral"]
+    assert rw.seed_residual < rc.seed_residual
+    assert rw.vcycles_done <= rc.vcycles_done
+    assert rw.final_residual <= max(1e-6, 1.5 * rc.final_residual)
+    assert rel_l2(pw - pw.mean(), pc - pc.mean()) < 1e-3
+
+
 @pytest.mark.parametrize("n,box_min", [(32, (200.0, 300.0, 100.0)), (32, (-500.0, -500.0, -500.0)),
                                         (64, (200.0, 300.0, 100.0))])
 def test_mg_rbgs_local_los_manufactured_solution(ctx, n, box_min):
