@@ -39,11 +39,15 @@ def log(msg):
 def make_workload(args):
     from baorec_hip import synth
     t0 = time.perf_counter()
-    if args.workload == "lightcone512":
+    if args.workload in ("lightcone512", "mg_lc512"):
         data = synth.lightcone_catalog(args.ndata, r_min=1000.0, r_max=1800.0, mesh=256, seed=2)
         rand = synth.lightcone_randoms(args.nrand, r_min=1000.0, r_max=1800.0, seed=3)
         ext = max(float(np.max(a) - np.min(a)) for a in rand[:3])
         cfg = dict(grid=(512, 512, 512), bias=2.2, f=0.75, R=15.0, los=None, pad=0.1 * ext)
+        if args.workload == "mg_lc512":
+            # the same catalogs through the multigrid solver: RBGS to 1e-6
+            # (or its attainable floor), FP64 levels <= 8^3
+            cfg.update(recon="multigrid", schedule="rbgs")
     elif args.workload == "box128":
         data = synth.box_catalog(1_000_000, L=1000.0, mesh=256, seed=1)
         rand = None
