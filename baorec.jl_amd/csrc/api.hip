@@ -1182,15 +1182,19 @@ int guarded(baorec_ctx *c, F &&fn) {
 
 // ------------------------------------------------------------- batching
 
-// A batch runs in groups of consecutive catalogs; each group is one launch
-// sequence over [quantity][catalog] buffers.  Grid and spectrum buffers of a
-// group stay within kBatchGroupBytes and its sorted placements within
-// kBatchPlacements (16 B each, uint32 cursors).  Small groups keep what one
-// FFT pass reads and writes near the size of the 256 MB Infinity Cache, so the
-// next pass finds part of it there: at 256^3 (0.93 GB per catalog) groups of
-// 2 ran the 64-catalog batch in 72.9 ms, against 77.9 (1), 75.3 (4), 77.3 (8)
-// and 76.5 ms (16).
-constexpr double kBatchGroupBytes = 2.0 * (1ull << 30);
+// A batch runs in scatter groups of consecutive catalogs: one multi-catalog
+// sort and accumulation per group, into density grids laid out
+// [catalog][cell] (at most kBatchScatterCats catalogs and kBatchPlacements
+// sorted placements of 16 B, uint32 cursors).  The rest of the pipeline runs
+// over FFT subgroups of a scatter group, whose other grid and spectrum
+// buffers stay within kBatchFFTBytes.  Small FFT subgroups keep what one pass
+// reads and writes near the size of the 256 MB Infinity Cache, so the next
+// pass finds part of it there: at 256^3 (0.86 GB of FFT buffers per catalog)
+// groups of 2 ran the 64-catalog batch in 72.9 ms, against 77.9 (1), 75.3
+// (4), 77.3 (8) and 76.5 ms (16) with scatter and FFT grouped alike.
+constexpr int kBatchScatterCats = 16;
+constexpr double kBatchScatterBytes = 8.0 * (1ull << 30);   // density grids of a scatter group
+constexpr double kBatchFFTBytes = 2.0 * (1ull << 30);
 constexpr int64_t kBatchPlacements = (int64_t)1 << 29;
 
 struct BatchGroup {
@@ -1199,13 +1203,19 @@ struct BatchGroup {
     size_t o64, o32;     // offsets of the group's tables in the packed arrays
 };
 
-// Per catalog: rho, delta_s (FP32), displacement (float4) and seven spectra.
-double batch_bytes_per_catalog(const Grid &G) { return 24.0 * G.R() + 7.0 * 8.0 * G.C(); }
+// FFT buffers per catalog: delta_s (FP32), displacement (float4) and seven
+// spectra.
+double batch_fft_bytes_per_catalog(const Grid &G) { return 20.0 * G.R() + 7.0 * 8.0 * G.C(); }
+
+// Catalogs per FFT subgroup.
+int batch_fft_group(const Grid &G) {
+    return (int)std::max(1.0, std::floor(kBatchFFTBytes / batch_fft_bytes_per_catalog(G)));
+}
 
 std::vector<BatchGroup> batch_groups(const Grid &G, const baorec_params *p, int ncat, const int64_t *off,
                                      std::vector<int64_t> &t64, std::vector<int> &t32) {
     const int per = p->mas == BAOREC_MAS_TSC ? 27 : 8;
-    int maxnb = (int)std::max(1.0, std::floor(kBatchGroupBytes / batch_bytes_per_catalog(G)));
+    int maxnb = (int)std::max(1.0, std::min((double)kBatchScatterCats, std::floor(kBatchScatterBytes / (4.0 * G.R()))));
     maxnb = std::min(maxnb, ncat);
     const int ng = (ncat + maxnb - 1) / maxnb;
     maxnb = (ncat + ng - 1) / ng;   // balanced group sizes
@@ -1233,6 +1243,10 @@ std::vector<BatchGroup> batch_groups(const Grid &G, const baorec_params *p, int 
     return gs;
 }
 
+void batch_fft_group_run(baorec_ctx *c, const Grid &G, const baorec_params *p, const int64_t *off, int c0, int nb,
+                         Slot *slots, const float *rho, float *ds, float4 *disp, float2 *cw, FFTBufs &f, float *x,
+                         float *y, float *z, float *ox, float *oy, float *oz, int32_t *err);
+
 // One group: run!(IterativeRecon, grid, x, y, z, w) (periodic box, no randoms)
 // followed by reconstructed_positions(recon, x, y, z; field = :sum) for each
 // catalog, every step batched over the group's catalogs.
@@ -1247,10 +1261,6 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
                        p->box_size[1], p->box_size[2], p->box_min[0], p->box_min[1], p->box_min[2]);
     hipchk(c, hipGetLastError(), "slots_set");
     float *rho = bufT<float>(c, "bt_rho", (size_t)maxnb * R);
-    float *ds = bufT<float>(c, "bt_delta_s", (size_t)maxnb * R);
-    float4 *disp = bufT<float4>(c, "bt_disp", (size_t)maxnb * R);
-    float2 *cw = bufT<float2>(c, "bt_cwork", (size_t)7 * maxnb * C);
-    const Work W{cw, cw + (int64_t)nb * C, cw + (int64_t)4 * nb * C, (int64_t)nb * C};
     const int64_t *d_off = d64 + g.o64, *d_beg = d_off + nb + 1;
     const int *d_cat = d32 + g.o32, *d_blk0 = d_cat + std::max(g.nblk, 1);
 
@@ -1295,9 +1305,28 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
         hipchk(c, launch_scatter_accum(a, 0, c->stream), "batch accumulate");
     }
 
-    // smooth! and the overdensity (src/recon.jl:42-58); per-catalog FP64 sums
+    // the rest in FFT subgroups [s0, s0 + nf) of the scatter group
+    const int nfmax = std::min(batch_fft_group(G), maxnb);
+    float *ds = bufT<float>(c, "bt_delta_s", (size_t)nfmax * R);
+    float4 *disp = bufT<float4>(c, "bt_disp", (size_t)nfmax * R);
+    float2 *cw = bufT<float2>(c, "bt_cwork", (size_t)7 * nfmax * C);
     FFTBufs f = fft_bufs(c, G);
     tables(c, G, slots, f);
+    for (int s0 = 0; s0 < nb; s0 += nfmax) {
+        const int nf = std::min(nfmax, nb - s0);
+        batch_fft_group_run(c, G, p, off, g.c0 + s0, nf, slots + s0, rho + (int64_t)s0 * R, ds, disp, cw, f, x, y,
+                            z, ox, oy, oz, err);
+    }
+}
+
+// Smoothing, overdensity, iterations, displacements and positions of nb
+// catalogs (from catalog c0) whose density grids are rho (R apart).
+void batch_fft_group_run(baorec_ctx *c, const Grid &G, const baorec_params *p, const int64_t *off, int c0, int nb,
+                         Slot *slots, const float *rho, float *ds, float4 *disp, float2 *cw, FFTBufs &f, float *x,
+                         float *y, float *z, float *ox, float *oy, float *oz, int32_t *err) {
+    const int64_t R = G.R(), C = G.C();
+    const Work W{cw, cw + (int64_t)nb * C, cw + (int64_t)4 * nb * C, (int64_t)nb * C};
+    // smooth! and the overdensity (src/recon.jl:42-58); per-catalog FP64 sums
     x_fwd(c, G, rho, W.C0, nb, &slots[0].sum[0], (int64_t)(sizeof(Slot) / sizeof(double)));
     strided_plain(c, G, f, 1, false, W.C0, nb);
     {
@@ -1327,7 +1356,7 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
 
     // reconstructed_positions(...; field = :sum), src/recon.jl:366-381
     for (int b = 0; b < nb; b++) {
-        const int64_t o = off[g.c0 + b], n = off[g.c0 + b + 1] - o;
+        const int64_t o = off[c0 + b], n = off[c0 + b + 1] - o;
         if (n == 0) continue;
         GatherArgs ga{};
         ga.x = x + o;
