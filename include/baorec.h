@@ -394,11 +394,13 @@ const char *baorec_io_last_error(void);
  * share the grid and the params (box_size / box_min must be set; no randoms).
  * Outputs: the reconstructed positions of every particle in ox, oy, oz.
  * Inputs are wrapped in place when mutate_inputs, as run! does.  Results are
- * bitwise those of the single-catalog calls.  Catalogs run in groups whose
- * grid and spectrum buffers stay within 2 GiB (0.93 GB per catalog at 256^3,
- * so pairs of catalogs; at least one catalog per group), one launch sequence
- * per group on the context's stream.  A particle outside
- * the grid gives BAOREC_ERR_OUT_OF_BOX.
+ * bitwise those of the single-catalog calls.  The sort and mass assignment
+ * run over scatter groups of up to 16 catalogs (density grids within 8 GiB,
+ * at most 2^29 placements); the rest runs over FFT subgroups of a scatter
+ * group whose grid and spectrum buffers stay within 2 GiB (0.86 GB per
+ * catalog at 256^3, so pairs of catalogs; at least one catalog per group),
+ * all on the context's stream.  A particle outside the grid gives
+ * BAOREC_ERR_OUT_OF_BOX.
  */
 int baorec_run_batch(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3],
                      int mem, int ncat, const int64_t *offsets, float *x, float *y,

@@ -3,9 +3,10 @@ and the CPU oracle.
 
 A batch runs, per catalog, the same kernels as run!(IterativeRecon, ...) +
 reconstructed_positions(...; field = :sum) with every launch covering all
-catalogs of a group; the per-catalog FP64 sums and the fixed-point
-mass assignment are deterministic, so the batch must reproduce the
-single-catalog outputs bit for bit.  One catalog per case is also checked
+catalogs of a group (a scatter group for the sort and the mass assignment,
+an FFT subgroup for the rest); the per-catalog FP64 sums and the
+fixed-point mass assignment are deterministic, so the batch must reproduce
+the single-catalog outputs bit for bit.  One catalog per case is also checked
 against the oracle (1e-4 Mpc/h RMS on the shifted positions).
 """
 import numpy as np
@@ -86,8 +87,9 @@ def test_batch_catalog_matches_oracle(ctx):
 
 
 def test_batch_several_groups_device_memory(ctx):
-    """At 256^3 a group holds 2 catalogs (2 GiB of grids), so 20 catalogs run
-    as ten groups; device-resident inputs."""
+    """At 256^3 the 20 catalogs scatter as two groups of 10 and run the FFT
+    pipeline in subgroups of 2 (2 GiB of grids and spectra) over each
+    group's density grids; device-resident inputs."""
     from baorec_hip.devmem import DeviceArray
     grid = (256, 256, 256)
     rng = np.random.default_rng(5)
@@ -97,7 +99,9 @@ def test_batch_several_groups_device_memory(ctx):
     dev = [DeviceArray.from_numpy(a.copy()) for a in (x, y, z, w)]
     got = bh.run_batch(bh.IterativeRecon(**kw), grid, *dev, off)
     got = [g.numpy() for g in got]
-    for c in (0, 1, 10, 19):   # both of the first group, later groups
+    # both of the first FFT subgroup, a middle one, the last of the first
+    # scatter group, the first and last of the second
+    for c in (0, 1, 5, 9, 10, 19):
         s = slice(off[c], off[c + 1])
         cx, cy, cz = x[s].copy(), y[s].copy(), z[s].copy()
         rec = bh.IterativeRecon(**kw)
