@@ -157,6 +157,7 @@ struct ScatterArgs {
     int wrap, mutate, tsc;
     int tx, ty, tz;             // tile extent (cells)
     int ntx, nty, ntz;          // tiles per axis
+    int sx, sy, sz;             // log2 of the tile extents (-1: not a power of two; set by the launchers)
     uint32_t *count;            // per (catalog, tile)
     uint32_t *cursor;           // per (catalog, tile): start, advanced to end
     float4 *sorted;             // (u_x, u_y, u_z, w) per placement

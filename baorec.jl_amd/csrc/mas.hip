@@ -166,6 +166,48 @@ __device__ __forceinline__ int axis_tiles(const ScatterArgs &a, float u, int n, 
     return cnt;
 }
 
+// Tile of cell index i along an axis: i >> sh for power-of-two tiles.
+__device__ __forceinline__ int tile_of(int i, int t, int sh) { return sh >= 0 ? i >> sh : i / t; }
+
+// axis_tiles for a mass assignment fixed at compile time.  CIC: the two
+// corners are consecutive cells (or n-1 and 0 with the wrap), so they lie in
+// one tile or two.
+template <bool TSC>
+__device__ __forceinline__ int axis_tiles_m(const ScatterArgs &a, float u, int n, int t, int sh, int (&tl)[3],
+                                            bool &ok) {
+    if constexpr (TSC) {
+        int idx[3];
+        float ww[3];
+        ok = tsc_axis(u, n, a.wrap, idx, ww) && ok;
+        if (!ok) return 0;
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int tt = tile_of(idx[k], t, sh);
+            bool seen = false;
+            for (int q = 0; q < cnt; q++) seen |= (tl[q] == tt);
+            if (!seen) tl[cnt++] = tt;
+        }
+        return cnt;
+    } else {
+        float w0, w1;
+        int i0 = 0, i1 = 0;
+        ok = cic_axis(u, n, a.wrap, i0, i1, w0, w1) && ok;
+        if (!ok) return 0;
+        tl[0] = tile_of(i0, t, sh);
+        tl[1] = tile_of(i1, t, sh);
+        return tl[1] != tl[0] ? 2 : 1;
+    }
+}
+
+int tile_shift(int t) { return t > 0 && (t & (t - 1)) == 0 ? __builtin_ctz(t) : -1; }
+
+void set_tile_shifts(ScatterArgs &a) {
+    a.sx = tile_shift(a.tx);
+    a.sy = tile_shift(a.ty);
+    a.sz = tile_shift(a.tz);
+}
+
 __global__ void k_scatter_count(ScatterArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.ntotal) return;
@@ -398,6 +440,7 @@ __device__ __forceinline__ void sort_range(int64_t n, int nblk, int blk, int64_t
     end = n * (blk + 1) / nblk;
 }
 
+template <bool TSC>
 __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint32_t *H,
                                                             const int *blk_cat,
                                                             const int64_t *blk_beg) {
@@ -436,9 +479,9 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
             PartU p = particle_at(a, cat, i, px[u], py[u], pz[u], pw[u], a.wrap != 0, a.wrap && a.mutate);
             bool ok = true;
             int tx[3], ty[3], tz[3];
-            const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
-            const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
-            const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
+            const int cx = axis_tiles_m<TSC>(a, p.u[0], a.nx, a.tx, a.sx, tx, ok);
+            const int cy = axis_tiles_m<TSC>(a, p.u[1], a.ny, a.ty, a.sy, ty, ok);
+            const int cz = axis_tiles_m<TSC>(a, p.u[2], a.nz, a.tz, a.sz, tz, ok);
             if (!ok) {
                 atomicOr(a.err, 1);
                 continue;
@@ -497,6 +540,7 @@ __global__ void __launch_bounds__(kScanCols * kScanGroups) k_sort_colscan(uint32
     }
 }
 
+template <bool TSC>
 __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, const uint32_t *H,
                                                              const int *blk_cat,
                                                              const int64_t *blk_beg) {
@@ -534,9 +578,9 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, cons
             PartU p = particle_at(a, cat, i, px[u], py[u], pz[u], pw[u], a.wrap && !a.mutate, false);
             bool ok = true;
             int tx[3], ty[3], tz[3];
-            const int cx = axis_tiles(a, p.u[0], a.nx, a.tx, a.ntx, tx, ok);
-            const int cy = axis_tiles(a, p.u[1], a.ny, a.ty, a.nty, ty, ok);
-            const int cz = axis_tiles(a, p.u[2], a.nz, a.tz, a.ntz, tz, ok);
+            const int cx = axis_tiles_m<TSC>(a, p.u[0], a.nx, a.tx, a.sx, tx, ok);
+            const int cy = axis_tiles_m<TSC>(a, p.u[1], a.ny, a.ty, a.sy, ty, ok);
+            const int cz = axis_tiles_m<TSC>(a, p.u[2], a.nz, a.tz, a.sz, tz, ok);
             if (!ok) continue;
             const float4 rec = make_float4(p.u[0], p.u[1], p.u[2], p.w);
             for (int k = 0; k < cz; k++)
@@ -594,11 +638,15 @@ bool sort_fits(const ScatterArgs &a) { return a.ntx * a.nty * a.ntz <= kSortMaxT
 
 hipError_t launch_sort(ScatterArgs &a, int nblk, uint32_t *H, const int *blk_cat, const int64_t *blk_beg,
                        const int *cat_blk0, hipStream_t s) {
+    set_tile_shifts(a);
     const int T = a.ntx * a.nty * a.ntz;
     const int64_t ntiles = (int64_t)T * a.ncat;
     hipError_t e;
     if (nblk > 0) {
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblk), dim3(kSortThreads), 0, s, a, H, blk_cat, blk_beg);
+        if (a.tsc)
+            hipLaunchKernelGGL(k_sort_hist<true>, dim3(nblk), dim3(kSortThreads), 0, s, a, H, blk_cat, blk_beg);
+        else
+            hipLaunchKernelGGL(k_sort_hist<false>, dim3(nblk), dim3(kSortThreads), 0, s, a, H, blk_cat, blk_beg);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_sort_colscan, dim3((unsigned)((ntiles + kScanCols - 1) / kScanCols)),
@@ -607,7 +655,10 @@ hipError_t launch_sort(ScatterArgs &a, int nblk, uint32_t *H, const int *blk_cat
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.count, a.cursor, ntiles);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (nblk > 0) {
-        hipLaunchKernelGGL(k_sort_place, dim3(nblk), dim3(kSortThreads), 0, s, a, H, blk_cat, blk_beg);
+        if (a.tsc)
+            hipLaunchKernelGGL(k_sort_place<true>, dim3(nblk), dim3(kSortThreads), 0, s, a, H, blk_cat, blk_beg);
+        else
+            hipLaunchKernelGGL(k_sort_place<false>, dim3(nblk), dim3(kSortThreads), 0, s, a, H, blk_cat, blk_beg);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_add_count, dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, s, a.cursor, a.count,
