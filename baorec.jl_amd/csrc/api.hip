@@ -406,7 +406,7 @@ void strided(baorec_ctx *c, const Grid &G, const StridedArgs &a0, int njobs, int
 }
 
 void x_inv(baorec_ctx *c, const Grid &G, const XInvArgs &a, int epi, int nbatch, const char *what) {
-    double out = a.out ? (epi == XE_DISP ? 16.0 : 4.0) : 0.0;
+    double out = a.out ? (epi == XE_DISP ? 12.0 : 4.0) : 0.0;
     double aux = (epi == XE_ITER_LOCAL || epi == XE_ITER_FIXED) ? 4.0 : 0.0;
     double fwd = a.fwd_out ? 8.0 * G.Cdense() : 0.0;
     static const char *const label[] = {"fft_x_c2r", "fft_x_c2r_od", "fft_x_c2r_od", "fft_x_c2r_iter",
@@ -634,11 +634,11 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
     }
 }
 
-// compute_displacements grids of nb catalogs: float4 (Psi_x, Psi_y, Psi_z, 0),
-// catalog b at disp + b * R.  The k and x tables come from slot[0] (the
-// catalogs share one box).
-void displacement_grid(baorec_ctx *c, const Grid &G, int kind, Slot *slot, const float *field,
-                       float4 *disp, bool c0_ready = false, int nb = 1, const Work *Wp = nullptr) {
+// compute_displacements grids of nb catalogs: 3 floats per cell (Psi_x,
+// Psi_y, Psi_z), catalog b at disp + 3 b R.  The k and x tables come from
+// slot[0] (the catalogs share one box).
+void displacement_grid(baorec_ctx *c, const Grid &G, int kind, Slot *slot, const float *field, float *disp,
+                       bool c0_ready = false, int nb = 1, const Work *Wp = nullptr) {
     FFTBufs f = fft_bufs(c, G);
     tables(c, G, slot, f);
     const Work W = Wp ? *Wp : work_single(f, G);
@@ -667,7 +667,7 @@ void displacement_grid(baorec_ctx *c, const Grid &G, int kind, Slot *slot, const
     a.in[1] = Hb + W.q; a.kxpow[1] = 0;
     a.in[2] = Hb + 2 * W.q; a.kxpow[2] = 0;
     a.nin = 3;
-    a.out = reinterpret_cast<float *>(disp);
+    a.out = disp;
     x_inv(c, G, a, XE_DISP, nb, "x displacement");
 }
 
@@ -1203,9 +1203,9 @@ struct BatchGroup {
     size_t o64, o32;     // offsets of the group's tables in the packed arrays
 };
 
-// FFT buffers per catalog: delta_s (FP32), displacement (float4) and seven
+// FFT buffers per catalog: delta_s (FP32), displacement (3 x FP32) and seven
 // spectra.
-double batch_fft_bytes_per_catalog(const Grid &G) { return 20.0 * G.R() + 7.0 * 8.0 * G.C(); }
+double batch_fft_bytes_per_catalog(const Grid &G) { return 16.0 * G.R() + 7.0 * 8.0 * G.C(); }
 
 // Catalogs per FFT subgroup.
 int batch_fft_group(const Grid &G) {
@@ -1244,7 +1244,7 @@ std::vector<BatchGroup> batch_groups(const Grid &G, const baorec_params *p, int 
 }
 
 void batch_fft_group_run(baorec_ctx *c, const Grid &G, const baorec_params *p, const int64_t *off, int c0, int nb,
-                         Slot *slots, const float *rho, float *ds, float4 *disp, float2 *cw, FFTBufs &f, float *x,
+                         Slot *slots, const float *rho, float *ds, float *disp, float2 *cw, FFTBufs &f, float *x,
                          float *y, float *z, float *ox, float *oy, float *oz, int32_t *err);
 
 // One group: run!(IterativeRecon, grid, x, y, z, w) (periodic box, no randoms)
@@ -1308,7 +1308,7 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
     // the rest in FFT subgroups [s0, s0 + nf) of the scatter group
     const int nfmax = std::min(batch_fft_group(G), maxnb);
     float *ds = bufT<float>(c, "bt_delta_s", (size_t)nfmax * R);
-    float4 *disp = bufT<float4>(c, "bt_disp", (size_t)nfmax * R);
+    float *disp = bufT<float>(c, "bt_disp", (size_t)3 * nfmax * R);
     float2 *cw = bufT<float2>(c, "bt_cwork", (size_t)7 * nfmax * C);
     FFTBufs f = fft_bufs(c, G);
     tables(c, G, slots, f);
@@ -1322,7 +1322,7 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
 // Smoothing, overdensity, iterations, displacements and positions of nb
 // catalogs (from catalog c0) whose density grids are rho (R apart).
 void batch_fft_group_run(baorec_ctx *c, const Grid &G, const baorec_params *p, const int64_t *off, int c0, int nb,
-                         Slot *slots, const float *rho, float *ds, float4 *disp, float2 *cw, FFTBufs &f, float *x,
+                         Slot *slots, const float *rho, float *ds, float *disp, float2 *cw, FFTBufs &f, float *x,
                          float *y, float *z, float *ox, float *oy, float *oz, int32_t *err) {
     const int64_t R = G.R(), C = G.C();
     const Work W{cw, cw + (int64_t)nb * C, cw + (int64_t)4 * nb * C, (int64_t)nb * C};
@@ -1363,7 +1363,7 @@ void batch_fft_group_run(baorec_ctx *c, const Grid &G, const baorec_params *p, c
         ga.y = y + o;
         ga.z = z + o;
         ga.n = n;
-        ga.disp = disp + (int64_t)b * R;
+        ga.disp = disp + (int64_t)3 * b * R;
         ga.slot = slots + b;
         ga.nx = G.nx;
         ga.ny = G.ny;
@@ -1379,7 +1379,7 @@ void batch_fft_group_run(baorec_ctx *c, const Grid &G, const baorec_params *p, c
         ga.oy = oy + o;
         ga.oz = oz + o;
         ga.err = err;
-        ProfScope ps(c, "gather", (double)n * (24.0 + 128.0));
+        ProfScope ps(c, "gather", (double)n * (24.0 + 96.0));
         hipchk(c, launch_gather(ga, c->stream), "batch gather");
     }
 }
@@ -1714,14 +1714,14 @@ static int shifts_impl(baorec_ctx *c, const baorec_params *p, int mem, const flo
         if (field < 0 || field > 2) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "field must be disp, rsd or sum");
         if (n < 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "negative particle count");
         Grid G;
-        float4 *disp;
+        float *disp;
         Slot *slot;
         if (!field3d) {
             if (!c->has_result) fail(c, BAOREC_ERR_NO_RESULT, "no cached result (run! first)");
             G = check_grid(c, c->result_grid, true);
             kind = c->result_kind;
             slot = slot_buf(c);
-            disp = bufT<float4>(c, "disp_cache", G.R());
+            disp = bufT<float>(c, "disp_cache", (size_t)3 * G.R());
             const float key[6] = {p->box_size[0], p->box_size[1], p->box_size[2],
                                   p->box_min[0], p->box_min[1], p->box_min[2]};
             if (!c->disp_valid || c->disp_gen != c->result_gen || memcmp(key, c->disp_box, sizeof key)) {
@@ -1741,7 +1741,7 @@ static int shifts_impl(baorec_ctx *c, const baorec_params *p, int mem, const flo
             slot = slot_buf(c) + 1;
             slot_from_params(c, slot, p, 0.0);
             const float *fd = stage_in(c, mem, field3d, G.R(), "in_field");
-            disp = bufT<float4>(c, "disp_tmp", G.R());
+            disp = bufT<float>(c, "disp_tmp", (size_t)3 * G.R());
             displacement_grid(c, G, kind, slot, fd, disp);
         }
         const float *dx = stage_in(c, mem, x, n, "sh_x");
@@ -1772,7 +1772,7 @@ static int shifts_impl(baorec_ctx *c, const baorec_params *p, int mem, const flo
         a.oz = o3;
         a.err = &slot->err;
         {
-            ProfScope ps(c, "gather", (double)n * (24.0 + 128.0));
+            ProfScope ps(c, "gather", (double)n * (24.0 + 96.0));
             hipchk(c, launch_gather(a, c->stream), "gather");
         }
         check_err_flag(c, slot);

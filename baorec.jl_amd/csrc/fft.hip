@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
             ix2[i][1] = s1 > 0.f ? 1.0f / s1 : 0.f;
         }
     }
-    float4 disp[8][2];
+    float3 disp[8][2];
     float rr[8][2];
     float qs[8][2];
 
@@ -274,15 +274,17 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
         }
     }
     if constexpr (EPI == XE_DISP) {
+        // 12-byte cells (Psi_x, Psi_y, Psi_z): cells 2m and 2m + 1 are three
+        // float2 (rbase is even, so 3 * rbase floats is 8-byte aligned)
         if (!valid) return;
-        float4 *o4 = reinterpret_cast<float4 *>(a.out) + rbase;
+        float2 *o2 = reinterpret_cast<float2 *>(a.out + 3 * rbase);
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const int m = PI::out_pos(t, i);
-            disp[i][0].w = 0.f;
-            disp[i][1].w = 0.f;
-            o4[2 * m] = disp[i][0];
-            o4[2 * m + 1] = disp[i][1];
+            const float3 d0 = disp[i][0], d1 = disp[i][1];
+            o2[3 * m] = make_float2(d0.x, d0.y);
+            o2[3 * m + 1] = make_float2(d0.z, d1.x);
+            o2[3 * m + 2] = make_float2(d1.y, d1.z);
         }
         return;
     }

@@ -61,7 +61,7 @@ enum XEpi : int {
     XE_OD_RAND = 2,   // data - alpha*randoms normalisation (src/recon.jl:78-85)
     XE_ITER_LOCAL = 3,// delta_r = delta_s - (x^2 a + x b + c) / x^2 (inputs A, B, C)
     XE_ITER_FIXED = 4,// delta_r = delta_s - fac * D
-    XE_DISP = 5,      // float4 (Psi_x, Psi_y, Psi_z, 0)
+    XE_DISP = 5,      // (Psi_x, Psi_y, Psi_z) per cell, 12 bytes
 };
 
 constexpr int kMaxXIn = 6;
@@ -74,7 +74,7 @@ struct XInvArgs {
     int64_t rstride;      // real elements between batch items
     int P;                // complex pitch of a kx row
     int64_t PZ;           // complex pitch of a kz plane
-    float *out;           // real output (float4 for XE_DISP)
+    float *out;           // real output (3 floats per cell for XE_DISP)
     const float *aux;     // delta_s for the iterative updates
     int nx, ny, nz;
     float scale;          // 1/(nx ny nz)
@@ -187,7 +187,7 @@ int scatter_tile_max();
 struct GatherArgs {
     const float *x, *y, *z;
     int64_t n;
-    const float4 *disp;         // float4 displacement grid
+    const float *disp;          // displacement grid, (Psi_x, Psi_y, Psi_z) per cell
     const Slot *slot;
     int nx, ny, nz;
     int field;                  // 0 disp, 1 rsd, 2 sum

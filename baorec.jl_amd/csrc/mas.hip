@@ -740,6 +740,8 @@ __device__ __forceinline__ bool gather_axis(float p, float bmin, float L, int n,
     return true;
 }
 
+__device__ __forceinline__ float3 load3(const float *p) { return make_float3(p[0], p[1], p[2]); }
+
 __global__ void k_gather(GatherArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -755,10 +757,11 @@ __global__ void k_gather(GatherArgs a) {
         return;
     }
     const int64_t nx = a.nx, nxy = (int64_t)a.nx * a.ny;
-#define G(ix, iy, iz) a.disp[(ix) + nx * (iy) + nxy * (iz)]
+    // 12-byte cells: one dwordx3 load per corner
+#define G(ix, iy, iz) load3(a.disp + 3 * ((ix) + nx * (iy) + nxy * (iz)))
     // term order of src/mas.jl:295-302
-    const float4 f0 = G(dx, dy, dz), f1 = G(dx, dy, uz), f2 = G(dx, uy, dz), f3 = G(dx, uy, uz);
-    const float4 f4 = G(ux, dy, dz), f5 = G(ux, dy, uz), f6 = G(ux, uy, dz), f7 = G(ux, uy, uz);
+    const float3 f0 = G(dx, dy, dz), f1 = G(dx, dy, uz), f2 = G(dx, uy, dz), f3 = G(dx, uy, uz);
+    const float3 f4 = G(ux, dy, dz), f5 = G(ux, dy, uz), f6 = G(ux, uy, dz), f7 = G(ux, uy, uz);
 #undef G
     float s[3];
 #pragma unroll

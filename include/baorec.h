@@ -397,7 +397,7 @@ const char *baorec_io_last_error(void);
  * bitwise those of the single-catalog calls.  The sort and mass assignment
  * run over scatter groups of up to 16 catalogs (density grids within 8 GiB,
  * at most 2^29 placements); the rest runs over FFT subgroups of a scatter
- * group whose grid and spectrum buffers stay within 2 GiB (0.86 GB per
+ * group whose grid and spectrum buffers stay within 2 GiB (0.80 GB per
  * catalog at 256^3, so pairs of catalogs; at least one catalog per group),
  * all on the context's stream.  A particle outside the grid gives
  * BAOREC_ERR_OUT_OF_BOX.
