@@ -134,13 +134,38 @@ __device__ __forceinline__ void xline_load(const float2 *__restrict__ src, bool 
     xn = (valid && t == 0) ? src[M] : make_float2(0.f, 0.f);
 }
 
+// Waves per SIMD the x C2R pass is compiled for: the local-LOS iteration
+// (three partial sums per cell) at 3 with n <= 512 (168 VGPRs, no spills);
+// the others fit 4 or more by themselves.
 template <int M, int EPI>
-__global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
+struct XWaves {
+    static constexpr int W = (EPI == XE_ITER_LOCAL && M <= 256) ? 3 : 1;
+};
+
+// An SGPR zero the compiler cannot see through: table reads indexed with it
+// are repeated per use instead of being kept live in registers.
+__device__ __forceinline__ int opaque_zero() {
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    return z;
+}
+
+template <int M, int EPI>
+__global__ void __launch_bounds__(XShape<M>::NT) __attribute__((amdgpu_waves_per_eu(XWaves<M, EPI>::W)))
+k_x_c2r(XInvArgs a) {
     using S = XShape<M>;
     using PI = Plan<M, true>;
     constexpr int NIN = XIn<EPI>::N;
     constexpr int TPL = S::TPL;
+    constexpr bool LXV = EPI == XE_ITER_LOCAL;
     __shared__ float2 lds[S::LPW * S::LS];
+    // The twiddles (stage factors and exp(2 pi i k / N), 3M entries), kx and
+    // (local LOS) the x coordinates in LDS, read per use: held in registers
+    // they kept the local-LOS pass at 2 waves per SIMD (226 VGPRs), whose
+    // arithmetic then hardly overlapped its memory traffic
+    __shared__ float2 twl[3 * M];
+    __shared__ float kxl[M + 1];
+    __shared__ float xvl[LXV ? 2 * M : 1];
     const int tid = threadIdx.x;
     const int li = tid / TPL, t = tid % TPL;
     const int64_t nlines = (int64_t)a.ny * a.nz;
@@ -155,20 +180,15 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
     const int lane = threadIdx.x & 63;
     const int plane = t == 0 ? lane : lane - t + (TPL - t);
 
-    TwCache<M, true> tc;
-    tc.load(t, g_tw);
-    float2 wk[8];   // exp(+2 pi i k / N) of this thread's bins
-    float kxv[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const int k = t + i * TPL;
-        wk[i] = cconj(g_tw[2 * M + k]);
-        kxv[i] = kx[k];
+    for (int i = tid; i < 3 * M; i += S::NT) twl[i] = g_tw[i];
+    for (int i = tid; i <= M; i += S::NT) kxl[i] = kx[i];
+    if constexpr (LXV) {
+        const float *xvg = a.xtab + b * a.xtabStride;
+        for (int i = tid; i < 2 * M; i += S::NT) xvl[i] = xvg[i];
     }
-    const float kxn = kx[M];
+    __syncthreads();
 
     float2 acc[8];
-    float ix2[8][2];
     const float *xv = a.xtab ? a.xtab + b * a.xtabStride : nullptr;
     float xy = 0.f, xz = 0.f;
     if constexpr (EPI == XE_ITER_LOCAL) {
@@ -184,15 +204,6 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
         } else {
             acc[i] = make_float2(0.f, 0.f);
         }
-        if constexpr (EPI == XE_ITER_LOCAL) {
-            // 1/x^2 once per cell (the reference divides by x^2 per pair,
-            // src/iterative.jl:33-34; the product differs by at most 1 ulp)
-            const float x0 = xv[2 * m], x1 = xv[2 * m + 1];
-            const float s0 = (x0 * x0 + xy * xy) + xz * xz;
-            const float s1 = (x1 * x1 + xy * xy) + xz * xz;
-            ix2[i][0] = s0 > 0.f ? 1.0f / s0 : 0.f;
-            ix2[i][1] = s1 > 0.f ? 1.0f / s1 : 0.f;
-        }
     }
     float3 disp[8][2];
     float rr[8][2];
@@ -206,11 +217,12 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
         float2 nxt[8], nn;
         if (q + 1 < NIN) xline_load<M>(a.in[q + 1] + b * a.cstride + crow, valid, t, nxt, nn);
         const int pw = a.kxpow[q];
+        const int z0 = opaque_zero();
         float2 v[8];
         float2 xk[8];
 #pragma unroll
-        for (int i = 0; i < 8; i++) xk[i] = pw ? cscale(cur[i], kpow(kxv[i], pw)) : cur[i];
-        const float2 xnyq = pw ? cscale(cn, kpow(kxn, pw)) : cn;
+        for (int i = 0; i < 8; i++) xk[i] = pw ? cscale(cur[i], kpow(kxl[z0 + t + i * TPL], pw)) : cur[i];
+        const float2 xnyq = pw ? cscale(cn, kpow(kxl[z0 + M], pw)) : cn;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             float2 xm;
@@ -225,10 +237,10 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
             const float2 xc = cconj(xm);
             const float2 s = cadd(xkk, xc);
             const float2 d = csub(xkk, xc);
-            const float2 wd = cmul(wk[i], d);
+            const float2 wd = cmul(cconj(twl[z0 + 2 * M + t + i * TPL]), d);  // exp(+2 pi i k / N) d
             v[i] = make_float2(s.x - wd.y, s.y + wd.x);  // s + i * w * d
         }
-        fft_line_c<M, true>(v, lds, lay, t, tc);
+        fft_line<M, true>(v, lds, lay, t, twl + z0);
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const float r0 = v[i].x * a.scale, r1 = v[i].y * a.scale;
@@ -251,12 +263,16 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
                 const float dv[2] = {r0, r1};
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
-                    const float xx = xv[2 * m + h];
+                    const float xx = xvl[z0 + 2 * m + h];
                     if (q == 0) qs[i][h] = (xx * xx) * dv[h];
                     else if (q == 1) qs[i][h] += xx * dv[h];
                     else {
                         const float u = qs[i][h] + dv[h];
-                        const float c = ix2[i][h];
+                        // 1/x^2 once per cell (the reference divides by
+                        // x^2 per pair, src/iterative.jl:33-34; the product
+                        // differs by at most 1 ulp)
+                        const float s2 = (xx * xx + xy * xy) + xz * xz;
+                        const float c = s2 > 0.f ? 1.0f / s2 : 0.f;
                         float *ac = &acc[i].x;
                         ac[h] = c > 0.f ? ac[h] - u * c : 0.f;
                     }
@@ -322,7 +338,7 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_c2r(XInvArgs a) {
             // forward R2C of the line just produced: acc is in the forward
             // plan's input order (inverse plans end with the forward's first radix)
             using PF = Plan<M, false>;
-            fft_line<M, false>(acc, lds, lay, t, g_tw);
+            fft_line<M, false>(acc, lds, lay, t, twl);
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < 8; i++) lds[lay(PF::out_pos(t, i))] = acc[i];
