@@ -131,26 +131,26 @@ __device__ __forceinline__ void cell_mcgs(T *v, const T *__restrict__ f, const M
 __global__ void k_mg_jacobi(float *__restrict__ vout, const float *__restrict__ vin,
                             const float *__restrict__ f, MGLevel L, float beta, float omega,
                             int has_los) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ix >= L.n[0]) return;
-    cell_jacobi(vout, vin, f, L, ix, blockIdx.y, blockIdx.z, beta, omega, has_los);
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= L.n[0] || iy >= L.n[1]) return;
+    cell_jacobi(vout, vin, f, L, ix, iy, blockIdx.z, beta, omega, has_los);
 }
 
 template <typename T>
 __global__ void k_mg_residual(T *__restrict__ r, const T *__restrict__ v, const T *__restrict__ f,
                               MGLevel L, float beta, int has_los) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ix >= L.n[0]) return;
-    cell_residual<T>(r, v, f, L, ix, blockIdx.y, blockIdx.z, beta, has_los);
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= L.n[0] || iy >= L.n[1]) return;
+    cell_residual<T>(r, v, f, L, ix, iy, blockIdx.z, beta, has_los);
 }
 
 __global__ void k_mg_residual_mixed(float *__restrict__ r, const double *__restrict__ v,
                                     const float *__restrict__ f, const double *fsum, double count, MGLevel L,
                                     float beta, int has_los) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ix >= L.n[0]) return;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= L.n[0] || iy >= L.n[1]) return;
     const double fshift = fsum ? *fsum / count : 0.0;
-    cell_residual_mixed(r, v, f, fshift, L, ix, blockIdx.y, blockIdx.z, beta, has_los);
+    cell_residual_mixed(r, v, f, fshift, L, ix, iy, blockIdx.z, beta, has_los);
 }
 
 // phi += e - mean(e): the correction of a V-cycle added to the FP64 solution,
@@ -166,9 +166,9 @@ __global__ void k_add_correction(T *__restrict__ phi, const float *__restrict__ 
 template <typename T>
 __global__ void k_mg_mcgs(T *__restrict__ v, const T *__restrict__ f, MGLevel L, float beta,
                           int has_los, int color) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y, iz = blockIdx.z;
-    if (ix >= L.n[0] || mg_color(ix, iy, iz) != color) return;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y * blockDim.y + threadIdx.y;
+    const int iz = blockIdx.z;
+    if (ix >= L.n[0] || iy >= L.n[1] || mg_color(ix, iy, iz) != color) return;
     cell_mcgs<T>(v, f, L, ix, iy, iz, beta, has_los);
 }
 
@@ -416,6 +416,16 @@ static inline dim3 cell_grid(int nx, int ny, int nz, int bx) {
     return dim3((unsigned)((nx + bx - 1) / bx), (unsigned)ny, (unsigned)nz);
 }
 static inline int bx_for(int nx) { return nx >= 256 ? 256 : (nx >= 64 ? 64 : nx); }
+// Stencil kernels: workgroups of 64 x 4 cells of one plane.  Four rows per
+// workgroup share their y-neighbour rows in the CU's cache; at 512^3 this ran
+// the RBGS lightcone solve in 64.0 ms against 67.0 with 256 x 1 (128 x 4,
+// 256 x 4 and 32 x 8 were slower).
+static inline void stencil_shape(const MGLevel &L, dim3 &grid, dim3 &block) {
+    const int by = 4 < L.n[1] ? 4 : L.n[1];
+    const int bx = L.n[0] >= 64 ? 64 : L.n[0];
+    block = dim3((unsigned)bx, (unsigned)by);
+    grid = dim3((unsigned)((L.n[0] + bx - 1) / bx), (unsigned)((L.n[1] + by - 1) / by), (unsigned)L.n[2]);
+}
 static inline unsigned red_blocks(int64_t n) {
     int64_t b = (n + 255) / 256;
     return (unsigned)(b > 1024 ? 1024 : (b < 1 ? 1 : b));
@@ -423,9 +433,9 @@ static inline unsigned red_blocks(int64_t n) {
 
 hipError_t launch_mg_jacobi(float *vout, const float *vin, const float *f, const MGLevel &L,
                             float beta, float omega, int has_los, hipStream_t s) {
-    const int bx = bx_for(L.n[0]);
-    hipLaunchKernelGGL(k_mg_jacobi, cell_grid(L.n[0], L.n[1], L.n[2], bx), dim3(bx), 0, s, vout,
-                       vin, f, L, beta, omega, has_los);
+    dim3 grid, block;
+    stencil_shape(L, grid, block);
+    hipLaunchKernelGGL(k_mg_jacobi, grid, block, 0, s, vout, vin, f, L, beta, omega, has_los);
     return hipGetLastError();
 }
 
@@ -437,18 +447,18 @@ hipError_t launch_mg_residual(float *r, const float *v, const float *f, const MG
 template <typename T>
 hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, float beta,
                                 int has_los, hipStream_t s) {
-    const int bx = bx_for(L.n[0]);
-    hipLaunchKernelGGL(k_mg_residual<T>, cell_grid(L.n[0], L.n[1], L.n[2], bx), dim3(bx), 0, s, r, v,
-                       f, L, beta, has_los);
+    dim3 grid, block;
+    stencil_shape(L, grid, block);
+    hipLaunchKernelGGL(k_mg_residual<T>, grid, block, 0, s, r, v, f, L, beta, has_los);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_mg_rbgs(T *v, const T *f, const MGLevel &L, float beta, int has_los, int color,
                           hipStream_t s) {
-    const int bx = bx_for(L.n[0]);
-    hipLaunchKernelGGL(k_mg_mcgs<T>, cell_grid(L.n[0], L.n[1], L.n[2], bx), dim3(bx), 0, s, v, f, L,
-                       beta, has_los, color);
+    dim3 grid, block;
+    stencil_shape(L, grid, block);
+    hipLaunchKernelGGL(k_mg_mcgs<T>, grid, block, 0, s, v, f, L, beta, has_los, color);
     return hipGetLastError();
 }
 
@@ -521,9 +531,9 @@ hipError_t launch_convert(T *dst, const U *src, int64_t n, hipStream_t s) {
 }
 hipError_t launch_mg_residual_mixed(float *r, const double *v, const float *f, const double *fsum, double count,
                                     const MGLevel &L, float beta, int has_los, hipStream_t s) {
-    const int bx = bx_for(L.n[0]);
-    hipLaunchKernelGGL(k_mg_residual_mixed, cell_grid(L.n[0], L.n[1], L.n[2], bx), dim3(bx), 0, s, r, v, f, fsum,
-                       count, L, beta, has_los);
+    dim3 grid, block;
+    stencil_shape(L, grid, block);
+    hipLaunchKernelGGL(k_mg_residual_mixed, grid, block, 0, s, r, v, f, fsum, count, L, beta, has_los);
     return hipGetLastError();
 }
 
