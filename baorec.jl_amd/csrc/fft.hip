@@ -50,16 +50,33 @@ __device__ void block_sum_store(double v, double *dst) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_sum_partials(const double *part, int64_t n, double *out,
-                                                      int64_t outStride) {
-    __shared__ double red[4];
+// out[b] = sum of the n partials of batch item b, in a fixed order: thread t
+// sums elements t + 1024 k over 8 independent accumulators (k mod 8), then
+// the accumulators, the waves and the workgroup in order.
+__global__ void __launch_bounds__(1024) k_sum_partials(const double *part, int64_t n, double *out,
+                                                       int64_t outStride) {
+    __shared__ double red[16];
     const double *p = part + (int64_t)blockIdx.x * n;
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) s += p[i];
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int64_t i = threadIdx.x;
+    for (; i + 7 * 1024 < n; i += 8 * 1024) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc[u] += p[i + u * 1024];
+    }
+#pragma unroll
+    for (int u = 0; u < 7; u++) {
+        const int64_t j = i + u * 1024;
+        if (j < n) acc[u] += p[j];
+    }
+    double s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) out[blockIdx.x * outStride] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; w++) t += red[w];
+        out[blockIdx.x * outStride] = t;
+    }
 }
 
 // ------------------------------------------------------------ x: R2C
@@ -723,7 +740,7 @@ static hipError_t x_r2c_m(const XFwdArgs &a, int nbatch, hipStream_t s) {
     dim3 grid(nwg, nbatch);
     hipLaunchKernelGGL(k_x_r2c<M>, grid, dim3(S::NT), 0, s, a);
     if (a.partials)
-        hipLaunchKernelGGL(k_sum_partials, dim3(nbatch), dim3(256), 0, s, a.partials, (int64_t)nwg, a.sums,
+        hipLaunchKernelGGL(k_sum_partials, dim3(nbatch), dim3(1024), 0, s, a.partials, (int64_t)nwg, a.sums,
                            a.sumStride);
     return hipGetLastError();
 }

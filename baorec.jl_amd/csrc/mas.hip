@@ -257,8 +257,18 @@ __global__ void __launch_bounds__(1024) k_scan(const uint32_t *count, uint32_t *
     const int tid = threadIdx.x;
     const int64_t per = (n + 1023) / 1024;
     const int64_t lo = tid * per, hi = min(n, lo + per);
+    // each thread's range in 16-byte loads when it is aligned (per a multiple
+    // of 4: the tile counts of one catalog, 4096 or 16384 tiles)
+    const bool vec = (per & 3) == 0 && (reinterpret_cast<uintptr_t>(count) & 15) == 0;
     uint32_t s = 0;
-    for (int64_t i = lo; i < hi; i++) s += count[i];
+    int64_t i0 = lo;
+    if (vec) {
+        for (; i0 + 3 < hi; i0 += 4) {
+            const uint4 c = *reinterpret_cast<const uint4 *>(count + i0);
+            s += ((c.x + c.y) + c.z) + c.w;
+        }
+    }
+    for (; i0 < hi; i0++) s += count[i0];
     part[tid] = s;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
