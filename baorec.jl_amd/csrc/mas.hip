@@ -338,16 +338,22 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
     const float bound = (float)cnt * __uint_as_float(a.wmax[cat]);
     const int e = bound > 0.f ? 61 - ilogbf(bound) : 0;
     const double S = ldexp(1.0, e);
-    for (int c = threadIdx.x; c < tcells; c += kAccumThreads) tile[c] = 0ull;
-    __syncthreads();
-    constexpr int U = 4;  // placements in flight per thread
-    for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += U * kAccumThreads) {
-        float4 rbuf[U];
+    constexpr int U = 4;  // placements per thread and step
+    auto load = [&](float4 (&rb)[U], uint32_t s0) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint32_t s = s0 + u * kAccumThreads;
-            rbuf[u] = s < end ? a.sorted[s] : make_float4(1.f, 1.f, 1.f, 0.f);
+            rb[u] = s < end ? a.sorted[s] : make_float4(1.f, 1.f, 1.f, 0.f);
         }
+    };
+    // the first step's placements are loaded while the tile is cleared, and
+    // each next step's while the current one is accumulated
+    float4 rbuf[U], rnext[U];
+    load(rbuf, beg + threadIdx.x);
+    for (int c = threadIdx.x; c < tcells; c += kAccumThreads) tile[c] = 0ull;
+    __syncthreads();
+    for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += U * kAccumThreads) {
+        if (s0 + U * kAccumThreads < end) load(rnext, s0 + U * kAccumThreads);
 #pragma unroll
         for (int u = 0; u < U; u++) {
             if (s0 + u * kAccumThreads >= end) break;
@@ -400,6 +406,8 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
             }
 #undef TI
         }
+#pragma unroll
+        for (int u = 0; u < U; u++) rbuf[u] = rnext[u];
     }
     __syncthreads();
     auto cell = [&](int c) { return (float)ldexp((double)(long long)tile[c], -e); };
