@@ -214,11 +214,13 @@ def test_run_iterative_box(ctx, los):
 
 
 @pytest.mark.parametrize("los,g", [((0.0, 0.0, 1.0), (16, 512, 512)), (None, (16, 512, 512)),
-                                   ((0.0, 0.0, 1.0), (32, 256, 256))])
+                                   ((0.0, 0.0, 1.0), (32, 256, 256)), (None, (1024, 16, 16)),
+                                   ((0.0, 0.0, 1.0), (1024, 16, 16)), (None, (512, 32, 16))])
 def test_run_iterative_box_long_axes(ctx, los, g):
     """y and z passes of length 256 and 512: the two-columns-per-thread
     kernel for single-output passes, the one-column kernel for multi-output
-    ones."""
+    ones; x lines of 512 and 1024 points (the x passes' LDS tables, the
+    local-LOS iteration pass at its largest line)."""
     x, y, z, w = synth.box_catalog(100000, L=1000.0, mesh=64, seed=23)
     kw = box_kw(los=los)
     rec = bh.IterativeRecon(**kw)
