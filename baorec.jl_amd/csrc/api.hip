@@ -30,8 +30,12 @@ struct baorec_ctx {
     int result_kind = 0;
     int32_t result_grid[3] = {0, 0, 0};
     uint64_t result_gen = 0;
-    // displacement grid cached for the result (Psi_x, Psi_y, Psi_z, 0)
+    // displacement grid cached for the result (Psi_x, Psi_y, Psi_z per cell)
     uint64_t disp_gen = 0;
+    // block tables of the data + randoms sort on the device, for these sizes
+    int64_t pair_n[2] = {-1, -1};
+    int pair_nblk = 0;
+    const void *pair_tab = nullptr;
     // generation of the result whose R2C spectrum is still in cwork[0]
     uint64_t spec_gen = 0;
     float disp_box[6] = {0, 0, 0, 0, 0, 0};
@@ -370,6 +374,84 @@ void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, f
     }
 }
 
+// cic! of the data and of the randoms into rho and rho + R (overwrite, no
+// wrap) as one two-catalog sort and accumulation: the same tiles, sums and
+// placements as two scatter() calls, in half the launches.  Falls back to two
+// calls when the tiles do not fit the sort.
+void scatter_pair(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, float *rho, float *x,
+                  float *y, float *z, const float *w, int64_t n, float *rx, float *ry, float *rz, const float *rw,
+                  int64_t nr) {
+    const int64_t per = p->mas == BAOREC_MAS_TSC ? 27 : 8;
+    ScatterArgs a{};
+    a.tsc = p->mas == BAOREC_MAS_TSC;
+    tiles_for(c, G, a, nr);
+    const int64_t ntot = n + nr;
+    if (!sort_fits(a) || ntot * per >= ((int64_t)1 << 32) || n == 0 || nr == 0) {
+        scatter(c, G, p, slot, rho, x, y, z, w, n, 0, 0, "data");
+        scatter(c, G, p, slot, rho + G.R(), rx, ry, rz, rw, nr, 0, 0, "randoms");
+        return;
+    }
+    a.x = x;
+    a.y = y;
+    a.z = z;
+    a.w = w;
+    a.x2 = rx;
+    a.y2 = ry;
+    a.z2 = rz;
+    a.w2 = rw;
+    a.n1 = n;
+    a.ncat = 2;
+    a.ntotal = ntot;
+    a.slot = slot;
+    a.slotStride = 0;
+    a.nx = G.nx;
+    a.ny = G.ny;
+    a.nz = G.nz;
+    a.wrap = 0;
+    a.mutate = 0;
+    const int64_t T = (int64_t)a.ntx * a.nty * a.ntz;
+    // block tables (int64: offsets 3 | block starts nblk + 1; int32: block
+    // catalogs nblk | first block per catalog 3), uploaded when the sizes change
+    int64_t *d64 = bufT<int64_t>(c, "pair_t64", 3 + kSortTableBlocks + 1);
+    int *d32 = bufT<int>(c, "pair_t32", kSortTableBlocks + 3);
+    if (c->pair_n[0] != n || c->pair_n[1] != nr || c->pair_tab != d64) {
+        const int64_t offs[3] = {0, n, ntot};
+        std::vector<int> blk_cat, cat_blk0;
+        std::vector<int64_t> blk_beg;
+        const int nblk = sort_blocks(ntot, offs, 2, blk_cat, blk_beg, cat_blk0);
+        if (nblk > kSortTableBlocks) fail(c, BAOREC_ERR_HIP, "sort block table overflow");
+        std::vector<int64_t> t64(offs, offs + 3);
+        t64.insert(t64.end(), blk_beg.begin(), blk_beg.end());
+        std::vector<int> t32(blk_cat);
+        t32.insert(t32.end(), cat_blk0.begin(), cat_blk0.end());
+        hipchk(c, hipMemcpy(d64, t64.data(), sizeof(int64_t) * t64.size(), hipMemcpyHostToDevice), "copy");
+        hipchk(c, hipMemcpy(d32, t32.data(), sizeof(int) * t32.size(), hipMemcpyHostToDevice), "copy");
+        c->pair_n[0] = n;
+        c->pair_n[1] = nr;
+        c->pair_nblk = nblk;
+        c->pair_tab = d64;
+    }
+    const int nblk = c->pair_nblk;
+    a.offsets = d64;
+    a.count = bufT<uint32_t>(c, "sc_count", (size_t)2 * T);
+    a.cursor = bufT<uint32_t>(c, "sc_cursor", (size_t)2 * T);
+    a.sorted = bufT<float4>(c, "sc_sorted", scatter_capacity(ntot, a.tsc));
+    a.err = &slot->err;
+    a.rho = rho;
+    a.rstride = G.R();
+    a.wmax = bufT<uint32_t>(c, "sc_wmax", 64);
+    hipchk(c, hipMemsetAsync(a.wmax, 0, sizeof(uint32_t) * 64, c->stream), "memset");
+    uint32_t *H = bufT<uint32_t>(c, "sc_hist", (size_t)nblk * T);
+    {
+        ProfScope ps(c, "scatter_bin", 28.0 * ntot + 16.0 * ntot * 1.2);
+        hipchk(c, launch_sort(a, nblk, H, d32, d64 + 3, d32 + nblk, c->stream), "scatter sort");
+    }
+    {
+        ProfScope ps(c, "scatter_accum", 16.0 * ntot * 1.2 + 4.0 * G.R() * 2);
+        hipchk(c, launch_scatter_accum(a, 0, c->stream), "scatter accumulate");
+    }
+}
+
 // ------------------------------------------------------------------ FFT
 
 struct FFTBufs {
@@ -503,8 +585,8 @@ void overdensity(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
     tables(c, G, slot, f);
     float *rho = bufT<float>(c, "rho2", (size_t)2 * G.R());
     const bool rand = nr > 0;
-    scatter(c, G, p, slot, rho, x, y, z, w, n, rand ? 0 : 1, 0, "data");
-    if (rand) scatter(c, G, p, slot, rho + G.R(), rx, ry, rz, rw, nr, 0, 0, "randoms");
+    if (rand) scatter_pair(c, G, p, slot, rho, x, y, z, w, n, rx, ry, rz, rw, nr);
+    else scatter(c, G, p, slot, rho, x, y, z, w, n, 1, 0, "data");
     smooth_to_half(c, G, f, rho, rand ? 2 : 1, slot->sum, p->smoothing_radius);
     XInvArgs a = xargs(G, f, slot);
     a.in[0] = f.cw;

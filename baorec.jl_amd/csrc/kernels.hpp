@@ -148,6 +148,10 @@ void set_twiddle_table(const float2 *dev_tw);
 struct ScatterArgs {
     float *x, *y, *z;           // positions (mutated when wrap && mutate)
     const float *w;
+    // optional second particle set (sort path, no wrap): particles
+    // [n1, ntotal) are x2[i - n1], ...; catalog 1 of a two-catalog sort
+    const float *x2, *y2, *z2, *w2;
+    int64_t n1;
     const int64_t *offsets;     // catalog c owns [offsets[c], offsets[c+1])
     int ncat;
     int64_t ntotal;
@@ -174,6 +178,8 @@ hipError_t launch_scatter_accum(const ScatterArgs &a, int accumulate, hipStream_
 hipError_t launch_zero(float *a, int64_t n, hipStream_t s);
 // Blocked counting sort (no global atomics); valid when sort_fits(a).
 bool sort_fits(const ScatterArgs &a);
+// Upper bound on the blocks sort_blocks makes for a handful of catalogs.
+constexpr int kSortTableBlocks = 1024;
 int sort_nblk(int64_t n);  // blocks of a single catalog (tables may then be NULL)
 int sort_blocks(int64_t n, const int64_t *offsets_host, int ncat, std::vector<int> &blk_cat,
                 std::vector<int64_t> &blk_beg, std::vector<int> &cat_blk0);

@@ -458,6 +458,17 @@ __device__ __forceinline__ void sort_range(int64_t n, int nblk, int blk, int64_t
     end = n * (blk + 1) / nblk;
 }
 
+// Particle arrays of catalog cat: the second set for catalog 1 of a
+// two-set sort (x2 != NULL), indexed from n1.
+struct SortSrc {
+    const float *x, *y, *z, *w;
+    int64_t i0;
+};
+__device__ __forceinline__ SortSrc sort_src(const ScatterArgs &a, int cat) {
+    if (a.x2 && cat == 1) return SortSrc{a.x2, a.y2, a.z2, a.w2, a.n1};
+    return SortSrc{a.x, a.y, a.z, a.w, 0};
+}
+
 template <bool TSC>
 __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint32_t *H,
                                                             const int *blk_cat,
@@ -475,6 +486,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
     const int T = a.ntx * a.nty * a.ntz;
     for (int i = threadIdx.x; i < T; i += kSortThreads) hist[i] = 0u;
     float wm = 0.f;
+    const SortSrc src = sort_src(a, cat);
     __syncthreads();
     // kSortUnroll particles per thread per step: their loads are issued
     // together, ahead of the LDS atomics
@@ -484,10 +496,10 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
         for (int u = 0; u < kSortUnroll; u++) {
             const int64_t i = i0 + (int64_t)u * kSortThreads;
             if (i < end) {
-                px[u] = a.x[i];
-                py[u] = a.y[i];
-                pz[u] = a.z[i];
-                pw[u] = a.w[i];
+                px[u] = src.x[i - src.i0];
+                py[u] = src.y[i - src.i0];
+                pz[u] = src.z[i - src.i0];
+                pw[u] = src.w[i - src.i0];
             }
         }
 #pragma unroll
@@ -576,6 +588,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, cons
     const uint32_t *row = H + (int64_t)blk * T;
     const uint32_t *start = a.cursor + (int64_t)cat * T;
     for (int i = threadIdx.x; i < T; i += kSortThreads) cur[i] = start[i] + row[i];
+    const SortSrc src = sort_src(a, cat);
     __syncthreads();
     for (int64_t i0 = beg + threadIdx.x; i0 < end; i0 += kSortUnroll * kSortThreads) {
         float px[kSortUnroll], py[kSortUnroll], pz[kSortUnroll], pw[kSortUnroll];
@@ -583,10 +596,10 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, cons
         for (int u = 0; u < kSortUnroll; u++) {
             const int64_t i = i0 + (int64_t)u * kSortThreads;
             if (i < end) {
-                px[u] = a.x[i];
-                py[u] = a.y[i];
-                pz[u] = a.z[i];
-                pw[u] = a.w[i];
+                px[u] = src.x[i - src.i0];
+                py[u] = src.y[i - src.i0];
+                pz[u] = src.z[i - src.i0];
+                pw[u] = src.w[i - src.i0];
             }
         }
 #pragma unroll
