@@ -1,6 +1,6 @@
 """GPU parity: every HIP entry point against the CPU oracle on seeded inputs.
 
-Tolerances (north-star): density grids within 1e-5 relative L2 (FP32);
+Tolerances: density grids within 1e-5 relative L2 (FP32);
 displacements and shifted positions within 1e-4 Mpc/h RMS.  Primitives that
 are pure FP32 arithmetic in the reference's order (scatter, gather, Jacobi,
 residual, restriction, prolongation) are held to tighter bounds.

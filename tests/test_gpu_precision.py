@@ -3,7 +3,7 @@
 Each case runs the HIP path (FP32 bulk arithmetic, FP64 global reductions)
 and the oracle twice: in the reference's FP32 operation order and in FP64.
 The HIP error against FP64 must be of the size of the FP32 algorithm's own
-error against FP64 (at most 2x it, or within the north-star tolerance), so
+error against FP64 (at most 2x it, or within the parity tolerance), so
 that the GPU adds no error beyond FP32 rounding.  Set BAOREC_PRECISION_OUT
 to a path to collect the table rows (markdown).
 """
