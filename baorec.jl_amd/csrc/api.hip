@@ -322,7 +322,7 @@ void tiles_for(baorec_ctx *c, const Grid &G, ScatterArgs &a, int64_t n) {
 // cic! into rho (overwrite unless accumulate) for one catalog.
 void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, float *rho, float *x,
              float *y, float *z, const float *w, int64_t n, int wrap, int accumulate,
-             const char *tag) {
+             const char *tag, double *sums = nullptr) {
     // placements are indexed with 32 bits; the bound is the worst case of a
     // particle touching 8 (CIC) or 27 (TSC) tiles
     const int64_t per = p->mas == BAOREC_MAS_TSC ? 27 : 8;
@@ -367,11 +367,14 @@ void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, f
         ProfScope ps(c, "scatter_bin", (double)n * (a.mutate && wrap ? 40.0 : 28.0) + 16.0 * n * 1.2);
         hipchk(c, launch_scatter(a, nullptr, c->stream), "scatter");
     }
+    // FP64 total of the grid (overwrite mode), from the tiles as they are written
+    if (sums && !accumulate) a.part = bufT<double>(c, "sc_part", (size_t)ntiles);
     {
         // overwrite mode writes every cell once (zeros in empty tiles)
         ProfScope ps(c, "scatter_accum", 16.0 * n * 1.2 + 4.0 * G.R() * (accumulate ? 2 : 1));
         hipchk(c, launch_scatter_accum(a, accumulate ? 1 : 0, c->stream), "scatter accumulate");
     }
+    if (a.part) hipchk(c, launch_sum_partials(a.part, ntiles, 1, sums, 1, c->stream), "grid total");
 }
 
 // cic! of the data and of the randoms into rho and rho + R (overwrite, no
@@ -387,8 +390,8 @@ void scatter_pair(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *sl
     tiles_for(c, G, a, nr);
     const int64_t ntot = n + nr;
     if (!sort_fits(a) || ntot * per >= ((int64_t)1 << 32) || n == 0 || nr == 0) {
-        scatter(c, G, p, slot, rho, x, y, z, w, n, 0, 0, "data");
-        scatter(c, G, p, slot, rho + G.R(), rx, ry, rz, rw, nr, 0, 0, "randoms");
+        scatter(c, G, p, slot, rho, x, y, z, w, n, 0, 0, "data", &slot->sum[0]);
+        scatter(c, G, p, slot, rho + G.R(), rx, ry, rz, rw, nr, 0, 0, "randoms", &slot->sum[1]);
         return;
     }
     a.x = x;
@@ -446,10 +449,13 @@ void scatter_pair(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *sl
         ProfScope ps(c, "scatter_bin", 28.0 * ntot + 16.0 * ntot * 1.2);
         hipchk(c, launch_sort(a, nblk, H, d32, d64 + 3, d32 + nblk, c->stream), "scatter sort");
     }
+    a.part = bufT<double>(c, "sc_part", (size_t)2 * T);
     {
         ProfScope ps(c, "scatter_accum", 16.0 * ntot * 1.2 + 4.0 * G.R() * 2);
         hipchk(c, launch_scatter_accum(a, 0, c->stream), "scatter accumulate");
     }
+    // FP64 totals of the data and randoms grids into slot->sum[0], [1]
+    hipchk(c, launch_sum_partials(a.part, T, 2, slot->sum, 1, c->stream), "grid totals");
 }
 
 // ------------------------------------------------------------------ FFT
@@ -493,7 +499,8 @@ void x_inv(baorec_ctx *c, const Grid &G, const XInvArgs &a, int epi, int nbatch,
     double fwd = a.fwd_out ? 8.0 * G.Cdense() : 0.0;
     static const char *const label[] = {"fft_x_c2r", "fft_x_c2r_od", "fft_x_c2r_od", "fft_x_c2r_iter",
                                         "fft_x_c2r_iter", "fft_x_c2r_disp"};
-    ProfScope ps(c, label[epi], (double)nbatch * (8.0 * G.Cdense() * a.nin + (out + aux) * G.R() + fwd));
+    const double in = a.rin[0] ? 4.0 * G.R() * a.nin : 8.0 * G.Cdense() * a.nin;
+    ProfScope ps(c, label[epi], (double)nbatch * (in + (out + aux) * G.R() + fwd));
     hipchk(c, launch_x_c2r(a, epi, nbatch, c->stream), what);
 }
 
@@ -503,8 +510,9 @@ StridedArgs sargs(const Grid &G, const FFTBufs &f, int axis, int mode, int mul) 
     a.ktab = f.ktab;
     a.ktabStride = 0;
     a.P = G.PP();
-    a.PZ = G.PZ();
+        a.PZ = G.PZ();
     a.Pv = G.P;
+    a.kyo = G.P;
     a.ny = G.ny;
     a.nz = G.nz;
     a.axis = axis;
@@ -561,20 +569,27 @@ XInvArgs xargs(const Grid &G, const FFTBufs &f, const Slot *slot) {
     return a;
 }
 
-// smooth! on nb contiguous real grids starting at rin, result in cw[0..nb)
-// (complex, fully transformed back along y and z; x inverse left to caller).
-void smooth_to_half(baorec_ctx *c, const Grid &G, FFTBufs &f, const float *rin, int nb, double *sums,
-                    float R) {
-    x_fwd(c, G, rin, f.cw, nb, sums, 1);
-    strided_plain(c, G, f, 1, false, f.cw, nb);
-    StridedArgs a = sargs(G, f, 2, 2, MUL_GAUSS);
-    a.R2 = R * R;
-    a.job[0].in = f.cw;
-    a.job[0].out[0] = f.cw;
-    a.job[0].nout = 1;
-    a.job[0].pw[0] = 0;
-    strided(c, G, a, 1, nb, "z smooth");
-    strided_plain(c, G, f, 1, true, f.cw, nb);
+// smooth! (src/utils.jl:45-56) of nb contiguous real grids, y and z axes,
+// in place.  The Gaussian exp(-R^2 k^2 / 2) is the product of one factor per
+// axis, so the smoothing runs one axis at a time on the real grid; here each
+// grid is read as complex lines of two adjacent x cells, whose transform
+// along y or z is that of the two real columns (the factor is real and even
+// in k).  The x factor is applied by the x pass that reads the grids
+// (XInvArgs::rin).
+void smooth_yz(baorec_ctx *c, const Grid &G, FFTBufs &f, float *rin, int nb, float R) {
+    for (int axis = 1; axis <= 2; axis++) {
+        StridedArgs a = sargs(G, f, axis, 2, MUL_GAUSS1);
+        a.P = G.nx / 2;
+        a.Pv = G.nx / 2;
+        a.PZ = (int64_t)(G.nx / 2) * G.ny;
+        a.cstride = G.R() / 2;
+        a.R2 = R * R;
+        a.job[0].in = reinterpret_cast<float2 *>(rin);
+        a.job[0].out[0] = reinterpret_cast<float2 *>(rin);
+        a.job[0].pw[0] = 0;
+        a.job[0].nout = 1;
+        strided(c, G, a, 1, nb, axis == 1 ? "y smooth" : "z smooth");
+    }
 }
 
 // setup_overdensity! (src/recon.jl:42-91) -> delta (device).
@@ -586,18 +601,21 @@ void overdensity(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
     float *rho = bufT<float>(c, "rho2", (size_t)2 * G.R());
     const bool rand = nr > 0;
     if (rand) scatter_pair(c, G, p, slot, rho, x, y, z, w, n, rx, ry, rz, rw, nr);
-    else scatter(c, G, p, slot, rho, x, y, z, w, n, 1, 0, "data");
-    smooth_to_half(c, G, f, rho, rand ? 2 : 1, slot->sum, p->smoothing_radius);
+    else scatter(c, G, p, slot, rho, x, y, z, w, n, 1, 0, "data", &slot->sum[0]);
+    smooth_yz(c, G, f, rho, rand ? 2 : 1, p->smoothing_radius);
+    // the x smoothing, the overdensity (FP64 totals from the scatter) and,
+    // for the solvers, R2C(delta) along x
     XInvArgs a = xargs(G, f, slot);
-    a.in[0] = f.cw;
+    a.rin[0] = rho;
     a.kxpow[0] = 0;
     a.nin = 1;
+    a.R2 = p->smoothing_radius * p->smoothing_radius;
     a.bias = p->bias;
     a.ran_min = p->ran_min;
     a.out = delta;
-    if (fwd_spectrum) a.fwd_out = f.cw;  // line-local in place: each line is read before it is written
+    if (fwd_spectrum) a.fwd_out = f.cw;
     if (rand) {
-        a.in[1] = f.cw + G.C();
+        a.rin[1] = rho + G.R();
         a.kxpow[1] = 0;
         a.nin = 2;
     }
@@ -1326,7 +1344,7 @@ std::vector<BatchGroup> batch_groups(const Grid &G, const baorec_params *p, int 
 }
 
 void batch_fft_group_run(baorec_ctx *c, const Grid &G, const baorec_params *p, const int64_t *off, int c0, int nb,
-                         Slot *slots, const float *rho, float *ds, float *disp, float2 *cw, FFTBufs &f, float *x,
+                         Slot *slots, float *rho, float *ds, float *disp, float2 *cw, FFTBufs &f, float *x,
                          float *y, float *z, float *ox, float *oy, float *oz, int32_t *err);
 
 // One group: run!(IterativeRecon, grid, x, y, z, w) (periodic box, no randoms)
@@ -1382,10 +1400,15 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
         ProfScope ps(c, "scatter_bin", pb);
         hipchk(c, launch_scatter(a, nullptr, c->stream), "batch scatter");
     }
+    a.part = bufT<double>(c, "bt_part", (size_t)maxnb * T);
     {
         ProfScope ps(c, "scatter_accum", 16.0 * ntot * 1.2 + 4.0 * R * nb);
         hipchk(c, launch_scatter_accum(a, 0, c->stream), "batch accumulate");
     }
+    // per-catalog FP64 totals of the density grids into slots[k].sum[0]
+    hipchk(c, launch_sum_partials(a.part, T, nb, &slots[0].sum[0], (int64_t)(sizeof(Slot) / sizeof(double)),
+                                  c->stream),
+           "grid totals");
 
     // the rest in FFT subgroups [s0, s0 + nf) of the scatter group
     const int nfmax = std::min(batch_fft_group(G), maxnb);
@@ -1404,28 +1427,19 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
 // Smoothing, overdensity, iterations, displacements and positions of nb
 // catalogs (from catalog c0) whose density grids are rho (R apart).
 void batch_fft_group_run(baorec_ctx *c, const Grid &G, const baorec_params *p, const int64_t *off, int c0, int nb,
-                         Slot *slots, const float *rho, float *ds, float *disp, float2 *cw, FFTBufs &f, float *x,
+                         Slot *slots, float *rho, float *ds, float *disp, float2 *cw, FFTBufs &f, float *x,
                          float *y, float *z, float *ox, float *oy, float *oz, int32_t *err) {
     const int64_t R = G.R(), C = G.C();
     const Work W{cw, cw + (int64_t)nb * C, cw + (int64_t)4 * nb * C, (int64_t)nb * C};
-    // smooth! and the overdensity (src/recon.jl:42-58); per-catalog FP64 sums
-    x_fwd(c, G, rho, W.C0, nb, &slots[0].sum[0], (int64_t)(sizeof(Slot) / sizeof(double)));
-    strided_plain(c, G, f, 1, false, W.C0, nb);
-    {
-        StridedArgs s = sargs(G, f, 2, 2, MUL_GAUSS);
-        s.R2 = p->smoothing_radius * p->smoothing_radius;
-        s.job[0].in = W.C0;
-        s.job[0].out[0] = W.C0;
-        s.job[0].nout = 1;
-        s.job[0].pw[0] = 0;
-        strided(c, G, s, 1, nb, "z smooth");
-    }
-    strided_plain(c, G, f, 1, true, W.C0, nb);
+    // smooth! and the overdensity (src/recon.jl:42-58); the per-catalog FP64
+    // totals came from the scatter
+    smooth_yz(c, G, f, rho, nb, p->smoothing_radius);
     {
         XInvArgs xa = xargs(G, f, slots);
-        xa.in[0] = W.C0;
+        xa.rin[0] = rho;
         xa.kxpow[0] = 0;
         xa.nin = 1;
+        xa.R2 = p->smoothing_radius * p->smoothing_radius;
         xa.bias = p->bias;
         xa.ran_min = p->ran_min;
         xa.out = ds;
@@ -1947,13 +1961,14 @@ int baorec_smooth(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], 
         if (mem == BAOREC_MEM_DEVICE) d = field;
         FFTBufs f = fft_bufs(c, G);
         tables(c, G, slot, f);
-        smooth_to_half(c, G, f, d, 1, nullptr, p->smoothing_radius);
+        smooth_yz(c, G, f, d, 1, p->smoothing_radius);
         XInvArgs a = xargs(G, f, slot);
-        a.in[0] = f.cw;
+        a.rin[0] = d;   // line-local in place: each line is read before it is written
         a.kxpow[0] = 0;
         a.nin = 1;
+        a.R2 = p->smoothing_radius * p->smoothing_radius;
         a.out = d;
-        x_inv(c, G, a, XE_PLAIN, 1, "x inverse");
+        x_inv(c, G, a, XE_PLAIN, 1, "x smooth");
         copy_out(c, mem, field, d, G.R());
         sync(c);
     });
@@ -2355,7 +2370,7 @@ int baorec_power_multipoles(baorec_ctx *c, const baorec_pk_params *pk, const int
         a.ny = G.ny;
         a.nz = G.nz;
         a.P = G.PP();
-    a.PZ = G.PZ();
+        a.PZ = G.PZ();
         a.Pv = G.P;
         for (int i = 0; i < 3; i++) {
             a.kf[i] = kf[i];

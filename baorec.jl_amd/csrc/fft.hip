@@ -79,6 +79,11 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double *part, int64
     }
 }
 
+hipError_t launch_sum_partials(const double *part, int64_t n, int nb, double *out, int64_t stride, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(nb), dim3(1024), 0, s, part, n, out, stride);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ x: R2C
 
 template <int M>
@@ -152,11 +157,55 @@ __device__ __forceinline__ void xline_load(const float2 *__restrict__ src, bool 
 }
 
 // Waves per SIMD the x C2R pass is compiled for: the local-LOS iteration
-// (three partial sums per cell) at 3 with n <= 512 (168 VGPRs, no spills);
-// the others fit 4 or more by themselves.
-template <int M, int EPI>
+// (three partial sums per cell) at 3 with n <= 512 (168 VGPRs, no spills),
+// and the passes that smooth real lines (forward and inverse transform per
+// input) at 3 (the randoms overdensity spills 4 VGPRs there, and still runs
+// 1.09 -> 0.87 ms at 512^3); the others fit 4 or more by themselves.
+// Input X'[k] = exp(-R2 kx^2 / 2) X[k] of the real line src (2M floats),
+// k = t + i*M/8, and X'[M] for thread t == 0, where X is the R2C of the line:
+// the packed forward transform and split of k_x_r2c (same arithmetic), then
+// the Gaussian's x factor with its exponent in FP64.  Uses lds (the line's
+// exchange area, barriers inside).
+template <int M, class Lay>
+__device__ __forceinline__ void xline_real(const float *__restrict__ src, bool valid, int t, float2 *lds,
+                                           const Lay &lay, const float2 *tw, const float *kxl, float R2,
+                                           float2 (&x)[8], float2 &xn) {
+    using PF = Plan<M, false>;
+    constexpr int TPL = M / 8;
+    const float2 *s2 = reinterpret_cast<const float2 *>(src);
+    float2 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = valid ? s2[PF::in_pos(t, i)] : make_float2(0.f, 0.f);
+    fft_line<M, false>(v, lds, lay, t, tw);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) lds[lay(PF::out_pos(t, i))] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int k = t + i * TPL;
+        const float2 zk = lds[lay(k)];
+        const float2 zc = cconj(lds[lay((M - k) & (M - 1))]);
+        const float2 e = cscale(cadd(zk, zc), 0.5f);
+        const float2 d = csub(zk, zc);
+        const float2 o = make_float2(0.5f * d.y, -0.5f * d.x);  // (zk - zc) / (2i)
+        const float2 X = cadd(e, cmul(tw[2 * M + k], o));
+        const float kk = kxl[k];
+        const double g = exp(-0.5 * (double)R2 * (double)(kk * kk));
+        x[i] = make_float2((float)((double)X.x * g), (float)((double)X.y * g));
+    }
+    xn = make_float2(0.f, 0.f);
+    if (t == 0) {
+        const float2 z0 = lds[lay(0)];
+        const float kk = kxl[M];
+        const double g = exp(-0.5 * (double)R2 * (double)(kk * kk));
+        xn = make_float2((float)((double)(z0.x - z0.y) * g), 0.f);
+    }
+}
+
+template <int M, int EPI, bool RIN = false>
 struct XWaves {
-    static constexpr int W = (EPI == XE_ITER_LOCAL && M <= 256) ? 3 : 1;
+    static constexpr int W = (EPI == XE_ITER_LOCAL || RIN) && M <= 256 ? 3 : 1;
 };
 
 // An SGPR zero the compiler cannot see through: table reads indexed with it
@@ -167,8 +216,8 @@ __device__ __forceinline__ int opaque_zero() {
     return z;
 }
 
-template <int M, int EPI>
-__global__ void __launch_bounds__(XShape<M>::NT) __attribute__((amdgpu_waves_per_eu(XWaves<M, EPI>::W)))
+template <int M, int EPI, bool RIN = false>
+__global__ void __launch_bounds__(XShape<M>::NT) __attribute__((amdgpu_waves_per_eu(XWaves<M, EPI, RIN>::W)))
 k_x_c2r(XInvArgs a) {
     using S = XShape<M>;
     using PI = Plan<M, true>;
@@ -228,11 +277,15 @@ k_x_c2r(XInvArgs a) {
 
     float2 cur[8], cn;
     const int64_t crow = (int64_t)iy * a.P + (int64_t)iz * a.PZ;
-    xline_load<M>(a.in[0] + b * a.cstride + crow, valid, t, cur, cn);
+    if constexpr (!RIN) xline_load<M>(a.in[0] + b * a.cstride + crow, valid, t, cur, cn);
 #pragma unroll
     for (int q = 0; q < NIN; q++) {
         float2 nxt[8], nn;
-        if (q + 1 < NIN) xline_load<M>(a.in[q + 1] + b * a.cstride + crow, valid, t, nxt, nn);
+        if constexpr (RIN) {
+            xline_real<M>(a.rin[q] + b * a.rstride + line * (2 * M), valid, t, lds, lay, twl, kxl, a.R2, cur, cn);
+        } else {
+            if (q + 1 < NIN) xline_load<M>(a.in[q + 1] + b * a.cstride + crow, valid, t, nxt, nn);
+        }
         const int pw = a.kxpow[q];
         const int z0 = opaque_zero();
         float2 v[8];
@@ -300,7 +353,7 @@ k_x_c2r(XInvArgs a) {
                 else { disp[i][0].z = r0; disp[i][1].z = r1; }
             }
         }
-        if (q + 1 < NIN) {
+        if (!RIN && q + 1 < NIN) {
 #pragma unroll
             for (int i = 0; i < 8; i++) cur[i] = nxt[i];
             cn = nn;
@@ -419,6 +472,11 @@ __device__ __forceinline__ float2 apply_mul(float2 F, float kx, float ky, float 
             return k2 > 0.f ? make_float2(-F.y / k2, F.x / k2) : make_float2(0.f, 0.f);
         case MUL_I:
             return make_float2(-F.y, F.x);
+        case MUL_GAUSS1: {
+            const float ka = a.axis == 1 ? ky : kz;
+            const double g = exp(-0.5 * (double)R2 * (double)(ka * ka));
+            return make_float2((float)((double)F.x * g), (float)((double)F.y * g));
+        }
         case MUL_MG_INV: {
             // exact inverse of the discrete operator for a fixed LOS (the
             // multigrid stencil has constant coefficients then)
@@ -462,7 +520,7 @@ k_strided(StridedArgs a) {
     }
     base += b * a.cstride;
     const float *kt = a.ktab + b * a.ktabStride;
-    const float *kax = a.axis == 1 ? kt + a.Pv : kt + a.Pv + a.ny;
+    const float *kax = a.axis == 1 ? kt + a.kyo : kt + a.kyo + a.ny;
     LayStrided lay{col, B};
     float2 v[8];
 
@@ -483,7 +541,7 @@ k_strided(StridedArgs a) {
             v[i] = valid ? J.in[base + PF::in_pos(t, i) * rs] : make_float2(0.f, 0.f);
         fft_line<M, false>(v, lds, lay, t, g_tw);
         const float kxv = valid ? kt[kxi] : 0.f;
-        const float kother = a.axis == 1 ? kt[a.Pv + a.ny + other] : kt[a.Pv + other];
+        const float kother = a.axis == 1 ? kt[a.kyo + a.ny + other] : kt[a.kyo + other];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const float kal = kax[PF::out_pos(t, i)];
@@ -549,7 +607,7 @@ k_strided2(StridedArgs a) {
     }
     base += b * a.cstride;
     const float *kt = a.ktab + b * a.ktabStride;
-    const float *kax = a.axis == 1 ? kt + a.Pv : kt + a.Pv + a.ny;
+    const float *kax = a.axis == 1 ? kt + a.kyo : kt + a.kyo + a.ny;
     const LayPair lay{2 * pc, B};
     float2 v[2][8];
     auto load = [&](const float2 *src, int pos, int i) {
@@ -578,7 +636,7 @@ k_strided2(StridedArgs a) {
         fft_line2<M, false>(v, lds, lay, t, g_tw);
         const float kx0v = valid ? kt[kx0] : 0.f;
         const float kx1v = kx0 + 1 < a.Pv ? kt[kx0 + 1] : 0.f;
-        const float kother = a.axis == 1 ? kt[a.Pv + a.ny + other] : kt[a.Pv + other];
+        const float kother = a.axis == 1 ? kt[a.kyo + a.ny + other] : kt[a.kyo + other];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             const float kal = kax[PF::out_pos(t, i)];
@@ -768,6 +826,12 @@ static hipError_t x_c2r_me(const XInvArgs &a, int nbatch, hipStream_t s) {
     using S = XShape<M>;
     const int64_t nlines = (int64_t)a.ny * a.nz;
     dim3 grid((unsigned)((nlines + S::LPW - 1) / S::LPW), nbatch);
+    if constexpr (EPI == XE_PLAIN || EPI == XE_OD_BOX || EPI == XE_OD_RAND) {
+        if (a.rin[0]) {
+            hipLaunchKernelGGL((k_x_c2r<M, EPI, true>), grid, dim3(S::NT), 0, s, a);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((k_x_c2r<M, EPI>), grid, dim3(S::NT), 0, s, a);
     return hipGetLastError();
 }

@@ -22,6 +22,8 @@ enum MulKind : int {
     MUL_I_INVK2 = 4,    // i/k^2, 0 at k = 0 (src/iterative.jl:219-223)
     MUL_I = 5,          // i (src/multigrid.jl:776-779)
     MUL_MG_INV = 6,     // 1 / symbol of the fixed-LOS multigrid operator, 0 at k = 0
+    MUL_GAUSS1 = 7,     // exp(-R^2 k_a^2 / 2) of the pass axis a only, exponent in FP64 (the
+                        // Gaussian's factor along a: smoothing one axis at a time)
 };
 
 constexpr int kMaxJobs = 3;
@@ -42,6 +44,7 @@ struct StridedArgs {
     int P;                     // complex pitch of a kx row (>= Pv, multiple of 16)
     int64_t PZ;                // complex pitch of a kz plane (P * (ny + 1))
     int Pv;                    // valid kx columns, nx/2 + 1
+    int kyo;                   // offset of ky in ktab (nx/2 + 1)
     int ny, nz;
     int axis;                  // 1: y, 2: z
     int mode;                  // 0 forward, 1 inverse, 2 forward*mul*inverse
@@ -68,6 +71,10 @@ constexpr int kMaxXIn = 6;
 
 struct XInvArgs {
     const float2 *in[kMaxXIn];
+    // real input lines instead of spectra (XE_PLAIN, XE_OD_BOX, XE_OD_RAND):
+    // input q is the R2C along x of rin[q], times exp(-R2 kx^2 / 2)
+    const float *rin[2];
+    float R2;
     int kxpow[kMaxXIn];
     int nin;
     int64_t cstride;      // complex elements between batch items
@@ -103,6 +110,8 @@ struct XFwdArgs {
 };
 // Number of workgroup partial sums per batch item of launch_x_r2c.
 int64_t x_r2c_partials(int nx, int ny, int nz);
+// out[b * stride] = sum of part[b * n .. b * n + n) in a fixed order, b < nb.
+hipError_t launch_sum_partials(const double *part, int64_t n, int nb, double *out, int64_t stride, hipStream_t s);
 
 // Local-LOS iteration, y inverse pass: from G_m = IFFT_z(k_z^m F) (m = 0,1,2)
 // it forms, per (kx, y, z), the three spectra the x pass needs
@@ -169,6 +178,7 @@ struct ScatterArgs {
     float *rho;                 // output grids, catalog c at c*rstride
     int64_t rstride;
     uint32_t *wmax;             // per catalog: max |w| as float bits (fixed-point scale)
+    double *part;               // optional: per (catalog, tile) FP64 sum of the cells written
 };
 
 hipError_t launch_scatter(ScatterArgs &a, uint32_t *scan_tmp, hipStream_t s);

@@ -292,6 +292,20 @@ __device__ __forceinline__ unsigned long long fixq(float c, double S) {
     return (unsigned long long)(long long)rint((double)c * S);
 }
 
+// FP64 total of the workgroup's values into *dst (fixed order: waves, then
+// their sums in order).  Every thread of the workgroup calls it.
+__device__ __forceinline__ void tile_sum_store(double v, double *dst) {
+    __shared__ double red[kAccumThreads / 64];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < kAccumThreads / 64; i++) s += red[i];
+        *dst = s;
+    }
+}
+
 // One workgroup per (catalog, tile).  Contributions (the reference's FP32
 // weight products) are summed exactly in 64-bit fixed point with integer LDS
 // atomics (float LDS atomics are several times slower on gfx950, and integer
@@ -317,7 +331,11 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
     // whole tile inside the grid, rows of float4: vector stores
     const bool vec = x0 + a.tx <= a.nx && y0 + a.ty <= a.ny && z0 + a.tz <= a.nz && (a.tx & 3) == 0 &&
                      (a.nx & 3) == 0 && (reinterpret_cast<uintptr_t>(rho) & 15) == 0;
+    // FP64 sum of the tile's cells (overwrite mode only): the grid's total
+    // without another pass over it
+    const bool sums = a.part != nullptr && accumulate == 0;
     if (cnt == 0) {   // no particle touches this tile
+        if (sums && threadIdx.x == 0) a.part[tid_global] = 0.0;
         if (accumulate != 0) return;
         if (vec) {
             const int tx4 = a.tx >> 2;
@@ -411,6 +429,7 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
     }
     __syncthreads();
     auto cell = [&](int c) { return (float)ldexp((double)(long long)tile[c], -e); };
+    double psum = 0.0;
     if (vec) {
         const int tx4 = a.tx >> 2;
         for (int c4 = threadIdx.x; c4 < tcells / 4; c4 += kAccumThreads) {
@@ -424,16 +443,20 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
                 v = make_float4(o.x + v.x, o.y + v.y, o.z + v.z, o.w + v.w);
             }
             *dst = v;
+            psum += ((double)v.x + (double)v.y) + ((double)v.z + (double)v.w);
         }
-        return;
+    } else {
+        for (int c = threadIdx.x; c < tcells; c += kAccumThreads) {
+            const int lx = c % a.tx, ly = (c / a.tx) % a.ty, lz = c / (a.tx * a.ty);
+            if (x0 + lx >= a.nx || y0 + ly >= a.ny || z0 + lz >= a.nz) continue;  // partial edge tile
+            const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
+            const float v = cell(c);
+            const float o = accumulate == 1 ? rho[g] + v : v;
+            rho[g] = o;
+            psum += (double)o;
+        }
     }
-    for (int c = threadIdx.x; c < tcells; c += kAccumThreads) {
-        const int lx = c % a.tx, ly = (c / a.tx) % a.ty, lz = c / (a.tx * a.ty);
-        if (x0 + lx >= a.nx || y0 + ly >= a.ny || z0 + lz >= a.nz) continue;  // partial edge tile
-        const int64_t g = (x0 + lx) + (int64_t)a.nx * ((y0 + ly) + (int64_t)a.ny * (z0 + lz));
-        const float v = cell(c);
-        rho[g] = accumulate == 1 ? rho[g] + v : v;
-    }
+    if (sums) tile_sum_store(psum, a.part + tid_global);
 }
 
 // ------------------------------------------------ blocked counting sort
