@@ -236,7 +236,10 @@ int baorec_read_cic(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[
                     int mem, float *out, const float *field, const float *x,
                     const float *y, const float *z, int64_t n);
 
-/* smooth!(field, R, box_size, plan), src/utils.jl:85-96 (in place). */
+/* smooth!(field, R, box_size, plan), src/utils.jl:85-96 (in place).  The
+ * Gaussian exp(-R^2 k^2 / 2) is applied one axis at a time, exp(-R^2 k_a^2 / 2)
+ * per axis with the exponent in FP64: the same function as the reference's
+ * single 3D factor, within FP32 rounding. */
 int baorec_smooth(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],
                   int mem, float *field);
 
