@@ -149,7 +149,7 @@ def test_setup_box_exact(ctx):
 
 # ------------------------------------------------------------------ smoothing / overdensity
 
-@pytest.mark.parametrize("shape", [(32, 32, 32), (512, 16, 32), (16, 512, 64), (256, 32, 16)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (512, 16, 32), (16, 512, 64), (256, 32, 16), (16, 32, 1024)])
 def test_smooth_matches_oracle(ctx, shape):
     """(nz, ny, nx): 512-long y and z axes take the two-columns-per-thread
     strided pass (fwd * Gaussian * inv in z, plain y)."""
@@ -171,17 +171,19 @@ def test_setup_overdensity_box(ctx):
     assert rel_l2(d, ref) < 1e-5
 
 
-def test_setup_overdensity_randoms(ctx):
+@pytest.mark.parametrize("g", [(64, 64, 64), (16, 16, 1024)])
+def test_setup_overdensity_randoms(ctx, g):
+    """(nz, ny, nx); 1024-point x lines run the largest smoothing x pass."""
     x, y, z, w = synth.lightcone_catalog(40000, r_min=300.0, r_max=600.0, mesh=64, seed=12)
     rx, ry, rz, rw = synth.lightcone_randoms(200000, r_min=300.0, r_max=600.0, seed=13)
     bs, bm = orc.setup_box(rx, ry, rz, 200.0)
     kw = dict(bias=2.2, f=0.75, smoothing_radius=15.0, box_size=list(bs), box_min=list(bm), los=None)
     rec = bh.IterativeRecon(**kw)
-    d = np.zeros((64, 64, 64), F32)
+    d = np.zeros(g, F32)
     bh.setup_overdensity_(d, rec, x, y, z, w, rx, ry, rz, rw)
-    ref = orc.setup_overdensity_randoms(np.zeros((64, 64, 64), F32), orc.IterativeRecon(**kw), x, y, z, w,
+    ref = orc.setup_overdensity_randoms(np.zeros(g, F32), orc.IterativeRecon(**kw), x, y, z, w,
                                         rx, ry, rz, rw)
-    ref64, dran, thr = orc.setup_overdensity_randoms(np.zeros((64, 64, 64), np.float64), orc.IterativeRecon(**kw),
+    ref64, dran, thr = orc.setup_overdensity_randoms(np.zeros(g, np.float64), orc.IterativeRecon(**kw),
                                                      x, y, z, w, rx, ry, rz, rw, return_aux=True)
     # cells whose smoothed randoms sit within 1e-4 of the threshold are decided
     # by summation order (the reference's atomic scatter included): excluded
