@@ -521,8 +521,7 @@ StridedArgs sargs(const Grid &G, const FFTBufs &f, int axis, int mode, int mul) 
     return a;
 }
 
-void x_fwd(baorec_ctx *c, const Grid &G, const float *in, float2 *out, int nbatch, double *sums,
-           int64_t sumStride) {
+void x_fwd(baorec_ctx *c, const Grid &G, const float *in, float2 *out, int nbatch) {
     XFwdArgs a{};
     a.in = in;
     a.out = out;
@@ -533,10 +532,6 @@ void x_fwd(baorec_ctx *c, const Grid &G, const float *in, float2 *out, int nbatc
     a.nx = G.nx;
     a.ny = G.ny;
     a.nz = G.nz;
-    a.sums = sums;
-    a.sumStride = sumStride;
-    a.partials = sums ? bufT<double>(c, "sum_partials", (size_t)nbatch * x_r2c_partials(G.nx, G.ny, G.nz))
-                      : nullptr;
     ProfScope ps(c, "fft_x_r2c", (double)nbatch * (4.0 * G.R() + 8.0 * G.Cdense()));
     hipchk(c, launch_x_r2c(a, nbatch, c->stream), "x r2c");
 }
@@ -659,7 +654,7 @@ void iterate_all(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slo
     for (int it = 1; it <= p->n_iter; it++) {
         const bool last = it == p->n_iter;
         // later iterations get R2C(delta_r) from the previous x pass (fwd_out)
-        if (it == 1 && !c0_ready) x_fwd(c, G, ds, C0, nb, nullptr, 0);
+        if (it == 1 && !c0_ready) x_fwd(c, G, ds, C0, nb);
         strided_plain(c, G, f, 1, false, C0, nb);
         if (!p->has_los) {
             StridedArgs z = sargs(G, f, 2, 2, MUL_INVK2);
@@ -743,7 +738,7 @@ void displacement_grid(baorec_ctx *c, const Grid &G, int kind, Slot *slot, const
     tables(c, G, slot, f);
     const Work W = Wp ? *Wp : work_single(f, G);
     float2 *C0 = W.C0, *Gb = W.Gb, *Hb = W.Hb;
-    if (!c0_ready) x_fwd(c, G, field, C0, nb, nullptr, 0);
+    if (!c0_ready) x_fwd(c, G, field, C0, nb);
     strided_plain(c, G, f, 1, false, C0, nb);
     StridedArgs z = sargs(G, f, 2, 2, kind == BAOREC_MULTIGRID ? MUL_I : MUL_I_INVK2);
     z.job[0].in = C0;
@@ -1062,7 +1057,7 @@ void mg_spectral_seed(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot
     const double g = s2 > 0 ? (double)p->beta / s2 : 0.0;
     FFTBufs fb = fft_bufs(c, G);
     tables(c, G, slot, fb);
-    x_fwd(c, G, f, fb.cw, 1, nullptr, 0);
+    x_fwd(c, G, f, fb.cw, 1);
     strided_plain(c, G, fb, 1, false, fb.cw, 1);
     StridedArgs z = sargs(G, fb, 2, 2, MUL_MG_INV);
     for (int a = 0; a < 3; a++) {
@@ -1981,7 +1976,7 @@ int baorec_rfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *in
         const float *d = stage_in(c, mem, in, G.R(), "in_field");
         FFTBufs f = fft_bufs(c, G);
         float2 *o = f.cw;
-        x_fwd(c, G, d, o, 1, nullptr, 0);
+        x_fwd(c, G, d, o, 1);
         strided_plain(c, G, f, 1, false, o, 1);
         strided_plain(c, G, f, 2, false, o, 1);
         for (int kz = 0; kz < G.nz; kz++)   // one plane of ny rows per copy
@@ -2338,7 +2333,7 @@ int baorec_power_multipoles(baorec_ctx *c, const baorec_pk_params *pk, const int
         }
         check_err_flag(c, slot);
         FFTBufs f = fft_bufs(c, G);
-        x_fwd(c, G, rho, f.cw, ng, nullptr, 0);
+        x_fwd(c, G, rho, f.cw, ng);
         strided_plain(c, G, f, 1, false, f.cw, ng);
         strided_plain(c, G, f, 2, false, f.cw, ng);
         // per-axis phase of half a cell and window factor

@@ -34,22 +34,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Workgroup total written to dst (one slot per workgroup; reduced later by
-// k_sum_partials in a fixed order, so the FP64 totals are deterministic).
-template <int NT>
-__device__ void block_sum_store(double v, double *dst) {
-    __shared__ double red[NT / 64];
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) red[w] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int i = 0; i < NT / 64; i++) s += red[i];
-        *dst = s;
-    }
-}
-
 // out[b] = sum of the n partials of batch item b, in a fixed order: thread t
 // sums elements t + 1024 k over 8 independent accumulators (k mod 8), then
 // the accumulators, the waves and the workgroup in order.
@@ -107,12 +91,8 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_r2c(XFwdArgs a) {
     const int b = blockIdx.y;
     const float2 *src = reinterpret_cast<const float2 *>(a.in + b * a.rstride) + line * M;
     float2 v[8];
-    double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        v[i] = valid ? src[PF::in_pos(t, i)] : make_float2(0.f, 0.f);
-        s += (double)v[i].x + (double)v[i].y;
-    }
+    for (int i = 0; i < 8; i++) v[i] = valid ? src[PF::in_pos(t, i)] : make_float2(0.f, 0.f);
     LayLine lay{li * S::LS};
     fft_line<M, false>(v, lds, lay, t, g_tw);
     __syncthreads();
@@ -136,7 +116,6 @@ __global__ void __launch_bounds__(XShape<M>::NT) k_x_r2c(XFwdArgs a) {
             dst[M] = make_float2(z0.x - z0.y, 0.f);
         }
     }
-    if (a.partials) block_sum_store<S::NT>(valid ? s : 0.0, a.partials + b * (int64_t)gridDim.x + blockIdx.x);
 }
 
 // ------------------------------------------------------------ x: C2R
@@ -797,16 +776,9 @@ static hipError_t x_r2c_m(const XFwdArgs &a, int nbatch, hipStream_t s) {
     const unsigned nwg = (unsigned)((nlines + S::LPW - 1) / S::LPW);
     dim3 grid(nwg, nbatch);
     hipLaunchKernelGGL(k_x_r2c<M>, grid, dim3(S::NT), 0, s, a);
-    if (a.partials)
-        hipLaunchKernelGGL(k_sum_partials, dim3(nbatch), dim3(1024), 0, s, a.partials, (int64_t)nwg, a.sums,
-                           a.sumStride);
     return hipGetLastError();
 }
 
-int64_t x_r2c_partials(int nx, int ny, int nz) {
-    int M = nx / 2, TPL = M / 8, LPW = TPL >= 256 ? 1 : 256 / TPL;
-    return ((int64_t)ny * nz + LPW - 1) / LPW;
-}
 
 hipError_t launch_x_r2c(const XFwdArgs &a, int nbatch, hipStream_t s) {
     switch (a.nx / 2) {

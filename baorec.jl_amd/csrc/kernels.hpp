@@ -104,12 +104,7 @@ struct XFwdArgs {
     int P;                // complex pitch of a kx row
     int64_t PZ;           // complex pitch of a kz plane
     int nx, ny, nz;
-    double *sums;         // optional FP64 total of each input (per batch item)
-    int64_t sumStride;
-    double *partials;     // workgroup partial sums (x_r2c_partials(..) per item), set with sums
 };
-// Number of workgroup partial sums per batch item of launch_x_r2c.
-int64_t x_r2c_partials(int nx, int ny, int nz);
 // out[b * stride] = sum of part[b * n .. b * n + n) in a fixed order, b < nb.
 hipError_t launch_sum_partials(const double *part, int64_t n, int nb, double *out, int64_t stride, hipStream_t s);
 
