@@ -283,6 +283,94 @@ def cpu_baseline(args, data, rand, cfg):
                         f"read-out {t_read:.1f} s"))
 
 
+def readout_bench(args):
+    """The drivers' read-outs after a 512^3 lightcone reconstruction
+    (examples/lightcone.jl:110-111): reconstructed_positions of the 20M
+    randoms with field :sum and with field :disp.  One reconstruction runs
+    first; a step is the two read-outs, from the displacement grid the first
+    read-out caches (the reference recomputes it per call,
+    src/recon.jl:271)."""
+    import baorec_hip as bh
+    from baorec_hip import _lib as L
+    from baorec_hip.devmem import DeviceArray, hip
+    args.workload = "lightcone512"
+    data, rand, cfg = make_workload(args)
+    args.workload = "readout512"
+    ctx = bh.Context(0)
+    lib = ctx.lib
+    p = L.Params()
+    lib.baorec_params_default(ctypes.byref(p), cfg["bias"], cfg["f"], cfg["R"])
+    p.box_pad = cfg["pad"]
+    p.has_los = 0
+    grid = (ctypes.c_int32 * 3)(*cfg["grid"])
+    d = [DeviceArray.from_numpy(a) for a in data]
+    r = [DeviceArray.from_numpy(a) for a in rand]
+    n, nr = len(data[0]), len(rand[0])
+    out = [DeviceArray((nr,)) for _ in range(3)]
+    vp = lambda a: ctypes.c_void_p(a.ptr)  # noqa: E731
+    ctx.check(lib.baorec_run_iterative(ctx.handle, ctypes.byref(p), grid, L.MEM_DEVICE, *(vp(a) for a in d), n,
+                                       *(vp(a) for a in r), nr))
+
+    def step():
+        for field in ("sum", "disp"):
+            ctx.check(lib.baorec_reconstructed_positions(ctx.handle, ctypes.byref(p), L.MEM_DEVICE, vp(r[0]),
+                                                         vp(r[1]), vp(r[2]), nr, None, grid, L.ITERATIVE,
+                                                         L.FIELD[field], *(vp(o) for o in out)))
+
+    h = hip()
+    for _ in range(max(args.warmup, 1)):   # the first read-out builds the displacement grid
+        step()
+    h.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    h.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    lib.baorec_profile(ctx.handle, 1)
+    step()
+    buf = ctypes.create_string_buffer(1 << 16)
+    lib.baorec_profile_report(ctx.handle, buf, len(buf))
+    lib.baorec_profile(ctx.handle, 0)
+    prof = json.loads(buf.value.decode())
+    st = prof["gather"]
+    achieved = st["bytes"] / st["count"] / (st["ms"] * 1e-3 / st["count"]) / 1e9
+    finite = all(np.isfinite(o.numpy()).all() for o in out)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import baorec_oracle as orc
+        workers = int(os.environ.get("BAOREC_ORACLE_WORKERS", os.cpu_count() or 1))
+        os.environ["BAOREC_ORACLE_WORKERS"] = str(workers)
+        nx, ny, nz = cfg["grid"]
+        m = 2_000_000
+        rec = orc.IterativeRecon(bias=cfg["bias"], f=cfg["f"], smoothing_radius=cfg["R"], los=None)
+        rec.box_size, rec.box_min = orc.setup_box(rand[0], rand[1], rand[2], cfg["pad"])
+        # any field of the grid's size: the read-out cost does not depend on it
+        rec.result_cache = np.random.default_rng(0).standard_normal((nz, ny, nx)).astype(np.float32)
+        t0 = time.perf_counter()
+        orc.reconstructed_positions(rec, rand[0][:m].copy(), rand[1][:m].copy(), rand[2][:m].copy(), kind="sum")
+        t1 = time.perf_counter()
+        cpu = {"value": round(m / (t1 - t0), 1), "unit": "particles/s", "cores": workers, "kind": "port",
+               "sample": f"FP32 numpy/C oracle: one :sum read-out of {m} randoms at {nx}^3, recomputing the "
+                         f"displacement grid as the reference does per call ({t1 - t0:.1f} s)"}
+    line = {
+        "metric": "randoms shifted/s, the drivers' two read-outs (:sum, :disp) after a 512\u00b3 lightcone recon",
+        "value": round(2 * nr * args.steps / dt, 1), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "readout512", "grid": list(cfg["grid"]), "n_randoms": nr, "fields": ["sum", "disp"],
+                   "displacement_grid": "cached from the first read-out"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_gather",
+                     "avg_launch_ms": round(st["ms"] / st["count"], 4)},
+        "cpu_baseline": cpu,
+        "kernels": {k: {"count": v["count"], "ms": round(v["ms"], 3),
+                        "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()},
+        "outputs_finite": finite,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def sky_bench(args):
     """Sky <-> Cartesian conversion of the 22M galaxies of the lightcone
     workload (2M tracers + 20M randoms): one step = sky_to_cartesian then
@@ -629,6 +717,9 @@ def main():
         return
     if args.workload == "batch64":
         batch_bench(args)
+        return
+    if args.workload == "readout512":
+        readout_bench(args)
         return
     data, rand, cfg = make_workload(args)
     g = gpu_bench(args, data, rand, cfg)
