@@ -252,14 +252,16 @@ __global__ void k_scatter_place(ScatterArgs a) {
 }
 
 // Exclusive scan of n counts into cursor (single workgroup of 1024 threads).
-__global__ void __launch_bounds__(1024) k_scan(const uint32_t *count, uint32_t *cursor, int64_t n) {
-    __shared__ uint32_t part[1024];
+__global__ void __launch_bounds__(1024) k_scan(const uint32_t *__restrict__ count, uint32_t *__restrict__ cursor,
+                                               int64_t n) {
+    __shared__ uint32_t part[16];
     const int tid = threadIdx.x;
     const int64_t per = (n + 1023) / 1024;
     const int64_t lo = tid * per, hi = min(n, lo + per);
     // each thread's range in 16-byte loads when it is aligned (per a multiple
     // of 4: the tile counts of one catalog, 4096 or 16384 tiles)
-    const bool vec = (per & 3) == 0 && (reinterpret_cast<uintptr_t>(count) & 15) == 0;
+    const bool vec = (per & 3) == 0 && (reinterpret_cast<uintptr_t>(count) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(cursor) & 15) == 0;
     uint32_t s = 0;
     int64_t i0 = lo;
     if (vec) {
@@ -269,18 +271,42 @@ __global__ void __launch_bounds__(1024) k_scan(const uint32_t *count, uint32_t *
         }
     }
     for (; i0 < hi; i0++) s += count[i0];
-    part[tid] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        uint32_t v = tid >= o ? part[tid - o] : 0u;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    // exclusive prefix of s over the workgroup: scan within each wave, then
+    // over the 16 wave totals
+    const int lane = tid & 63, wv = tid >> 6;
+    uint32_t v = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
     }
-    uint32_t run = tid ? part[tid - 1] : 0u;
-    for (int64_t i = lo; i < hi; i++) {
-        cursor[i] = run;
-        run += count[i];
+    if (lane == 63) part[wv] = v;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int k = 0; k < 16; k++) {
+            const uint32_t t = part[k];
+            part[k] = acc;
+            acc += t;
+        }
+    }
+    __syncthreads();
+    uint32_t run = part[wv] + v - s;
+    i0 = lo;
+    if (vec) {
+        for (; i0 + 3 < hi; i0 += 4) {
+            const uint4 c = *reinterpret_cast<const uint4 *>(count + i0);
+            uint4 o;
+            o.x = run;
+            o.y = o.x + c.x;
+            o.z = o.y + c.y;
+            o.w = o.z + c.z;
+            run = o.w + c.w;
+            *reinterpret_cast<uint4 *>(cursor + i0) = o;
+        }
+    }
+    for (; i0 < hi; i0++) {
+        cursor[i0] = run;
+        run += count[i0];
     }
 }
 
