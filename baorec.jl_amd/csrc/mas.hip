@@ -80,20 +80,38 @@ struct PartU {
     bool ok;
 };
 
+// Box of one catalog, read once into registers by a workgroup whose
+// particles all belong to that catalog (the sort kernels): the wrap's
+// write-back stores could alias the slot, so reading it per particle would
+// reload it after every store.
+struct BoxGeom {
+    float box_min[3], box_size[3];
+};
+__device__ __forceinline__ BoxGeom box_geom(const ScatterArgs &a, int cat) {
+    const Slot &sl = a.slot[cat * a.slotStride];
+    BoxGeom g;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        g.box_min[k] = sl.box_min[k];
+        g.box_size[k] = sl.box_size[k];
+    }
+    return g;
+}
+
 // Grid coordinates of particle i of catalog cat from its loaded position and
 // weight (wrapped in place first when apply_wrap; see load_particle).
-__device__ __forceinline__ PartU particle_at(const ScatterArgs &a, int cat, int64_t i, float x, float y, float z,
-                                             float w, bool apply_wrap, bool write_back) {
+__device__ __forceinline__ PartU particle_at(const ScatterArgs &a, const BoxGeom &sl, int cat, int64_t i, float x,
+                                             float y, float z, float w, bool apply_wrap, bool write_back) {
     PartU p;
     p.cat = cat;
-    const Slot &sl = a.slot[cat * a.slotStride];
     if (apply_wrap) {  // src/mas.jl:56-60: one-sided, box_min[1]/box_size[1] for all axes
         const float b0 = sl.box_min[0], L0 = sl.box_size[0];
         const bool wx = (x - b0) > L0, wy = (y - b0) > L0, wz = (z - b0) > L0;
         x = wx ? x - L0 : x;
         y = wy ? y - L0 : y;
         z = wz ? z - L0 : z;
-        if (write_back) {  // only the coordinates that moved are stored
+        if (write_back) {  // only the coordinates that moved are stored (single-set sorts:
+                           // the two-set sort runs without the wrap)
             if (wx) a.x[i] = x;
             if (wy) a.y[i] = y;
             if (wz) a.z[i] = z;
@@ -518,6 +536,36 @@ __device__ __forceinline__ SortSrc sort_src(const ScatterArgs &a, int cat) {
     return SortSrc{a.x, a.y, a.z, a.w, 0};
 }
 
+// f(t) for every tile t a particle's assignment touches (tiles from
+// axis_tiles_m).  CIC: the base tile, then the rare second tile along each
+// axis unrolled (no loops over the per-axis counts).
+template <bool TSC, typename F>
+__device__ __forceinline__ void each_tile(const ScatterArgs &a, const int (&tx)[3], const int (&ty)[3],
+                                          const int (&tz)[3], int cx, int cy, int cz, F &&f) {
+    if constexpr (TSC) {
+        for (int k = 0; k < cz; k++)
+            for (int j = 0; j < cy; j++)
+                for (int l = 0; l < cx; l++) f(tx[l] + a.ntx * (ty[j] + a.nty * tz[k]));
+    } else {
+        const int b = tx[0] + a.ntx * (ty[0] + a.nty * tz[0]);
+        const int dx = tx[1] - tx[0], dy = a.ntx * (ty[1] - ty[0]), dz = a.ntx * a.nty * (tz[1] - tz[0]);
+        f(b);
+        if (cx > 1) f(b + dx);
+        if (cy > 1) {
+            f(b + dy);
+            if (cx > 1) f(b + dy + dx);
+        }
+        if (cz > 1) {
+            f(b + dz);
+            if (cx > 1) f(b + dz + dx);
+            if (cy > 1) {
+                f(b + dz + dy);
+                if (cx > 1) f(b + dz + dy + dx);
+            }
+        }
+    }
+}
+
 template <bool TSC>
 __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint32_t *H,
                                                             const int *blk_cat,
@@ -536,6 +584,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
     for (int i = threadIdx.x; i < T; i += kSortThreads) hist[i] = 0u;
     float wm = 0.f;
     const SortSrc src = sort_src(a, cat);
+    const BoxGeom geo = box_geom(a, cat);
     __syncthreads();
     // kSortUnroll particles per thread per step: their loads are issued
     // together, ahead of the LDS atomics
@@ -555,7 +604,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
         for (int u = 0; u < kSortUnroll; u++) {
             const int64_t i = i0 + (int64_t)u * kSortThreads;
             if (i >= end) break;
-            PartU p = particle_at(a, cat, i, px[u], py[u], pz[u], pw[u], a.wrap != 0, a.wrap && a.mutate);
+            PartU p = particle_at(a, geo, cat, i, px[u], py[u], pz[u], pw[u], a.wrap != 0, a.wrap && a.mutate);
             bool ok = true;
             int tx[3], ty[3], tz[3];
             const int cx = axis_tiles_m<TSC>(a, p.u[0], a.nx, a.tx, a.sx, tx, ok);
@@ -566,9 +615,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(ScatterArgs a, uint3
                 continue;
             }
             wm = fmaxf(wm, fabsf(p.w));
-            for (int k = 0; k < cz; k++)
-                for (int j = 0; j < cy; j++)
-                    for (int l = 0; l < cx; l++) atomicAdd(&hist[tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
+            each_tile<TSC>(a, tx, ty, tz, cx, cy, cz, [&](int t) { atomicAdd(&hist[t], 1u); });
         }
     }
     for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o, 64));
@@ -638,6 +685,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, cons
     const uint32_t *start = a.cursor + (int64_t)cat * T;
     for (int i = threadIdx.x; i < T; i += kSortThreads) cur[i] = start[i] + row[i];
     const SortSrc src = sort_src(a, cat);
+    const BoxGeom geo = box_geom(a, cat);
     __syncthreads();
     for (int64_t i0 = beg + threadIdx.x; i0 < end; i0 += kSortUnroll * kSortThreads) {
         float px[kSortUnroll], py[kSortUnroll], pz[kSortUnroll], pw[kSortUnroll];
@@ -655,7 +703,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, cons
         for (int u = 0; u < kSortUnroll; u++) {
             const int64_t i = i0 + (int64_t)u * kSortThreads;
             if (i >= end) break;
-            PartU p = particle_at(a, cat, i, px[u], py[u], pz[u], pw[u], a.wrap && !a.mutate, false);
+            PartU p = particle_at(a, geo, cat, i, px[u], py[u], pz[u], pw[u], a.wrap && !a.mutate, false);
             bool ok = true;
             int tx[3], ty[3], tz[3];
             const int cx = axis_tiles_m<TSC>(a, p.u[0], a.nx, a.tx, a.sx, tx, ok);
@@ -663,12 +711,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_place(ScatterArgs a, cons
             const int cz = axis_tiles_m<TSC>(a, p.u[2], a.nz, a.tz, a.sz, tz, ok);
             if (!ok) continue;
             const float4 rec = make_float4(p.u[0], p.u[1], p.u[2], p.w);
-            for (int k = 0; k < cz; k++)
-                for (int j = 0; j < cy; j++)
-                    for (int l = 0; l < cx; l++) {
-                        const uint32_t slot = atomicAdd(&cur[tx[l] + a.ntx * (ty[j] + a.nty * tz[k])], 1u);
-                        a.sorted[slot] = rec;
-                    }
+            each_tile<TSC>(a, tx, ty, tz, cx, cy, cz, [&](int t) { a.sorted[atomicAdd(&cur[t], 1u)] = rec; });
         }
     }
 }
