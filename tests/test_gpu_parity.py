@@ -138,9 +138,15 @@ def test_read_cic_matches_oracle(ctx):
     assert np.max(np.abs(out - ref)) < 1e-5
 
 
-def test_setup_box_exact(ctx):
+@pytest.mark.parametrize("n", [100001, 4099, 7])
+def test_setup_box_exact(ctx, n):
+    """Extremes planted at the first and last particles and near the start
+    (where the block and thread ranges of the reduction begin and end) must
+    all be found, for lengths that are not multiples of the block size."""
     rng = np.random.default_rng(6)
-    x, y, z = (rng.standard_normal(100001).astype(F32) * 300 + 50 for _ in range(3))
+    x, y, z = (rng.standard_normal(n).astype(F32) * 300 + 50 for _ in range(3))
+    x[-1], y[0], z[-1] = 5000.0, -5000.0, -4000.0
+    x[1], z[2] = -3000.0, 3500.0
     bs, bm = bh.setup_box(x, y, z, 500.0, ctx)
     rs, rm = orc.setup_box(x, y, z, 500.0)
     np.testing.assert_array_equal(bs, rs)
