@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(kAccumThreads) k_scatter_accum(ScatterArgs a, 
     const float bound = (float)cnt * __uint_as_float(a.wmax[cat]);
     const int e = bound > 0.f ? 61 - ilogbf(bound) : 0;
     const double S = ldexp(1.0, e);
-    constexpr int U = 4;  // placements per thread and step
+    constexpr int U = 8;  // placements per thread and step
     auto load = [&](float4 (&rb)[U], uint32_t s0) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
