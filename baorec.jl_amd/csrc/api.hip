@@ -1082,6 +1082,9 @@ void mg_spectral_seed(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot
 }
 
 // Solves A phi = f on level 0.  v0 (device) receives the potential.
+// Partials of the fixed-order FP64 grid reductions (at most 1024 blocks).
+static double *red_part(baorec_ctx *c) { return bufT<double>(c, "red_part", 1024); }
+
 void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot, float *v0,
                      const float *f0) {
     if (p->mg_schedule == BAOREC_MG_REFERENCE) {
@@ -1110,8 +1113,8 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     const int64_t N = G.R();
     double *red = bufT<double>(c, "mg_red", 8);
     hipchk(c, hipMemsetAsync(red, 0, sizeof(double) * 8, c->stream), "memset");
-    if (p->has_los) hipchk(c, launch_sum<float>(f0, N, red, c->stream), "sum f");
-    hipchk(c, launch_sumsq<float>(f0, N, red + 1, c->stream), "norm f");
+    if (p->has_los) hipchk(c, launch_sum<float>(f0, N, red, red_part(c), c->stream), "sum f");
+    hipchk(c, launch_sumsq<float>(f0, N, red + 1, red_part(c), c->stream), "norm f");
     float *rb[2] = {bufT<float>(c, "mg_r0", N), bufT<float>(c, "mg_r1", N)};
     double *phi = refine ? bufT<double>(c, "mg_phi64", N) : nullptr;   // FP64 solution
     float *phi32 = refine ? nullptr : bufT<float>(c, "mg_phi32", N);      // FP32 solution
@@ -1143,7 +1146,7 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
             hipchk(c, hipMemcpyAsync(phi32, seed, sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream), "copy");
     }
     residual(rb[0]);   // of the start: f (minus its mean with a fixed LOS) for v = 0
-    hipchk(c, launch_sumsq<float>(rb[0], N, red + 5, c->stream), "norm r0");
+    hipchk(c, launch_sumsq<float>(rb[0], N, red + 5, red_part(c), c->stream), "norm r0");
     double h[8];
     hipchk(c, hipMemcpyAsync(h, red, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
     sync(c);
@@ -1166,7 +1169,7 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
         // v += e - mean(e): v stays zero-mean (A annihilates constants), so an
         // inconsistent f does not make it drift by a constant every cycle
         hipchk(c, hipMemsetAsync(red + 4, 0, sizeof(double), c->stream), "memset");
-        hipchk(c, launch_sum<float>(vcur(P, 0), N, red + 4, c->stream), "sum e");
+        hipchk(c, launch_sum<float>(vcur(P, 0), N, red + 4, red_part(c), c->stream), "sum e");
         {
             ProfScope ps(c, "mg_correct", (refine ? 20.0 : 12.0) * N);
             if (refine)
@@ -1177,8 +1180,8 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
         cyc++;
         residual(rn);
         hipchk(c, hipMemsetAsync(red + 2, 0, 2 * sizeof(double), c->stream), "memset");
-        hipchk(c, launch_sumsq<float>(rn, N, red + 2, c->stream), "norm r");
-        hipchk(c, launch_sumsq_diff<float>(rn, r, N, red + 3, c->stream), "norm dr");
+        hipchk(c, launch_sumsq<float>(rn, N, red + 2, red_part(c), c->stream), "norm r");
+        hipchk(c, launch_sumsq_diff<float>(rn, r, N, red + 3, red_part(c), c->stream), "norm dr");
         hipchk(c, hipMemcpyAsync(h + 2, red + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
         sync(c);
         rel = fn > 0 ? sqrt(h[2]) / fn : 0.0;
@@ -1715,17 +1718,21 @@ int baorec_run_multigrid(baorec_ctx *c, baorec_params *p, const int32_t grid[3],
     return run_impl(c, BAOREC_MULTIGRID, p, grid, mem, x, y, z, w, n, rx, ry, rz, rw, nr);
 }
 
-int baorec_result_info(const baorec_ctx *c, int32_t grid[3], int32_t *kind) {
+int baorec_result_info(const baorec_ctx *c, int32_t grid[3], int32_t *kind, uint64_t *generation) {
     if (!c || !grid || !kind) return BAOREC_ERR_INVALID_ARGUMENT;
     if (!c->has_result) return BAOREC_ERR_NO_RESULT;
     for (int a = 0; a < 3; a++) grid[a] = c->result_grid[a];
     *kind = c->result_kind;
+    if (generation) *generation = c->result_gen;
     return BAOREC_OK;
 }
 
-int baorec_result_copy(baorec_ctx *c, int mem, float *dst) {
+int baorec_result_copy(baorec_ctx *c, int mem, float *dst, uint64_t generation) {
     return guarded(c, [&] {
         if (!c->has_result) fail(c, BAOREC_ERR_NO_RESULT, "no cached result (run! first)");
+        if (generation != c->result_gen)
+            fail(c, BAOREC_ERR_NO_RESULT,
+                 "stale result: a later run! on this context replaced the cached result");
         if (!dst) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "dst is NULL");
         const int64_t N = (int64_t)c->result_grid[0] * c->result_grid[1] * c->result_grid[2];
         float *r = bufT<float>(c, "result", N);
@@ -2383,8 +2390,8 @@ int baorec_power_multipoles(baorec_ctx *c, const baorec_pk_params *pk, const int
         }
         // FP64 totals sum(w), sum(w^2) for the normalisation and shot noise
         hipchk(c, hipMemsetAsync(dout + 5 * nb, 0, 2 * sizeof(double), c->stream), "memset");
-        hipchk(c, launch_sum<float>(wd, n, dout + 5 * nb, c->stream), "sum w");
-        hipchk(c, launch_sumsq<float>(wd, n, dout + 5 * nb + 1, c->stream), "sum w2");
+        hipchk(c, launch_sum<float>(wd, n, dout + 5 * nb, red_part(c), c->stream), "sum w");
+        hipchk(c, launch_sumsq<float>(wd, n, dout + 5 * nb + 1, red_part(c), c->stream), "sum w2");
         std::vector<double> h((size_t)5 * nb + 2);
         hipchk(c, hipMemcpyAsync(h.data(), dout, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream), "D2H");
         sync(c);

@@ -319,13 +319,14 @@ hipError_t launch_mg_reduce_t(T *coarse, const U *fine, const int fine_n[3], hip
 template <typename T, typename U>
 hipError_t launch_mg_prolong_t(T *fine, const U *coarse, const int coarse_n[3], int add,
                                hipStream_t s);
-// sum of squares of a grid into out (FP64, atomically accumulated)
+// FP64 sum, sum of squares and sum of squared differences of a grid, written
+// to *out (overwritten) in a fixed order; part holds 1024 doubles of partials
 template <typename T>
-hipError_t launch_sumsq(const T *a, int64_t n, double *out, hipStream_t s);
+hipError_t launch_sumsq(const T *a, int64_t n, double *out, double *part, hipStream_t s);
 template <typename T>
-hipError_t launch_sumsq_diff(const T *a, const T *b, int64_t n, double *out, hipStream_t s);
+hipError_t launch_sumsq_diff(const T *a, const T *b, int64_t n, double *out, double *part, hipStream_t s);
 template <typename T>
-hipError_t launch_sum(const T *a, int64_t n, double *out, hipStream_t s);
+hipError_t launch_sum(const T *a, int64_t n, double *out, double *part, hipStream_t s);
 template <typename T>
 hipError_t launch_add_const(T *a, int64_t n, const double *num, double den, double sign,
                             hipStream_t s);

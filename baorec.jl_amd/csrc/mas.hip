@@ -9,8 +9,10 @@
 //           back), locate the 1..8 tiles its corners touch, count them;
 //   scan    exclusive prefix sum of the per-tile counts (one workgroup);
 //   place   per particle again: append (u_x, u_y, u_z, w) to each touched tile;
-//   accum   one workgroup per tile: LDS float atomics for every corner inside
-//           the tile, then one coalesced write of the tile (no HBM atomics).
+//   accum   one workgroup per tile: 64-bit fixed-point integer LDS atomics
+//           for every corner inside the tile (exact, order-independent sums),
+//           then one rounding to FP32 and one coalesced write of the tile (no
+//           HBM atomics).
 // u is the reference's FP32 grid coordinate, so the corner indices and
 // weights recomputed in accum are bit-identical to the reference's.
 #include <math.h>

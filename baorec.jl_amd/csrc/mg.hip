@@ -353,39 +353,23 @@ __global__ void k_mg_xvec(float *xv, const Slot *slot, int nx, int ny, int nz) {
     }
 }
 
-template <typename T>
-__global__ void k_sumsq(const T *a, int64_t n, double *out) {
+// FP64 reductions of a grid in a fixed order, so repeated runs agree bit for
+// bit: each block sums a fixed grid-stride share (thread, then wave, then the
+// four waves in order) into part[blockIdx.x]; launch_reduce then sums the
+// partials with k_sum_partials (fixed order as well).
+template <int OP, typename T>
+__global__ void __launch_bounds__(256) k_reduce_part(const T *a, const T *b, int64_t n, double *part) {
+    __shared__ double red[4];
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const double v = (double)a[i];
-        s += v * v;
+        const double v = OP == 2 ? (double)a[i] - (double)b[i] : (double)a[i];
+        s += OP == 0 ? v : v * v;
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
-}
-
-// sum of (a - b)^2 in FP64
-template <typename T>
-__global__ void k_sumsq_diff(const T *a, const T *b, int64_t n, double *out) {
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const double v = (double)a[i] - (double)b[i];
-        s += v * v;
-    }
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
-}
-
-template <typename T>
-__global__ void k_sum(const T *a, int64_t n, double *out) {
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        s += (double)a[i];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 // a += sign * (*num) / den  (mean removal with a device-side FP64 total)
@@ -503,20 +487,26 @@ hipError_t launch_mg_xvec(float *xv, const Slot *slot, int nx, int ny, int nz, h
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_sumsq(const T *a, int64_t n, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_sumsq<T>, dim3(red_blocks(n)), dim3(256), 0, s, a, n, out);
-    return hipGetLastError();
+template <int OP, typename T>
+static hipError_t launch_reduce(const T *a, const T *b, int64_t n, double *out, double *part,
+                                hipStream_t s) {
+    const unsigned nb = red_blocks(n);
+    hipLaunchKernelGGL((k_reduce_part<OP, T>), dim3(nb), dim3(256), 0, s, a, b, n, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_sum_partials(part, nb, 1, out, 1, s);
 }
 template <typename T>
-hipError_t launch_sumsq_diff(const T *a, const T *b, int64_t n, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_sumsq_diff<T>, dim3(red_blocks(n)), dim3(256), 0, s, a, b, n, out);
-    return hipGetLastError();
+hipError_t launch_sumsq(const T *a, int64_t n, double *out, double *part, hipStream_t s) {
+    return launch_reduce<1, T>(a, nullptr, n, out, part, s);
 }
 template <typename T>
-hipError_t launch_sum(const T *a, int64_t n, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_sum<T>, dim3(red_blocks(n)), dim3(256), 0, s, a, n, out);
-    return hipGetLastError();
+hipError_t launch_sumsq_diff(const T *a, const T *b, int64_t n, double *out, double *part, hipStream_t s) {
+    return launch_reduce<2, T>(a, b, n, out, part, s);
+}
+template <typename T>
+hipError_t launch_sum(const T *a, int64_t n, double *out, double *part, hipStream_t s) {
+    return launch_reduce<0, T>(a, nullptr, n, out, part, s);
 }
 template <typename T>
 hipError_t launch_add_const(T *a, int64_t n, const double *num, double den, double sign,
@@ -560,11 +550,11 @@ template hipError_t launch_mg_reduce_t<double, double>(double *, const double *,
 template hipError_t launch_mg_prolong_t<float, float>(float *, const float *, const int *, int, hipStream_t);
 template hipError_t launch_mg_prolong_t<float, double>(float *, const double *, const int *, int, hipStream_t);
 template hipError_t launch_mg_prolong_t<double, double>(double *, const double *, const int *, int, hipStream_t);
-template hipError_t launch_sumsq<float>(const float *, int64_t, double *, hipStream_t);
-template hipError_t launch_sumsq<double>(const double *, int64_t, double *, hipStream_t);
-template hipError_t launch_sumsq_diff<float>(const float *, const float *, int64_t, double *, hipStream_t);
-template hipError_t launch_sum<float>(const float *, int64_t, double *, hipStream_t);
-template hipError_t launch_sum<double>(const double *, int64_t, double *, hipStream_t);
+template hipError_t launch_sumsq<float>(const float *, int64_t, double *, double *, hipStream_t);
+template hipError_t launch_sumsq<double>(const double *, int64_t, double *, double *, hipStream_t);
+template hipError_t launch_sumsq_diff<float>(const float *, const float *, int64_t, double *, double *, hipStream_t);
+template hipError_t launch_sum<float>(const float *, int64_t, double *, double *, hipStream_t);
+template hipError_t launch_sum<double>(const double *, int64_t, double *, double *, hipStream_t);
 template hipError_t launch_add_const<float>(float *, int64_t, const double *, double, double, hipStream_t);
 template hipError_t launch_add_const<double>(double *, int64_t, const double *, double, double, hipStream_t);
 template hipError_t launch_convert<double, float>(double *, const float *, int64_t, hipStream_t);

@@ -153,9 +153,29 @@ function run!(recon::AbstractRecon, grid_size::Tuple{Int,Int,Int}, x, y, z, w,
         recon.box_min = collect(p.box_min)
     end
     out = _similar_grid(x, grid_size)
-    check(ctx, ccall((:baorec_result_copy, libbaorec), Cint, (Ptr{Cvoid}, Cint, Ptr{Float32}), ctx.ptr,
-                     _mem(out), _ptr(out)))
+    gen = Ref{UInt64}(0)
+    check(ctx, ccall((:baorec_result_info, libbaorec), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ref{Int32}, Ref{UInt64}),
+                     ctx.ptr, zeros(Int32, 3), Ref{Int32}(0), gen))
+    check(ctx, ccall((:baorec_result_copy, libbaorec), Cint, (Ptr{Cvoid}, Cint, Ptr{Float32}, UInt64),
+                     ctx.ptr, _mem(out), _ptr(out), gen[]))
     recon.result_cache = out
+    RESULT_GEN[objectid(recon)] = (ctx.ptr, gen[])
+    out
+end
+
+# The context keeps the result of its last run! only.  Each recon remembers the
+# generation of its own; a read-out of a recon whose result a later run! on the
+# same context replaced passes the recon's own result_cache as the field, as the
+# reference reads recon.result_cache (src/recon.jl:382-388).
+const RESULT_GEN = Dict{UInt64,Tuple{Ptr{Cvoid},UInt64}}()
+
+function _current_result(recon, ctx)
+    key = get(RESULT_GEN, objectid(recon), nothing)
+    key === nothing && return false
+    gen = Ref{UInt64}(0)
+    st = ccall((:baorec_result_info, libbaorec), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ref{Int32}, Ref{UInt64}),
+               ctx.ptr, zeros(Int32, 3), Ref{Int32}(0), gen)
+    st == 0 && key == (ctx.ptr, gen[])
 end
 
 const GRIDARGS = (Ptr{Cvoid}, Ref{Params}, Ptr{Int32}, Cint, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
@@ -185,7 +205,12 @@ function _shifts(cfun::Symbol, recon, x, y, z, field3d, field::Symbol, ctx)
     p = params(recon)
     out = (similar(x), similar(y), similar(z))
     # field3d === nothing: the context's cached result (its displacement grids
-    # are kept, so repeated read-outs of one result only gather)
+    # are kept, so repeated read-outs of one result only gather), or the
+    # recon's own result_cache when a later run! replaced the context's
+    if field3d === nothing && !_current_result(recon, ctx)
+        recon.result_cache === nothing && error("no cached result (run! first)")
+        field3d = recon.result_cache
+    end
     grid = field3d === nothing ? size(recon.result_cache) : size(field3d)
     mem = _mem(x, y, z, field3d, out...)
     st = ccall(Libdl.dlsym(Libdl.dlopen(libbaorec), cfun), Cint, SHIFTARGS, ctx.ptr, p, mem, _ptr(x),

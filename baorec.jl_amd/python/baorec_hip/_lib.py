@@ -109,8 +109,8 @@ SIGNATURES = {
     "baorec_version": (ctypes.c_char_p, []),
     "baorec_run_iterative": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),
     "baorec_run_multigrid": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),
-    "baorec_result_info": (INT, [P, I32P, I32P]),
-    "baorec_result_copy": (INT, [P, INT, FP]),
+    "baorec_result_info": (INT, [P, I32P, I32P, ctypes.POINTER(ctypes.c_uint64)]),
+    "baorec_result_copy": (INT, [P, INT, FP, ctypes.c_uint64]),
     "baorec_setup_overdensity": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),
     "baorec_reconstructed_overdensity": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),
     "baorec_reconstructed_potential": (INT, [P, PP, I32P, INT, FP, FP, FP, FP, FP, I64, FP, FP, FP, FP, I64]),

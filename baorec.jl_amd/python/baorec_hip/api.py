@@ -185,14 +185,31 @@ class MultigridRecon(_Recon):
 
 @dataclass
 class ResultCache:
-    """recon.result_cache: the field lives on the GPU inside the context."""
+    """recon.result_cache (src/recon.jl:153, 180, 234, 261).  The field lives
+    on the GPU inside the context, which keeps only the result of its last
+    run!; `generation` identifies this recon's.  `host` is this recon's own
+    host copy when run_ made one: read-outs of a result that a later run! on
+    the same context replaced use it as an explicit field, as the reference
+    reads each recon's own result_cache."""
     ctx: Context
     grid: tuple
     kind: int
+    generation: int
+    host: Optional[np.ndarray] = dfield(default=None, repr=False)
+
+    def is_current(self) -> bool:
+        g = (ctypes.c_int32 * 3)()
+        k = ctypes.c_int32()
+        gen = ctypes.c_uint64()
+        rc = self.ctx.lib.baorec_result_info(self.ctx.handle, g, ctypes.byref(k), ctypes.byref(gen))
+        return rc == L.OK and gen.value == self.generation
 
     def numpy(self) -> np.ndarray:
+        if self.host is not None:
+            return self.host.copy()
         out = np.empty(self.grid[::-1], dtype=F32)
-        self.ctx.check(self.ctx.lib.baorec_result_copy(self.ctx.handle, L.MEM_HOST, _ptr(out)))
+        self.ctx.check(self.ctx.lib.baorec_result_copy(self.ctx.handle, L.MEM_HOST, _ptr(out),
+                                                       ctypes.c_uint64(self.generation)))
         return out
 
 
@@ -297,8 +314,16 @@ def run_(recon: _Recon, grid_size, x, y, z, w, rx=None, ry=None, rz=None, rw=Non
         recon.final_residual = p.final_residual
         recon.final_change = p.final_change
         recon.seed_residual = p.seed_residual
-    recon.result_cache = ResultCache(ctx, tuple(int(v) for v in grid_size), recon.kind)
-    return recon.result_cache.numpy() if copy else recon.result_cache
+    g = (ctypes.c_int32 * 3)()
+    k = ctypes.c_int32()
+    gen = ctypes.c_uint64()
+    ctx.check(ctx.lib.baorec_result_info(ctx.handle, g, ctypes.byref(k), ctypes.byref(gen)))
+    cache = ResultCache(ctx, tuple(int(v) for v in grid_size), recon.kind, int(gen.value))
+    recon.result_cache = cache
+    if not copy:
+        return cache
+    cache.host = cache.numpy()
+    return cache.host.copy()
 
 
 def _grid_call(name, recon, delta, x, y, z, w, rx, ry, rz, rw, pass_p_mut=False):
@@ -343,6 +368,15 @@ def _shifts(name, recon, x, y, z, field3d, field):
         outs = [_dev_empty((n,), x) for _ in range(3)]
     else:
         outs = [np.empty(n, dtype=F32) for _ in range(3)]
+    if field3d is None and isinstance(recon.result_cache, ResultCache):
+        field3d = recon.result_cache
+    if isinstance(field3d, ResultCache) and not field3d.is_current():
+        # a later run! on this context replaced the cached result: read this
+        # recon's own copy, or refuse rather than read another recon's field
+        if field3d.host is None or _is_device(x):
+            raise BaorecError(L.ERR_NO_RESULT, "stale result: a later run! on this context replaced "
+                                               "the cached result of this recon")
+        field3d = field3d.host
     if field3d is None or isinstance(field3d, ResultCache):
         fptr, grid = None, (ctypes.c_int32 * 3)(0, 0, 0)
     else:

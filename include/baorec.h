@@ -162,9 +162,16 @@ int baorec_run_multigrid(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3
                          int64_t n, const float *rx, const float *ry,
                          const float *rz, const float *rw, int64_t nr);
 
-/* recon.result_cache: dimensions, kind, and a copy of the cached grid. */
-int baorec_result_info(const baorec_ctx *ctx, int32_t grid[3], int32_t *kind);
-int baorec_result_copy(baorec_ctx *ctx, int mem, float *dst);
+/* recon.result_cache (src/recon.jl:153, 180, 234, 261).  The context keeps
+ * the result of its last run!; each run! gives it a new generation number.
+ * A binding that hands out several recons on one context records the
+ * generation with each recon and passes it back: baorec_result_copy returns
+ * BAOREC_ERR_NO_RESULT ("stale result") when a later run! has replaced that
+ * result, instead of copying another recon's grid.  `generation` may be NULL
+ * in baorec_result_info. */
+int baorec_result_info(const baorec_ctx *ctx, int32_t grid[3], int32_t *kind,
+                       uint64_t *generation);
+int baorec_result_copy(baorec_ctx *ctx, int mem, float *dst, uint64_t generation);
 
 /* ------------------------------------------------ grid-level entry points */
 

@@ -16,6 +16,7 @@ loops that are not FFTs (CIC scatter/gather, multigrid) live in the C library
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -46,6 +47,23 @@ def lib():
             build()
         _LIB = ctypes.CDLL(path)
     return _LIB
+
+
+@contextlib.contextmanager
+def planted(k: int):
+    """Within the block, the C functions come from ``liboracle_plant<k>.so``:
+    the oracle with deliberate mistake k (listed in oracle_impl.h).  Tests use
+    it to show that a check fails on a plausible mistake."""
+    global _LIB
+    path = os.path.join(_HERE, "_build", f"liboracle_plant{int(k)}.so")
+    if not os.path.exists(path):
+        build()
+    saved = lib()
+    _LIB = ctypes.CDLL(path)
+    try:
+        yield
+    finally:
+        _LIB = saved
 
 
 def _p(a):

@@ -11,6 +11,27 @@
  * the same algorithm in FP64, used to state errors against FP64.
  */
 
+/* ORACLE_PLANT selects a deliberately wrong variant, built only as the
+ * separate test libraries _build/liboracle_plant<k>.so, so that the tests can
+ * show that their checks catch the mistake (0, the default, is the oracle):
+ *   1  the sign of the operator's first-derivative term sG is flipped;
+ *   2  the cell-centre position p = x/cell drops box_min;
+ *   3  prolongation feeds fine cell 2c (not 2c+1) from coarse cell c;
+ *   4  fmg starts each level from v = 0 instead of the prolonged coarse solve. */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+#if ORACLE_PLANT == 1
+#define PLANT_GSIGN -
+#else
+#define PLANT_GSIGN
+#endif
+#if ORACLE_PLANT == 2
+#define PLANT_XC(a, i) (L->xv[a][i] - (REAL)L->bmin[a])
+#else
+#define PLANT_XC(a, i) (L->xv[a][i])
+#endif
+
 #define CAT2(a, b) a##b
 #define CAT(a, b) CAT2(a, b)
 #define FN(name) CAT(name, SFX)
@@ -137,12 +158,14 @@ typedef struct {
     int n[3];
     REAL cell[3], cell2[3], icell2[3], losn[3];
     REAL *xv[3];
+    float bmin[3];
 } FN(mglevel);
 
 static void FN(mg_setup)(FN(mglevel) *L, int nx, int ny, int nz,
                          const float *box_size, const float *box_min,
                          int has_los, const float *los) {
     L->n[0] = nx; L->n[1] = ny; L->n[2] = nz;
+    for (int a = 0; a < 3; a++) L->bmin[a] = box_min[a];
     for (int a = 0; a < 3; a++) {
         if (sizeof(REAL) == sizeof(float)) {
             float c = box_size[a] / (float)L->n[a];
@@ -177,9 +200,9 @@ static void FN(mg_free)(FN(mglevel) *L) {
     int64_t ixp = WRAPP(ix, nx), ixm = WRAPM(ix, nx);                           \
     int64_t iyp = WRAPP(iy, ny), iym = WRAPM(iy, ny);                           \
     int64_t izp = WRAPP(iz, nz), izm = WRAPM(iz, nz);                           \
-    REAL px = has_los ? L->losn[0] : L->xv[0][ix] / L->cell[0];                 \
-    REAL py = has_los ? L->losn[1] : L->xv[1][iy] / L->cell[1];                 \
-    REAL pz = has_los ? L->losn[2] : L->xv[2][iz] / L->cell[2];                 \
+    REAL px = has_los ? L->losn[0] : PLANT_XC(0, ix) / L->cell[0];              \
+    REAL py = has_los ? L->losn[1] : PLANT_XC(1, iy) / L->cell[1];              \
+    REAL pz = has_los ? L->losn[2] : PLANT_XC(2, iz) / L->cell[2];              \
     REAL g = beta / (L->cell2[0] * (px * px) + L->cell2[1] * (py * py) +         \
                      L->cell2[2] * (pz * pz));                                  \
     REAL gpx2 = L->icell2[0] + g * (px * px);                                   \
@@ -195,7 +218,7 @@ static void FN(mg_free)(FN(mglevel) *L) {
                             v[IDX(ixm, iy, izp)] - v[IDX(ixp, iy, izm)]) +     \
                (py * pz) * (v[IDX(ix, iyp, izp)] + v[IDX(ix, iym, izm)] -      \
                             v[IDX(ix, iym, izp)] - v[IDX(ix, iyp, izm)]));     \
-    REAL sG = g * (px * (v[IDX(ixp, iy, iz)] - v[IDX(ixm, iy, iz)]) +           \
+    REAL sG = PLANT_GSIGN g * (px * (v[IDX(ixp, iy, iz)] - v[IDX(ixm, iy, iz)]) +           \
                    py * (v[IDX(ix, iyp, iz)] - v[IDX(ix, iym, iz)]) +           \
                    pz * (v[IDX(ix, iy, izp)] - v[IDX(ix, iy, izm)]));
 
@@ -253,6 +276,9 @@ static void FN(mg_prolong)(REAL *v1h, const REAL *v2h, int cx, int cy, int cz) {
 #define CI(a, b, c) ((a) + (int64_t)cx * ((b) + (int64_t)cy * (c)))
 #define FI(a, b, c) ((a) + fx * ((b) + fy * (c)))
                 int64_t x0 = 2 * ix + 1, y0 = 2 * iy + 1, z0 = 2 * iz + 1;
+#if ORACLE_PLANT == 3
+                x0 = 2 * ix; y0 = 2 * iy; z0 = 2 * iz;
+#endif
                 int64_t xp = x0 + 1 >= fx ? x0 + 1 - fx : x0 + 1;
                 int64_t yp = y0 + 1 >= fy ? y0 + 1 - fy : y0 + 1;
                 int64_t zp = z0 + 1 >= fz ? z0 + 1 - fz : z0 + 1;
@@ -359,6 +385,9 @@ static void FN(mg_fmg)(REAL *v, const REAL *f, int nx, int ny, int nz,
         free(f2h);
         FN(mg_prolong)(v, v2h, cx, cy, cz);
         free(v2h);
+#if ORACLE_PLANT == 4
+        for (int64_t I = 0; I < N; I++) v[I] = 0;
+#endif
     } else {
         for (int64_t I = 0; I < N; I++) v[I] = 0;
     }
@@ -418,3 +447,5 @@ void FN(oracle_mg_fmg)(REAL *v, const REAL *f, int nx, int ny, int nz,
 #undef MG_STENCIL_BODY
 #undef WRAPP
 #undef WRAPM
+#undef PLANT_GSIGN
+#undef PLANT_XC

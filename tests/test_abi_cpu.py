@@ -70,7 +70,7 @@ def test_null_context_calls_fail_cleanly():
     g = (ctypes.c_int32 * 3)(16, 16, 16)
     assert lib.baorec_run_iterative(None, ctypes.byref(p), g, 0, None, None, None, None, 0,
                                     None, None, None, None, 0) == L.ERR_INVALID_ARGUMENT
-    assert lib.baorec_result_info(None, g, ctypes.byref(ctypes.c_int32())) == L.ERR_INVALID_ARGUMENT
+    assert lib.baorec_result_info(None, g, ctypes.byref(ctypes.c_int32()), None) == L.ERR_INVALID_ARGUMENT
     assert lib.baorec_profile(None, 1) == L.ERR_INVALID_ARGUMENT
 
 

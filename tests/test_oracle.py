@@ -372,6 +372,126 @@ def test_mg_fmg_reduces_residual_local_los():
     assert r / r0 < 1e-2
 
 
+def gaussian_blob_operator(n, L, box_min, beta, centre, s):
+    """phi = exp(-|x - c|^2 / 2 s^2) at the cell centres b + (i + 1/2) L/n and
+    the continuum operator the stencil of src/multigrid.jl:213-326
+    discretises with a local LOS:  r = f - A v with f = 0 gives
+    L phi = lap phi + beta [(rhat.grad)^2 phi + (2/|x|) rhat.grad phi],
+    evaluated from the closed-form derivatives of the Gaussian."""
+    h = L / n
+    xs = [box_min[a] + (np.arange(n) + 0.5) * h for a in range(3)]
+    X, Y, Z = xs[0][None, None, :], xs[1][None, :, None], xs[2][:, None, None]
+    dx, dy, dz = X - centre[0], Y - centre[1], Z - centre[2]
+    phi = np.exp(-(dx ** 2 + dy ** 2 + dz ** 2) / (2 * s * s))
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2)
+    dr = (dx * X + dy * Y + dz * Z) / r                   # (x - c) . rhat
+    lap = ((dx ** 2 + dy ** 2 + dz ** 2) / s ** 4 - 3 / s ** 2) * phi
+    rr = (dr ** 2 / s ** 4 - 1 / s ** 2) * phi           # (rhat.grad)^2 phi
+    rg = -dr / s ** 2 * phi                               # rhat.grad phi
+    return phi, lap + beta * (rr + 2 / r * rg)
+
+
+BLOB = dict(L=600.0, box_min=(50.0, -300.0, 100.0), beta=0.4, centre=(350.0, 0.0, 400.0), s=50.0)
+
+
+def blob_operator_errors(sizes, dtype=F64):
+    errs = []
+    for n in sizes:
+        phi, Lphi = gaussian_blob_operator(n, BLOB["L"], BLOB["box_min"], BLOB["beta"], BLOB["centre"],
+                                           BLOB["s"])
+        r = orc.mg_residual(phi.astype(dtype), np.zeros(phi.shape, dtype), [BLOB["L"]] * 3, BLOB["box_min"],
+                            BLOB["beta"], los=None)
+        errs.append(np.linalg.norm(r - Lphi) / np.linalg.norm(Lphi))
+    return errs
+
+
+def test_mg_operator_local_los_second_order_against_analytic():
+    """The local-LOS operator (position-dependent p = x/cell, cross terms and
+    the first-derivative term sG, src/multigrid.jl:213-326) converges to the
+    continuum operator at second order: error ratio 4 per halving of the cell,
+    on a Gaussian blob whose tails vanish at the periodic faces."""
+    e = blob_operator_errors((32, 64, 128))
+    print("operator rel. errors", e)
+    assert 3.6 < e[0] / e[1] < 4.4 and 3.6 < e[1] / e[2] < 4.4
+    assert e[2] < 5e-3
+    # the FP32 instantiation (the GPU's comparison target) is the same operator
+    e32 = blob_operator_errors((64,), F32)[0]
+    assert abs(e32 - e[1]) < 0.02 * e[1]
+
+
+@pytest.mark.parametrize("plant", [1, 2])
+def test_mg_operator_analytic_check_catches_planted(plant):
+    """A flipped sign of sG (1) or p measured from box_min instead of the
+    observer (2) breaks the second-order convergence above."""
+    with orc.planted(plant):
+        e = blob_operator_errors((32, 64, 128))
+    print(f"plant {plant}: operator rel. errors", e)
+    assert not (3.6 < e[0] / e[1] < 4.4 and 3.6 < e[1] / e[2] < 4.4 and e[2] < 5e-3)
+
+
+def smooth_field(n, L, R, seed):
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal((n, n, n))
+    k = 2 * np.pi * np.fft.fftfreq(n, L / n)
+    k2 = k[None, None, :] ** 2 + k[None, :, None] ** 2 + k[:, None, None] ** 2
+    f = np.fft.ifftn(np.fft.fftn(f) * np.exp(-0.5 * k2 * R * R)).real
+    return f - f.mean()
+
+
+def fmg_fixed_los_errors(ncycles_list):
+    """fmg (src/multigrid.jl:722-752) against the exact inverse of the same
+    discrete operator (its Fourier symbol, fixed LOS), for a smooth f."""
+    n, L, beta, los = 64, 1000.0, 0.4, (0.0, 0.6, 0.8)
+    f = smooth_field(n, L, 40.0, 3)
+    A = mg_symbol((n,) * 3, (L,) * 3, beta, los)
+    fk = np.fft.fftn(f)
+    phi = np.fft.ifftn(np.where(A > 1e-14, fk / np.where(A > 1e-14, A, 1), 0)).real
+    out = []
+    for nc in ncycles_list:
+        v = orc.fmg(f, [L] * 3, [0.0] * 3, beta, 0.4, 5, nc, los=los)
+        out.append(np.linalg.norm(v - v.mean() - phi) / np.linalg.norm(phi))
+    return out
+
+
+def test_fmg_fixed_los_full_multigrid_accuracy():
+    """Full multigrid's defining property: one V-cycle per level, started
+    from the prolonged coarse solution, already lands near the discrete
+    solution (1e-2 here, where one V-cycle from v = 0 leaves 0.23); the
+    reference's 6 V-cycles per level reach 1e-5."""
+    e1, e6 = fmg_fixed_los_errors((1, 6))
+    print(f"fmg vs exact inverse: 1 cycle {e1:.2e}, 6 cycles {e6:.2e}")
+    assert e1 < 0.03 and e6 < 3e-5
+
+
+@pytest.mark.parametrize("plant", [3, 4])
+def test_fmg_accuracy_catches_planted(plant):
+    """A wrong prolongation index map (3) or a level started from v = 0
+    instead of the prolonged coarse solution (4) fails the FMG accuracy."""
+    with orc.planted(plant):
+        e1, e6 = fmg_fixed_los_errors((1, 6))
+    print(f"plant {plant}: fmg 1 cycle {e1:.2e}, 6 cycles {e6:.2e}")
+    assert not (e1 < 0.03 and e6 < 3e-5)
+
+
+def test_fmg_local_los_recovers_manufactured_solution():
+    """Local LOS, observer outside the box: f = A v_true (A the oracle's
+    operator, pinned against the continuum above), so f lies in A's range;
+    the reference schedule (6 V-cycles per level, 5+5 damped Jacobi) returns
+    v_true up to a constant within 1e-4, one cycle per level within 0.1."""
+    from scipy.ndimage import gaussian_filter
+    n, L, beta, bm = 64, 1000.0, 0.4, [200.0, 300.0, 100.0]
+    rng = np.random.default_rng(3)
+    vt = gaussian_filter(rng.standard_normal((n, n, n)), 3.0, mode="wrap")
+    vt -= vt.mean()
+    f = -orc.mg_residual(vt, np.zeros_like(vt), [L] * 3, bm, beta, los=None)
+    errs = []
+    for nc in (1, 6):
+        v = orc.fmg(f, [L] * 3, bm, beta, 0.4, 5, nc, los=None)
+        errs.append(np.linalg.norm(v - v.mean() - vt) / np.linalg.norm(vt))
+    print("local LOS fmg errors", errs)
+    assert errs[0] < 0.1 and errs[1] < 1e-4
+
+
 # ------------------------------------------------------------------ planted mistake
 
 def test_planted_sign_error_is_caught(monkeypatch):
