@@ -1,6 +1,7 @@
 // C ABI of libbaorec_hip: context, device buffers and the reconstruction
 // pipelines built from the kernels in fft.hip, mas.hip and mg.hip.
 #include <math.h>
+#include <cmath>
 #include <stdio.h>
 #include <string.h>
 
@@ -60,6 +61,8 @@ struct baorec_ctx {
     std::map<std::array<int, 2>, int> tune_pair;
     int force_tile[3] = {0, 0, 0};   // set while tuning
     int force_pair = -1;
+    // learned warm start: the network of baorec_warmstart_init / _load
+    baorec::UNet *net = nullptr;
     // comoving-distance table of the last cosmology used
     int cosmo_valid = 0;
     baorec_cosmology cosmo{};
@@ -1081,6 +1084,134 @@ void mg_spectral_seed(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot
     x_inv(c, G, a, XE_PLAIN, 1, "x multigrid seed");
 }
 
+// ------------------------------------------------------------ learned warm start
+
+// Direction of the spectral seed's fixed LOS (mg_spectral_seed): the given
+// LOS, or the box centre seen from the origin.
+static void seed_los(const baorec_params *p, float nc[3]) {
+    double d[3];
+    if (p->has_los) {
+        for (int a = 0; a < 3; a++) d[a] = p->los[a];
+    } else {
+        for (int a = 0; a < 3; a++) d[a] = (double)p->box_min[a] + 0.5 * (double)p->box_size[a];
+    }
+    const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    for (int a = 0; a < 3; a++) nc[a] = r > 0 ? (float)(d[a] / r) : (a == 2 ? 1.f : 0.f);
+}
+
+static MGLevel cells_of(const Grid &G, const baorec_params *p) {
+    MGLevel L{};
+    const int n[3] = {G.nx, G.ny, G.nz};
+    for (int a = 0; a < 3; a++) {
+        L.n[a] = n[a];
+        const float cell = p->box_size[a] / (float)n[a];
+        L.cell[a] = cell;
+        L.cell2[a] = cell * cell;
+        L.icell2[a] = 1.0f / L.cell2[a];
+        L.losn[a] = p->has_los ? p->los[a] / cell : 0.f;
+    }
+    return L;
+}
+
+static int ilog2i(int n) {
+    int k = 0;
+    while ((1 << k) < n) k++;
+    return k;
+}
+
+// The network's input for nb grids f (each G.R() cells, device): per item the
+// rms s of f (FP64), the spectral seed phis, and the 8 feature channels in the
+// network's input tensor.  Returns the input tensor.  The slot must hold the
+// box (its sums and error flag are left as they are).
+static uint16_t *warm_inputs(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const float *f,
+                             int nb, float *phis, double *ss) {
+    const int64_t N = G.R();
+    const MGLevel L = cells_of(G, p);
+    std::vector<double> h(nb);
+    double *tmp = bufT<double>(c, "warm_sum", 1);
+    for (int b = 0; b < nb; b++) {
+        hipchk(c, launch_sumsq<float>(f + b * N, N, tmp, bufT<double>(c, "red_part", 1024), c->stream), "rms f");
+        hipchk(c, hipMemcpyAsync(&h[b], tmp, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        mg_spectral_seed(c, G, p, slot, L, f + b * N, phis + b * N);
+    }
+    sync(c);
+    for (int b = 0; b < nb; b++) h[b] = sqrt(h[b] / (double)N);
+    hipchk(c, hipMemcpyAsync(ss, h.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c->stream), "H2D");
+    uint16_t *X = nullptr;
+    hipchk(c, unet_input(c->net, nb, ilog2i(G.nx), ilog2i(G.ny), ilog2i(G.nz), &X), "network buffers");
+    FeatArgs a{};
+    a.lx = ilog2i(G.nx);
+    a.ly = ilog2i(G.ny);
+    a.lz = ilog2i(G.nz);
+    a.batch = nb;
+    for (int k = 0; k < 3; k++) {
+        a.bmin[k] = p->box_min[k];
+        a.h[k] = L.cell[k];
+    }
+    seed_los(p, a.nc);
+    a.ih2 = L.icell2[0];
+    a.href = 100.f * L.cell[0];
+    hipchk(c, launch_unet_features(f, phis, ss, a, X, c->stream), "features");
+    return X;
+}
+
+static void need_net(baorec_ctx *c) {
+    if (!unet_ready(c->net))
+        fail(c, BAOREC_ERR_NO_RESULT, "no warm-start network (baorec_warmstart_init or baorec_warmstart_load first)");
+}
+
+// The learned seed of the multigrid solve on level 0 of P: the spectral seed
+// of f plus the network's correction.  The network runs on the pyramid level
+// whose cell is closest to the one it was trained at (restricting f there),
+// and its correction is prolonged back to level 0.
+static void learned_seed(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const MGLevel &L0,
+                         const float *f0, float *seed) {
+    need_net(c);
+    mg_spectral_seed(c, G, p, slot, L0, f0, seed);
+    // the level of the network
+    const float ht = unet_h_train(c->net);
+    int lev = 0;
+    Grid g = G;
+    std::vector<Grid> gs{G};
+    while (mg_recurse(g) && g.nx / 2 >= 16 && g.ny / 2 >= 16 && g.nz / 2 >= 16) {
+        const float hc = p->box_size[0] / (float)g.nx, hn = 2.f * hc;
+        if (!(ht > 0.f) || fabsf(logf(hn / ht)) >= fabsf(logf(hc / ht))) break;
+        g = Grid{g.nx / 2, g.ny / 2, g.nz / 2, g.nx / 4 + 1};
+        gs.push_back(g);
+        lev++;
+    }
+    const Grid Gc = gs[lev];
+    // f on that level
+    const float *fc = f0;
+    for (int l = 1; l <= lev; l++) {
+        float *dst = bufT<float>(c, "warm_f" + std::to_string(l), gs[l].R());
+        const int fn[3] = {gs[l - 1].nx, gs[l - 1].ny, gs[l - 1].nz};
+        hipchk(c, launch_mg_reduce(dst, fc, fn, c->stream), "restrict f");
+        fc = dst;
+    }
+    float *phis = bufT<float>(c, "warm_phis", Gc.R());
+    double *ss = bufT<double>(c, "warm_ss", 1);
+    warm_inputs(c, Gc, p, slot, fc, 1, phis, ss);
+    {
+        ProfScope ps(c, "unet_forward", 0.0);
+        hipchk(c, unet_forward(c->net, c->stream), "network");
+    }
+    const float hc = p->box_size[0] / (float)Gc.nx;
+    if (lev == 0) {
+        hipchk(c, launch_unet_apply_seed(unet_output(c->net), ss, hc * hc, Gc.R(), 1, seed, c->stream), "correction");
+        return;
+    }
+    float *e = bufT<float>(c, "warm_e" + std::to_string(lev), Gc.R());
+    hipchk(c, launch_unet_apply_seed(unet_output(c->net), ss, hc * hc, Gc.R(), 0, e, c->stream), "correction");
+    // prolong the correction to level 0 and add it to the spectral seed
+    for (int l = lev; l >= 1; l--) {
+        const int cn[3] = {gs[l].nx, gs[l].ny, gs[l].nz};
+        float *dst = l == 1 ? seed : bufT<float>(c, "warm_e" + std::to_string(l - 1), gs[l - 1].R());
+        hipchk(c, launch_mg_prolong(dst, e, cn, l == 1 ? 1 : 0, c->stream), "prolong correction");
+        e = dst;
+    }
+}
+
 // Solves A phi = f on level 0.  v0 (device) receives the potential.
 // Partials of the fixed-order FP64 grid reductions (at most 1024 blocks).
 static double *red_part(baorec_ctx *c) { return bufT<double>(c, "red_part", 1024); }
@@ -1098,66 +1229,67 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     if (p->mg_schedule != BAOREC_MG_RBGS) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mg_schedule");
     // RBGS in correction form: each V-cycle (FP32 fine levels, FP64 coarse
     // ones) solves A e = r from e = 0 for the residual r = f - A v of the
-    // solution so far, and v += e.
-    // The periodic operator annihilates constants; with a fixed LOS it is
-    // symmetric, so its range is exactly the zero-mean fields and f is taken
-    // minus its mean (the consistent projection).  The system is then
-    // consistent and converges to the tolerance: v is kept in FP64 and its
-    // residual computed in FP64 (iterative refinement), since in FP32 the fine
-    // residual of a converged v is rounding in A v, about 1e-4 of ||f|| at
-    // 512^3.  With a local LOS the first-derivative term makes the operator
-    // non-symmetric and the range is not orthogonal to constants: f is used
-    // as given, the residual has a floor far above FP32 rounding, and v and r
-    // stay in FP32.
-    const bool refine = p->has_los != 0;
+    // solution so far, and v += e.  v is kept in FP64 and its fine residual
+    // computed in FP64 (iterative refinement): in FP32 alone the residual of
+    // a converged v is rounding in A v, about 1e-4 of ||f|| at 512^3.
+    // The periodic operator annihilates constants, so its range is the
+    // orthogonal complement of the left null vector w (A^T w = 0): only
+    // f - alpha with alpha = (w . f) / (w . 1) is consistent.  With a fixed LOS
+    // A is symmetric, w = 1 and alpha is the mean of f.  With a local LOS the
+    // first-derivative term makes A non-symmetric and w != 1; a V-cycle leaves
+    // the residual's constant component unreduced, so after each residual its
+    // mean is moved from the residual into the shift of f (alpha += mean(r),
+    // r -= mean(r)).  alpha converges to (w . f) / (w . 1) and the residual to
+    // zero (without it, an arbitrary f stalls at a floor: 1e-2 of ||f|| on the
+    // 512^3 lightcone).  The solution is that of f minus this constant.
     const int64_t N = G.R();
+    // FP64 scalars: [0] N alpha, [1] sum f^2, [2] |r|^2, [3] |dr|^2, [4] sum e,
+    // [5] |r0|^2, [6] sum r, [7] sum f
     double *red = bufT<double>(c, "mg_red", 8);
     hipchk(c, hipMemsetAsync(red, 0, sizeof(double) * 8, c->stream), "memset");
-    if (p->has_los) hipchk(c, launch_sum<float>(f0, N, red, red_part(c), c->stream), "sum f");
+    hipchk(c, launch_sum<float>(f0, N, red + 7, red_part(c), c->stream), "sum f");
+    hipchk(c, hipMemcpyAsync(red, red + 7, sizeof(double), hipMemcpyDeviceToDevice, c->stream), "copy");
     hipchk(c, launch_sumsq<float>(f0, N, red + 1, red_part(c), c->stream), "norm f");
     float *rb[2] = {bufT<float>(c, "mg_r0", N), bufT<float>(c, "mg_r1", N)};
-    double *phi = refine ? bufT<double>(c, "mg_phi64", N) : nullptr;   // FP64 solution
-    float *phi32 = refine ? nullptr : bufT<float>(c, "mg_phi32", N);      // FP32 solution
-    if (refine)
-        hipchk(c, hipMemsetAsync(phi, 0, sizeof(double) * N, c->stream), "memset");
-    else
-        hipchk(c, launch_fill(phi32, N, 0.f, c->stream), "fill");
+    double *phi = bufT<double>(c, "mg_phi64", N);   // FP64 solution
+    hipchk(c, hipMemsetAsync(phi, 0, sizeof(double) * N, c->stream), "memset");
     MGPyr P = mg_build(c, G, p, slot, v0, rb[0], p->fp64_coarse_max);
-    const double *fsum = p->has_los ? red : nullptr;
     auto residual = [&](float *r) {
-        if (refine) {
+        {
             ProfScope ps(c, "mg_residual_fp64", 16.0 * N);
-            hipchk(c, launch_mg_residual_mixed(r, phi, f0, fsum, (double)N, P.L[0], p->beta, p->has_los,
+            hipchk(c, launch_mg_residual_mixed(r, phi, f0, red, (double)N, P.L[0], p->beta, p->has_los,
                                                c->stream), "residual");
-        } else {
-            ProfScope ps(c, "mg_residual", 12.0 * N);
-            hipchk(c, launch_mg_residual(r, phi32, f0, P.L[0], p->beta, p->has_los, c->stream), "residual");
+        }
+        if (!p->has_los) {   // alpha += mean(r), r -= mean(r)
+            ProfScope ps(c, "mg_shift", 8.0 * N);
+            hipchk(c, launch_sum<float>(r, N, red + 6, red_part(c), c->stream), "sum r");
+            hipchk(c, launch_accumulate(red, red + 6, c->stream), "shift");
+            hipchk(c, launch_add_const<float>(r, N, red + 6, (double)N, -1.0, c->stream), "r - mean");
         }
     };
     if (p->mg_warm_start) {
-        if (p->mg_warm_start != 1) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mg_warm_start");
+        if (p->mg_warm_start != 1 && p->mg_warm_start != 2)
+            fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mg_warm_start");
         if (!fft_side_ok(G.nx) || !fft_side_ok(G.ny) || !fft_side_ok(G.nz))
-            fail(c, BAOREC_ERR_UNSUPPORTED, "the spectral warm start needs grid sides that are powers of two in [16, 1024]");
+            fail(c, BAOREC_ERR_UNSUPPORTED, "the warm starts need grid sides that are powers of two in [16, 1024]");
         float *seed = bufT<float>(c, "mg_seed", N);
-        mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed);
-        if (refine)
-            hipchk(c, launch_convert<double, float>(phi, seed, N, c->stream), "seed to FP64");
+        if (p->mg_warm_start == 1)
+            mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed);
         else
-            hipchk(c, hipMemcpyAsync(phi32, seed, sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream), "copy");
+            learned_seed(c, G, p, slot, P.L[0], f0, seed);
+        hipchk(c, launch_convert<double, float>(phi, seed, N, c->stream), "seed to FP64");
     }
-    residual(rb[0]);   // of the start: f (minus its mean with a fixed LOS) for v = 0
+    residual(rb[0]);   // of the start: f minus its mean for v = 0
     hipchk(c, launch_sumsq<float>(rb[0], N, red + 5, red_part(c), c->stream), "norm r0");
     double h[8];
     hipchk(c, hipMemcpyAsync(h, red, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
     sync(c);
-    const double f2 = h[1] - (p->has_los ? h[0] * h[0] / (double)N : 0.0);
+    const double f2 = h[1] - h[7] * h[7] / (double)N;   // ||f - mean(f)||^2
     const double fn = f2 > 0 ? sqrt(f2) : 0.0;
     p->seed_residual = fn > 0 ? sqrt(h[5]) / fn : 0.0;
-    // Stop at ||f - A v|| <= tol ||f||; or once a V-cycle changes the
-    // residual by <= tol ||f||, or two V-cycles in a row reduce it by less
-    // than 10 %: with a local LOS the operator is not symmetric, its range
-    // misses a direction and an arbitrary f has a residual floor above tol
-    // (the iteration has then converged to the attainable solution).
+    // Stop at ||f - alpha - A v|| <= tol ||f - mean(f)||; or once a V-cycle
+    // changes the residual by <= tol ||f||, or two V-cycles in a row reduce
+    // it by less than 10 % (a safeguard: the consistent system converges).
     int cyc = 0, stall = 0;
     double rel = p->seed_residual, chg = rel;
     while (rel > p->tolerance && chg > p->tolerance && stall < 2 && cyc < p->max_vcycles) {
@@ -1171,11 +1303,8 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
         hipchk(c, hipMemsetAsync(red + 4, 0, sizeof(double), c->stream), "memset");
         hipchk(c, launch_sum<float>(vcur(P, 0), N, red + 4, red_part(c), c->stream), "sum e");
         {
-            ProfScope ps(c, "mg_correct", (refine ? 20.0 : 12.0) * N);
-            if (refine)
-                hipchk(c, launch_add_correction<double>(phi, vcur(P, 0), N, red + 4, c->stream), "v += e");
-            else
-                hipchk(c, launch_add_correction<float>(phi32, vcur(P, 0), N, red + 4, c->stream), "v += e");
+            ProfScope ps(c, "mg_correct", 20.0 * N);
+            hipchk(c, launch_add_correction<double>(phi, vcur(P, 0), N, red + 4, c->stream), "v += e");
         }
         cyc++;
         residual(rn);
@@ -1191,10 +1320,10 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     p->vcycles_done = cyc;
     p->final_residual = rel;
     p->final_change = chg;
-    if (refine)
-        hipchk(c, launch_convert<float, double>(v0, phi, N, c->stream), "phi to FP32");
-    else
-        hipchk(c, hipMemcpyAsync(v0, phi32, sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream), "copy");
+    hipchk(c, hipMemcpyAsync(h, red, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+    sync(c);
+    p->rhs_shift = h[0] / (double)N;
+    hipchk(c, launch_convert<float, double>(v0, phi, N, c->stream), "phi to FP32");
 }
 
 // ------------------------------------------------------------ entry helpers
@@ -1510,23 +1639,32 @@ CosmoArgs cosmo_args(const baorec_cosmology *c) {
     return a;
 }
 
+// Knots of range(z_tab_min, z_tab_max, length = z_tab_num) (src/cosmo.jl:88).
+void cosmo_knots(const baorec_cosmology *cp, double *h) {
+    const int num = cp->z_tab_num;
+    const long double a = cp->z_tab_min, b = cp->z_tab_max;
+    for (int i = 0; i < num; i++) h[i] = (double)(a + (b - a) * (long double)i / (long double)(num - 1));
+    h[num - 1] = cp->z_tab_max;
+}
+
+void check_cosmology(baorec_ctx *c, const baorec_cosmology *cp) {
+    if (!cp) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "cosmology is NULL");
+    if (cp->z_tab_num < 2 || !(cp->z_tab_max > cp->z_tab_min) || cp->z_tab_min < 0.0 || !(cp->h > 0.f))
+        fail(c, BAOREC_ERR_INVALID_ARGUMENT, "bad cosmology table (need z_tab_num >= 2, 0 <= z_tab_min < z_tab_max, h > 0)");
+}
+
 // The table for cosmology c on the device (rebuilt when the parameters change).
 CosmoTable cosmo_table(baorec_ctx *c, const baorec_cosmology *cp) {
-    if (!cp) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "cosmology is NULL");
+    check_cosmology(c, cp);
     const int num = cp->z_tab_num;
-    if (num < 2 || !(cp->z_tab_max > cp->z_tab_min) || cp->z_tab_min < 0.0 || !(cp->h > 0.f))
-        fail(c, BAOREC_ERR_INVALID_ARGUMENT, "bad cosmology table (need z_tab_num >= 2, 0 <= z_tab_min < z_tab_max, h > 0)");
     double *zk = bufT<double>(c, "cosmo_z", num);
     double *rk = bufT<double>(c, "cosmo_r", num);
     int *guide = bufT<int>(c, "cosmo_guide", num);
     CosmoTable t{zk, rk, num, cp->z_tab_min, (double)(num - 1) / (cp->z_tab_max - cp->z_tab_min), guide, num,
                  0.0, 0.0};
     if (!c->cosmo_valid || memcmp(&c->cosmo, cp, sizeof(*cp)) != 0) {
-        // knots of range(z_tab_min, z_tab_max, length = z_tab_num)
         std::vector<double> h(num);
-        const long double a = cp->z_tab_min, b = cp->z_tab_max;
-        for (int i = 0; i < num; i++) h[i] = (double)(a + (b - a) * (long double)i / (long double)(num - 1));
-        h[num - 1] = cp->z_tab_max;
+        cosmo_knots(cp, h.data());
         hipchk(c, hipMemcpyAsync(zk, h.data(), sizeof(double) * num, hipMemcpyHostToDevice, c->stream), "H2D");
         double *seg = bufT<double>(c, "cosmo_seg", num);
         hipchk(c, launch_cosmo_table(cosmo_args(cp), zk, seg, rk, num, c->stream), "cosmology table");
@@ -1662,6 +1800,7 @@ int baorec_destroy(baorec_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto &kv : c->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
+    unet_destroy(c->net);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return BAOREC_OK;
@@ -2679,6 +2818,173 @@ int baorec_run_batch(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int
         copy_out(c, mem, ox, o1, n);
         copy_out(c, mem, oy, o2, n);
         copy_out(c, mem, oz, o3, n);
+        sync(c);
+    });
+}
+
+// ------------------------------------------------------- learned warm start
+
+int64_t baorec_warmstart_nparams(void) { return unet_count_params(); }
+
+int baorec_warmstart_init(baorec_ctx *c, uint64_t seed, float h_train) {
+    return guarded(c, [&] {
+        if (!(h_train > 0.f)) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "h_train must be positive");
+        if (!c->net) c->net = unet_create();
+        if (!c->net) fail(c, BAOREC_ERR_OUT_OF_MEMORY, "network");
+        hipchk(c, unet_init(c->net, seed, c->stream), "network init");
+        unet_set_h_train(c->net, h_train);
+    });
+}
+
+namespace {
+struct NetFileHeader {
+    char magic[8];   // "BRUNET01"
+    int64_t nparams;
+    int64_t step;
+    float h_train;
+    int32_t reserved;
+};
+}  // namespace
+
+int baorec_warmstart_save(baorec_ctx *c, const char *path) {
+    return guarded(c, [&] {
+        need_net(c);
+        if (!path) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "path is NULL");
+        const int64_t n = unet_nparams(c->net);
+        std::vector<float> h((size_t)n);
+        hipchk(c, unet_get_params(c->net, h.data(), 0, c->stream), "D2H parameters");
+        NetFileHeader hd{};
+        memcpy(hd.magic, "BRUNET01", 8);
+        hd.nparams = n;
+        hd.step = unet_step(c->net);
+        hd.h_train = unet_h_train(c->net);
+        FILE *fp = fopen(path, "wb");
+        if (!fp) fail(c, BAOREC_ERR_INVALID_ARGUMENT, std::string("cannot write ") + path);
+        const bool ok = fwrite(&hd, sizeof hd, 1, fp) == 1 && fwrite(h.data(), sizeof(float), (size_t)n, fp) == (size_t)n;
+        if (fclose(fp) != 0 || !ok) fail(c, BAOREC_ERR_INVALID_ARGUMENT, std::string("write failed: ") + path);
+    });
+}
+
+int baorec_warmstart_load(baorec_ctx *c, const char *path) {
+    return guarded(c, [&] {
+        if (!path) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "path is NULL");
+        FILE *fp = fopen(path, "rb");
+        if (!fp) fail(c, BAOREC_ERR_INVALID_ARGUMENT, std::string("cannot read ") + path);
+        NetFileHeader hd{};
+        const int64_t n = unet_count_params();
+        std::vector<float> h((size_t)n);
+        const bool ok = fread(&hd, sizeof hd, 1, fp) == 1 && memcmp(hd.magic, "BRUNET01", 8) == 0 &&
+                        hd.nparams == n && fread(h.data(), sizeof(float), (size_t)n, fp) == (size_t)n;
+        fclose(fp);
+        if (!ok) fail(c, BAOREC_ERR_INVALID_ARGUMENT, std::string("not a warm-start network file of this build: ") + path);
+        for (int64_t i = 0; i < n; i++)
+            if (!std::isfinite(h[(size_t)i])) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "non-finite parameter in the file");
+        if (!c->net) c->net = unet_create();
+        if (!c->net) fail(c, BAOREC_ERR_OUT_OF_MEMORY, "network");
+        hipchk(c, unet_set_params(c->net, h.data(), c->stream), "H2D parameters");
+        unet_set_h_train(c->net, hd.h_train);
+    });
+}
+
+int baorec_warmstart_params(baorec_ctx *c, int which, float *host, int64_t n) {
+    return guarded(c, [&] {
+        need_net(c);
+        if (!host || n != unet_nparams(c->net)) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "need nparams floats");
+        if (which == 1)
+            hipchk(c, unet_set_params(c->net, host, c->stream), "H2D parameters");
+        else if (which == 0 || which == 2)
+            hipchk(c, unet_get_params(c->net, host, which == 2, c->stream), "D2H parameters");
+        else
+            fail(c, BAOREC_ERR_INVALID_ARGUMENT, "which must be 0 (get), 1 (set) or 2 (gradients)");
+    });
+}
+
+// forward, loss against t, backward (and an Adam step when lr > 0)
+static double warm_loss_backward(baorec_ctx *c, const Grid &G, int nb, const float *t, float lr) {
+    const int64_t n = (int64_t)nb * G.R();
+    {
+        ProfScope ps(c, "unet_forward", 0.0);
+        hipchk(c, unet_forward(c->net, c->stream), "network forward");
+    }
+    const int lb = unet_loss_blocks(n);
+    double *part = bufT<double>(c, "warm_loss_part", (size_t)lb);
+    double *tot = bufT<double>(c, "warm_loss", 1);
+    hipchk(c, launch_unet_loss(unet_output(c->net), t, ilog2i(G.nx), ilog2i(G.ny), ilog2i(G.nz), n,
+                               (float)(1.0 / (3.0 * (double)n)), unet_output_grad(c->net), part, c->stream), "loss");
+    hipchk(c, launch_sum_partials(part, lb, 1, tot, 1, c->stream), "loss sum");
+    {
+        ProfScope ps(c, "unet_backward", 0.0);
+        hipchk(c, unet_backward(c->net, c->stream), "network backward");
+    }
+    if (lr > 0.f) hipchk(c, unet_adam(c->net, lr, c->stream), "adam");
+    double h = 0.0;
+    hipchk(c, hipMemcpyAsync(&h, tot, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+    sync(c);
+    return h;
+}
+
+int baorec_warmstart_step(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, const float *f,
+                          const float *phi, int32_t nbatch, float lr, double *loss) {
+    return guarded(c, [&] {
+        need_net(c);
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, true);
+        if (nbatch < 1 || !f || !phi) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "need nbatch >= 1 grids f and phi");
+        if (!(lr >= 0.f)) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "lr must be >= 0");
+        const int64_t N = G.R(), n = N * nbatch;
+        const float *df = stage_in(c, mem, f, n, "warm_in_f");
+        const float *dphi = stage_in(c, mem, phi, n, "warm_in_phi");
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        float *phis = bufT<float>(c, "warm_phis_b", n);
+        double *ss = bufT<double>(c, "warm_ss_b", nbatch);
+        warm_inputs(c, G, p, slot, df, nbatch, phis, ss);
+        float *t = bufT<float>(c, "warm_t", n);
+        const float ih2 = cells_of(G, p).icell2[0];
+        hipchk(c, launch_unet_target(dphi, phis, ss, ih2, N, n, t, c->stream), "target");
+        const double l = warm_loss_backward(c, G, nbatch, t, lr);
+        if (loss) *loss = l;
+    });
+}
+
+int baorec_warmstart_eval(baorec_ctx *c, const int32_t grid[3], int mem, const uint16_t *features,
+                          const float *target, int32_t nbatch, float *out, double *loss) {
+    return guarded(c, [&] {
+        need_net(c);
+        Grid G = check_grid(c, grid, true);
+        if (nbatch < 1 || !features || !target) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "need features and target");
+        const int64_t N = G.R(), n = N * nbatch;
+        uint16_t *X = nullptr;
+        hipchk(c, unet_input(c->net, nbatch, ilog2i(G.nx), ilog2i(G.ny), ilog2i(G.nz), &X), "network buffers");
+        hipchk(c, hipMemcpyAsync(X, features, sizeof(uint16_t) * 8 * n,
+                                 mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                 c->stream), "features");
+        const float *t = stage_in(c, mem, target, n, "warm_in_t");
+        const double l = warm_loss_backward(c, G, nbatch, t, 0.f);
+        if (loss) *loss = l;
+        if (out)
+            hipchk(c, hipMemcpyAsync(out, unet_output(c->net), sizeof(float) * n,
+                                     mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                     c->stream), "output");
+        sync(c);
+    });
+}
+
+int baorec_warmstart_predict(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, const float *f,
+                             float *seed) {
+    return guarded(c, [&] {
+        need_net(c);
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, true);
+        if (!f || !seed) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "f and seed are needed");
+        const float *df = stage_in(c, mem, f, G.R(), "warm_in_f");
+        float *out = stage_out(c, mem, seed, G.R(), "warm_out");
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        learned_seed(c, G, p, slot, cells_of(G, p), df, out);
+        copy_out(c, mem, seed, out, G.R());
         sync(c);
     });
 }

@@ -327,6 +327,8 @@ template <typename T>
 hipError_t launch_sumsq_diff(const T *a, const T *b, int64_t n, double *out, double *part, hipStream_t s);
 template <typename T>
 hipError_t launch_sum(const T *a, int64_t n, double *out, double *part, hipStream_t s);
+// *a += *b (one FP64 scalar on the device)
+hipError_t launch_accumulate(double *a, const double *b, hipStream_t s);
 template <typename T>
 hipError_t launch_add_const(T *a, int64_t n, const double *num, double den, double sign,
                             hipStream_t s);
@@ -339,5 +341,80 @@ hipError_t launch_mg_residual_mixed(float *r, const double *v, const float *f, c
 // phi += e - esum/n (sum in FP64, stored in phi's type)
 template <typename T>
 hipError_t launch_add_correction(T *phi, const float *e, int64_t n, const double *esum, hipStream_t s);
+
+// ------------------------------------------------------------------ U-Net (unet.hip)
+// Tensors are BF16 (stored as uint16_t), channels last [batch][z][y][x][c];
+// grid sides are powers of two (log2 in lx, ly, lz).
+struct ConvArgs {
+    const uint16_t *in1;   // C1 channels, full resolution
+    const uint16_t *in2;   // C2 channels, half resolution when up2 (nearest upsampling on read)
+    const uint16_t *w;     // packed B operand (launch_unet_pack)
+    const float *bias;     // CO entries or NULL
+    uint16_t *out;         // CO channels
+    int lx, ly, lz;
+};
+struct WgradArgs {
+    const uint16_t *in1, *in2;   // the forward convolution's inputs
+    const uint16_t *dout;        // gradient of its output (after the ReLU mask)
+    float *part;                 // [nchunks][27][C1 + C2][CO] partial sums
+    int lx, ly, lz, batch;
+    int64_t per;                 // bricks of 16 x 4 x 4 voxels per chunk
+};
+struct FeatArgs {
+    int lx, ly, lz, batch;
+    float bmin[3], h[3];   // box_min and cell size per axis
+    float nc[3];           // fixed LOS of the spectral seed (unit vector)
+    float ih2;             // 1 / h^2
+    float href;            // reference length of the 1/|x| feature
+};
+hipError_t launch_unet_conv(const ConvArgs &a, int c1, int c2, int co, bool relu, bool up2, int batch,
+                            hipStream_t s);
+hipError_t launch_unet_wgrad(const WgradArgs &a, int c1, int c2, int co, bool up2, int nchunks, hipStream_t s);
+hipError_t launch_unet_sum_chunks(const float *part, int nch, int64_t n, float *out, hipStream_t s);
+hipError_t launch_unet_pack(const float *w, int cin, int co, int transposed, uint16_t *out, hipStream_t s);
+hipError_t launch_unet_relu_mask(uint16_t *dy, const uint16_t *y, int64_t n, hipStream_t s);
+hipError_t launch_unet_pool(const uint16_t *in, uint16_t *out, int lx, int ly, int lz, int lc, int batch,
+                            hipStream_t s);
+hipError_t launch_unet_pool_bwd(const uint16_t *dc, uint16_t *df, int lx, int ly, int lz, int lc, int batch,
+                                hipStream_t s);
+hipError_t launch_unet_split(const uint16_t *dcat, uint16_t *dskip, uint16_t *dcoarse, int c1, int c2, int lx,
+                             int ly, int lz, int batch, hipStream_t s);
+hipError_t launch_unet_head(const uint16_t *x, const float *w, const float *b, float *out, int64_t n,
+                            hipStream_t s);
+int unet_head_bwd_blocks(int64_t n);
+hipError_t launch_unet_head_bwd(const uint16_t *x, const float *w, const float *dout, uint16_t *dx, float *part,
+                                int64_t n, hipStream_t s);
+hipError_t launch_unet_features(const float *f, const float *phis, const double *ss, const FeatArgs &a, uint16_t *X,
+                                hipStream_t s);
+hipError_t launch_unet_target(const float *phi, const float *phis, const double *ss, float ih2, int64_t nv, int64_t n,
+                              float *t, hipStream_t s);
+int unet_loss_blocks(int64_t n);
+hipError_t launch_unet_loss(const float *out, const float *t, int lx, int ly, int lz, int64_t n, float scale,
+                            float *dout, double *part, hipStream_t s);
+hipError_t launch_unet_adam(float *w, float *m, float *v, const float *g, int64_t n, float lr_t, float b1, float b2,
+                            float eps, float wd, hipStream_t s);
+hipError_t launch_unet_apply_seed(const float *e, const double *ss, float h2, int64_t n, int add, float *out,
+                                  hipStream_t s);
+
+
+// The network object (unet.hip): parameters, Adam state, activations.
+struct UNet;
+UNet *unet_create();
+int64_t unet_count_params();
+void unet_destroy(UNet *n);
+int64_t unet_nparams(const UNet *n);
+float unet_h_train(const UNet *n);
+void unet_set_h_train(UNet *n, float h);
+int64_t unet_step(const UNet *n);
+bool unet_ready(const UNet *n);
+hipError_t unet_init(UNet *n, uint64_t seed, hipStream_t s);
+hipError_t unet_set_params(UNet *n, const float *host, hipStream_t s);
+hipError_t unet_get_params(const UNet *n, float *host, int grads, hipStream_t s);
+hipError_t unet_input(UNet *n, int batch, int lx, int ly, int lz, uint16_t **X);
+float *unet_output(UNet *n);
+float *unet_output_grad(UNet *n);
+hipError_t unet_forward(UNet *n, hipStream_t s);
+hipError_t unet_backward(UNet *n, hipStream_t s);
+hipError_t unet_adam(UNet *n, float lr, hipStream_t s);
 
 }  // namespace baorec

@@ -508,6 +508,14 @@ template <typename T>
 hipError_t launch_sum(const T *a, int64_t n, double *out, double *part, hipStream_t s) {
     return launch_reduce<0, T>(a, nullptr, n, out, part, s);
 }
+__global__ void k_accumulate(double *a, const double *b) {
+    if (threadIdx.x == 0) *a += *b;
+}
+hipError_t launch_accumulate(double *a, const double *b, hipStream_t s) {
+    hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, s, a, b);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_add_const(T *a, int64_t n, const double *num, double den, double sign,
                             hipStream_t s) {

@@ -54,6 +54,7 @@ mutable struct Params
     final_residual::Float64
     final_change::Float64
     seed_residual::Float64
+    rhs_shift::Float64
     Params() = new()
 end
 

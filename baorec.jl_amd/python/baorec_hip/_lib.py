@@ -59,6 +59,7 @@ class Params(ctypes.Structure):
         ("final_residual", ctypes.c_double),
         ("final_change", ctypes.c_double),
         ("seed_residual", ctypes.c_double),
+        ("rhs_shift", ctypes.c_double),
     ]
 
 

@@ -166,6 +166,7 @@ class MultigridRecon(_Recon):
     final_residual: float = float("nan")
     final_change: float = float("nan")
     seed_residual: float = float("nan")
+    rhs_shift: float = float("nan")
 
     kind = L.MULTIGRID
 
@@ -314,6 +315,7 @@ def run_(recon: _Recon, grid_size, x, y, z, w, rx=None, ry=None, rz=None, rw=Non
         recon.final_residual = p.final_residual
         recon.final_change = p.final_change
         recon.seed_residual = p.seed_residual
+        recon.rhs_shift = p.rhs_shift
     g = (ctypes.c_int32 * 3)()
     k = ctypes.c_int32()
     gen = ctypes.c_uint64()
@@ -339,6 +341,7 @@ def _grid_call(name, recon, delta, x, y, z, w, rx, ry, rz, rw, pass_p_mut=False)
         recon.final_residual = p.final_residual
         recon.final_change = p.final_change
         recon.seed_residual = p.seed_residual
+        recon.rhs_shift = p.rhs_shift
     return delta
 
 
@@ -605,6 +608,7 @@ def fmg(f, recon: MultigridRecon, v=None):
     recon.final_residual = p.final_residual
     recon.final_change = p.final_change
     recon.seed_residual = p.seed_residual
+    recon.rhs_shift = p.rhs_shift
     return v
 
 

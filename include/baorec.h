@@ -120,6 +120,7 @@ typedef struct baorec_params {
     double final_residual;    /* RBGS: last relative residual ||f - A v|| / ||f|| */
     double final_change;      /* RBGS: last V-cycle's residual change / ||f|| */
     double seed_residual;     /* RBGS: relative residual of the starting v */
+    double rhs_shift;         /* RBGS: the constant alpha subtracted from f */
 } baorec_params;
 
 /* Defaults of the Julia constructors (src/recon.jl:2-33); beta = f/bias in

@@ -26,19 +26,37 @@ def rms3(a, b):
                                   for p, q in zip(a, b)])))
 
 
-def assert_fp32_parity(gpu, ref32, ref64, tol=1e-5, what=""):
-    """Parity with the FP32 reference algorithm.  Where the FP32 algorithm is
-    itself ill-conditioned (randoms-normalised cells near the threshold, whose
-    division amplifies FFT rounding), two FP32 implementations can differ by
-    more than tol; then the HIP result must be at least as close to the FP64
-    oracle as the FP32 reference is (within 1.5x)."""
-    e32 = rel_l2(gpu, ref32)
-    e_gpu = rel_l2(gpu, ref64)
-    e_ref = rel_l2(ref32, ref64)
-    branch = "FP32 oracle" if e32 < tol else "FP64 fallback"
-    print(f"{what}: rel L2 vs FP32 oracle {e32:.2e}; vs FP64: HIP {e_gpu:.2e}, FP32 reference {e_ref:.2e}"
-          f" [{branch}]")
-    assert e32 < tol or e_gpu <= 1.5 * e_ref + 1e-6, (e32, e_gpu, e_ref)
+def randoms_mask(dran, thr, margin=3.0):
+    """Cells whose smoothed randoms are clear of the normalisation threshold
+    thr (src/recon.jl:81-89): above margin * thr, or below thr by more than
+    1e-4 (there delta is 0 in every implementation).  In cells with
+    thr < rho_r <= margin * thr the division by a small rho_r amplifies the
+    FFT rounding of the smoothed randoms, so two FP32 implementations of the
+    reference algorithm differ there by more than 1e-5 (each is that far from
+    FP64)."""
+    return (dran > margin * thr) | (dran < thr * (1 - 1e-4))
+
+
+def assert_density_parity(gpu, ref32, ref64, mask, tol=1e-5, what=""):
+    """Density parity: relative L2 against the FP32 oracle below tol on the
+    masked cells, no fallback.  The error of both against the FP64 oracle is
+    printed and checked separately (HIP no worse than 1.5x the FP32
+    reference algorithm's own error, on all cells)."""
+    e32 = rel_l2(gpu[mask], ref32[mask])
+    e_gpu, e_ref = rel_l2(gpu, ref64), rel_l2(ref32, ref64)
+    print(f"{what}: rel L2 vs FP32 oracle {e32:.2e} on {int(mask.sum())} of {mask.size} cells; "
+          f"all cells vs FP64: HIP {e_gpu:.2e}, FP32 reference {e_ref:.2e}")
+    assert e32 < tol, e32
+    assert e_gpu <= 1.5 * e_ref + 1e-7, (e_gpu, e_ref)
+
+
+def assert_fp64_precision(gpu, ref32, ref64, what=""):
+    """Precision of a result the parity bounds do not cover directly (the
+    multigrid potential of a thresholded lightcone density): its error against
+    the FP64 oracle is no worse than 1.5x the FP32 reference algorithm's."""
+    e_gpu, e_ref = rel_l2(gpu, ref64), rel_l2(ref32, ref64)
+    print(f"{what}: vs FP64: HIP {e_gpu:.2e}, FP32 reference {e_ref:.2e}; vs FP32 oracle {rel_l2(gpu, ref32):.2e}")
+    assert e_gpu <= 1.5 * e_ref + 1e-7, (e_gpu, e_ref)
 
 
 def box_kw(L=1000.0, los=(0.0, 0.0, 1.0)):
@@ -193,11 +211,7 @@ def test_setup_overdensity_randoms(ctx, g):
                                         rx, ry, rz, rw)
     ref64, dran, thr = orc.setup_overdensity_randoms(np.zeros(g, np.float64), orc.IterativeRecon(**kw),
                                                      x, y, z, w, rx, ry, rz, rw, return_aux=True)
-    # cells whose smoothed randoms sit within 1e-4 of the threshold are decided
-    # by summation order (the reference's atomic scatter included): excluded
-    keep = np.abs(dran / thr - 1) > 1e-4
-    print("threshold-ambiguous cells:", int((~keep).sum()))
-    assert_fp32_parity(d[keep], ref[keep], ref64[keep], what="setup_overdensity randoms")
+    assert_density_parity(d, ref, ref64, randoms_mask(dran, thr), what="setup_overdensity randoms")
 
 
 # ------------------------------------------------------------------ iterative recon end to end
@@ -313,7 +327,9 @@ def test_run_iterative_lightcone(ctx):
                               dtype=np.float64)
     np.testing.assert_array_equal(rec.box_size, o.box_size)
     np.testing.assert_array_equal(rec.box_min, o.box_min)
-    assert_fp32_parity(d, ref, ref64, what="iterative lightcone delta")
+    _, dran, thr = orc.setup_overdensity_randoms(np.zeros((n, n, n)), orc.IterativeRecon(**kw, box_size=o.box_size,
+                                                 box_min=o.box_min), x, y, z, w, rx, ry, rz, rw, return_aux=True)
+    assert_density_parity(d, ref, ref64, randoms_mask(dran, thr), what="iterative lightcone delta")
     got = bh.reconstructed_positions(rec, x, y, z, field="sum")
     exp = orc.reconstructed_positions(o, x, y, z, kind="sum")
     assert rms3(got, exp) < 1e-4
@@ -375,7 +391,8 @@ def test_mg_residual_local_los_against_analytic_operator(ctx, n):
     """The HIP residual with a local LOS, applied to a Gaussian blob, against
     the continuum operator lap + beta [(rhat.grad)^2 + (2/|x|) rhat.grad]
     evaluated in closed form: within 2 % of the FP64 oracle's own
-    truncation error (which falls 4x per halving of the cell), so the GPU's
+    truncation error (which falls 4x per halving of the cell) plus twice the
+    FP32 oracle's rounding of the stencil, so the GPU's
     first-derivative term, cross terms and position-dependent coefficients
     are pinned without reference to the oracle's code."""
     from test_oracle import BLOB, gaussian_blob_operator
@@ -388,8 +405,11 @@ def test_mg_residual_local_los_against_analytic_operator(ctx, n):
     e_gpu = rel_l2(r.astype(np.float64), lphi)
     r64 = orc.mg_residual(phi, np.zeros_like(phi), [BLOB["L"]] * 3, BLOB["box_min"], BLOB["beta"], los=None)
     e64 = rel_l2(r64, lphi)
-    print(f"{n}^3: HIP {e_gpu:.4e}, FP64 oracle {e64:.4e} against the continuum operator")
-    assert abs(e_gpu - e64) < 0.02 * e64
+    r32 = orc.mg_residual(v, np.zeros_like(v), [BLOB["L"]] * 3, BLOB["box_min"], F32(BLOB["beta"]), los=None)
+    rnd = rel_l2(r32.astype(np.float64), r64)   # FP32 rounding of the stencil, grows as 1/h^2
+    print(f"{n}^3: HIP {e_gpu:.4e}, FP64 oracle {e64:.4e} against the continuum operator; "
+          f"FP32 rounding {rnd:.1e}")
+    assert abs(e_gpu - e64) < 0.02 * e64 + 2 * rnd
 
 
 @pytest.mark.parametrize("los", [None, (0.0, 0.0, 1.0)])
@@ -435,7 +455,7 @@ def test_run_multigrid_lightcone(ctx):
     ref = orc.run_multigrid(o, (n, n, n), x, y, z, w, rx, ry, rz, rw, box_pad=200.0)
     ref64 = orc.run_multigrid(orc.MultigridRecon(**kw), (n, n, n), x, y, z, w, rx, ry, rz, rw, box_pad=200.0,
                               dtype=np.float64)
-    assert_fp32_parity(phi, ref, ref64, what="multigrid lightcone phi")
+    assert_fp64_precision(phi, ref, ref64, what="multigrid lightcone phi")
     got = bh.reconstructed_positions(rec, x, y, z, field="sum")
     exp = orc.reconstructed_positions(o, x, y, z, kind="sum")
     assert rms3(got, exp) < 1e-4
@@ -616,45 +636,33 @@ def test_mg_rbgs_local_los_manufactured_solution(ctx, n, box_min):
     assert err < 3e-5
 
 
-def test_mg_rbgs_local_los_stops_at_attainable_residual(ctx):
-    """Random f with a local LOS has a residual floor (the non-symmetric
-    operator's range misses a direction): the solve stops by stagnation
-    instead of running to max_vcycles, and the floor it reports is the
-    one the reference's own damped-Jacobi V-cycles reach (FP64 oracle) within
-    a factor 2 (the two iterations' fixed points differ in the unreachable
-    direction)."""
+@pytest.mark.parametrize("box_min", [(200.0, 300.0, 100.0), (-500.0, -500.0, -500.0)])
+def test_mg_rbgs_local_los_random_f_reaches_tolerance(ctx, box_min):
+    """Random f with a local LOS is not in the range of the non-symmetric
+    operator; the RBGS solve subtracts the constant alpha that makes it
+    consistent and reaches 1e-6.  alpha must be (w . f) / (w . 1), w the left
+    null vector of the oracle's operator (sparse LU), and v must solve the
+    FP64 oracle's system for f - alpha to FP32 rounding."""
+    from test_oracle import left_null_vector, operator_matrix
     n = 32
-    kw = mg_rec(None)
+    kw = dict(bias=2.0, f=0.8, smoothing_radius=15.0, box_size=[1000.0] * 3, box_min=list(box_min), los=None)
+    o = orc.MultigridRecon(**kw)
     rng = np.random.default_rng(92)
     f = rng.standard_normal((n, n, n)).astype(F32)
     rec = bh.MultigridRecon(**kw, schedule="rbgs", tolerance=1e-6)
-    bh.fmg(f, rec)
-    assert rec.vcycles_done < 60 and rec.final_residual > rec.tolerance
-    o = orc.MultigridRecon(**kw)
+    v = bh.fmg(f, rec)
+    M = operator_matrix(n, o.box_size[0], list(o.box_min), float(o.beta))
+    w = left_null_vector(M)
     f64 = f.astype(np.float64)
-    v = np.zeros_like(f64)
-    for _ in range(40):
-        orc.mg_vcycle(v, f64, o.box_size, o.box_min, o.beta, 0.4, 5, los=None)
-    floor = np.linalg.norm(orc.mg_residual(v, f64, o.box_size, o.box_min, o.beta, los=None)) / np.linalg.norm(f64)
-    print(f"rbgs floor {rec.final_residual:.2e} after {rec.vcycles_done} V-cycles; Jacobi V-cycles {floor:.2e}")
-    assert 1e-5 < floor and 0.5 * floor < rec.final_residual < 2.0 * floor
-
-
-@pytest.mark.parametrize("los", [None, (0.0, 0.0, 1.0)])
-def test_mg_rbgs_is_deterministic(ctx, los):
-    """The RBGS stop test and mean removal read FP64 norms and sums that are
-    reduced in a fixed order: two runs give the same V-cycle count, residual
-    and potential bit for bit."""
-    n = 64
-    (x, y, z, w), (rx, ry, rz, rw) = lightcone_case(71, 72)
-    kw = dict(bias=2.2, f=0.75, smoothing_radius=15.0, los=None if los is None else list(los))
-    out = []
-    for _ in range(2):
-        rec = bh.MultigridRecon(**kw, box_pad=200.0, schedule="rbgs", tolerance=1e-6)
-        phi = bh.run_(rec, (n, n, n), x, y, z, w, rx, ry, rz, rw)
-        out.append((phi, rec.vcycles_done, rec.final_residual))
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    assert out[0][1:] == out[1][1:]
+    alpha = float(w @ f64.ravel()) / float(w.sum())
+    res = np.linalg.norm(M @ v.astype(np.float64).ravel() - (f64.ravel() - alpha)) / np.linalg.norm(f64 - f64.mean())
+    floor = np.linalg.norm(M @ v.astype(np.float64).ravel() - (f64.ravel() - f64.mean()))
+    print(f"rbgs local LOS: {rec.vcycles_done} V-cycles to {rec.final_residual:.2e}; shift {rec.rhs_shift:.6e} "
+          f"vs (w.f)/(w.1) {alpha:.6e} (mean f {f64.mean():.3e}); FP64 residual of f - alpha {res:.2e}")
+    assert rec.final_residual <= 1e-6 and rec.vcycles_done < 40
+    assert abs(rec.rhs_shift - alpha) < 1e-6 * f64.std()
+    assert res < 1e-5
+    assert floor > 10 * res   # the plain mean would not have been consistent
 
 
 # ------------------------------------------------------------------ edge cases

@@ -492,6 +492,90 @@ def test_fmg_local_los_recovers_manufactured_solution():
     assert errs[0] < 0.1 and errs[1] < 1e-4
 
 
+def operator_matrix(n, L, box_min, beta, los=None):
+    """The oracle's multigrid operator M (r = f - M v, src/multigrid.jl:213-326)
+    on an n^3 periodic grid as a sparse matrix, read off by applying it to
+    the 64 indicator fields of the cells (i mod 4, j mod 4, k mod 4): the
+    19-point stencil reaches one cell of each such class per row."""
+    import scipy.sparse as sp
+    N = n ** 3
+    idx = np.arange(N).reshape(n, n, n)
+    Z, Y, X = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    rows, cols, vals = [], [], []
+    for cz in range(4):
+        for cy in range(4):
+            for cx in range(4):
+                v = np.zeros((n, n, n))
+                v[cz::4, cy::4, cx::4] = 1.0
+                Mv = -orc.mg_residual(v, np.zeros_like(v), [L] * 3, box_min, beta, los=los)
+                for dz in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dx in (-1, 0, 1):
+                            zz, yy, xx = (Z + dz) % n, (Y + dy) % n, (X + dx) % n
+                            m = (zz % 4 == cz) & (yy % 4 == cy) & (xx % 4 == cx)
+                            rows.append(idx[m])
+                            cols.append(idx[zz[m], yy[m], xx[m]])
+                            vals.append(Mv[m])
+    M = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(N, N))
+    M.eliminate_zeros()
+    return M
+
+
+def left_null_vector(M):
+    """w with M^T w = 0, mean 1 (one equation replaced by w_0 = 1; sparse LU)."""
+    import scipy.sparse.linalg as spl
+    N = M.shape[0]
+    MT = M.T.tolil()
+    MT[0, :] = 0
+    MT[0, 0] = 1.0
+    b = np.zeros(N)
+    b[0] = 1.0
+    w = spl.spsolve(MT.tocsc(), b)
+    return w / w.mean()
+
+
+def test_local_los_operator_left_null_vector():
+    """With a local LOS the operator is not symmetric: it annihilates constants
+    (zero row sums), but its left null vector w is not constant, so f is in
+    its range only when w . f = 0.  The reference's own V-cycles on an
+    arbitrary zero-mean f stall at a residual floor; on f - (w.f)/(w.1) they
+    converge to rounding.  Moving the residual's mean into the shift of f
+    after every V-cycle reaches that same shift (w.f)/(w.1) without w."""
+    n, L, bm, beta = 16, 1000.0, [200.0, 300.0, 100.0], 0.375
+    M = operator_matrix(n, L, bm, beta)
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal((n, n, n))
+    np.testing.assert_allclose(M @ v.ravel(), -orc.mg_residual(v, np.zeros_like(v), [L] * 3, bm, beta).ravel(),
+                               atol=1e-12)
+    assert np.abs(M @ np.ones(n ** 3)).max() < 1e-12
+    w = left_null_vector(M)
+    assert np.abs(M.T @ w).max() < 1e-12 and np.sqrt(np.mean((w - 1) ** 2)) > 1e-2
+    f = rng.standard_normal((n, n, n))
+    f -= f.mean()
+    alpha = float(w @ f.ravel()) / float(w.sum())
+
+    def cycles(ff, shift_each_cycle):
+        ff = ff.copy()
+        vv = np.zeros_like(ff)
+        shift, res = 0.0, []
+        for _ in range(24):
+            orc.mg_vcycle(vv, ff, [L] * 3, bm, beta, 0.4, 5, los=None)
+            r = orc.mg_residual(vv, ff, [L] * 3, bm, beta, los=None)
+            res.append(np.linalg.norm(r) / np.linalg.norm(f))
+            if shift_each_cycle:
+                ff -= r.mean()
+                shift += r.mean()
+        return res, shift
+
+    stalled, _ = cycles(f, False)
+    projected, _ = cycles(f - alpha, False)
+    shifted, shift = cycles(f, True)
+    print(f"alpha {alpha:.3e}: floor {stalled[-1]:.1e}, projected {projected[-1]:.1e}, "
+          f"shifted {shifted[-1]:.1e} (shift {shift:.6e})")
+    assert stalled[-1] > 1e-5 and projected[-1] < 1e-12 and shifted[-1] < 1e-12
+    assert abs(shift - alpha) < 1e-12
+
+
 # ------------------------------------------------------------------ planted mistake
 
 def test_planted_sign_error_is_caught(monkeypatch):
