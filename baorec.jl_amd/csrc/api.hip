@@ -7,6 +7,7 @@
 
 #include <array>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1724,6 +1725,55 @@ double time_min(baorec_ctx *c, int reps, F &&fn) {
     return best;
 }
 
+// ----------------------------------------- allocation-free sky conversion
+//
+// baorec_sky_to_cartesian_noalloc keeps its comoving-distance table and error
+// flag in `scratch`, a file-scope device array (one copy per GPU, held until
+// the process ends), instead of context buffers.  Host code never uses the
+// array's name as an address: baorec_create looks up each device's copy with
+// hipGetSymbolAddress once, with that device current, and the entry point
+// passes that address to the kernels.
+namespace {
+constexpr int kNoallocMaxKnots = 131072;
+// zk, rk, seg (FP64, kNoallocMaxKnots each), guide (int32), error flag (int32)
+constexpr size_t kNoallocZk = 0;
+constexpr size_t kNoallocRk = kNoallocZk + sizeof(double) * kNoallocMaxKnots;
+constexpr size_t kNoallocSeg = kNoallocRk + sizeof(double) * kNoallocMaxKnots;
+constexpr size_t kNoallocGuide = kNoallocSeg + sizeof(double) * kNoallocMaxKnots;
+constexpr size_t kNoallocErr = kNoallocGuide + sizeof(int) * kNoallocMaxKnots;
+constexpr size_t kNoallocBytes = kNoallocErr + 256;
+constexpr int kNoallocMaxDevices = 64;
+
+struct NoallocDevice {
+    unsigned char *ptr = nullptr;   // this device's copy of `scratch`
+    int table_valid = 0;            // the table in it is for `cosmo`
+    baorec_cosmology cosmo{};
+    double r0 = 0.0, dr_inv = 0.0;
+};
+NoallocDevice g_noalloc[kNoallocMaxDevices];
+double g_noalloc_knots[kNoallocMaxKnots];   // host staging of the knots
+std::mutex g_noalloc_lock;                  // guards the two arrays above
+
+__global__ void k_zero_flag(int *e) {
+    if (threadIdx.x == 0) *e = 0;
+}
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+// Once per device, with that device current (called by baorec_create).
+static hipError_t noalloc_setup(int device) {
+    if (device < 0 || device >= kNoallocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_noalloc_lock);
+    NoallocDevice &d = g_noalloc[device];
+    if (d.ptr) return hipSuccess;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    d.ptr = static_cast<unsigned char *>(p);
+    return hipSuccess;
+}
+
 extern "C" {
 
 void baorec_params_default(baorec_params *p, float bias, float f, float smoothing_radius) {
@@ -1786,6 +1836,12 @@ int baorec_create(baorec_ctx **out, int device) {
     set_twiddle_table(tw.data());
     if (hipDeviceSynchronize() != hipSuccess) {
         g_create_err = "twiddle upload failed";
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return BAOREC_ERR_HIP;
+    }
+    if (noalloc_setup(device) != hipSuccess) {
+        g_create_err = "hipGetSymbolAddress of the allocation-free scratch array failed";
         (void)hipStreamDestroy(c->stream);
         delete c;
         return BAOREC_ERR_HIP;
@@ -2386,6 +2442,64 @@ int baorec_sky_to_cartesian(baorec_ctx *c, const baorec_cosmology *cp, int mem, 
         copy_out(c, mem, y, oy, n);
         copy_out(c, mem, z, oz, n);
         cosmo_check(c, e, "sky_to_cartesian");
+    });
+}
+
+int baorec_sky_to_cartesian_noalloc(baorec_ctx *c, const baorec_cosmology *cp, const float *ra,
+                                    const float *dec, const float *red, int64_t n, float *x, float *y,
+                                    float *z) {
+    return guarded(c, [&] {
+        check_cosmology(c, cp);
+        const int num = cp->z_tab_num;
+        if (num > kNoallocMaxKnots)
+            fail(c, BAOREC_ERR_UNSUPPORTED, "baorec_sky_to_cartesian_noalloc: z_tab_num " + std::to_string(num) +
+                                                " exceeds its scratch limit of " +
+                                                std::to_string(kNoallocMaxKnots) + " knots");
+        if (n < 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "negative count");
+        if (n > 0 && (!ra || !dec || !red || !x || !y || !z))
+            fail(c, BAOREC_ERR_INVALID_ARGUMENT, "NULL array");
+        std::lock_guard<std::mutex> lk(g_noalloc_lock);
+        NoallocDevice &d = g_noalloc[c->device];
+        if (!d.ptr) fail(c, BAOREC_ERR_HIP, "allocation-free scratch not set up on this device");
+        double *zk = reinterpret_cast<double *>(d.ptr + kNoallocZk);
+        double *rk = reinterpret_cast<double *>(d.ptr + kNoallocRk);
+        double *seg = reinterpret_cast<double *>(d.ptr + kNoallocSeg);
+        int *guide = reinterpret_cast<int *>(d.ptr + kNoallocGuide);
+        int *err = reinterpret_cast<int *>(d.ptr + kNoallocErr);
+        if (!d.table_valid || memcmp(&d.cosmo, cp, sizeof(*cp)) != 0) {
+            d.table_valid = 0;
+            cosmo_knots(cp, g_noalloc_knots);
+            hipchk(c, hipMemcpyAsync(zk, g_noalloc_knots, sizeof(double) * num, hipMemcpyHostToDevice, c->stream),
+                   "H2D knots");
+            hipchk(c, launch_cosmo_table(cosmo_args(cp), zk, seg, rk, num, c->stream), "cosmology table");
+            double ends[2];
+            hipchk(c, hipMemcpyAsync(&ends[0], rk, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+            hipchk(c, hipMemcpyAsync(&ends[1], rk + num - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream),
+                   "D2H");
+            sync(c);
+            d.r0 = ends[0];
+            d.dr_inv = (double)num / (ends[1] - ends[0]);
+            CosmoTable g{zk, rk, num, cp->z_tab_min, (double)(num - 1) / (cp->z_tab_max - cp->z_tab_min),
+                         guide, num, d.r0, d.dr_inv};
+            hipchk(c, launch_cosmo_guide(g, guide, c->stream), "cosmology guide");
+            d.cosmo = *cp;
+            d.table_valid = 1;
+        }
+        if (n == 0) {
+            sync(c);
+            return;
+        }
+        CosmoTable t{zk, rk, num, cp->z_tab_min, (double)(num - 1) / (cp->z_tab_max - cp->z_tab_min), guide,
+                     num, d.r0, d.dr_inv};
+        hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(64), 0, c->stream, err);
+        hipchk(c, hipGetLastError(), "clear flag");
+        const float hh = (cp->h * 100.0f) / 100.0f;   // cosmo.H0 / 100 in Float32, as the driver uses
+        hipchk(c, launch_sky_to_cart(t, hh, ra, dec, red, x, y, z, n, err, c->stream), "sky_to_cartesian");
+        int h = 0;
+        hipchk(c, hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, c->stream), "D2H");
+        sync(c);
+        if (h) fail(c, BAOREC_ERR_OUT_OF_BOX, "sky_to_cartesian: value outside the cosmology table "
+                                              "(the reference raises BoundsError)");
     });
 }
 

@@ -136,6 +136,7 @@ SIGNATURES = {
     "baorec_comoving_distance": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, I64, FP]),
     "baorec_sky_to_cartesian": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
     "baorec_cartesian_to_sky": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
+    "baorec_sky_to_cartesian_noalloc": (INT, [P, ctypes.POINTER(Cosmology), FP, FP, FP, I64, FP, FP, FP]),
     "baorec_pk_params_default": (None, [ctypes.POINTER(PkParams)]),
     "baorec_power_multipoles": (INT, [P, ctypes.POINTER(PkParams), I32P, INT, FP, FP, FP, FP, I64,
                                       FP, FP, FP, FP, FP]),

@@ -322,6 +322,25 @@ int baorec_sky_to_cartesian(baorec_ctx *ctx, const baorec_cosmology *c, int mem,
                             const float *ra, const float *dec, const float *red,
                             int64_t n, float *x, float *y, float *z);
 
+/*
+ * Allocation-free sky_to_cartesian.  No counterpart in the reference: for
+ * callers that cannot allocate device memory during the call (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * conversion and results as baorec_sky_to_cartesian with mem =
+ * BAOREC_MEM_DEVICE: ra, dec, red, x, y, z are device pointers on the
+ * context's GPU, and the call makes no hipMalloc, hipFree or other allocating
+ * call.  Its distance table (zk, rk, interval integrals, guide) and error flag
+ * live in a static device array of 3.67 MB that each GPU the library runs on
+ * holds until the process ends; z_tab_num is therefore limited to 131072
+ * knots (more returns BAOREC_ERR_UNSUPPORTED).  The table is kept between
+ * calls and rebuilt when the cosmology changes.  All calls in a process share
+ * that array per GPU: the library serialises them with a lock, and two
+ * processes each have their own copy.
+ */
+int baorec_sky_to_cartesian_noalloc(baorec_ctx *ctx, const baorec_cosmology *c,
+                                    const float *ra, const float *dec, const float *red,
+                                    int64_t n, float *x, float *y, float *z);
+
 /* cartesian_to_sky(x, y, z, cosmo) (examples/lightcone.jl:50-80): RA is
  * rem(lon - 360, 360) as the driver computes it (values in (-360, 0]). */
 int baorec_cartesian_to_sky(baorec_ctx *ctx, const baorec_cosmology *c, int mem,
