@@ -3085,6 +3085,19 @@ int baorec_warmstart_eval(baorec_ctx *c, const int32_t grid[3], int mem, const u
     });
 }
 
+int baorec_warmstart_tensor(baorec_ctx *c, int kind, int index, uint16_t *host, int64_t n) {
+    return guarded(c, [&] {
+        need_net(c);
+        const uint16_t *d = nullptr;
+        int64_t cnt = 0;
+        if (unet_tensor(c->net, kind, index, &d, &cnt) != hipSuccess)
+            fail(c, BAOREC_ERR_INVALID_ARGUMENT, "no such tensor (run baorec_warmstart_eval or _step first)");
+        if (!host || n != cnt) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "need " + std::to_string(cnt) + " entries");
+        hipchk(c, hipMemcpyAsync(host, d, sizeof(uint16_t) * cnt, hipMemcpyDeviceToHost, c->stream), "D2H tensor");
+        sync(c);
+    });
+}
+
 int baorec_warmstart_predict(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, const float *f,
                              float *seed) {
     return guarded(c, [&] {

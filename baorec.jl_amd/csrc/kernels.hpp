@@ -411,6 +411,7 @@ hipError_t unet_init(UNet *n, uint64_t seed, hipStream_t s);
 hipError_t unet_set_params(UNet *n, const float *host, hipStream_t s);
 hipError_t unet_get_params(const UNet *n, float *host, int grads, hipStream_t s);
 hipError_t unet_input(UNet *n, int batch, int lx, int ly, int lz, uint16_t **X);
+hipError_t unet_tensor(const UNet *n, int kind, int i, const uint16_t **ptr, int64_t *count);
 float *unet_output(UNet *n);
 float *unet_output_grad(UNet *n);
 hipError_t unet_forward(UNet *n, hipStream_t s);

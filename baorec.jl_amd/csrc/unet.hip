@@ -993,6 +993,28 @@ hipError_t unet_input(UNet *n, int batch, int lx, int ly, int lz, uint16_t **X) 
     *X = n->X;
     return hipSuccess;
 }
+// Device tensor of the last forward/backward (BF16 bits) and its length:
+// kind 0 the output of layer i, 1 its gradient after the ReLU mask (what the
+// weight and input gradients of layer i used), 2 pooled tensor i (0..2),
+// 3 its gradient, 4 the input features.
+hipError_t unet_tensor(const UNet *n, int kind, int i, const uint16_t **ptr, int64_t *count) {
+    *ptr = nullptr;
+    *count = 0;
+    if (!n->X) return hipErrorNotReady;
+    if ((kind == 0 || kind == 1) && i >= 0 && i < kNL) {
+        *ptr = kind == 0 ? n->A[i] : n->dA[i];
+        *count = n->nvox(kLayers[i].level) * kLayers[i].co;
+    } else if ((kind == 2 || kind == 3) && i >= 0 && i < 3) {
+        *ptr = kind == 2 ? n->Pl[i] : n->dP[i];
+        *count = n->nvox(i + 1) * (16 << i);
+    } else if (kind == 4) {
+        *ptr = n->X;
+        *count = n->nvox(0) * 8;
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipSuccess;
+}
 float *unet_output(UNet *n) { return n->out; }
 float *unet_output_grad(UNet *n) { return n->dout; }
 hipError_t unet_forward(UNet *n, hipStream_t s) { return n->forward(s); }

@@ -136,6 +136,16 @@ SIGNATURES = {
     "baorec_comoving_distance": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, I64, FP]),
     "baorec_sky_to_cartesian": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
     "baorec_cartesian_to_sky": (INT, [P, ctypes.POINTER(Cosmology), INT, FP, FP, FP, I64, FP, FP, FP]),
+    "baorec_warmstart_nparams": (I64, []),
+    "baorec_warmstart_init": (INT, [P, ctypes.c_uint64, ctypes.c_float]),
+    "baorec_warmstart_save": (INT, [P, ctypes.c_char_p]),
+    "baorec_warmstart_load": (INT, [P, ctypes.c_char_p]),
+    "baorec_warmstart_params": (INT, [P, INT, FP, I64]),
+    "baorec_warmstart_step": (INT, [P, PP, I32P, INT, FP, FP, ctypes.c_int32, ctypes.c_float,
+                                    ctypes.POINTER(ctypes.c_double)]),
+    "baorec_warmstart_eval": (INT, [P, I32P, INT, FP, FP, ctypes.c_int32, FP, ctypes.POINTER(ctypes.c_double)]),
+    "baorec_warmstart_predict": (INT, [P, PP, I32P, INT, FP, FP]),
+    "baorec_warmstart_tensor": (INT, [P, INT, INT, FP, I64]),
     "baorec_sky_to_cartesian_noalloc": (INT, [P, ctypes.POINTER(Cosmology), FP, FP, FP, I64, FP, FP, FP]),
     "baorec_pk_params_default": (None, [ctypes.POINTER(PkParams)]),
     "baorec_power_multipoles": (INT, [P, ctypes.POINTER(PkParams), I32P, INT, FP, FP, FP, FP, I64,

@@ -161,7 +161,7 @@ class MultigridRecon(_Recon):
     max_vcycles: int = 200
     rb_sweeps: int = 2
     fp64_coarse_max: int = 8
-    warm_start: str = "none"          # RBGS start: "none" (v = 0) or "spectral"
+    warm_start: str = "none"          # RBGS start: "none" (v = 0), "spectral" or "learned"
     vcycles_done: int = 0
     final_residual: float = float("nan")
     final_change: float = float("nan")
@@ -180,7 +180,7 @@ class MultigridRecon(_Recon):
         p.max_vcycles = int(self.max_vcycles)
         p.rb_sweeps = int(self.rb_sweeps)
         p.fp64_coarse_max = int(self.fp64_coarse_max)
-        p.mg_warm_start = {"none": 0, "spectral": 1}[self.warm_start]
+        p.mg_warm_start = {"none": 0, "spectral": 1, "learned": 2}[self.warm_start]
         return p
 
 
@@ -680,6 +680,106 @@ def cartesian_to_sky(x, y, z, cosmo: Cosmology, ctx: Optional[Context] = None):
     """cartesian_to_sky (examples/lightcone.jl:50-80): (RA, Dec, redshift), with
     RA = rem(lon - 360, 360) as the driver computes it."""
     return _sky("baorec_cartesian_to_sky", x, y, z, cosmo, ctx)
+
+
+# ------------------------------------------------------------------ learned warm start
+
+def warmstart_nparams() -> int:
+    """Number of FP32 parameters of the warm-start network."""
+    return int(L.load().baorec_warmstart_nparams())
+
+
+def warmstart_init(seed: int = 0, h_train: float = 1.0, ctx: Optional[Context] = None):
+    """A fresh warm-start network in the context (He-normal weights); h_train
+    is the cell size (Mpc/h) of the grids it is trained on."""
+    ctx = ctx or default_context()
+    ctx.check(ctx.lib.baorec_warmstart_init(ctx.handle, ctypes.c_uint64(int(seed)), float(h_train)))
+
+
+def warmstart_save(path, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    ctx.check(ctx.lib.baorec_warmstart_save(ctx.handle, os.fsencode(path)))
+
+
+def warmstart_load(path, ctx: Optional[Context] = None):
+    """Loads a network written by warmstart_save into the context; RBGS solves
+    with warm_start="learned" then start from its seed."""
+    ctx = ctx or default_context()
+    ctx.check(ctx.lib.baorec_warmstart_load(ctx.handle, os.fsencode(path)))
+
+
+def warmstart_params(which: str = "params", ctx: Optional[Context] = None) -> np.ndarray:
+    """The network's parameters (which="params") or the gradients of the last
+    step / eval (which="grads"), FP32."""
+    ctx = ctx or default_context()
+    out = np.empty(warmstart_nparams(), F32)
+    ctx.check(ctx.lib.baorec_warmstart_params(ctx.handle, {"params": 0, "grads": 2}[which], _ptr(out), out.size))
+    return out
+
+
+def warmstart_set_params(params, ctx: Optional[Context] = None):
+    ctx = ctx or default_context()
+    a = np.ascontiguousarray(params, dtype=F32)
+    ctx.check(ctx.lib.baorec_warmstart_params(ctx.handle, 1, _ptr(a), a.size))
+
+
+def warmstart_step(recon: MultigridRecon, f, phi, lr: float = 1e-3) -> float:
+    """One training step on a batch of (f, phi) pairs, arrays of shape
+    (nbatch, nz, ny, nx) (or one grid); phi is the converged local-LOS
+    potential of f.  lr = 0 computes the loss and gradients only.  Returns the
+    loss."""
+    ctx = recon.context()
+    f = _f32(f)
+    phi = _f32(phi)
+    shape = f.shape if f.ndim == 4 else (1,) + tuple(f.shape)
+    if tuple(phi.shape if phi.ndim == 4 else (1,) + tuple(phi.shape)) != tuple(shape):
+        raise ValueError("f and phi shapes differ")
+    loss = ctypes.c_double()
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_warmstart_step(ctx.handle, ctypes.byref(p), _grid(shape[1:][::-1]), _mem(f, phi),
+                                            _ptr(f), _ptr(phi), shape[0], float(lr), ctypes.byref(loss)))
+    return loss.value
+
+
+def warmstart_eval(features, target, ctx: Optional[Context] = None):
+    """The network on explicit features (uint16 BF16 bit patterns of shape
+    (nbatch, nz, ny, nx, 8)) against a target (nbatch, nz, ny, nx): returns
+    (output, loss); the parameter gradients are then warmstart_params("grads")."""
+    ctx = ctx or default_context()
+    X = np.ascontiguousarray(features, dtype=np.uint16)
+    t = np.ascontiguousarray(target, dtype=F32)
+    if X.ndim != 5 or X.shape[-1] != 8 or X.shape[:4] != t.shape:
+        raise ValueError("features must be (nbatch, nz, ny, nx, 8) and match the target")
+    out = np.empty(t.shape, F32)
+    loss = ctypes.c_double()
+    ctx.check(ctx.lib.baorec_warmstart_eval(ctx.handle, _grid(t.shape[1:][::-1]), L.MEM_HOST,
+                                            X.ctypes.data_as(ctypes.c_void_p), _ptr(t), t.shape[0], _ptr(out),
+                                            ctypes.byref(loss)))
+    return out, loss.value
+
+
+def warmstart_tensor(kind: str, index: int, shape, ctx: Optional[Context] = None) -> np.ndarray:
+    """A BF16 tensor of the last warmstart_eval / warmstart_step as uint16 bit
+    patterns of the given shape: kind "act" (layer output), "grad" (its
+    gradient after the ReLU mask), "pool", "pool_grad" or "input"."""
+    ctx = ctx or default_context()
+    out = np.empty(shape, np.uint16)
+    k = {"act": 0, "grad": 1, "pool": 2, "pool_grad": 3, "input": 4}[kind]
+    ctx.check(ctx.lib.baorec_warmstart_tensor(ctx.handle, k, int(index), out.ctypes.data_as(ctypes.c_void_p),
+                                              out.size))
+    return out
+
+
+def warmstart_predict(recon: MultigridRecon, f):
+    """The learned seed (spectral seed + the network's correction) of a
+    multigrid solve of f on its grid."""
+    ctx = recon.context()
+    f = _f32(f)
+    out = _new_grid(f.shape[::-1], f)
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_warmstart_predict(ctx.handle, ctypes.byref(p), _grid(f.shape[::-1]), _mem(f, out),
+                                               _ptr(f), _ptr(out)))
+    return out
 
 
 # ------------------------------------------------------------------ catalog I/O

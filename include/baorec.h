@@ -68,22 +68,26 @@ typedef enum { BAOREC_MAS_CIC = 0, BAOREC_MAS_TSC = 1 } baorec_mas;
  * Gauss-Seidel V-cycles (rb_sweeps pre- and post-smoothing sweeps) in the
  * four-colour ordering that generalises red-black to the operator's 19-point
  * stencil (its cross-derivative terms couple diagonal neighbours), with levels
- * of side <= fp64_coarse_max in FP64.  With a fixed LOS the V-cycles refine
+ * of side <= fp64_coarse_max in FP64.  The V-cycles refine
  * an FP64 solution: its fine residual is computed in FP64 and each V-cycle
  * solves for the correction (FP32 fine levels), so the residual can fall
  * below the FP32 rounding of A v (about 1e-4 of ||f|| at 512^3).  It stops
  * when the relative residual
  * (FP64 norm) is <= `tolerance`, when one V-cycle changes the residual by
  * <= tolerance * ||f||, or when two V-cycles in a row reduce it by less than
- * 10 %: with a local LOS the operator is not symmetric and an arbitrary f has
- * a residual floor above zero (final_residual reports where it stopped).  The
- * solution is returned with zero mean (A annihilates constants).
+ * 10 % (a safeguard).  The solution is returned with zero mean (A
+ * annihilates constants).
  * mg_warm_start = 1 starts RBGS from a spectral seed instead of v = 0: the
  * exact inverse of the discrete operator with the LOS fixed (the given LOS,
  * or for a local LOS the direction of the box centre seen from the origin,
  * without the operator's first-derivative term), one forward and inverse
  * FFT.  With a fixed LOS that is the solution itself up to FP32 rounding; the
- * V-cycles then continue to the same stopping rule. */
+ * V-cycles then continue to the same stopping rule.  mg_warm_start = 2 adds
+ * the learned correction of the context's network (baorec_warmstart_*).
+ * With a local LOS the operator is not symmetric and only f - alpha, alpha
+ * = (w . f) / (w . 1) for its left null vector w, is in its range: the solve
+ * moves the residual's mean into alpha after every V-cycle and reports it in
+ * rhs_shift (with a fixed LOS, w = 1 and alpha is the mean of f). */
 typedef enum { BAOREC_MG_REFERENCE = 0, BAOREC_MG_RBGS = 1 } baorec_mg_schedule;
 
 /*
@@ -114,7 +118,7 @@ typedef struct baorec_params {
     float box_pad;            /* run with randoms: setup_box pad, 500 */
     float ran_min;            /* randoms threshold factor, 0.01 */
     int32_t mutate_inputs;    /* write wrapped positions back (wrap=true), 1 */
-    int32_t mg_warm_start;    /* RBGS: 0 from v = 0; 1 from the spectral seed, 0 */
+    int32_t mg_warm_start;    /* RBGS: 0 from v = 0; 1 the spectral seed; 2 the learned seed */
     /* ---- outputs of the last run ---- */
     int32_t vcycles_done;     /* RBGS: V-cycles used on the finest level */
     double final_residual;    /* RBGS: last relative residual ||f - A v|| / ||f|| */
@@ -461,6 +465,60 @@ int64_t baorec_tune_report(baorec_ctx *ctx, char *buf, int64_t len);
  * baorec_tune_save (entries the library cannot use are rejected). */
 int baorec_tune_save(baorec_ctx *ctx, const char *path);
 int baorec_tune_load(baorec_ctx *ctx, const char *path);
+
+/* ---------------------------------------------------- learned warm start */
+
+/*
+ * Extension, no reference counterpart (BAOrec.jl's fmg starts every level
+ * from v = 0, src/multigrid.jl:722-752).  A 3D U-Net held in the context
+ * predicts, from the right-hand side f of the multigrid solve, the
+ * correction that turns the spectral seed (mg_warm_start = 1) into the
+ * local-LOS potential; mg_warm_start = 2 starts RBGS from spectral seed +
+ * correction, and the V-cycles then run to the same tolerance as a cold
+ * start.  Network: levels of 16, 32, 64, 128 channels, two 3^3 convolutions
+ * (+ bias, ReLU) per level down and up, 2x2x2 average pooling, nearest
+ * upsampling concatenated after the skip input, a 1x1 head; BF16 weights and
+ * activations on the matrix cores, FP32 accumulation, periodic padding.
+ * Input features per cell (8 channels): f / s, phi_spectral / (s h^2),
+ * rhat - nhat (local LOS minus the seed's fixed LOS, 3), 100 h / |x|, 1 inside
+ * the survey (f != 0), 0; s is the rms of f, h the cell size.  Output: the
+ * correction / (s h^2).  In the solve the network runs on the pyramid level
+ * whose cell is closest to h_train (f restricted there) and its correction is
+ * prolonged back.
+ */
+/* Number of parameters (FP32): per layer W[27][cin][cout] then b[cout], and
+ * the head's 16 weights and bias. */
+int64_t baorec_warmstart_nparams(void);
+/* A fresh network with He-normal weights from `seed`; h_train is the cell
+ * size (Mpc/h) of the grids it will be trained on. */
+int baorec_warmstart_init(baorec_ctx *ctx, uint64_t seed, float h_train);
+/* Network file: "BRUNET01", nparams, Adam step, h_train, parameters (FP32). */
+int baorec_warmstart_save(baorec_ctx *ctx, const char *path);
+int baorec_warmstart_load(baorec_ctx *ctx, const char *path);
+/* which = 0: copy the parameters to host[n]; 1: set them from host[n]
+ * (resets Adam); 2: copy the gradients of the last step / eval. */
+int baorec_warmstart_params(baorec_ctx *ctx, int which, float *host, int64_t n);
+/* One training step on nbatch pairs (f, phi) of grid-sized fields (phi the
+ * converged local-LOS potential of f): features, target
+ * (phi - phi_spectral) / (s h^2), forward, loss = mean square of the
+ * finite-difference gradient of (output - target), backward, and an Adam step
+ * of rate lr (lr = 0: gradients only).  Box, beta and LOS come from p. */
+int baorec_warmstart_step(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3], int mem,
+                          const float *f, const float *phi, int32_t nbatch, float lr, double *loss);
+/* The network on given features (nbatch x cells x 8 BF16 bit patterns,
+ * channels last) against a given target: output (FP32), loss, and the
+ * parameter gradients (read with baorec_warmstart_params(ctx, 2, ...)). */
+int baorec_warmstart_eval(baorec_ctx *ctx, const int32_t grid[3], int mem, const uint16_t *features,
+                          const float *target, int32_t nbatch, float *out, double *loss);
+/* A tensor of the last eval / step (BF16 bit patterns, channels last), for
+ * checking each layer on its own inputs: kind 0 the output of layer `index`
+ * (0..13), 1 its gradient after the ReLU mask, 2 pooled tensor `index`
+ * (0..2), 3 its gradient, 4 the input features.  n must be its length. */
+int baorec_warmstart_tensor(baorec_ctx *ctx, int kind, int index, uint16_t *host, int64_t n);
+/* The learned seed of a multigrid solve on this grid (spectral seed + the
+ * network's correction), as mg_warm_start = 2 starts from it. */
+int baorec_warmstart_predict(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3], int mem,
+                             const float *f, float *seed);
 
 /* ------------------------------------------------------------- profiling */
 
