@@ -1114,6 +1114,27 @@ static MGLevel cells_of(const Grid &G, const baorec_params *p) {
     return L;
 }
 
+// cells_of plus this grid's cell centres (for the local-LOS residual)
+static MGLevel level_with_xv(baorec_ctx *c, const Grid &G, const baorec_params *p, const Slot *slot,
+                             const std::string &key) {
+    MGLevel L = cells_of(G, p);
+    float *xv = bufT<float>(c, key, (size_t)G.nx + G.ny + G.nz);
+    hipchk(c, launch_mg_xvec(xv, slot, G.nx, G.ny, G.nz, c->stream), "xvec");
+    L.xv = xv;
+    return L;
+}
+
+// The spectral seed refined by one preconditioned Richardson step:
+// phi_s = A_c^-1 f, r_s = f - A phi_s (the local-LOS operator), then
+// out = phi_s + A_c^-1 r_s.  phis and rs (may be NULL) receive phi_s and
+// psi = A_c^-1 r_s separately.
+static void spectral_refined(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const MGLevel &L,
+                             const float *f, float *phis, float *rs, float *psi) {
+    mg_spectral_seed(c, G, p, slot, L, f, phis);
+    hipchk(c, launch_mg_residual(rs, phis, f, L, p->beta, p->has_los, c->stream), "seed residual");
+    mg_spectral_seed(c, G, p, slot, L, rs, psi);
+}
+
 static int ilog2i(int n) {
     int k = 0;
     while ((1 << k) < n) k++;
@@ -1121,19 +1142,20 @@ static int ilog2i(int n) {
 }
 
 // The network's input for nb grids f (each G.R() cells, device): per item the
-// rms s of f (FP64), the spectral seed phis, and the 8 feature channels in the
-// network's input tensor.  Returns the input tensor.  The slot must hold the
-// box (its sums and error flag are left as they are).
+// rms s of f (FP64), the spectral seed phis, its local-LOS residual rs and
+// psi = A_c^-1 rs, and the 8 feature channels in the network's input
+// tensor.  Returns the input tensor.  The slot must hold the box (its sums
+// and error flag are left as they are).
 static uint16_t *warm_inputs(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const float *f,
-                             int nb, float *phis, double *ss) {
+                             int nb, float *phis, float *rs, float *psi, double *ss) {
     const int64_t N = G.R();
-    const MGLevel L = cells_of(G, p);
+    const MGLevel L = level_with_xv(c, G, p, slot, "warm_xv");
     std::vector<double> h(nb);
     double *tmp = bufT<double>(c, "warm_sum", 1);
     for (int b = 0; b < nb; b++) {
         hipchk(c, launch_sumsq<float>(f + b * N, N, tmp, bufT<double>(c, "red_part", 1024), c->stream), "rms f");
         hipchk(c, hipMemcpyAsync(&h[b], tmp, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
-        mg_spectral_seed(c, G, p, slot, L, f + b * N, phis + b * N);
+        spectral_refined(c, G, p, slot, L, f + b * N, phis + b * N, rs + b * N, psi + b * N);
     }
     sync(c);
     for (int b = 0; b < nb; b++) h[b] = sqrt(h[b] / (double)N);
@@ -1152,7 +1174,7 @@ static uint16_t *warm_inputs(baorec_ctx *c, const Grid &G, const baorec_params *
     seed_los(p, a.nc);
     a.ih2 = L.icell2[0];
     a.href = 100.f * L.cell[0];
-    hipchk(c, launch_unet_features(f, phis, ss, a, X, c->stream), "features");
+    hipchk(c, launch_unet_features(f, phis, rs, psi, ss, a, X, c->stream), "features");
     return X;
 }
 
@@ -1191,8 +1213,10 @@ static void learned_seed(baorec_ctx *c, const Grid &G, const baorec_params *p, S
         fc = dst;
     }
     float *phis = bufT<float>(c, "warm_phis", Gc.R());
+    float *rs = bufT<float>(c, "warm_rs", Gc.R());
+    float *psi = bufT<float>(c, "warm_psi", Gc.R());
     double *ss = bufT<double>(c, "warm_ss", 1);
-    warm_inputs(c, Gc, p, slot, fc, 1, phis, ss);
+    warm_inputs(c, Gc, p, slot, fc, 1, phis, rs, psi, ss);
     {
         ProfScope ps(c, "unet_forward", 0.0);
         hipchk(c, unet_forward(c->net, c->stream), "network");
@@ -1269,15 +1293,29 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
         }
     };
     if (p->mg_warm_start) {
-        if (p->mg_warm_start != 1 && p->mg_warm_start != 2)
+        if (p->mg_warm_start < 1 || p->mg_warm_start > 3)
             fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mg_warm_start");
         if (!fft_side_ok(G.nx) || !fft_side_ok(G.ny) || !fft_side_ok(G.nz))
             fail(c, BAOREC_ERR_UNSUPPORTED, "the warm starts need grid sides that are powers of two in [16, 1024]");
         float *seed = bufT<float>(c, "mg_seed", N);
-        if (p->mg_warm_start == 1)
+        if (p->mg_warm_start == 1) {
             mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed);
-        else
-            learned_seed(c, G, p, slot, P.L[0], f0, seed);
+        } else {
+            if (p->mg_seed_steps < 0 || p->mg_seed_steps > 16)
+                fail(c, BAOREC_ERR_INVALID_ARGUMENT, "mg_seed_steps must be in [0, 16]");
+            if (p->mg_warm_start == 3)
+                mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed);
+            else
+                learned_seed(c, G, p, slot, P.L[0], f0, seed);
+            // spectral Richardson steps: seed += A_c^-1 (f - A seed)
+            float *rs = bufT<float>(c, "mg_seed_r", N);
+            float *psi = bufT<float>(c, "mg_seed_psi", N);
+            for (int k = 0; k < p->mg_seed_steps; k++) {
+                hipchk(c, launch_mg_residual(rs, seed, f0, P.L[0], p->beta, p->has_los, c->stream), "seed residual");
+                mg_spectral_seed(c, G, p, slot, P.L[0], rs, psi);
+                hipchk(c, launch_add_grid(seed, psi, N, c->stream), "seed step");
+            }
+        }
         hipchk(c, launch_convert<double, float>(phi, seed, N, c->stream), "seed to FP64");
     }
     residual(rb[0]);   // of the start: f minus its mean for v = 0
@@ -1796,6 +1834,7 @@ void baorec_params_default(baorec_params *p, float bias, float f, float smoothin
     p->box_pad = 500.0f;
     p->ran_min = 0.01f;
     p->mutate_inputs = 1;
+    p->mg_seed_steps = 3;
 }
 
 const char *baorec_version(void) { return "baorec_hip 0.1.0 (gfx950)"; }
@@ -3052,8 +3091,10 @@ int baorec_warmstart_step(baorec_ctx *c, const baorec_params *p, const int32_t g
         Slot *slot = slot_buf(c) + 1;
         slot_from_params(c, slot, p, 0.0);
         float *phis = bufT<float>(c, "warm_phis_b", n);
+        float *rs = bufT<float>(c, "warm_rs_b", n);
+        float *psi = bufT<float>(c, "warm_psi_b", n);
         double *ss = bufT<double>(c, "warm_ss_b", nbatch);
-        warm_inputs(c, G, p, slot, df, nbatch, phis, ss);
+        warm_inputs(c, G, p, slot, df, nbatch, phis, rs, psi, ss);
         float *t = bufT<float>(c, "warm_t", n);
         const float ih2 = cells_of(G, p).icell2[0];
         hipchk(c, launch_unet_target(dphi, phis, ss, ih2, N, n, t, c->stream), "target");

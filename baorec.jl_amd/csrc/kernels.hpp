@@ -384,8 +384,10 @@ hipError_t launch_unet_head(const uint16_t *x, const float *w, const float *b, f
 int unet_head_bwd_blocks(int64_t n);
 hipError_t launch_unet_head_bwd(const uint16_t *x, const float *w, const float *dout, uint16_t *dx, float *part,
                                 int64_t n, hipStream_t s);
-hipError_t launch_unet_features(const float *f, const float *phis, const double *ss, const FeatArgs &a, uint16_t *X,
-                                hipStream_t s);
+hipError_t launch_unet_features(const float *f, const float *phis, const float *rs, const float *psi,
+                                const double *ss, const FeatArgs &a, uint16_t *X, hipStream_t s);
+// a += b (FP32 grids)
+hipError_t launch_add_grid(float *a, const float *b, int64_t n, hipStream_t s);
 hipError_t launch_unet_target(const float *phi, const float *phis, const double *ss, float ih2, int64_t nv, int64_t n,
                               float *t, hipStream_t s);
 int unet_loss_blocks(int64_t n);

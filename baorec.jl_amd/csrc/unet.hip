@@ -379,14 +379,15 @@ __global__ void __launch_bounds__(256) k_head_bwd(const uint16_t *__restrict__ x
 
 // Input features of the network per voxel (8 BF16 channels):
 //   0  f / s                       the solve's right-hand side
-//   1  phi_s / (s h^2)             the spectral seed (fixed-LOS inverse)
-//   2-4  rhat - nhat_c             local LOS minus the seed's fixed LOS
-//   5  h_ref / |x|                 strength of the first-derivative term
-//   6  1 where f != 0 (inside the survey), else 0
-//   7  0
+//   1  phi_s / (s h^2)             the spectral seed (fixed-LOS inverse of f)
+//   2  r_s / s                     its residual under the local-LOS operator
+//   3  psi / (s h^2)               the fixed-LOS inverse of that residual
+//   4-6  rhat - nhat_c             local LOS minus the seed's fixed LOS
+//   7  h_ref / |x|                 strength of the first-derivative term
 // s = rms(f) of the item (FP64 on the device), h the cell size.
-__global__ void k_features(const float *__restrict__ f, const float *__restrict__ phis, const double *__restrict__ ss,
-                           FeatArgs a, uint16_t *__restrict__ X) {
+__global__ void k_features(const float *__restrict__ f, const float *__restrict__ phis, const float *__restrict__ rs,
+                           const float *__restrict__ psi, const double *__restrict__ ss, FeatArgs a,
+                           uint16_t *__restrict__ X) {
     const int nx = 1 << a.lx, ny = 1 << a.ly, nz = 1 << a.lz;
     const int64_t nv = (int64_t)nx * ny * nz;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv * a.batch;
@@ -400,20 +401,25 @@ __global__ void k_features(const float *__restrict__ f, const float *__restrict_
                     pz = a.bmin[2] + (z + 0.5f) * a.h[2];
         const float r = sqrtf(px * px + py * py + pz * pz);
         const float ir = r > 0.f ? 1.f / r : 0.f;
-        const float fv = f[i];
         uint16_t o[8];
-        o[0] = f2bf(fv * inv);
+        o[0] = f2bf(f[i] * inv);
         o[1] = f2bf(phis[i] * inv * a.ih2);
-        o[2] = f2bf(px * ir - a.nc[0]);
-        o[3] = f2bf(py * ir - a.nc[1]);
-        o[4] = f2bf(pz * ir - a.nc[2]);
-        o[5] = f2bf(a.href * ir);
-        o[6] = f2bf(fv != 0.f ? 1.f : 0.f);
-        o[7] = 0;
+        o[2] = f2bf(rs[i] * inv);
+        o[3] = f2bf(psi[i] * inv * a.ih2);
+        o[4] = f2bf(px * ir - a.nc[0]);
+        o[5] = f2bf(py * ir - a.nc[1]);
+        o[6] = f2bf(pz * ir - a.nc[2]);
+        o[7] = f2bf(a.href * ir);
         uint4 q;
         memcpy(&q, o, 16);
         *reinterpret_cast<uint4 *>(X + i * 8) = q;
     }
+}
+
+// a += b
+__global__ void k_add(float *__restrict__ a, const float *__restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] += b[i];
 }
 
 // target = (phi - phi_s) / (s h^2) and the loss
@@ -582,10 +588,14 @@ hipError_t launch_unet_head_bwd(const uint16_t *x, const float *w, const float *
     hipLaunchKernelGGL(k_head_bwd, dim3(blocks(n)), dim3(256), 0, s, x, w, dout, dx, part, n);
     return hipGetLastError();
 }
-hipError_t launch_unet_features(const float *f, const float *phis, const double *ss, const FeatArgs &a, uint16_t *X,
-                                hipStream_t s) {
+hipError_t launch_unet_features(const float *f, const float *phis, const float *rs, const float *psi,
+                                const double *ss, const FeatArgs &a, uint16_t *X, hipStream_t s) {
     const int64_t n = ((int64_t)a.batch << (a.lx + a.ly + a.lz));
-    hipLaunchKernelGGL(k_features, dim3(blocks(n)), dim3(256), 0, s, f, phis, ss, a, X);
+    hipLaunchKernelGGL(k_features, dim3(blocks(n)), dim3(256), 0, s, f, phis, rs, psi, ss, a, X);
+    return hipGetLastError();
+}
+hipError_t launch_add_grid(float *a, const float *b, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_add, dim3(blocks(n)), dim3(256), 0, s, a, b, n);
     return hipGetLastError();
 }
 hipError_t launch_unet_target(const float *phi, const float *phis, const double *ss, float ih2, int64_t nv, int64_t n,

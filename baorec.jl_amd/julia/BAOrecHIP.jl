@@ -50,6 +50,7 @@ mutable struct Params
     ran_min::Float32
     mutate_inputs::Int32
     mg_warm_start::Int32
+    mg_seed_steps::Int32
     vcycles_done::Int32
     final_residual::Float64
     final_change::Float64

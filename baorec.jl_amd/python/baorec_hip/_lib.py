@@ -55,6 +55,7 @@ class Params(ctypes.Structure):
         ("ran_min", ctypes.c_float),
         ("mutate_inputs", ctypes.c_int32),
         ("mg_warm_start", ctypes.c_int32),
+        ("mg_seed_steps", ctypes.c_int32),
         ("vcycles_done", ctypes.c_int32),
         ("final_residual", ctypes.c_double),
         ("final_change", ctypes.c_double),

@@ -161,7 +161,8 @@ class MultigridRecon(_Recon):
     max_vcycles: int = 200
     rb_sweeps: int = 2
     fp64_coarse_max: int = 8
-    warm_start: str = "none"          # RBGS start: "none" (v = 0), "spectral" or "learned"
+    warm_start: str = "none"          # RBGS start: "none" (v = 0), "spectral", "learned" or "refined"
+    seed_steps: int = 3               # spectral Richardson steps of the learned and refined starts
     vcycles_done: int = 0
     final_residual: float = float("nan")
     final_change: float = float("nan")
@@ -180,7 +181,8 @@ class MultigridRecon(_Recon):
         p.max_vcycles = int(self.max_vcycles)
         p.rb_sweeps = int(self.rb_sweeps)
         p.fp64_coarse_max = int(self.fp64_coarse_max)
-        p.mg_warm_start = {"none": 0, "spectral": 1, "learned": 2}[self.warm_start]
+        p.mg_warm_start = {"none": 0, "spectral": 1, "learned": 2, "refined": 3}[self.warm_start]
+        p.mg_seed_steps = int(self.seed_steps)
         return p
 
 

@@ -2,15 +2,22 @@
 """Reconstructions per second of the BAOrec.jl hot path on one MI355X.
 
 One step = one reconstruction as the reference's GPU benchmark times it
-(test/benchmarks_gpu.jl:114-126): run!(IterativeRecon, 512^3, data, randoms)
+(test/benchmarks_gpu.jl:114-126, 140-151): run!(recon, 512^3, data, randoms)
 followed by reconstructed_positions(recon, data; field=:sum), with the
 catalogs already resident in HBM.
 
-Default workload (the 512^3 lightcone configuration): 2,000,000 lognormal
-tracers in the shell r in [1000, 1800] Mpc/h over a 90x90 deg cap with
-n(r) ~ r^-0.5, 20,000,000 unclustered randoms in the same geometry, box padded
-10 %, local radial LOS, CIC, FP32, R = 15 Mpc/h, b = 2.2, f = 0.75,
-3 iterations.  Synthetic data (the reference's mocks are not available).
+Catalogs of the 512^3 lightcone configuration: 2,000,000 lognormal tracers in
+the shell r in [1000, 1800] Mpc/h over a 90x90 deg cap with n(r) ~ r^-0.5,
+20,000,000 unclustered randoms in the same geometry, box padded 10 %, local
+radial LOS, CIC, FP32, R = 15 Mpc/h, b = 2.2, f = 0.75.  Synthetic data (the
+reference's mocks are not available).
+
+Default workload lc512_warm: the multigrid reconstruction of these catalogs
+(RBGS V-cycles to relative residual 1e-6) started from the learned seed is the
+headline value; the same reconstruction from the plain cold start, from the
+spectral seed and from the refined spectral seed is timed beside it, and the
+iterative reconstruction (3 iterations) of configurations 1 (128^3 box, 1M
+tracers, LOS z) and 2 (these catalogs) is timed briefly.
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" fields).
 """
@@ -68,12 +75,17 @@ def make_workload(args):
     return data, rand, cfg
 
 
-def gpu_bench(args, data, rand, cfg):
+WARM_STARTS = {"none": 0, "spectral": 1, "learned": 2, "refined": 3}
+NET = os.path.join(ROOT, "baorec.jl_amd", "weights", "unet_lc128.bin")
+
+
+def gpu_bench(args, data, rand, cfg, warm="none", steps=None, host_leg=True, ctx=None):
     import baorec_hip as bh
     from baorec_hip import _lib as L
     from baorec_hip.devmem import DeviceArray, hip
 
-    ctx = bh.Context(0)
+    steps = args.steps if steps is None else steps
+    ctx = ctx or bh.Context(0)
     lib = ctx.lib
     p = L.Params()
     lib.baorec_params_default(ctypes.byref(p), cfg["bias"], cfg["f"], cfg["R"])
@@ -84,7 +96,10 @@ def gpu_bench(args, data, rand, cfg):
         p.tolerance = 1e-6
         p.rb_sweeps = 2
         p.fp64_coarse_max = 8
-        p.mg_warm_start = 1 if args.warm_start else 0
+        p.mg_warm_start = WARM_STARTS[warm]
+        p.mg_seed_steps = args.seed_steps
+        if warm == "learned":
+            ctx.check(lib.baorec_warmstart_load(ctx.handle, os.fsencode(args.net)))
     if cfg["los"] is not None:
         p.has_los = 1
         for a in range(3):
@@ -143,11 +158,16 @@ def gpu_bench(args, data, rand, cfg):
         log(f"warmup step {i + 1}/{args.warmup} done")
     h.hipDeviceSynchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         step()
     h.hipDeviceSynchronize()
     dt = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {dt:.3f} s")
+    log(f"{cfg.get('name', args.workload)} ({warm}): timed {steps} steps in {dt:.3f} s")
+    if not host_leg:
+        pos = [o.numpy() for o in out]
+        mg = {"vcycles": p.vcycles_done, "seed_residual": p.seed_residual, "final_residual": p.final_residual,
+              "rhs_shift": p.rhs_shift} if multigrid else None
+        return dict(dt=dt, steps=steps, pos=pos, mg=mg)
 
     # the same step with host-resident catalogs and outputs (PCIe transfers
     # included); reported beside the device-resident value, never as it
@@ -182,11 +202,11 @@ def gpu_bench(args, data, rand, cfg):
     prof = json.loads(buf.value.decode())
     pos = [o.numpy() for o in out]
     finite = all(np.isfinite(a).all() for a in pos)
-    mg = {"schedule": cfg["schedule"], "warm_start": "spectral" if args.warm_start else "none",
+    mg = {"schedule": cfg["schedule"], "warm_start": warm,
           "vcycles": p.vcycles_done, "seed_residual": p.seed_residual, "final_residual": p.final_residual,
-          "final_change": p.final_change} if multigrid else None
-    return dict(dt=dt, prof=prof, finite=finite, box=(list(p.box_size), list(p.box_min)), dt_host=dt_host,
-                mg=mg, tuned=tuned)
+          "final_change": p.final_change, "rhs_shift": p.rhs_shift} if multigrid else None
+    return dict(dt=dt, steps=steps, prof=prof, finite=finite, box=(list(p.box_size), list(p.box_min)),
+                dt_host=dt_host, mg=mg, tuned=tuned, pos=pos)
 
 
 def kernel_symbol(label, grid, rand):
@@ -681,25 +701,110 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     print(json.dumps({"calibration_copy_bytes": nbytes, "copies": reps}), flush=True)
 
 
+def warm_bench(args):
+    """The headline: the 512^3 lightcone multigrid reconstruction (RBGS to
+    relative residual 1e-6, local LOS) started from the learned seed, against
+    the plain cold start and the two non-learned seeds, each a full timed
+    run of run! + reconstructed_positions(:sum) on the same device-resident
+    catalogs; plus the iterative forms of configurations 1 and 2."""
+    import baorec_hip as bh
+    if not os.path.exists(args.net):
+        raise SystemExit(f"warm-start network {args.net} not found")
+    args.workload = "mg_lc512"
+    data, rand, cfg = make_workload(args)
+    args.workload = "lc512_warm"
+    cfg["name"] = "mg_lc512"
+    ctx = bh.Context(0)
+    forms = {}
+    for warm in ("none", "spectral", "refined"):
+        forms[warm] = gpu_bench(args, data, rand, cfg, warm=warm, host_leg=False, ctx=ctx)
+    g = gpu_bench(args, data, rand, cfg, warm="learned", ctx=ctx)
+    forms["learned"] = g
+    ref = forms["none"]["pos"]
+    comp = {}
+    for k, v in forms.items():
+        rms = float(np.sqrt(np.mean([np.mean((a.astype(np.float64) - b) ** 2) for a, b in zip(v["pos"], ref)])))
+        comp[k] = {"recons_per_s": round(v["steps"] / v["dt"], 3), "ms_per_step": round(v["dt"] / v["steps"] * 1e3, 3),
+                   "vcycles": v["mg"]["vcycles"], "seed_residual": v["mg"]["seed_residual"],
+                   "final_residual": v["mg"]["final_residual"], "positions_rms_vs_plain_mpch": rms}
+    # configurations 1 and 2 (iterative), timed briefly on the same context
+    others = {}
+    for wl, nsteps in (("lightcone512", 40), ("box128", 200)):
+        args.workload = wl
+        d2, r2, c2 = (data, rand, dict(grid=(512, 512, 512), bias=2.2, f=0.75, R=15.0, los=None,
+                                       pad=cfg["pad"])) if wl == "lightcone512" else make_workload(args)
+        c2["name"] = wl
+        o = gpu_bench(args, d2, r2, c2, steps=nsteps, host_leg=False, ctx=ctx)
+        others[wl] = {"recons_per_s": round(nsteps / o["dt"], 2), "ms_per_step": round(o["dt"] / nsteps * 1e3, 3),
+                      "steps": nsteps}
+    args.workload = "lc512_warm"
+    prof = g["prof"]
+    roof = roofline(prof, cfg["grid"], "mg_lc512", True)
+    cpu = None if args.no_cpu else cpu_baseline(args, data, rand, dict(cfg, recon="iterative"))
+    total_ms = sum(v["ms"] for v in prof.values())
+    total_bytes = sum(v["bytes"] for v in prof.values())
+    plain = comp["none"]
+    line = {
+        "metric": "recons/s at 512\u00b3 grid, 2M particles: learned warm-start vs plain at equal residual",
+        "value": round(args.steps / g["dt"], 4),
+        "unit": "recons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(g["dt"] / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 solve, bf16 network",
+        "data": "synthetic",
+        "config": {"workload": "mg_lc512, learned warm start", "grid": list(cfg["grid"]),
+                   "n_data": len(data[0]), "n_randoms": len(rand[0]), "recon": "multigrid (RBGS, 2+2 sweeps, "
+                   "FP64 levels <= 8^3)", "tolerance": 1e-6, "los": "local", "box_pad": round(cfg["pad"], 2),
+                   "network": os.path.relpath(args.net, ROOT), **g["mg"]},
+        "warm_vs_plain": {"speedup": round(comp["learned"]["recons_per_s"] / plain["recons_per_s"], 3),
+                          "vcycle_ratio": round(plain["vcycles"] / max(comp["learned"]["vcycles"], 1), 3),
+                          "forms": comp},
+        "other_configs": others,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": {k: {"count": v["count"], "ms": round(v["ms"], 3),
+                        "GBps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in prof.items()},
+        "profiled_step": {"kernel_ms": round(total_ms, 3), "algorithmic_GB": round(total_bytes / 1e9, 3)},
+        "outputs_finite": g["finite"],
+        "pcie_inclusive": {"value": round(1.0 / g["dt_host"], 4), "unit": "recons/s",
+                           "note": "catalogs in host memory, copied in and out by the library each step"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 50 for lc512_warm, about 3 s per form; 10 otherwise)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="lightcone512")
+    ap.add_argument("--workload", default="lc512_warm",
+                    help="lc512_warm (default: the 512^3 lightcone multigrid recon, learned warm start against "
+                         "the plain cold start), lightcone512, mg_lc512, box128, mg512, mg512_zlos, mg512_ref, "
+                         "batch64, readout512, sky22m, catalog2m, pk512")
     ap.add_argument("--ndata", type=int, default=2_000_000)
     ap.add_argument("--nrand", type=int, default=20_000_000)
     ap.add_argument("--ncat", type=int, default=64, help="catalogs of the batch64 workload")
     ap.add_argument("--gen-workers", type=int, default=8,
                     help="processes generating the batch64 catalogs (1 under a profiler)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--warm-start", action="store_true",
-                    help="multigrid workloads: start RBGS from the spectral seed")
+    ap.add_argument("--warm-start", default="none", choices=sorted(WARM_STARTS),
+                    help="multigrid workloads: RBGS start (cold, spectral seed, learned seed, refined spectral)")
+    ap.add_argument("--net", default=NET, help="warm-start network file (learned starts)")
+    ap.add_argument("--seed-steps", type=int, default=3,
+                    help="spectral Richardson steps of the learned and refined starts")
     ap.add_argument("--autotune", action="store_true",
                     help="run baorec_autotune on the workload's catalog before timing")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run 1 GiB device-to-device copies (byte-counter calibration)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 50 if args.workload == "lc512_warm" else 10
     if args.calibrate:
         copy_calibration()
         return
@@ -722,8 +827,11 @@ def main():
     if args.workload == "readout512":
         readout_bench(args)
         return
+    if args.workload == "lc512_warm":
+        warm_bench(args)
+        return
     data, rand, cfg = make_workload(args)
-    g = gpu_bench(args, data, rand, cfg)
+    g = gpu_bench(args, data, rand, cfg, warm=args.warm_start)
     prof = g["prof"]
     roof = roofline(prof, cfg["grid"], args.workload, rand is not None)
     total_ms = sum(v["ms"] for v in prof.values())

@@ -82,8 +82,12 @@ typedef enum { BAOREC_MAS_CIC = 0, BAOREC_MAS_TSC = 1 } baorec_mas;
  * or for a local LOS the direction of the box centre seen from the origin,
  * without the operator's first-derivative term), one forward and inverse
  * FFT.  With a fixed LOS that is the solution itself up to FP32 rounding; the
- * V-cycles then continue to the same stopping rule.  mg_warm_start = 2 adds
- * the learned correction of the context's network (baorec_warmstart_*).
+ * V-cycles then continue to the same stopping rule.  mg_warm_start = 3 then
+ * takes mg_seed_steps preconditioned Richardson steps v += A_c^-1 (f - A v)
+ * (A_c^-1: that spectral inverse, A: the local-LOS operator).
+ * mg_warm_start = 2 adds the learned correction of the context's network
+ * (baorec_warmstart_*) to the spectral seed and then takes the same steps,
+ * so 2 against 3 isolates what the network contributes.
  * With a local LOS the operator is not symmetric and only f - alpha, alpha
  * = (w . f) / (w . 1) for its left null vector w, is in its range: the solve
  * moves the residual's mean into alpha after every V-cycle and reports it in
@@ -118,7 +122,9 @@ typedef struct baorec_params {
     float box_pad;            /* run with randoms: setup_box pad, 500 */
     float ran_min;            /* randoms threshold factor, 0.01 */
     int32_t mutate_inputs;    /* write wrapped positions back (wrap=true), 1 */
-    int32_t mg_warm_start;    /* RBGS: 0 from v = 0; 1 the spectral seed; 2 the learned seed */
+    int32_t mg_warm_start;    /* RBGS: 0 from v = 0; 1 the spectral seed; 2 the learned seed;
+                                 3 the spectral seed refined by spectral Richardson steps */
+    int32_t mg_seed_steps;    /* RBGS, warm starts 2 and 3: spectral Richardson steps, 3 */
     /* ---- outputs of the last run ---- */
     int32_t vcycles_done;     /* RBGS: V-cycles used on the finest level */
     double final_residual;    /* RBGS: last relative residual ||f - A v|| / ||f|| */
@@ -479,10 +485,11 @@ int baorec_tune_load(baorec_ctx *ctx, const char *path);
  * (+ bias, ReLU) per level down and up, 2x2x2 average pooling, nearest
  * upsampling concatenated after the skip input, a 1x1 head; BF16 weights and
  * activations on the matrix cores, FP32 accumulation, periodic padding.
- * Input features per cell (8 channels): f / s, phi_spectral / (s h^2),
- * rhat - nhat (local LOS minus the seed's fixed LOS, 3), 100 h / |x|, 1 inside
- * the survey (f != 0), 0; s is the rms of f, h the cell size.  Output: the
- * correction / (s h^2).  In the solve the network runs on the pyramid level
+ * Input features per cell (8 channels): f / s, phi_s / (s h^2) (phi_s the
+ * spectral seed), r_s / s (its residual under the local-LOS operator),
+ * psi / (s h^2) (psi the spectral inverse of r_s), rhat - nhat (local LOS minus
+ * the seed's fixed LOS, 3), 100 h / |x|; s is the rms of f, h the cell size.
+ * Output: the correction / (s h^2).  In the solve the network runs on the pyramid level
  * whose cell is closest to h_train (f restricted there) and its correction is
  * prolonged back.
  */
