@@ -122,9 +122,33 @@ function params(recon::AbstractRecon)
         p.jacobi_damping_factor = recon.jacobi_damping_factor
         p.jacobi_niterations = recon.jacobi_niterations
         p.vcycle_niterations = recon.vcycle_niterations
+        o = MG_OPTIONS[]
+        if o !== nothing   # extension: RBGS solve and its warm starts
+            p.mg_schedule = o.schedule === :rbgs ? 1 : 0
+            p.tolerance = o.tolerance
+            p.mg_warm_start = Dict(:none => 0, :spectral => 1, :learned => 2, :refined => 3)[o.warm_start]
+            p.mg_seed_steps = o.seed_steps
+        end
     end
     p
 end
+
+# Extension (no reference counterpart): MultigridRecon solves through the RBGS
+# schedule to `tolerance`, started cold (:none), from the spectral seed, from
+# the refined spectral seed or from the learned seed (the network loaded with
+# warmstart_load).  `nothing` restores the reference's fmg schedule.
+const MG_OPTIONS = Ref{Any}(nothing)
+multigrid_solver!(; schedule::Symbol = :rbgs, tolerance::Float64 = 1e-6, warm_start::Symbol = :none,
+                  seed_steps::Integer = 3) =
+    (MG_OPTIONS[] = (schedule = schedule, tolerance = tolerance, warm_start = warm_start,
+                     seed_steps = Int32(seed_steps)))
+reference_multigrid!() = (MG_OPTIONS[] = nothing)
+
+# The warm-start network of the context (baorec_warmstart_load / _save).
+warmstart_load(path::AbstractString; ctx::Context = default_context()) =
+    check(ctx, ccall((:baorec_warmstart_load, libbaorec), Cint, (Ptr{Cvoid}, Cstring), ctx.ptr, path))
+warmstart_save(path::AbstractString; ctx::Context = default_context()) =
+    check(ctx, ccall((:baorec_warmstart_save, libbaorec), Cint, (Ptr{Cvoid}, Cstring), ctx.ptr, path))
 
 _grid(gs) = Int32[gs[1], gs[2], gs[3]]
 _kind(::IterativeRecon) = ITERATIVE
