@@ -801,7 +801,7 @@ MGPyr mg_build(baorec_ctx *c, const Grid &G0, const baorec_params *p, Slot *slot
     }
     const int nl = (int)P.g.size();
     size_t xvtot = 0;
-    for (auto &q : P.g) xvtot += q.nx + q.ny + q.nz;
+    for (auto &q : P.g) xvtot += 2 * (q.nx + q.ny + q.nz);
     float *xv = bufT<float>(c, "mg_xv", xvtot);
     size_t off = 0;
     for (int i = 0; i < nl; i++) {
@@ -844,8 +844,9 @@ MGPyr mg_build(baorec_ctx *c, const Grid &G0, const baorec_params *p, Slot *slot
             L.losn[a] = p->has_los ? p->los[a] / cell : 0.f;
         }
         L.xv = xv + off;
+        L.pv = L.xv + (q.nx + q.ny + q.nz);
         hipchk(c, launch_mg_xvec(xv + off, slot, q.nx, q.ny, q.nz, c->stream), "mg xvec");
-        off += q.nx + q.ny + q.nz;
+        off += 2 * (q.nx + q.ny + q.nz);
         P.L.push_back(L);
         P.cur.push_back(0);
     }
@@ -966,6 +967,17 @@ void mg_fmg_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
 
 // Multicolour Gauss-Seidel V-cycle (RBGS schedule) with FP64 coarse levels.
 void mg_smooth_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int sweeps) {
+    // whole sweeps in one launch each, ping-pong between A and B; below 128^3
+    // cells the per-colour launches are as fast
+    if (!P.dbl[l] && mg_gs4_ok(P.L[l]) && P.g[l].R() >= ((int64_t)1 << 21)) {
+        for (int s = 0; s < sweeps; s++) {
+            ProfScope ps(c, "mg_gs4", 12.0 * P.g[l].R());
+            hipchk(c, launch_mg_gs4(vother(P, l), vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
+                   "gs4");
+            P.cur[l] ^= 1;
+        }
+        return;
+    }
     for (int s = 0; s < sweeps; s++)
         for (int col = 0; col < 4; col++) {
             ProfScope ps(c, P.dbl[l] ? "mg_mcgs64" : "mg_mcgs", (P.dbl[l] ? 8.0 : 4.0) * 2.5 * P.g[l].R());
@@ -1118,9 +1130,10 @@ static MGLevel cells_of(const Grid &G, const baorec_params *p) {
 static MGLevel level_with_xv(baorec_ctx *c, const Grid &G, const baorec_params *p, const Slot *slot,
                              const std::string &key) {
     MGLevel L = cells_of(G, p);
-    float *xv = bufT<float>(c, key, (size_t)G.nx + G.ny + G.nz);
+    float *xv = bufT<float>(c, key, 2 * ((size_t)G.nx + G.ny + G.nz));
     hipchk(c, launch_mg_xvec(xv, slot, G.nx, G.ny, G.nz, c->stream), "xvec");
     L.xv = xv;
+    L.pv = xv + (G.nx + G.ny + G.nz);
     return L;
 }
 
@@ -2301,9 +2314,10 @@ static MGLevel level_for(baorec_ctx *c, const Grid &G, const baorec_params *p, S
         L.icell2[a] = 1.0f / L.cell2[a];
         L.losn[a] = p->has_los ? p->los[a] / cell : 0.f;
     }
-    float *xv = bufT<float>(c, "mgp_xv", G.nx + G.ny + G.nz);
+    float *xv = bufT<float>(c, "mgp_xv", 2 * ((size_t)G.nx + G.ny + G.nz));
     hipchk(c, launch_mg_xvec(xv, slot, G.nx, G.ny, G.nz, c->stream), "xvec");
     L.xv = xv;
+    L.pv = xv + (G.nx + G.ny + G.nz);
     return L;
 }
 
@@ -2328,6 +2342,43 @@ int baorec_mg_jacobi(baorec_ctx *c, const baorec_params *p, const int32_t grid[3
         }
         if (a != dv)
             hipchk(c, hipMemcpyAsync(dv, a, sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream), "copy");
+        copy_out(c, mem, v, dv, G.R());
+        sync(c);
+    });
+}
+
+int baorec_mg_gauss_seidel(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem, float *v,
+                           const float *f, int sweeps, int fused) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, false);
+        if (sweeps < 0) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "sweeps < 0");
+        if (G.nx % 2 || G.ny % 2 || G.nz % 2)
+            fail(c, BAOREC_ERR_UNSUPPORTED, "the 4-colour Gauss-Seidel sweep needs even grid sides");
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        float *dv = const_cast<float *>(stage_in(c, mem, v, G.R(), "mgp_v"));
+        if (mem == BAOREC_MEM_DEVICE) dv = v;
+        const float *df = stage_in(c, mem, f, G.R(), "mgp_f");
+        MGLevel L = level_for(c, G, p, slot);
+        if (fused) {
+            if (!mg_gs4_ok(L))
+                fail(c, BAOREC_ERR_UNSUPPORTED, "the one-launch sweep needs sides that are multiples of 64, 16, 8");
+            float *tmp = bufT<float>(c, "mgp_t", G.R());
+            float *a = dv, *b = tmp;
+            for (int i = 0; i < sweeps; i++) {
+                hipchk(c, launch_mg_gs4(b, a, df, L, p->beta, p->has_los, c->stream), "gs4");
+                std::swap(a, b);
+            }
+            if (a != dv)
+                hipchk(c, hipMemcpyAsync(dv, a, sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream),
+                       "copy");
+        } else {
+            for (int i = 0; i < sweeps; i++)
+                for (int col = 0; col < 4; col++)
+                    hipchk(c, launch_mg_rbgs<float>(dv, df, L, p->beta, p->has_los, col, c->stream), "mcgs");
+        }
         copy_out(c, mem, v, dv, G.R());
         sync(c);
     });

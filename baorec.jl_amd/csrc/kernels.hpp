@@ -226,6 +226,7 @@ struct MGLevel {
     int n[3];
     float cell[3], cell2[3], icell2[3], losn[3];
     const float *xv;   // x[n0] | y[n1] | z[n2] cell centres of this level
+    const float *pv;   // the same divided by the cell size (FP32), = xv + n0 + n1 + n2
 };
 
 // ------------------------------------------------------------- cosmology
@@ -305,12 +306,19 @@ hipError_t launch_mg_residual(float *r, const float *v, const float *f, const MG
 hipError_t launch_mg_reduce(float *coarse, const float *fine, const int fine_n[3], hipStream_t s);
 hipError_t launch_mg_prolong(float *fine, const float *coarse, const int coarse_n[3], int add,
                              hipStream_t s);
+// xv[0 .. n) cell centres (x | y | z), xv[n .. 2n) centres / cell size, n = nx + ny + nz
 hipError_t launch_mg_xvec(float *xv, const Slot *slot, int nx, int ny, int nz, hipStream_t s);
 
 // FP64-capable red-black Gauss-Seidel pieces (RBGS schedule)
 template <typename T>
 hipError_t launch_mg_rbgs(T *v, const T *f, const MGLevel &L, float beta, int has_los, int color,
                           hipStream_t s);
+// One whole 4-colour sweep out of place (vout != vin), bitwise the four
+// in-place launch_mg_rbgs colour passes; for levels with mg_gs4_ok (sides
+// multiples of 64, 16 and 8).
+bool mg_gs4_ok(const MGLevel &L);
+hipError_t launch_mg_gs4(float *vout, const float *vin, const float *f, const MGLevel &L, float beta, int has_los,
+                         hipStream_t s);
 template <typename T>
 hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, float beta,
                                 int has_los, hipStream_t s);

@@ -5,6 +5,8 @@
 // templated ones also run FP64 coarse levels.
 #include <math.h>
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace baorec {
@@ -14,19 +16,14 @@ struct Coef {
     T px, py, pz, g, gpx2, gpy2, gpz2;
 };
 
-// src/multigrid.jl:161-168 (per-cell coefficients)
+// src/multigrid.jl:161-168 (per-cell coefficients) from the LOS components
+// p = (px, py, pz) in cell units
 template <typename T>
-__device__ __forceinline__ Coef<T> coef(const MGLevel &L, int ix, int iy, int iz, T beta, int has_los) {
+__device__ __forceinline__ Coef<T> coef_p(const MGLevel &L, T px, T py, T pz, T beta) {
     Coef<T> c;
-    if (has_los) {
-        c.px = (T)L.losn[0];
-        c.py = (T)L.losn[1];
-        c.pz = (T)L.losn[2];
-    } else {
-        c.px = (T)L.xv[ix] / (T)L.cell[0];
-        c.py = (T)L.xv[L.n[0] + iy] / (T)L.cell[1];
-        c.pz = (T)L.xv[L.n[0] + L.n[1] + iz] / (T)L.cell[2];
-    }
+    c.px = px;
+    c.py = py;
+    c.pz = pz;
     c.g = beta / ((T)L.cell2[0] * (c.px * c.px) + (T)L.cell2[1] * (c.py * c.py) +
                   (T)L.cell2[2] * (c.pz * c.pz));
     c.gpx2 = (T)L.icell2[0] + c.g * (c.px * c.px);
@@ -35,12 +32,37 @@ __device__ __forceinline__ Coef<T> coef(const MGLevel &L, int ix, int iy, int iz
     return c;
 }
 
+// p is the fixed LOS, or the cell centre / cell size (local LOS)
+template <typename T>
+__device__ __forceinline__ Coef<T> coef(const MGLevel &L, int ix, int iy, int iz, T beta, int has_los) {
+    if (has_los) return coef_p<T>(L, (T)L.losn[0], (T)L.losn[1], (T)L.losn[2], beta);
+    if constexpr (std::is_same<T, float>::value)   // xv / cell, precomputed per level
+        return coef_p<T>(L, L.pv[ix], L.pv[L.n[0] + iy], L.pv[L.n[0] + L.n[1] + iz], beta);
+    return coef_p<T>(L, (T)L.xv[ix] / (T)L.cell[0], (T)L.xv[L.n[0] + iy] / (T)L.cell[1],
+                     (T)L.xv[L.n[0] + L.n[1] + iz] / (T)L.cell[2], beta);
+}
+
 template <typename T>
 struct Nbr {
     T A, B, C, X, G;
 };
 
-// Neighbour sums of src/multigrid.jl:170-185 in the reference's order.
+// Neighbour sums of src/multigrid.jl:170-185 in the reference's order, from
+// the 18 neighbour values (p/m: +1/-1 along the two axes named).
+template <typename T>
+__device__ __forceinline__ Nbr<T> nbr_of(const Coef<T> &c, T vxp, T vxm, T vyp, T vym, T vzp, T vzm, T xpyp,
+                                         T xmym, T xmyp, T xpym, T xpzp, T xmzm, T xmzp, T xpzm, T ypzp, T ymzm,
+                                         T ymzp, T ypzm) {
+    Nbr<T> r;
+    r.A = c.gpx2 * (vxp + vxm);
+    r.B = c.gpy2 * (vyp + vym);
+    r.C = c.gpz2 * (vzp + vzm);
+    r.X = (c.g / (T)2) * ((c.px * c.py) * (xpyp + xmym - xmyp - xpym) + (c.px * c.pz) * (xpzp + xmzm - xmzp - xpzm) +
+                          (c.py * c.pz) * (ypzp + ymzm - ymzp - ypzm));
+    r.G = c.g * (c.px * (vxp - vxm) + c.py * (vyp - vym) + c.pz * (vzp - vzm));
+    return r;
+}
+
 template <typename T>
 __device__ __forceinline__ Nbr<T> nbr(const T *__restrict__ v, const MGLevel &L, int ix, int iy, int iz,
                                       const Coef<T> &c) {
@@ -50,20 +72,11 @@ __device__ __forceinline__ Nbr<T> nbr(const T *__restrict__ v, const MGLevel &L,
     const int izp = iz + 1 == nz ? 0 : iz + 1, izm = iz == 0 ? nz - 1 : iz - 1;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
 #define V(a, b, d) v[(a) + sy * (b) + sz * (d)]
-    const T vxp = V(ixp, iy, iz), vxm = V(ixm, iy, iz);
-    const T vyp = V(ix, iyp, iz), vym = V(ix, iym, iz);
-    const T vzp = V(ix, iy, izp), vzm = V(ix, iy, izm);
-    Nbr<T> r;
-    r.A = c.gpx2 * (vxp + vxm);
-    r.B = c.gpy2 * (vyp + vym);
-    r.C = c.gpz2 * (vzp + vzm);
-    r.X = (c.g / (T)2) *
-          ((c.px * c.py) * (V(ixp, iyp, iz) + V(ixm, iym, iz) - V(ixm, iyp, iz) - V(ixp, iym, iz)) +
-           (c.px * c.pz) * (V(ixp, iy, izp) + V(ixm, iy, izm) - V(ixm, iy, izp) - V(ixp, iy, izm)) +
-           (c.py * c.pz) * (V(ix, iyp, izp) + V(ix, iym, izm) - V(ix, iym, izp) - V(ix, iyp, izm)));
-    r.G = c.g * (c.px * (vxp - vxm) + c.py * (vyp - vym) + c.pz * (vzp - vzm));
+    return nbr_of<T>(c, V(ixp, iy, iz), V(ixm, iy, iz), V(ix, iyp, iz), V(ix, iym, iz), V(ix, iy, izp),
+                     V(ix, iy, izm), V(ixp, iyp, iz), V(ixm, iym, iz), V(ixm, iyp, iz), V(ixp, iym, iz),
+                     V(ixp, iy, izp), V(ixm, iy, izm), V(ixm, iy, izp), V(ixp, iy, izm), V(ix, iyp, izp),
+                     V(ix, iym, izm), V(ix, iym, izp), V(ix, iyp, izm));
 #undef V
-    return r;
 }
 
 // Per-cell operations, shared by the grid-wide kernels below and the
@@ -117,15 +130,21 @@ __device__ __forceinline__ void cell_residual_mixed(float *__restrict__ r, const
 // colour updates in place without races.
 __device__ __forceinline__ int mg_color(int ix, int iy, int iz) { return ((ix + iy) & 1) + 2 * ((iy + iz) & 1); }
 
+// The Gauss-Seidel value of one cell: (f + neighbour sums) / diagonal.
+template <typename T>
+__device__ __forceinline__ T mcgs_value(const Coef<T> &c, T f, const Nbr<T> &s, int has_los) {
+    T j = f + s.A + s.B + s.C + s.X;
+    if (!has_los) j += s.G;
+    return j / ((T)2 * (c.gpx2 + c.gpy2 + c.gpz2));
+}
+
 template <typename T>
 __device__ __forceinline__ void cell_mcgs(T *v, const T *__restrict__ f, const MGLevel &L, int ix, int iy, int iz,
                                           float beta, int has_los) {
     const Coef<T> c = coef<T>(L, ix, iy, iz, (T)beta, has_los);
     const Nbr<T> s = nbr<T>(v, L, ix, iy, iz, c);
     const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
-    T j = f[I] + s.A + s.B + s.C + s.X;
-    if (!has_los) j += s.G;
-    v[I] = j / ((T)2 * (c.gpx2 + c.gpy2 + c.gpz2));
+    v[I] = mcgs_value<T>(c, f[I], s, has_los);
 }
 
 __global__ void k_mg_jacobi(float *__restrict__ vout, const float *__restrict__ vin,
@@ -170,6 +189,207 @@ __global__ void k_mg_mcgs(T *__restrict__ v, const T *__restrict__ f, MGLevel L,
     const int iz = blockIdx.z;
     if (ix >= L.n[0] || iy >= L.n[1] || mg_color(ix, iy, iz) != color) return;
     cell_mcgs<T>(v, f, L, ix, iy, iz, beta, has_los);
+}
+
+// One whole 4-colour Gauss-Seidel sweep in one launch, out of place: vout
+// holds what the four in-place colour passes of k_mg_mcgs leave in v (the
+// same cell arithmetic on the same values, so the same bits).
+//
+// A workgroup owns a 64 x 16 column of tz planes and marches up z with a
+// ring of kGsRing planes of old values in LDS, each 72 x 24 (a halo of 4).
+// The colour-c value of a cell depends on colour < c values of its 18
+// neighbours (new) and colour > c values (old), so the colour-3 cells of the
+// tile need colour 2 on the tile grown by 1, colour 1 grown by 2, colour 0
+// grown by 3, and old values grown by 4 (every colour is computed on the halo
+// cells too, redundantly with the neighbouring workgroups).  Step k loads
+// plane P = z0 - 4 + k and updates colour s on plane P - 1 - 2s (s = 0..3):
+// the four stages of a step read and write disjoint (plane, colour) sets, so
+// one barrier per step orders everything, and the tile of plane P - 8
+// (finished in the step before) is written out as float4 rows.
+//
+// Latency: the next plane's float4 loads and the next step's f values are
+// in flight during a step's arithmetic; the coefficients' cell positions
+// come from LDS tables.  LDS rows keep even x then odd x (36 + 36), so the
+// colour cells of a row, every second x, and their x neighbours are
+// consecutive words.
+constexpr int kGsTX = 64, kGsTY = 16, kGsHalo = 4;
+constexpr int kGsW = kGsTX + 2 * kGsHalo;      // 72 words per LDS row
+constexpr int kGsRows = kGsTY + 2 * kGsHalo;   // 24 rows per plane
+constexpr int kGsPlane = kGsW * kGsRows;
+constexpr int kGsRing = 10;                    // planes P-9 .. P
+constexpr int kGsThreads = 256;
+constexpr int kGsMaxTz = 128;
+constexpr int kGsChunks = (kGsW / 4) * kGsRows;   // float4 loads per plane (432)
+// colour cells per stage: (8 + e) rows x (32 + e) columns, e = 3 - s
+constexpr int kGsStage0 = 11 * 35, kGsStage1 = kGsStage0 + 10 * 34, kGsStage2 = kGsStage1 + 9 * 33,
+              kGsCells = kGsStage2 + 8 * 32;   // 1278
+constexpr int kGsIters = (kGsCells + kGsThreads - 1) / kGsThreads;   // 5
+
+__device__ __forceinline__ int gs_word(int xl, int yl) { return yl * kGsW + (xl & 1) * (kGsW / 2) + (xl >> 1); }
+__device__ __forceinline__ int gs_wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// Local cell (xl, yl) and local plane ql = Q - z0 + 4 of a thread's cell in
+// stage s at step k (parity par of the stage planes), from its stage row r
+// and column i.
+__device__ __forceinline__ void gs_cell(int s, int r, int i, int par, int &xl, int &yl) {
+    yl = 1 + s + 2 * r + ((((s >> 1) ^ (1 + s)) ^ par) & 1);   // rows with (y + Q) & 1 == s >> 1
+    xl = 1 + s + 2 * i + (((s & 1) ^ (1 + s + yl)) & 1);       // columns with (x + y) & 1 == s & 1
+}
+
+__global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout, const float *__restrict__ vin,
+                                                       const float *__restrict__ f, MGLevel L, float beta,
+                                                       int has_los, int tz) {
+    __shared__ __align__(16) float lds[kGsRing * kGsPlane];
+    __shared__ float ptab[kGsW + kGsRows + kGsMaxTz + 8];   // p of LDS column | row | local plane
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const int x0 = blockIdx.x * kGsTX, y0 = blockIdx.y * kGsTY, z0 = blockIdx.z * tz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int tid = threadIdx.x;
+    float *tz_tab = ptab + kGsW + kGsRows;
+    if (!has_los)
+        for (int q = tid; q < kGsW + kGsRows + tz + 8; q += kGsThreads) {
+            int g;
+            if (q < kGsW)
+                g = gs_wrap(x0 - kGsHalo + q, nx);
+            else if (q < kGsW + kGsRows)
+                g = nx + gs_wrap(y0 - kGsHalo + q - kGsW, ny);
+            else
+                g = nx + ny + gs_wrap(z0 - kGsHalo + q - kGsW - kGsRows, nz);
+            ptab[q] = L.pv[g];
+        }
+
+    // plane loads: chunk q = row * 18 + j covers x0 - 4 + 4j .. +3 of LDS row
+    // `row`; threads past the 432 chunks load chunk 0 again and do not store
+    int64_t ld_off[2];
+    int ld_word[2];
+    bool ld_ok[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int q = tid + kGsThreads * u;
+        ld_ok[u] = q < kGsChunks;
+        const int qq = ld_ok[u] ? q : 0;
+        const int row = qq / (kGsW / 4), j = qq % (kGsW / 4);
+        ld_off[u] = gs_wrap(x0 - kGsHalo + 4 * j, nx) + sy * gs_wrap(y0 - kGsHalo + row, ny);
+        ld_word[u] = row * kGsW + 2 * j;
+    }
+    float4 pre[2];
+    // unconditional (a step past the last plane reloads the last plane), so
+    // that no branch makes the compiler wait for the loads before the step's
+    // arithmetic
+    auto fetch = [&](int P) {
+        const float *src = vin + sz * gs_wrap(min(P, z0 + tz + kGsHalo - 1), nz);
+#pragma unroll
+        for (int u = 0; u < 2; u++) pre[u] = *reinterpret_cast<const float4 *>(src + ld_off[u]);
+    };
+    auto store = [&](int slot) {
+        float *pl = lds + slot * kGsPlane;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+            if (ld_ok[u]) {
+                pl[ld_word[u]] = pre[u].x;
+                pl[ld_word[u] + 1] = pre[u].z;
+                pl[ld_word[u] + kGsW / 2] = pre[u].y;
+                pl[ld_word[u] + kGsW / 2 + 1] = pre[u].w;
+            }
+    };
+    // the thread's cells: iteration it handles cell jc = tid + 256 it of the
+    // stage list; desc = s | r << 2 | i << 8, or -1
+    int desc[kGsIters];
+#pragma unroll
+    for (int it = 0; it < kGsIters; it++) {
+        const int jc = tid + kGsThreads * it;
+        int s, jj, w, magic;   // stage, index in the stage, columns, ceil(2^16 / w)
+        if (jc < kGsStage0) {
+            s = 0; jj = jc; w = 35; magic = 1873;
+        } else if (jc < kGsStage1) {
+            s = 1; jj = jc - kGsStage0; w = 34; magic = 1928;
+        } else if (jc < kGsStage2) {
+            s = 2; jj = jc - kGsStage1; w = 33; magic = 1986;
+        } else {
+            s = 3; jj = jc - kGsStage2; w = 32; magic = 2048;
+        }
+        const int r = (jj * magic) >> 16, i = jj - r * w;   // exact for jj < 400
+        desc[it] = jc < kGsCells ? (s | (r << 2) | (i << 8)) : -1;
+    }
+    // stage s updates plane Q = z0 - 5 - 2s + k, needed on z0 - 3 + s .. z0 + tz + 2 - s
+    auto active = [&](int s, int k) { return k >= 2 + 3 * s && k <= tz + 7 + s; };
+    // f of the thread's cells at step k (a valid dummy address where inactive)
+    auto fetch_f = [&](int k, float *fr) {
+        const int par = (z0 + k + 1) & 1;
+#pragma unroll
+        for (int it = 0; it < kGsIters; it++) {
+            const int d = desc[it];
+            const int s = d & 3;
+            int64_t off = 0;
+            if (d >= 0 && active(s, k)) {
+                int xl, yl;
+                gs_cell(s, (d >> 2) & 63, d >> 8, par, xl, yl);
+                off = gs_wrap(x0 - kGsHalo + xl, nx) + sy * gs_wrap(y0 - kGsHalo + yl, ny) +
+                      sz * gs_wrap(z0 - 5 - 2 * s + k, nz);
+            }
+            fr[it] = f[off];
+        }
+    };
+    // write-out: thread -> row tid / 16, x 4 (tid % 16) of the 64 x 16 tile
+    const int wo_row = tid >> 4, wo_x = 4 * (tid & 15);
+    const int64_t wo_off = (x0 + wo_x) + sy * (y0 + wo_row);
+    const int wo_word = (kGsHalo + wo_row) * kGsW + ((kGsHalo + wo_x) >> 1);
+    const int last_load = tz + 7;
+
+    auto step = [&](int k, const float *fcur, float *fnext) {
+        if (k <= last_load) store(k % kGsRing);
+        fetch(z0 - kGsHalo + k + 1);
+        __syncthreads();
+        if (k >= 12) {   // plane z0 - 12 + k, finished in step k - 1
+            const float *pl = lds + ((k - 8) % kGsRing) * kGsPlane;
+            const float2 ev = *reinterpret_cast<const float2 *>(pl + wo_word);
+            const float2 od = *reinterpret_cast<const float2 *>(pl + wo_word + kGsW / 2);
+            *reinterpret_cast<float4 *>(vout + sz * (z0 - 12 + k) + wo_off) = make_float4(ev.x, od.x, ev.y, od.y);
+        }
+        fetch_f(k + 1, fnext);
+        const int par = (z0 + k + 1) & 1;
+        const int k10 = k % kGsRing;
+#pragma unroll
+        for (int it = 0; it < kGsIters; it++) {
+            const int d = desc[it];
+            const int s = d & 3;
+            if (d < 0 || !active(s, k)) continue;
+            int xl, yl;
+            gs_cell(s, (d >> 2) & 63, d >> 8, par, xl, yl);
+            const int ql = k - 1 - 2 * s;   // local plane of Q, >= 1
+            int m = k10 + kGsRing - 1 - 2 * s;   // ring slot of Q
+            m = m >= kGsRing ? m - kGsRing : m;
+            const int mm = m == 0 ? kGsRing - 1 : m - 1, mp = m == kGsRing - 1 ? 0 : m + 1;
+            const float *pm = lds + mm * kGsPlane;
+            float *p0 = lds + m * kGsPlane;
+            const float *pp = lds + mp * kGsPlane;
+            const int wc = gs_word(xl, yl);
+            // x neighbours sit in the other parity half: x+1 at the same or next index
+            const int ox = (xl & 1) ? -(kGsW / 2) + 1 : kGsW / 2;     // word(x+1) - word(x)
+            const int oxm = (xl & 1) ? -(kGsW / 2) : kGsW / 2 - 1;    // word(x-1) - word(x)
+#define A0(o) p0[wc + (o)]
+#define AM(o) pm[wc + (o)]
+#define AP(o) pp[wc + (o)]
+            const Coef<float> c = has_los ? coef_p<float>(L, L.losn[0], L.losn[1], L.losn[2], beta)
+                                          : coef_p<float>(L, ptab[xl], ptab[kGsW + yl], tz_tab[ql], beta);
+            const Nbr<float> sn = nbr_of<float>(
+                c, A0(ox), A0(oxm), A0(kGsW), A0(-kGsW), AP(0), AM(0), A0(kGsW + ox), A0(-kGsW + oxm),
+                A0(kGsW + oxm), A0(-kGsW + ox), AP(ox), AM(oxm), AP(oxm), AM(ox), AP(kGsW), AM(-kGsW), AP(-kGsW),
+                AM(kGsW));
+#undef A0
+#undef AM
+#undef AP
+            p0[wc] = mcgs_value<float>(c, fcur[it], sn, has_los);
+        }
+    };
+
+    float fa[kGsIters], fb[kGsIters];
+    fetch(z0 - kGsHalo);
+    fetch_f(0, fa);
+    for (int k = 0; k < tz + 12; k += 2) {   // tz + 12 is even: the f buffers alternate
+        step(k, fa, fb);
+        step(k + 1, fb, fa);
+    }
 }
 
 // 27-point full weighting around fine cell 2c+1 (src/multigrid.jl:586-640).
@@ -349,7 +569,9 @@ __global__ void k_mg_xvec(float *xv, const Slot *slot, int nx, int ny, int nz) {
         int ax = idx < nx ? 0 : (idx < nx + ny ? 1 : 2);
         int i = idx - (ax == 0 ? 0 : (ax == 1 ? nx : nx + ny));
         const float h = slot->box_size[ax] / (float)n3[ax];
-        xv[idx] = (float)((double)slot->box_min[ax] + ((double)i + 0.5) * (double)h);
+        const float x = (float)((double)slot->box_min[ax] + ((double)i + 0.5) * (double)h);
+        xv[idx] = x;
+        xv[tot + idx] = x / h;
     }
 }
 
@@ -434,6 +656,27 @@ hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, 
     dim3 grid, block;
     stencil_shape(L, grid, block);
     hipLaunchKernelGGL(k_mg_residual<T>, grid, block, 0, s, r, v, f, L, beta, has_los);
+    return hipGetLastError();
+}
+
+// z planes per workgroup of k_mg_gs4: 128, or fewer (a divisor of nz, at
+// least 8) so that a launch has at least 512 workgroups, two per CU; each
+// workgroup also loads 8 halo planes and runs 12 pipeline steps beyond tz.
+static int gs4_tz(const MGLevel &L) {
+    const int64_t cols = (int64_t)(L.n[0] / kGsTX) * (L.n[1] / kGsTY);
+    int tz = kGsMaxTz;
+    while (tz > 8 && (L.n[2] % tz || cols * (L.n[2] / tz) < 512)) tz /= 2;
+    return tz;
+}
+
+bool mg_gs4_ok(const MGLevel &L) { return L.n[0] % kGsTX == 0 && L.n[1] % kGsTY == 0 && L.n[2] % 8 == 0; }
+
+hipError_t launch_mg_gs4(float *vout, const float *vin, const float *f, const MGLevel &L, float beta, int has_los,
+                         hipStream_t s) {
+    if (!mg_gs4_ok(L) || vout == vin) return hipErrorInvalidValue;
+    const int tz = gs4_tz(L);
+    const dim3 grid((unsigned)(L.n[0] / kGsTX), (unsigned)(L.n[1] / kGsTY), (unsigned)(L.n[2] / tz));
+    hipLaunchKernelGGL(k_mg_gs4, grid, dim3(kGsThreads), 0, s, vout, vin, f, L, beta, has_los, tz);
     return hipGetLastError();
 }
 

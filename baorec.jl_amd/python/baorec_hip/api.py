@@ -564,6 +564,17 @@ def jacobi_(v, f, recon: MultigridRecon):
     return v
 
 
+def gauss_seidel_(v, f, recon: MultigridRecon, sweeps=1, fused=True):
+    """The RBGS schedule's smoother (extension): `sweeps` 4-colour
+    Gauss-Seidel sweeps in place on v; fused=True runs each sweep as one
+    launch (bit for bit the per-colour launches of fused=False)."""
+    ctx = recon.context()
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_mg_gauss_seidel(ctx.handle, ctypes.byref(p), _grid(v.shape[::-1]), _mem(v, f),
+                                             _ptr(v), _ptr(_f32(f)), int(sweeps), int(bool(fused))))
+    return v
+
+
 def residual_(r, v, f, recon: MultigridRecon):
     """residual!, src/multigrid.jl:378-395."""
     ctx = recon.context()

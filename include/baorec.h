@@ -287,6 +287,15 @@ int baorec_x_vec(const int32_t grid[3], const float box_size[3],
  * The box, beta, los, damping and sweep counts come from p. */
 int baorec_mg_jacobi(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],
                      int mem, float *v, const float *f);
+/* The RBGS schedule's smoother (an extension: the reference smooths with
+ * jacobi!, src/multigrid.jl:193-210): `sweeps` multicolour Gauss-Seidel
+ * sweeps in place on v, the colours ((ix+iy)&1) + 2((iy+iz)&1) in order 0..3,
+ * each cell v = (f + neighbour sums of src/multigrid.jl:170-185) / diagonal.
+ * fused = 0: one launch per colour; fused = 1: one launch per sweep (sides
+ * multiples of 64, 16 and 8, else BAOREC_ERR_UNSUPPORTED), bit for bit the
+ * same result.  Sides must be even. */
+int baorec_mg_gauss_seidel(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],
+                           int mem, float *v, const float *f, int sweeps, int fused);
 int baorec_mg_residual(baorec_ctx *ctx, const baorec_params *p,
                        const int32_t grid[3], int mem, float *r, const float *v,
                        const float *f);

@@ -398,6 +398,33 @@ def mg_residual(v, f, box_size, box_min, beta, los=None):
     return r
 
 
+def mg_colors(shape):
+    """Colour ((ix+iy)&1) + 2((iy+iz)&1) of every cell of an (nz, ny, nx) grid:
+    the RBGS schedule's smoother order (an extension; the reference smooths
+    with jacobi!, src/multigrid.jl:193-210)."""
+    nz, ny, nx = shape
+    iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return ((ix + iy) & 1) + 2 * ((iy + iz) & 1)
+
+
+def mg_gauss_seidel(v, f, box_size, box_min, beta, los=None, sweeps=1):
+    """Multicolour Gauss-Seidel sweeps (in place on v) stated through the
+    residual: v_i <- (f_i + sum_j a_ij v_j) / d_i = v_i + r_i / d_i with
+    r = f - A v of residual! (src/multigrid.jl:213-326), one colour at a time
+    (no two cells of a colour are stencil neighbours, so a colour's updates
+    are independent).  d_i, the operator's diagonal, is -r_i of the colour's
+    indicator with f = 0: its neighbours carry no weight there."""
+    col = mg_colors(v.shape)
+    zero = np.zeros_like(v)
+    diag = [-mg_residual((col == c).astype(v.dtype), zero, box_size, box_min, beta, los) for c in range(4)]
+    for _ in range(sweeps):
+        for c in range(4):
+            m = col == c
+            r = mg_residual(v, f, box_size, box_min, beta, los)
+            v[m] += r[m] / diag[c][m]
+    return v
+
+
 def mg_prolong(v2h: np.ndarray) -> np.ndarray:
     """prolong!, src/multigrid.jl:398-453 (fresh fine array)."""
     cz, cy, cx = v2h.shape
@@ -585,10 +612,27 @@ def comoving_distance(c: Cosmology, z) -> np.ndarray:
                                   epsrel=1e-10, limit=200)[0] for zi in z])
 
 
+_TABLES = {}
+
+
 def distance_table(c: Cosmology):
-    """Knots and distances of comoving_distance_interp (src/cosmo.jl:88-93)."""
-    zk = np.linspace(c.z_tab_min, c.z_tab_max, c.z_tab_num)
-    return zk, comoving_distance(c, zk)
+    """Knots and distances of comoving_distance_interp (src/cosmo.jl:88-93):
+    the integral of 1/H from 0 to the first knot by comoving_distance, then
+    knot interval by knot interval with an 8-point Gauss-Legendre rule,
+    summed in order (1e5 adaptive quadratures would take minutes).  Cached
+    per cosmology; the arrays are read-only."""
+    import dataclasses
+    key = dataclasses.astuple(c)
+    if key not in _TABLES:
+        zk = np.linspace(c.z_tab_min, c.z_tab_max, c.z_tab_num)
+        xg, wg = np.polynomial.legendre.leggauss(8)
+        mid, half = 0.5 * (zk[1:] + zk[:-1]), 0.5 * (zk[1:] - zk[:-1])
+        seg = C_KMS * half * (inv_hubble(c, mid[:, None] + half[:, None] * xg[None, :]) @ wg)
+        dist = comoving_distance(c, zk[:1])[0] + np.concatenate([[0.0], np.cumsum(seg)])
+        zk.setflags(write=False)
+        dist.setflags(write=False)
+        _TABLES[key] = (zk, dist)
+    return _TABLES[key]
 
 
 def _interp(xk, yk, x):

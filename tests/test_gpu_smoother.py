@@ -1,0 +1,62 @@
+"""The RBGS schedule's smoother on the GPU: the one-launch 4-colour sweep
+(k_mg_gs4) against the per-colour launches (bit for bit) and against the
+oracle's multicolour Gauss-Seidel (FP64, within FP32 rounding)."""
+import numpy as np
+import pytest
+
+import baorec_hip as bh
+import baorec_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+F32 = np.float32
+
+
+def recon(shape, los, ctx):
+    nz, ny, nx = shape
+    return bh.MultigridRecon(bias=2.0, f=0.8, smoothing_radius=15.0, box_size=[4.0 * nx, 4.0 * ny, 4.0 * nz],
+                             box_min=[200.0, -300.0, 500.0], los=los, ctx=ctx)
+
+
+def fields(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape).astype(F32), rng.standard_normal(shape).astype(F32)
+
+
+@pytest.mark.parametrize("shape,los,sweeps", [
+    ((8, 16, 64), None, 1),
+    ((24, 32, 64), [0.0, 0.0, 1.0], 2),
+    ((40, 48, 192), None, 3),
+    ((256, 256, 256), None, 2),
+    ((512, 512, 512), None, 1),
+])
+def test_fused_sweep_bitwise_per_colour(ctx, shape, los, sweeps):
+    """Shapes: one workgroup column, halos wrapping onto the tile itself
+    (64 x 16), non-cubic sides, and the solver's 256^3 and 512^3 levels."""
+    v, f = fields(shape, sum(shape))
+    rec = recon(shape, los, ctx)
+    a = bh.gauss_seidel_(v.copy(), f, rec, sweeps=sweeps, fused=True)
+    b = bh.gauss_seidel_(v.copy(), f, rec, sweeps=sweeps, fused=False)
+    assert np.all(np.isfinite(a))
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("los", [None, [0.3, -0.4, 0.866]])
+def test_fused_sweep_matches_oracle(ctx, los):
+    shape = (32, 32, 128)
+    v, f = fields(shape, 5)
+    rec = recon(shape, los, ctx)
+    got = bh.gauss_seidel_(v.copy(), f, rec, sweeps=2, fused=True)
+    exp = orc.mg_gauss_seidel(v.astype(np.float64), f.astype(np.float64), rec.box_size, rec.box_min, rec.beta,
+                              los=los, sweeps=2)
+    err = float(np.linalg.norm(got - exp) / np.linalg.norm(exp))
+    print(f"los {los}: rel L2 vs FP64 oracle {err:.2e}, max abs {np.max(np.abs(got - exp)):.2e}")
+    assert err < 2e-6
+
+
+def test_fused_sweep_rejects_unsupported_shape(ctx):
+    shape = (8, 16, 48)   # x side not a multiple of 64
+    v, f = fields(shape, 1)
+    with pytest.raises(bh.BaorecError):
+        bh.gauss_seidel_(v, f, recon(shape, None, ctx), fused=True)
+    bh.gauss_seidel_(v, f, recon(shape, None, ctx), fused=False)

@@ -576,6 +576,52 @@ def test_local_los_operator_left_null_vector():
     assert abs(shift - alpha) < 1e-12
 
 
+def textbook_gauss_seidel(M, v, f, order):
+    """Forward Gauss-Seidel x <- (D + L)^-1 (f - U x) on the system permuted to
+    the cell order `order` (D, L, U: diagonal, strictly lower and upper parts)."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import spsolve_triangular
+    Mp = M[order][:, order].tocsr()
+    x = v.ravel()[order]
+    rhs = f.ravel()[order] - sp.triu(Mp, k=1) @ x
+    x = spsolve_triangular(sp.tril(Mp, k=0).tocsr(), rhs, lower=True)
+    out = np.empty_like(x)
+    out[order] = x
+    return out.reshape(v.shape)
+
+
+@pytest.mark.parametrize("los", [None, [0.0, 0.0, 1.0]])
+def test_mg_gauss_seidel_is_textbook_colour_ordered_gauss_seidel(los):
+    """One sweep of the oracle's multicolour Gauss-Seidel equals forward
+    Gauss-Seidel on the operator matrix with the cells ordered by colour (any
+    order inside a colour: no two cells of a colour are coupled)."""
+    n, L, bm, beta = 8, 1000.0, [200.0, 300.0, 100.0], 0.375
+    M = operator_matrix(n, L, bm, beta, los=los)
+    col = orc.mg_colors((n, n, n)).ravel()
+    for c in range(4):   # the colouring decouples each colour
+        sel = np.flatnonzero(col == c)
+        blk = M[sel][:, sel]
+        assert abs(blk - sp_diag(blk)).max() == 0.0
+    rng = np.random.default_rng(4)
+    v, f = rng.standard_normal((n, n, n)), rng.standard_normal((n, n, n))
+    got = orc.mg_gauss_seidel(v.copy(), f, [L] * 3, bm, beta, los=los)
+    order = np.concatenate([rng.permutation(np.flatnonzero(col == c)) for c in range(4)])
+    np.testing.assert_allclose(got, textbook_gauss_seidel(M, v, f, order), rtol=1e-12, atol=1e-12)
+
+
+def sp_diag(m):
+    import scipy.sparse as sp
+    return sp.diags(m.diagonal())
+
+
+def test_mg_gauss_seidel_check_catches_planted_colouring(monkeypatch):
+    """A 2-colour (red-black) ordering leaves edge neighbours in one colour
+    and fails the textbook comparison."""
+    monkeypatch.setattr(orc, "mg_colors", lambda shape: np.indices(shape).sum(axis=0) % 2 * 1)
+    with pytest.raises(AssertionError):
+        test_mg_gauss_seidel_is_textbook_colour_ordered_gauss_seidel(None)
+
+
 # ------------------------------------------------------------------ planted mistake
 
 def test_planted_sign_error_is_caught(monkeypatch):
@@ -614,6 +660,18 @@ def test_comoving_distance_einstein_de_sitter():
     z = np.array([0.1, 0.5, 1.0, 2.0, 3.0])
     exact = 2 * orc.C_KMS / (100 * h) * (1 - 1 / np.sqrt(1 + z))
     np.testing.assert_allclose(orc.comoving_distance(c, z), exact, rtol=2e-6)
+
+
+@pytest.mark.parametrize("zmin,zmax,num", [(0.0, 3.0, 100000), (0.1, 10.0, 2001), (0.0, 1.0, 2)])
+def test_distance_table_matches_adaptive_quadrature(zmin, zmax, num):
+    """The interval-by-interval table against an adaptive quadrature of the
+    whole integral at knots spread over the table."""
+    c = orc.Cosmology(z_tab_min=zmin, z_tab_max=zmax, z_tab_num=num)
+    zk, dk = orc.distance_table(c)
+    assert len(zk) == num and zk[0] == zmin and zk[-1] == zmax
+    idx = np.unique(np.linspace(0, num - 1, 40).astype(int))
+    np.testing.assert_allclose(dk[idx], orc.comoving_distance(c, zk[idx]), rtol=1e-10, atol=1e-9)
+    assert orc.distance_table(c) is orc.distance_table(c)
 
 
 def test_sky_cartesian_round_trip_and_geometry():
