@@ -864,14 +864,31 @@ float *vother(MGPyr &P, int l) { return P.cur[l] ? P.A[l] : P.B[l]; }
 double *vcurd(MGPyr &P, int l) { return P.cur[l] ? P.Bd[l] : P.Ad[l]; }
 double *votherd(MGPyr &P, int l) { return P.cur[l] ? P.Ad[l] : P.Bd[l]; }
 
-void mg_jacobi_n(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int n) {
-    for (int i = 0; i < n; i++) {
-        ProfScope ps(c, "mg_jacobi", 12.0 * P.g[l].R());
-        hipchk(c, launch_mg_jacobi(vother(P, l), vcur(P, l), P.F[l], P.L[l], p->beta,
-                                   p->jacobi_damping_factor, p->has_los, c->stream),
-               "jacobi");
-        P.cur[l] ^= 1;
+// n damped Jacobi sweeps (jacobi!, src/multigrid.jl:193-210) ping-ponging
+// between a (the input) and b; returns the one holding the result.  On
+// grids of at least 128^3 cells whose sides are multiples of 64, 16 and 8,
+// sweeps run 3 (or 2) per launch, temporally blocked (the same bits).
+static float *jacobi_sweeps(baorec_ctx *c, const MGLevel &L, int64_t R, float *a, float *b, const float *f,
+                            const baorec_params *p, int n) {
+    const bool tb = mg_jacobi_tb_ok(L) && R >= ((int64_t)1 << 21);
+    for (int left = n; left > 0;) {
+        const int k = !tb || left == 1 ? 1 : (left == 2 || left == 4 ? 2 : 3);
+        ProfScope ps(c, k == 1 ? "mg_jacobi" : "mg_jacobi_tb", 12.0 * R);
+        if (k == 1)
+            hipchk(c, launch_mg_jacobi(b, a, f, L, p->beta, p->jacobi_damping_factor, p->has_los, c->stream),
+                   "jacobi");
+        else
+            hipchk(c, launch_mg_jacobi_tb(b, a, f, L, p->beta, p->jacobi_damping_factor, p->has_los, k, c->stream),
+                   "jacobi (blocked)");
+        std::swap(a, b);
+        left -= k;
     }
+    return a;
+}
+
+void mg_jacobi_n(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int n) {
+    float *res = jacobi_sweeps(c, P.L[l], P.g[l].R(), vcur(P, l), vother(P, l), P.F[l], p, n);
+    if (res != vcur(P, l)) P.cur[l] ^= 1;
 }
 
 void zero_level(baorec_ctx *c, MGPyr &P, int l) {
@@ -966,13 +983,20 @@ void mg_fmg_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
 }
 
 // Multicolour Gauss-Seidel V-cycle (RBGS schedule) with FP64 coarse levels.
-void mg_smooth_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int sweeps) {
-    // whole sweeps in one launch each, ping-pong between A and B; below 128^3
-    // cells the per-colour launches are as fast
-    if (!P.dbl[l] && mg_gs4_ok(P.L[l]) && P.g[l].R() >= ((int64_t)1 << 21)) {
+// whole sweeps in one launch each (ping-pong between A and B) on levels of
+// at least 128^3 cells; below, the per-colour launches are as fast
+static bool mg_use_gs4(const MGPyr &P, int l) {
+    return !P.dbl[l] && mg_gs4_ok(P.L[l]) && P.g[l].R() >= ((int64_t)1 << 21);
+}
+
+// sum_part (may be NULL; one-launch sweeps only): the last sweep leaves the
+// per-workgroup sums of the result there (mg_gs4_blocks partials)
+void mg_smooth_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int sweeps, double *sum_part = nullptr) {
+    if (mg_use_gs4(P, l)) {
         for (int s = 0; s < sweeps; s++) {
             ProfScope ps(c, "mg_gs4", 12.0 * P.g[l].R());
-            hipchk(c, launch_mg_gs4(vother(P, l), vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
+            hipchk(c, launch_mg_gs4(vother(P, l), vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream,
+                                    s + 1 == sweeps ? sum_part : nullptr),
                    "gs4");
             P.cur[l] ^= 1;
         }
@@ -992,7 +1016,8 @@ void mg_smooth_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int sw
         }
 }
 
-void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
+// sum_part: as mg_smooth_rb's, for the post-smoothing of level l
+void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, double *sum_part = nullptr) {
     if (l == P.lc) {
         if (P.dbl[l])
             mg_coarse<double>(c, P, p, 0, 1);
@@ -1041,7 +1066,7 @@ void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
         else
             hipchk(c, launch_mg_prolong(vcur(P, l), vcur(P, l + 1), cn, 1, c->stream), "prolong");
     }
-    mg_smooth_rb(c, P, l, p, coarsest ? 16 * p->rb_sweeps : p->rb_sweeps);
+    mg_smooth_rb(c, P, l, p, coarsest ? 16 * p->rb_sweeps : p->rb_sweeps, sum_part);
 }
 
 // Spectral seed of the RBGS solve: the exact inverse of the discrete
@@ -1050,7 +1075,7 @@ void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
 // from the origin (the operator's first-derivative term, which falls off as
 // 1/|x|, is left out).  The k = 0 mode is zero.
 void mg_spectral_seed(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const MGLevel &L0,
-                      const float *f, float *out) {
+                      const float *f, float *out, bool accumulate) {
     double d[3];
     if (p->has_los) {
         for (int a = 0; a < 3; a++) d[a] = p->los[a];
@@ -1094,6 +1119,7 @@ void mg_spectral_seed(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot
     a.kxpow[0] = 0;
     a.nin = 1;
     a.out = out;
+    a.accumulate = accumulate ? 1 : 0;
     x_inv(c, G, a, XE_PLAIN, 1, "x multigrid seed");
 }
 
@@ -1143,9 +1169,9 @@ static MGLevel level_with_xv(baorec_ctx *c, const Grid &G, const baorec_params *
 // psi = A_c^-1 r_s separately.
 static void spectral_refined(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const MGLevel &L,
                              const float *f, float *phis, float *rs, float *psi) {
-    mg_spectral_seed(c, G, p, slot, L, f, phis);
+    mg_spectral_seed(c, G, p, slot, L, f, phis, false);
     hipchk(c, launch_mg_residual(rs, phis, f, L, p->beta, p->has_los, c->stream), "seed residual");
-    mg_spectral_seed(c, G, p, slot, L, rs, psi);
+    mg_spectral_seed(c, G, p, slot, L, rs, psi, false);
 }
 
 static int ilog2i(int n) {
@@ -1203,7 +1229,7 @@ static void need_net(baorec_ctx *c) {
 static void learned_seed(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, const MGLevel &L0,
                          const float *f0, float *seed) {
     need_net(c);
-    mg_spectral_seed(c, G, p, slot, L0, f0, seed);
+    mg_spectral_seed(c, G, p, slot, L0, f0, seed, false);
     // the level of the network
     const float ht = unet_h_train(c->net);
     int lev = 0;
@@ -1292,7 +1318,41 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     double *phi = bufT<double>(c, "mg_phi64", N);   // FP64 solution
     hipchk(c, hipMemsetAsync(phi, 0, sizeof(double) * N, c->stream), "memset");
     MGPyr P = mg_build(c, G, p, slot, v0, rb[0], p->fp64_coarse_max);
-    auto residual = [&](float *r) {
+    // With sides that are multiples of 64, 16 and 8 the refinement step
+    // (phi += e - mean(e), r = f - alpha - A phi, sum r) is one z-marching
+    // pass (k_mg_refine) and the shift with both norms another; otherwise
+    // one launch per operation.
+    const bool fused = mg_refine_ok(P.L[0]);
+    double *phib = fused ? bufT<double>(c, "mg_phi64b", N) : nullptr;
+    double *rpart = fused ? bufT<double>(c, "mg_refine_part", mg_refine_blocks(P.L[0])) : nullptr;
+    double *npart = bufT<double>(c, "mg_norm_part", 2048);
+    // the V-cycle's last sweep on level 0 sums the correction e (unless
+    // level 0 runs inside mg_coarse or by per-colour launches)
+    double *epart = P.lc != 0 && mg_use_gs4(P, 0) ? bufT<double>(c, "mg_gs4_part", mg_gs4_blocks(P.L[0])) : nullptr;
+    // phi += e - mean(e) (e != NULL), r = residual, alpha += mean(r) and
+    // r -= mean(r) (local LOS), norms[0] = |r|^2 and, with rprev, norms[1] =
+    // |r - rprev|^2
+    auto residual = [&](float *r, const float *e, const float *rprev, double *norms) {
+        if (fused) {
+            {
+                ProfScope ps(c, "mg_refine", (e ? 28.0 : 16.0) * N);
+                hipchk(c, launch_mg_refine(phib, phi, e, red + 4, f0, red, r, rpart, P.L[0], p->beta, p->has_los,
+                                           c->stream), "refine");
+            }
+            if (e) std::swap(phi, phib);
+            ProfScope ps(c, "mg_shift_norms", (p->has_los ? 4.0 : 8.0) * N + (rprev ? 4.0 : 0.0) * N);
+            if (!p->has_los) {
+                hipchk(c, launch_sum_partials(rpart, mg_refine_blocks(P.L[0]), 1, red + 6, 1, c->stream), "sum r");
+                hipchk(c, launch_accumulate(red, red + 6, c->stream), "shift");
+            }
+            hipchk(c, launch_shift_norms(r, rprev, N, p->has_los ? nullptr : red + 6, norms, npart, c->stream),
+                   "shift, norms");
+            return;
+        }
+        if (e) {
+            ProfScope ps(c, "mg_correct", 20.0 * N);
+            hipchk(c, launch_add_correction<double>(phi, e, N, red + 4, c->stream), "v += e");
+        }
         {
             ProfScope ps(c, "mg_residual_fp64", 16.0 * N);
             hipchk(c, launch_mg_residual_mixed(r, phi, f0, red, (double)N, P.L[0], p->beta, p->has_los,
@@ -1304,6 +1364,8 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
             hipchk(c, launch_accumulate(red, red + 6, c->stream), "shift");
             hipchk(c, launch_add_const<float>(r, N, red + 6, (double)N, -1.0, c->stream), "r - mean");
         }
+        hipchk(c, launch_sumsq<float>(r, N, norms, red_part(c), c->stream), "norm r");
+        if (rprev) hipchk(c, launch_sumsq_diff<float>(r, rprev, N, norms + 1, red_part(c), c->stream), "norm dr");
     };
     if (p->mg_warm_start) {
         if (p->mg_warm_start < 1 || p->mg_warm_start > 3)
@@ -1312,27 +1374,24 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
             fail(c, BAOREC_ERR_UNSUPPORTED, "the warm starts need grid sides that are powers of two in [16, 1024]");
         float *seed = bufT<float>(c, "mg_seed", N);
         if (p->mg_warm_start == 1) {
-            mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed);
+            mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed, false);
         } else {
             if (p->mg_seed_steps < 0 || p->mg_seed_steps > 16)
                 fail(c, BAOREC_ERR_INVALID_ARGUMENT, "mg_seed_steps must be in [0, 16]");
             if (p->mg_warm_start == 3)
-                mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed);
+                mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed, false);
             else
                 learned_seed(c, G, p, slot, P.L[0], f0, seed);
             // spectral Richardson steps: seed += A_c^-1 (f - A seed)
             float *rs = bufT<float>(c, "mg_seed_r", N);
-            float *psi = bufT<float>(c, "mg_seed_psi", N);
             for (int k = 0; k < p->mg_seed_steps; k++) {
                 hipchk(c, launch_mg_residual(rs, seed, f0, P.L[0], p->beta, p->has_los, c->stream), "seed residual");
-                mg_spectral_seed(c, G, p, slot, P.L[0], rs, psi);
-                hipchk(c, launch_add_grid(seed, psi, N, c->stream), "seed step");
+                mg_spectral_seed(c, G, p, slot, P.L[0], rs, seed, true);   // seed += A_c^-1 rs
             }
         }
         hipchk(c, launch_convert<double, float>(phi, seed, N, c->stream), "seed to FP64");
     }
-    residual(rb[0]);   // of the start: f minus its mean for v = 0
-    hipchk(c, launch_sumsq<float>(rb[0], N, red + 5, red_part(c), c->stream), "norm r0");
+    residual(rb[0], nullptr, nullptr, red + 5);   // of the start (f minus its mean for v = 0)
     double h[8];
     hipchk(c, hipMemcpyAsync(h, red, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
     sync(c);
@@ -1349,20 +1408,15 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
         float *r = rb[cyc & 1], *rn = rb[(cyc + 1) & 1];
         P.F[0] = r;
         zero_level(c, P, 0);
-        mg_vcycle_rb(c, P, 0, p);
+        mg_vcycle_rb(c, P, 0, p, epart);
         // v += e - mean(e): v stays zero-mean (A annihilates constants), so an
         // inconsistent f does not make it drift by a constant every cycle
-        hipchk(c, hipMemsetAsync(red + 4, 0, sizeof(double), c->stream), "memset");
-        hipchk(c, launch_sum<float>(vcur(P, 0), N, red + 4, red_part(c), c->stream), "sum e");
-        {
-            ProfScope ps(c, "mg_correct", 20.0 * N);
-            hipchk(c, launch_add_correction<double>(phi, vcur(P, 0), N, red + 4, c->stream), "v += e");
-        }
+        if (epart)   // the last sweep's partial sums
+            hipchk(c, launch_sum_partials(epart, mg_gs4_blocks(P.L[0]), 1, red + 4, 1, c->stream), "sum e");
+        else
+            hipchk(c, launch_sum<float>(vcur(P, 0), N, red + 4, red_part(c), c->stream), "sum e");
         cyc++;
-        residual(rn);
-        hipchk(c, hipMemsetAsync(red + 2, 0, 2 * sizeof(double), c->stream), "memset");
-        hipchk(c, launch_sumsq<float>(rn, N, red + 2, red_part(c), c->stream), "norm r");
-        hipchk(c, launch_sumsq_diff<float>(rn, r, N, red + 3, red_part(c), c->stream), "norm dr");
+        residual(rn, vcur(P, 0), r, red + 2);
         hipchk(c, hipMemcpyAsync(h + 2, red + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
         sync(c);
         rel = fn > 0 ? sqrt(h[2]) / fn : 0.0;
@@ -2334,12 +2388,7 @@ int baorec_mg_jacobi(baorec_ctx *c, const baorec_params *p, const int32_t grid[3
         const float *df = stage_in(c, mem, f, G.R(), "mgp_f");
         float *tmp = bufT<float>(c, "mgp_t", G.R());
         MGLevel L = level_for(c, G, p, slot);
-        float *a = dv, *b = tmp;
-        for (int i = 0; i < p->jacobi_niterations; i++) {
-            hipchk(c, launch_mg_jacobi(b, a, df, L, p->beta, p->jacobi_damping_factor, p->has_los, c->stream),
-                   "jacobi");
-            std::swap(a, b);
-        }
+        float *a = jacobi_sweeps(c, L, G.R(), dv, tmp, df, p, p->jacobi_niterations);
         if (a != dv)
             hipchk(c, hipMemcpyAsync(dv, a, sizeof(float) * G.R(), hipMemcpyDeviceToDevice, c->stream), "copy");
         copy_out(c, mem, v, dv, G.R());

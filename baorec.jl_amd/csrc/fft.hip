@@ -379,8 +379,16 @@ k_x_c2r(XInvArgs a) {
     }
     if (valid && a.out) {
         float2 *o2 = reinterpret_cast<float2 *>(a.out + rbase);
+        if (EPI == XE_PLAIN && a.accumulate) {
 #pragma unroll
-        for (int i = 0; i < 8; i++) o2[PI::out_pos(t, i)] = acc[i];
+            for (int i = 0; i < 8; i++) {
+                const float2 o = o2[PI::out_pos(t, i)];
+                o2[PI::out_pos(t, i)] = make_float2(o.x + acc[i].x, o.y + acc[i].y);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) o2[PI::out_pos(t, i)] = acc[i];
+        }
     }
     if constexpr (EPI != XE_DISP) {
         if (a.fwd_out) {

@@ -95,6 +95,7 @@ struct XInvArgs {
     float ran_min;
     float2 *fwd_out;      // optional: R2C of the result line written here (pitch P),
                           // so the next transform skips reading the real grid
+    int accumulate;       // XE_PLAIN: out += result instead of out = result
 };
 
 struct XFwdArgs {
@@ -313,12 +314,19 @@ hipError_t launch_mg_xvec(float *xv, const Slot *slot, int nx, int ny, int nz, h
 template <typename T>
 hipError_t launch_mg_rbgs(T *v, const T *f, const MGLevel &L, float beta, int has_los, int color,
                           hipStream_t s);
+// 2 or 3 damped Jacobi sweeps in one launch, out of place (vout != vin),
+// bitwise that many launch_mg_jacobi calls; sides multiples of 64, 16, 8.
+bool mg_jacobi_tb_ok(const MGLevel &L);
+hipError_t launch_mg_jacobi_tb(float *vout, const float *vin, const float *f, const MGLevel &L, float beta,
+                               float omega, int has_los, int sweeps, hipStream_t s);
 // One whole 4-colour sweep out of place (vout != vin), bitwise the four
 // in-place launch_mg_rbgs colour passes; for levels with mg_gs4_ok (sides
 // multiples of 64, 16 and 8).
+// part (may be NULL): mg_gs4_blocks(L) FP64 partial sums of vout, fixed order.
 bool mg_gs4_ok(const MGLevel &L);
+int64_t mg_gs4_blocks(const MGLevel &L);
 hipError_t launch_mg_gs4(float *vout, const float *vin, const float *f, const MGLevel &L, float beta, int has_los,
-                         hipStream_t s);
+                         hipStream_t s, double *part = nullptr);
 template <typename T>
 hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, float beta,
                                 int has_los, hipStream_t s);
@@ -349,6 +357,20 @@ hipError_t launch_mg_residual_mixed(float *r, const double *v, const float *f, c
 // phi += e - esum/n (sum in FP64, stored in phi's type)
 template <typename T>
 hipError_t launch_add_correction(T *phi, const float *e, int64_t n, const double *esum, hipStream_t s);
+// The FP64 refinement pass of the RBGS solve in one launch: phi_out = phi_in
+// + (e - sum(e)/N) (e may be NULL: phi_in as is, phi_out unused), r =
+// (float)((f - *fsum/N) - A phi_out), and the per-workgroup FP64 sums of r in
+// part[mg_refine_blocks(L)] (fixed order).  Sides multiples of 64, 16, 8.
+bool mg_refine_ok(const MGLevel &L);
+int64_t mg_refine_blocks(const MGLevel &L);
+hipError_t launch_mg_refine(double *phi_out, const double *phi_in, const float *e, const double *esum, const float *f,
+                            const double *fsum, float *r, double *part, const MGLevel &L, float beta, int has_los,
+                            hipStream_t s);
+// r += (float)(-*rsum / n) (rsum may be NULL), then out[0] = sum r^2 and,
+// with rprev, out[1] = sum (r - rprev)^2, FP64 in a fixed order; part holds
+// 2 x 1024 partials.
+hipError_t launch_shift_norms(float *r, const float *rprev, int64_t n, const double *rsum, double *out,
+                              double *part, hipStream_t s);
 
 // ------------------------------------------------------------------ U-Net (unet.hip)
 // Tensors are BF16 (stored as uint16_t), channels last [batch][z][y][x][c];

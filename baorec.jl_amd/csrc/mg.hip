@@ -16,16 +16,27 @@ struct Coef {
     T px, py, pz, g, gpx2, gpy2, gpz2;
 };
 
+// Reciprocal for the Gauss-Seidel smoother's FP32 arithmetic: v_rcp_f32
+// (1 ulp) instead of an IEEE division.  The smoother only approximates the
+// solve inside the FP64-refined iteration, so its rounding does not reach
+// the converged potential; the reference-ordered kernels (Jacobi, residual)
+// keep IEEE divisions.
+template <typename T, bool FAST>
+__device__ __forceinline__ T mg_div(T a, T b) {
+    if constexpr (FAST && std::is_same<T, float>::value) return a * __builtin_amdgcn_rcpf(b);
+    return a / b;
+}
+
 // src/multigrid.jl:161-168 (per-cell coefficients) from the LOS components
 // p = (px, py, pz) in cell units
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ Coef<T> coef_p(const MGLevel &L, T px, T py, T pz, T beta) {
     Coef<T> c;
     c.px = px;
     c.py = py;
     c.pz = pz;
-    c.g = beta / ((T)L.cell2[0] * (c.px * c.px) + (T)L.cell2[1] * (c.py * c.py) +
-                  (T)L.cell2[2] * (c.pz * c.pz));
+    c.g = mg_div<T, FAST>(beta, (T)L.cell2[0] * (c.px * c.px) + (T)L.cell2[1] * (c.py * c.py) +
+                                    (T)L.cell2[2] * (c.pz * c.pz));
     c.gpx2 = (T)L.icell2[0] + c.g * (c.px * c.px);
     c.gpy2 = (T)L.icell2[1] + c.g * (c.py * c.py);
     c.gpz2 = (T)L.icell2[2] + c.g * (c.pz * c.pz);
@@ -33,13 +44,13 @@ __device__ __forceinline__ Coef<T> coef_p(const MGLevel &L, T px, T py, T pz, T 
 }
 
 // p is the fixed LOS, or the cell centre / cell size (local LOS)
-template <typename T>
+template <typename T, bool FAST = false>
 __device__ __forceinline__ Coef<T> coef(const MGLevel &L, int ix, int iy, int iz, T beta, int has_los) {
-    if (has_los) return coef_p<T>(L, (T)L.losn[0], (T)L.losn[1], (T)L.losn[2], beta);
+    if (has_los) return coef_p<T, FAST>(L, (T)L.losn[0], (T)L.losn[1], (T)L.losn[2], beta);
     if constexpr (std::is_same<T, float>::value)   // xv / cell, precomputed per level
-        return coef_p<T>(L, L.pv[ix], L.pv[L.n[0] + iy], L.pv[L.n[0] + L.n[1] + iz], beta);
-    return coef_p<T>(L, (T)L.xv[ix] / (T)L.cell[0], (T)L.xv[L.n[0] + iy] / (T)L.cell[1],
-                     (T)L.xv[L.n[0] + L.n[1] + iz] / (T)L.cell[2], beta);
+        return coef_p<T, FAST>(L, L.pv[ix], L.pv[L.n[0] + iy], L.pv[L.n[0] + L.n[1] + iz], beta);
+    return coef_p<T, FAST>(L, (T)L.xv[ix] / (T)L.cell[0], (T)L.xv[L.n[0] + iy] / (T)L.cell[1],
+                           (T)L.xv[L.n[0] + L.n[1] + iz] / (T)L.cell[2], beta);
 }
 
 template <typename T>
@@ -83,17 +94,23 @@ __device__ __forceinline__ Nbr<T> nbr(const T *__restrict__ v, const MGLevel &L,
 // single-workgroup coarse-level kernel (same arithmetic in both).
 
 // One damped Jacobi update (src/multigrid.jl:137-191 and the broadcast at
-// :207), out of place: vout = (1 - w) vin + w jac(vin).
+// :207): (1 - w) v + w jac(v) from the cell's f, neighbour sums and value.
+__device__ __forceinline__ float jacobi_value(const Coef<float> &c, float f, const Nbr<float> &s, float v,
+                                             float omega, int has_los) {
+    float j = f + s.A + s.B + s.C + s.X;
+    if (!has_los) j += s.G;
+    j /= 2.0f * (c.gpx2 + c.gpy2 + c.gpz2);
+    return (1.0f - omega) * v + omega * j;
+}
+
+// out of place: vout = (1 - w) vin + w jac(vin)
 __device__ __forceinline__ void cell_jacobi(float *__restrict__ vout, const float *__restrict__ vin,
                                             const float *__restrict__ f, const MGLevel &L, int ix, int iy,
                                             int iz, float beta, float omega, int has_los) {
     const Coef<float> c = coef<float>(L, ix, iy, iz, beta, has_los);
     const Nbr<float> s = nbr<float>(vin, L, ix, iy, iz, c);
     const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
-    float j = f[I] + s.A + s.B + s.C + s.X;
-    if (!has_los) j += s.G;
-    j /= 2.0f * (c.gpx2 + c.gpy2 + c.gpz2);
-    vout[I] = (1.0f - omega) * vin[I] + omega * j;
+    vout[I] = jacobi_value(c, f[I], s, vin[I], omega, has_los);
 }
 
 // r = f - A v (src/multigrid.jl:328-376 and the broadcast at :393).
@@ -130,18 +147,20 @@ __device__ __forceinline__ void cell_residual_mixed(float *__restrict__ r, const
 // colour updates in place without races.
 __device__ __forceinline__ int mg_color(int ix, int iy, int iz) { return ((ix + iy) & 1) + 2 * ((iy + iz) & 1); }
 
-// The Gauss-Seidel value of one cell: (f + neighbour sums) / diagonal.
+// The Gauss-Seidel value of one cell: (f + neighbour sums) / diagonal, with
+// the smoother's reciprocal (mg_div<T, true>); its coefficients come from
+// coef<T, true>.
 template <typename T>
 __device__ __forceinline__ T mcgs_value(const Coef<T> &c, T f, const Nbr<T> &s, int has_los) {
     T j = f + s.A + s.B + s.C + s.X;
     if (!has_los) j += s.G;
-    return j / ((T)2 * (c.gpx2 + c.gpy2 + c.gpz2));
+    return mg_div<T, true>(j, (T)2 * (c.gpx2 + c.gpy2 + c.gpz2));
 }
 
 template <typename T>
 __device__ __forceinline__ void cell_mcgs(T *v, const T *__restrict__ f, const MGLevel &L, int ix, int iy, int iz,
                                           float beta, int has_los) {
-    const Coef<T> c = coef<T>(L, ix, iy, iz, (T)beta, has_los);
+    const Coef<T> c = coef<T, true>(L, ix, iy, iz, (T)beta, has_los);
     const Nbr<T> s = nbr<T>(v, L, ix, iy, iz, c);
     const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
     v[I] = mcgs_value<T>(c, f[I], s, has_los);
@@ -227,6 +246,10 @@ constexpr int kGsIters = (kGsCells + kGsThreads - 1) / kGsThreads;   // 5
 
 __device__ __forceinline__ int gs_word(int xl, int yl) { return yl * kGsW + (xl & 1) * (kGsW / 2) + (xl >> 1); }
 __device__ __forceinline__ int gs_wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+__device__ __forceinline__ double wave_sum64(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 
 // Local cell (xl, yl) and local plane ql = Q - z0 + 4 of a thread's cell in
 // stage s at step k (parity par of the stage planes), from its stage row r
@@ -236,10 +259,14 @@ __device__ __forceinline__ void gs_cell(int s, int r, int i, int par, int &xl, i
     xl = 1 + s + 2 * i + (((s & 1) ^ (1 + s + yl)) & 1);       // columns with (x + y) & 1 == s & 1
 }
 
+// part (may be NULL): per workgroup the FP64 sum of the values it writes,
+// in a fixed order (the total of vout without another pass over it).
 __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout, const float *__restrict__ vin,
                                                        const float *__restrict__ f, MGLevel L, float beta,
-                                                       int has_los, int tz) {
+                                                       int has_los, int tz, double *part) {
     __shared__ __align__(16) float lds[kGsRing * kGsPlane];
+    __shared__ double red[kGsThreads / 64];
+    double acc = 0.0;
     __shared__ float ptab[kGsW + kGsRows + kGsMaxTz + 8];   // p of LDS column | row | local plane
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     const int x0 = blockIdx.x * kGsTX, y0 = blockIdx.y * kGsTY, z0 = blockIdx.z * tz;
@@ -345,6 +372,7 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
             const float2 ev = *reinterpret_cast<const float2 *>(pl + wo_word);
             const float2 od = *reinterpret_cast<const float2 *>(pl + wo_word + kGsW / 2);
             *reinterpret_cast<float4 *>(vout + sz * (z0 - 12 + k) + wo_off) = make_float4(ev.x, od.x, ev.y, od.y);
+            if (part) acc += ((double)ev.x + (double)od.x) + ((double)ev.y + (double)od.y);
         }
         fetch_f(k + 1, fnext);
         const int par = (z0 + k + 1) & 1;
@@ -370,8 +398,8 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
 #define A0(o) p0[wc + (o)]
 #define AM(o) pm[wc + (o)]
 #define AP(o) pp[wc + (o)]
-            const Coef<float> c = has_los ? coef_p<float>(L, L.losn[0], L.losn[1], L.losn[2], beta)
-                                          : coef_p<float>(L, ptab[xl], ptab[kGsW + yl], tz_tab[ql], beta);
+            const Coef<float> c = has_los ? coef_p<float, true>(L, L.losn[0], L.losn[1], L.losn[2], beta)
+                                          : coef_p<float, true>(L, ptab[xl], ptab[kGsW + yl], tz_tab[ql], beta);
             const Nbr<float> sn = nbr_of<float>(
                 c, A0(ox), A0(oxm), A0(kGsW), A0(-kGsW), AP(0), AM(0), A0(kGsW + ox), A0(-kGsW + oxm),
                 A0(kGsW + oxm), A0(-kGsW + ox), AP(ox), AM(oxm), AP(oxm), AM(ox), AP(kGsW), AM(-kGsW), AP(-kGsW),
@@ -389,6 +417,362 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
     for (int k = 0; k < tz + 12; k += 2) {   // tz + 12 is even: the f buffers alternate
         step(k, fa, fb);
         step(k + 1, fb, fa);
+    }
+    if (part) {
+        acc = wave_sum64(acc);
+        if ((tid & 63) == 0) red[tid >> 6] = acc;
+        __syncthreads();
+        if (tid == 0)
+            part[blockIdx.x + gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] =
+                (red[0] + red[1]) + (red[2] + red[3]);
+    }
+}
+
+// S damped Jacobi sweeps in one launch (temporal blocking), out of place:
+// vout is what S launches of k_mg_jacobi leave (the same cell arithmetic on
+// the same values, so the same bits).  A workgroup owns a 64 x 16 column of
+// tz planes and marches up z; sweep level t (t = 1..S) of the tile, grown by
+// S - t cells in x and y, lives in a ring of 4 LDS planes (level 0: the old
+// values, loaded).  Step k loads plane P = z0 - S + k and computes level t
+// on plane P - 2t + 1: level t - 1 there and on both neighbouring planes was
+// finished in earlier steps, and the stages of one step touch disjoint
+// planes, so one barrier per step orders them.  Level S (the tile itself)
+// goes straight to vout as float4 rows.
+constexpr int kJbTX = 64, kJbTY = 16, kJbThreads = 256, kJbRing = 4, kJbMaxTz = 128;
+
+template <int S>
+struct JbShape {
+    static constexpr int W = kJbTX + 2 * S, H = kJbTY + 2 * S, Plane = W * H;
+    // cells of the levels below S: level t covers (64 + 2e) x (16 + 2e), e = S - t
+    static constexpr int cells(int t) { return (kJbTX + 2 * (S - t)) * (kJbTY + 2 * (S - t)); }
+    static constexpr int Inner = S == 2 ? cells(1) : cells(1) + cells(2);
+    static constexpr int Iters = (Inner + kJbThreads - 1) / kJbThreads;
+    static constexpr int Chunks = (W + 3) / 4 * H;   // float4 columns per plane row (last may overhang)
+};
+
+template <int S>
+__global__ void __launch_bounds__(kJbThreads) k_mg_jacobi_tb(float *__restrict__ vout, const float *__restrict__ vin,
+                                                             const float *__restrict__ f, MGLevel L, float beta,
+                                                             float omega, int has_los, int tz) {
+    static_assert(S == 2 || S == 3, "2 or 3 sweeps per pass");
+    using J = JbShape<S>;
+    __shared__ __align__(16) float lds[S * kJbRing * J::Plane];
+    __shared__ float ptab[J::W + J::H + kJbMaxTz + 2 * S];
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const int x0 = blockIdx.x * kJbTX, y0 = blockIdx.y * kJbTY, z0 = blockIdx.z * tz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int tid = threadIdx.x;
+    float *pz_tab = ptab + J::W + J::H;
+    if (!has_los)
+        for (int q = tid; q < J::W + J::H + tz + 2 * S; q += kJbThreads) {
+            int g;
+            if (q < J::W)
+                g = gs_wrap(x0 - S + q, nx);
+            else if (q < J::W + J::H)
+                g = nx + gs_wrap(y0 - S + q - J::W, ny);
+            else
+                g = nx + ny + gs_wrap(z0 - S + q - J::W - J::H, nz);
+            ptab[q] = L.pv[g];
+        }
+    auto plane = [&](int t, int Pl) {   // level t, local plane Pl = P - z0 + S >= 0
+        return lds + (t * kJbRing + (Pl & (kJbRing - 1))) * J::Plane;
+    };
+    // level-0 loads: element q = row * W + col, one float per element (W is
+    // not a multiple of 4 at S = 3); threads past the plane reload element 0
+    constexpr int NL = (J::Plane + kJbThreads - 1) / kJbThreads;
+    int64_t ld_off[NL];
+#pragma unroll
+    for (int u = 0; u < NL; u++) {
+        const int q = tid + kJbThreads * u, qq = q < J::Plane ? q : 0;
+        ld_off[u] = gs_wrap(x0 - S + qq % J::W, nx) + sy * gs_wrap(y0 - S + qq / J::W, ny);
+    }
+    float pre[NL];
+    auto fetch = [&](int P) {
+        const float *src = vin + sz * gs_wrap(min(P, z0 + tz - 1 + S), nz);
+#pragma unroll
+        for (int u = 0; u < NL; u++) pre[u] = src[ld_off[u]];
+    };
+    auto store = [&](int Pl) {
+        float *pl = plane(0, Pl);
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+            const int q = tid + kJbThreads * u;
+            if (q < J::Plane) pl[q] = pre[u];
+        }
+    };
+    // inner levels: iteration it handles cell jc = tid + 256 it of the list
+    // (level 1 cells, then level 2 at S = 3); desc = t | xl << 2 | yl << 10
+    int desc[J::Iters];
+#pragma unroll
+    for (int it = 0; it < J::Iters; it++) {
+        const int jc = tid + kJbThreads * it;
+        int t = 1, jj = jc;
+        if (S == 3 && jc >= J::cells(1)) {
+            t = 2;
+            jj = jc - J::cells(1);
+        }
+        const int e = S - t, w = kJbTX + 2 * e;
+        const int yl = S - e + jj / w, xl = S - e + jj % w;
+        desc[it] = jc < J::Inner ? (t | (xl << 2) | (yl << 10)) : -1;
+    }
+    // level t is computed on plane Q = z0 - S + k - 2t + 1, needed on
+    // z0 - (S - t) .. z0 + tz - 1 + (S - t)
+    auto active = [&](int t, int k) { return k >= 3 * t - 1 && k <= tz + 2 * S + t - 2; };
+    auto f_off = [&](int xl, int yl, int Q) {
+        return gs_wrap(x0 - S + xl, nx) + sy * gs_wrap(y0 - S + yl, ny) + sz * gs_wrap(Q, nz);
+    };
+    auto fetch_f = [&](int k, float *fr) {
+#pragma unroll
+        for (int it = 0; it < J::Iters; it++) {
+            const int d = desc[it], t = d & 3;
+            const int64_t off = d >= 0 && active(t, k) ? f_off((d >> 2) & 255, d >> 10, z0 - S + k - 2 * t + 1) : 0;
+            fr[it] = f[off];
+        }
+    };
+    // level S: 4 consecutive cells, row tid / 16, x 4 (tid % 16)
+    const int oy = tid >> 4, ox = 4 * (tid & 15);
+    const int64_t o_off = (x0 + ox) + sy * (y0 + oy);
+    auto fetch_fo = [&](int k, float4 &fo) {
+        const int Q = z0 - S + k - 2 * S + 1;
+        fo = *reinterpret_cast<const float4 *>(f + sz * (active(S, k) ? Q : z0) + o_off);
+    };
+    // one cell of level t from level t - 1 (local cell xl, yl, local plane ql)
+    auto cell = [&](int t, int xl, int yl, int ql, float fv) {
+        const float *pm = plane(t - 1, ql - 1), *p0 = plane(t - 1, ql), *pp = plane(t - 1, ql + 1);
+        const int wc = yl * J::W + xl;
+        const Coef<float> c = has_los ? coef_p<float>(L, L.losn[0], L.losn[1], L.losn[2], beta)
+                                      : coef_p<float>(L, ptab[xl], ptab[J::W + yl], pz_tab[ql], beta);
+#define A0(o) p0[wc + (o)]
+#define AM(o) pm[wc + (o)]
+#define AP(o) pp[wc + (o)]
+        const Nbr<float> sn = nbr_of<float>(c, A0(1), A0(-1), A0(J::W), A0(-J::W), AP(0), AM(0), A0(J::W + 1),
+                                            A0(-J::W - 1), A0(J::W - 1), A0(-J::W + 1), AP(1), AM(-1), AP(-1),
+                                            AM(1), AP(J::W), AM(-J::W), AP(-J::W), AM(J::W));
+        const float v = jacobi_value(c, fv, sn, A0(0), omega, has_los);
+#undef A0
+#undef AM
+#undef AP
+        return v;
+    };
+
+    auto step = [&](int k, const float *fcur, float *fnext, const float4 &focur, float4 &fonext) {
+        if (k <= tz + 2 * S - 1) store(k);
+        fetch(z0 - S + k + 1);
+        fetch_f(k + 1, fnext);
+        fetch_fo(k + 1, fonext);
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < J::Iters; it++) {
+            const int d = desc[it], t = d & 3;
+            if (d < 0 || !active(t, k)) continue;
+            const int xl = (d >> 2) & 255, yl = d >> 10, ql = k - 2 * t + 1;
+            plane(t, ql)[yl * J::W + xl] = cell(t, xl, yl, ql, fcur[it]);
+        }
+        if (active(S, k)) {
+            const int ql = k - 2 * S + 1;
+            const float fo[4] = {focur.x, focur.y, focur.z, focur.w};
+            float o[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) o[j] = cell(S, S + ox + j, S + oy, ql, fo[j]);
+            *reinterpret_cast<float4 *>(vout + sz * (z0 - S + ql) + o_off) = make_float4(o[0], o[1], o[2], o[3]);
+        }
+    };
+
+    float fa[J::Iters], fb[J::Iters];
+    float4 foa, fob;
+    fetch(z0 - S);
+    fetch_f(0, fa);
+    fetch_fo(0, foa);
+    const int nsteps = tz + 3 * S - 1;
+    int k = 0;
+    for (; k + 1 < nsteps; k += 2) {   // the f buffers alternate
+        step(k, fa, fb, foa, fob);
+        step(k + 1, fb, fa, fob, foa);
+    }
+    if (k < nsteps) step(k, fa, fb, foa, fob);
+}
+
+// One FP64 refinement pass of the RBGS solve (z-marching, LDS ring):
+//   phi_out = phi_in + (e - sum(e) / N)          (with e; without, phi_in is used)
+//   r = (float)((f - alpha) - A phi_out),  alpha = *fsum / N,
+// the arithmetic of k_add_correction followed by k_mg_residual_mixed, plus
+// each workgroup's FP64 sum of r in a fixed order (part[block]).  A
+// workgroup owns a 64 x 16 column of tz planes; step k loads plane z0-1+k of
+// phi_in (+ e) into a ring of 4 planes of 66 x 18 doubles (a halo of 1),
+// writes its interior to phi_out, and computes r on the plane below.  Every
+// value of phi and r is read or written once per pass.
+constexpr int kRfTX = 64, kRfTY = 16;
+constexpr int kRfW = kRfTX + 2, kRfRows = kRfTY + 2, kRfPlane = kRfW * kRfRows;   // 66 x 18
+constexpr int kRfRing = 4, kRfThreads = 256, kRfMaxTz = 128;
+constexpr int kRfLoads = (kRfPlane + kRfThreads - 1) / kRfThreads;   // 5
+
+__global__ void __launch_bounds__(kRfThreads)
+    k_mg_refine(double *__restrict__ phi_out, const double *__restrict__ phi_in, const float *__restrict__ e,
+                const double *esum, const float *__restrict__ f, const double *fsum, float *__restrict__ r,
+                double *__restrict__ part, MGLevel L, float beta, int has_los, int tz) {
+    __shared__ double ring[kRfRing * kRfPlane];
+    __shared__ double ptab[kRfW + kRfRows + kRfMaxTz + 2];   // p of LDS column | row | local plane
+    __shared__ double red[kRfThreads / 64];
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const int x0 = blockIdx.x * kRfTX, y0 = blockIdx.y * kRfTY, z0 = blockIdx.z * tz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const double count = (double)sz * nz;
+    const double me = e ? *esum / count : 0.0;
+    const double alpha = fsum ? *fsum / count : 0.0;
+    const int tid = threadIdx.x;
+    double *pz_tab = ptab + kRfW + kRfRows;
+    if (!has_los)
+        for (int q = tid; q < kRfW + kRfRows + tz + 2; q += kRfThreads) {
+            int a, g;
+            if (q < kRfW) {
+                a = 0;
+                g = gs_wrap(x0 - 1 + q, nx);
+            } else if (q < kRfW + kRfRows) {
+                a = 1;
+                g = nx + gs_wrap(y0 - 1 + q - kRfW, ny);
+            } else {
+                a = 2;
+                g = nx + ny + gs_wrap(z0 - 1 + q - kRfW - kRfRows, nz);
+            }
+            ptab[q] = (double)L.xv[g] / (double)L.cell[a];   // as coef<double>
+        }
+    // plane loads: element q = row * 66 + col (threads past 1188 reload 0)
+    int64_t ld_off[kRfLoads];
+    bool ld_in[kRfLoads];   // interior column and row: written to phi_out
+#pragma unroll
+    for (int u = 0; u < kRfLoads; u++) {
+        const int q = tid + kRfThreads * u, qq = q < kRfPlane ? q : 0;
+        const int row = qq / kRfW, col = qq % kRfW;
+        ld_off[u] = gs_wrap(x0 - 1 + col, nx) + sy * gs_wrap(y0 - 1 + row, ny);
+        ld_in[u] = q < kRfPlane && col >= 1 && col <= kRfTX && row >= 1 && row <= kRfTY;
+    }
+    double pre[kRfLoads];
+    float pre_e[kRfLoads];
+    auto fetch = [&](int P) {
+        const int64_t zo = sz * gs_wrap(min(P, z0 + tz), nz);
+#pragma unroll
+        for (int u = 0; u < kRfLoads; u++) {
+            pre[u] = phi_in[zo + ld_off[u]];
+            if (e) pre_e[u] = e[zo + ld_off[u]];
+        }
+    };
+    auto store = [&](int P) {
+        double *pl = ring + (P - z0 + 1) % kRfRing * kRfPlane;
+        const int64_t zo = sz * gs_wrap(P, nz);
+        const bool own = P >= z0 && P < z0 + tz;
+#pragma unroll
+        for (int u = 0; u < kRfLoads; u++) {
+            const int q = tid + kRfThreads * u;
+            if (q >= kRfPlane) continue;
+            const double v = e ? (double)pre[u] + ((double)pre_e[u] - me) : pre[u];
+            pl[q] = v;
+            if (e && own && ld_in[u]) phi_out[zo + ld_off[u]] = v;
+        }
+    };
+    // f of the tile's plane Q: float4 chunk tid (row tid / 16, x 4 (tid % 16)),
+    // staged through LDS (two slots); cells: x = tid % 64, rows tid / 64 + 4j
+    __shared__ __align__(16) float ftile[2][kRfTX * kRfTY];
+    const int64_t f_off = (x0 + 4 * (tid & 15)) + sy * (y0 + (tid >> 4));
+    float4 fpre = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto fetch_f = [&](int Q) { fpre = *reinterpret_cast<const float4 *>(f + sz * min(Q, z0 + tz - 1) + f_off); };
+    const int cx = tid & 63, cy0 = tid >> 6;
+    const int64_t c_off = (x0 + cx) + sy * (y0 + cy0);
+    double acc = 0.0;
+    fetch(z0 - 1);
+    fetch_f(z0);
+    for (int k = 0; k < tz + 2; k++) {
+        const int P = z0 - 1 + k;
+        store(P);
+        if (k >= 2) *reinterpret_cast<float4 *>(&ftile[k & 1][4 * tid]) = fpre;   // f of plane P - 1
+        fetch(P + 1);
+        if (k >= 2) fetch_f(P);
+        __syncthreads();
+        if (k < 2) continue;
+        const int Q = P - 1, ql = Q - z0 + 1;
+        const double *pm = ring + (ql - 1) % kRfRing * kRfPlane;
+        const double *p0 = ring + ql % kRfRing * kRfPlane;
+        const double *pp = ring + (ql + 1) % kRfRing * kRfPlane;
+        const float *ft = ftile[k & 1];
+#pragma unroll 1
+        for (int j = 0; j < 4; j++) {
+            const int cy = cy0 + 4 * j;
+            const int xl = cx + 1, yl = cy + 1, wc = yl * kRfW + xl;
+            const Coef<double> c = has_los ? coef_p<double>(L, (double)L.losn[0], (double)L.losn[1],
+                                                            (double)L.losn[2], (double)beta)
+                                           : coef_p<double>(L, ptab[xl], ptab[kRfW + yl], pz_tab[ql], (double)beta);
+#define A0(o) p0[wc + (o)]
+#define AM(o) pm[wc + (o)]
+#define AP(o) pp[wc + (o)]
+            const Nbr<double> sn = nbr_of<double>(
+                c, A0(1), A0(-1), A0(kRfW), A0(-kRfW), AP(0), AM(0), A0(kRfW + 1), A0(-kRfW - 1), A0(kRfW - 1),
+                A0(-kRfW + 1), AP(1), AM(-1), AP(-1), AM(1), AP(kRfW), AM(-kRfW), AP(-kRfW), AM(kRfW));
+            double t = 2.0 * (c.gpx2 + c.gpy2 + c.gpz2) * A0(0) - (sn.A + sn.B + sn.C + sn.X);
+#undef A0
+#undef AM
+#undef AP
+            if (!has_los) t -= sn.G;
+            const float rv = (float)(((double)ft[cy * kRfTX + cx] - alpha) - t);
+            r[sz * Q + c_off + sy * 4 * j] = rv;
+            acc += (double)rv;
+        }
+    }
+    acc = wave_sum64(acc);
+    if ((tid & 63) == 0) red[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0)
+        part[blockIdx.x + gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] =
+            (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// r += (float)(-(*rsum) / n) when rsum (the mean removal of k_add_const),
+// then per block the FP64 sums of r^2 and of (r - rprev)^2 (when rprev), in
+// a fixed order: part[block] and part[nblocks + block].
+__global__ void __launch_bounds__(256) k_shift_norms(float *r, const float *rprev, int64_t n, const double *rsum,
+                                                     double *part) {
+    __shared__ double red[2][4];
+    const float cst = rsum ? (float)(-1.0 * (*rsum) / (double)n) : 0.f;
+    double s2 = 0.0, d2 = 0.0;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    auto cell = [&](float v, const float *pv, int64_t i, float &out) {
+        if (rsum) v += cst;
+        out = v;
+        s2 += (double)v * (double)v;
+        if (rprev) {
+            const double d = (double)v - (double)pv[i];
+            d2 += d * d;
+        }
+    };
+    if ((n & 3) == 0 && ((uintptr_t)r & 15) == 0 && ((uintptr_t)rprev & 15) == 0) {
+        float4 *r4 = reinterpret_cast<float4 *>(r);
+        const float4 *p4 = reinterpret_cast<const float4 *>(rprev);
+        for (int64_t i = t0; i < (n >> 2); i += stride) {
+            const float4 v = r4[i];
+            const float4 q = rprev ? p4[i] : v;
+            const float pq[4] = {q.x, q.y, q.z, q.w};
+            float4 o;
+            cell(v.x, pq, 0, o.x);
+            cell(v.y, pq, 1, o.y);
+            cell(v.z, pq, 2, o.z);
+            cell(v.w, pq, 3, o.w);
+            if (rsum) r4[i] = o;
+        }
+    } else {
+        for (int64_t i = t0; i < n; i += stride) {
+            float o;
+            cell(r[i], rprev, i, o);
+            if (rsum) r[i] = o;
+        }
+    }
+    s2 = wave_sum64(s2);
+    d2 = wave_sum64(d2);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = s2;
+        red[1][threadIdx.x >> 6] = d2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        part[gridDim.x + blockIdx.x] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     }
 }
 
@@ -579,14 +963,39 @@ __global__ void k_mg_xvec(float *xv, const Slot *slot, int nx, int ny, int nz) {
 // bit: each block sums a fixed grid-stride share (thread, then wave, then the
 // four waves in order) into part[blockIdx.x]; launch_reduce then sums the
 // partials with k_sum_partials (fixed order as well).
+template <int OP>
+__device__ __forceinline__ double red_term(double a, double b) {
+    const double v = OP == 2 ? a - b : a;
+    return OP == 0 ? v : v * v;
+}
+
 template <int OP, typename T>
 __global__ void __launch_bounds__(256) k_reduce_part(const T *a, const T *b, int64_t n, double *part) {
     __shared__ double red[4];
     double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const double v = OP == 2 ? (double)a[i] - (double)b[i] : (double)a[i];
-        s += OP == 0 ? v : v * v;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    bool vec = false;
+    if constexpr (sizeof(T) == 4)   // float4 loads, two in flight: thread t sums chunks t, t + stride, ...
+        vec = (n & 3) == 0 && ((uintptr_t)a & 15) == 0 && (OP != 2 || ((uintptr_t)b & 15) == 0);
+    if (vec) {
+        const float4 *a4 = reinterpret_cast<const float4 *>(a), *b4 = reinterpret_cast<const float4 *>(b);
+        const int64_t n4 = n >> 2;
+        double s1 = 0.0;
+        int64_t i = t0;
+        for (; i + stride < n4; i += 2 * stride) {
+            const float4 x = a4[i], y = a4[i + stride];
+            const float4 u = OP == 2 ? b4[i] : x, w = OP == 2 ? b4[i + stride] : y;
+            s += red_term<OP>(x.x, u.x) + red_term<OP>(x.y, u.y) + red_term<OP>(x.z, u.z) + red_term<OP>(x.w, u.w);
+            s1 += red_term<OP>(y.x, w.x) + red_term<OP>(y.y, w.y) + red_term<OP>(y.z, w.z) + red_term<OP>(y.w, w.w);
+        }
+        if (i < n4) {
+            const float4 x = a4[i], u = OP == 2 ? b4[i] : x;
+            s += red_term<OP>(x.x, u.x) + red_term<OP>(x.y, u.y) + red_term<OP>(x.z, u.z) + red_term<OP>(x.w, u.w);
+        }
+        s += s1;
+    } else {
+        for (int64_t i = t0; i < n; i += stride)
+            s += red_term<OP>((double)a[i], OP == 2 ? (double)b[i] : 0.0);
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -671,13 +1080,75 @@ static int gs4_tz(const MGLevel &L) {
 
 bool mg_gs4_ok(const MGLevel &L) { return L.n[0] % kGsTX == 0 && L.n[1] % kGsTY == 0 && L.n[2] % 8 == 0; }
 
+int64_t mg_gs4_blocks(const MGLevel &L) {
+    return (int64_t)(L.n[0] / kGsTX) * (L.n[1] / kGsTY) * (L.n[2] / gs4_tz(L));
+}
+
 hipError_t launch_mg_gs4(float *vout, const float *vin, const float *f, const MGLevel &L, float beta, int has_los,
-                         hipStream_t s) {
+                         hipStream_t s, double *part) {
     if (!mg_gs4_ok(L) || vout == vin) return hipErrorInvalidValue;
     const int tz = gs4_tz(L);
     const dim3 grid((unsigned)(L.n[0] / kGsTX), (unsigned)(L.n[1] / kGsTY), (unsigned)(L.n[2] / tz));
-    hipLaunchKernelGGL(k_mg_gs4, grid, dim3(kGsThreads), 0, s, vout, vin, f, L, beta, has_los, tz);
+    hipLaunchKernelGGL(k_mg_gs4, grid, dim3(kGsThreads), 0, s, vout, vin, f, L, beta, has_los, tz, part);
     return hipGetLastError();
+}
+
+// z planes per workgroup of k_mg_jacobi_tb: at least 512 workgroups
+static int jacobi_tb_tz(const MGLevel &L) {
+    const int64_t cols = (int64_t)(L.n[0] / kJbTX) * (L.n[1] / kJbTY);
+    int tz = kJbMaxTz;
+    while (tz > 8 && (L.n[2] % tz || cols * (L.n[2] / tz) < 512)) tz /= 2;
+    return tz;
+}
+
+bool mg_jacobi_tb_ok(const MGLevel &L) {
+    return L.n[0] % kJbTX == 0 && L.n[1] % kJbTY == 0 && L.n[2] % 8 == 0 && L.n[2] >= 8;
+}
+
+hipError_t launch_mg_jacobi_tb(float *vout, const float *vin, const float *f, const MGLevel &L, float beta,
+                               float omega, int has_los, int sweeps, hipStream_t s) {
+    if (!mg_jacobi_tb_ok(L) || vout == vin || (sweeps != 2 && sweeps != 3)) return hipErrorInvalidValue;
+    const int tz = jacobi_tb_tz(L);
+    const dim3 grid((unsigned)(L.n[0] / kJbTX), (unsigned)(L.n[1] / kJbTY), (unsigned)(L.n[2] / tz));
+    if (sweeps == 2)
+        hipLaunchKernelGGL(k_mg_jacobi_tb<2>, grid, dim3(kJbThreads), 0, s, vout, vin, f, L, beta, omega, has_los, tz);
+    else
+        hipLaunchKernelGGL(k_mg_jacobi_tb<3>, grid, dim3(kJbThreads), 0, s, vout, vin, f, L, beta, omega, has_los, tz);
+    return hipGetLastError();
+}
+
+// z planes per workgroup of k_mg_refine: at least 1024 workgroups
+static int refine_tz(const MGLevel &L) {
+    const int64_t cols = (int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY);
+    int tz = kRfMaxTz;
+    while (tz > 8 && (L.n[2] % tz || cols * (L.n[2] / tz) < 1024)) tz /= 2;
+    return tz;
+}
+
+bool mg_refine_ok(const MGLevel &L) { return L.n[0] % kRfTX == 0 && L.n[1] % kRfTY == 0 && L.n[2] % 8 == 0; }
+
+int64_t mg_refine_blocks(const MGLevel &L) {
+    return (int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY) * (L.n[2] / refine_tz(L));
+}
+
+hipError_t launch_mg_refine(double *phi_out, const double *phi_in, const float *e, const double *esum, const float *f,
+                            const double *fsum, float *r, double *part, const MGLevel &L, float beta, int has_los,
+                            hipStream_t s) {
+    if (!mg_refine_ok(L) || (e && phi_out == phi_in)) return hipErrorInvalidValue;
+    const int tz = refine_tz(L);
+    const dim3 grid((unsigned)(L.n[0] / kRfTX), (unsigned)(L.n[1] / kRfTY), (unsigned)(L.n[2] / tz));
+    hipLaunchKernelGGL(k_mg_refine, grid, dim3(kRfThreads), 0, s, phi_out, phi_in, e, esum, f, fsum, r, part, L,
+                       beta, has_los, tz);
+    return hipGetLastError();
+}
+
+hipError_t launch_shift_norms(float *r, const float *rprev, int64_t n, const double *rsum, double *out,
+                              double *part, hipStream_t s) {
+    const unsigned nb = red_blocks(n);
+    hipLaunchKernelGGL(k_shift_norms, dim3(nb), dim3(256), 0, s, r, rprev, n, rsum, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_sum_partials(part, nb, rprev ? 2 : 1, out, 1, s);
 }
 
 template <typename T>

@@ -221,7 +221,9 @@ def kernel_symbol(label, grid, rand):
             "fft_x_c2r_disp": f"k_x_c2r<{h}, 5>", "scatter_accum": "k_scatter_accum<0>", "zero": "k_zero4",
             "gather": "k_gather", "mg_jacobi": "k_mg_jacobi", "mg_residual": "k_mg_residual<float>",
             "mg_reduce": "k_mg_reduce<float, float>", "mg_prolong": "k_mg_prolong<float, float>",
-            "mg_mcgs": "k_mg_mcgs<float>", "mg_mcgs64": "k_mg_mcgs<double>",
+            "mg_mcgs": "k_mg_mcgs<float>", "mg_mcgs64": "k_mg_mcgs<double>", "mg_gs4": "k_mg_gs4",
+            "mg_refine": "k_mg_refine", "mg_residual_fp64": "k_mg_residual_mixed",
+            "mg_correct": "k_add_correction<double>",
             "mg_coarse": "k_mg_coarse<float>", "mg_coarse64": "k_mg_coarse<double>"}.get(label)
 
 
@@ -516,6 +518,75 @@ def catalog_bench(args):
     shutil.rmtree(d, ignore_errors=True)
 
 
+def fft_bench(args):
+    """The 512^3 real transforms against the vendor library: the library's
+    baorec_rfft3d / baorec_irfft3d (device-resident, one launch sequence
+    each) beside rocFFT through torch.fft.rfftn / irfftn on the same device,
+    each timed over the same number of calls.  Algorithmic bytes per 3D
+    R2C: the real grid read once and the half spectrum written once (and the
+    reverse for C2R)."""
+    import torch
+    import baorec_hip as bh
+    from baorec_hip import _lib as L
+    from baorec_hip.devmem import DeviceArray, hip
+    n = 512
+    ctx = bh.Context(0)
+    lib = ctx.lib
+    grid = (ctypes.c_int32 * 3)(n, n, n)
+    rng = np.random.default_rng(1)
+    host = rng.standard_normal((n, n, n), dtype=np.float32)
+    x = DeviceArray.from_numpy(host)
+    spec = DeviceArray((n * n * (n // 2 + 1) * 2,))
+    back = DeviceArray((n * n * n,))
+    h = hip()
+
+    def ours(fwd):
+        if fwd:
+            ctx.check(lib.baorec_rfft3d(ctx.handle, grid, L.MEM_DEVICE, ctypes.c_void_p(x.ptr),
+                                        ctypes.c_void_p(spec.ptr)))
+        else:
+            ctx.check(lib.baorec_irfft3d(ctx.handle, grid, L.MEM_DEVICE, ctypes.c_void_p(spec.ptr),
+                                         ctypes.c_void_p(back.ptr)))
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        h.hipDeviceSynchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        h.hipDeviceSynchronize()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    res = {"ours_r2c_ms": timed(lambda: ours(True), args.steps),
+           "ours_c2r_ms": timed(lambda: ours(False), args.steps)}
+    err = float(np.max(np.abs(back.numpy().reshape(n, n, n) - host)))
+    xt = torch.from_numpy(host).cuda()
+    holder = {}
+
+    def t_r2c():
+        holder["k"] = torch.fft.rfftn(xt)
+
+    def t_c2r():
+        holder["x"] = torch.fft.irfftn(holder["k"], s=(n, n, n))
+
+    res["rocfft_r2c_ms"] = timed(t_r2c, args.steps)
+    res["rocfft_c2r_ms"] = timed(t_c2r, args.steps)
+    err_t = float((holder["x"] - xt).abs().max().item())
+    nbytes = n ** 3 * 4 + n * n * (n // 2 + 1) * 8
+    line = {"metric": "3D R2C + C2R ms, 512^3 FP32 (library vs rocFFT)",
+            "value": round(res["ours_r2c_ms"] + res["ours_c2r_ms"], 4), "unit": "ms", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "fft512", "grid": [n, n, n]},
+            "library": {k: round(v, 4) for k, v in res.items()},
+            "GBps_algorithmic": {k: round(nbytes / (v * 1e-3) / 1e9, 1) for k, v in res.items()},
+            "round_trip_max_abs_err": {"library": err, "rocfft": err_t},
+            "vendor": "rocFFT via torch " + torch.__version__}
+    print(json.dumps(line), flush=True)
+
+
 def pk_bench(args):
     """Power-spectrum multipoles of a 1M-tracer periodic box at 512^3 with
     TSC and interlacing, dk = 0.005 (the configuration of the reference's
@@ -786,7 +857,7 @@ def main():
     ap.add_argument("--workload", default="lc512_warm",
                     help="lc512_warm (default: the 512^3 lightcone multigrid recon, learned warm start against "
                          "the plain cold start), lightcone512, mg_lc512, box128, mg512, mg512_zlos, mg512_ref, "
-                         "batch64, readout512, sky22m, catalog2m, pk512")
+                         "batch64, readout512, sky22m, catalog2m, pk512, fft512")
     ap.add_argument("--ndata", type=int, default=2_000_000)
     ap.add_argument("--nrand", type=int, default=20_000_000)
     ap.add_argument("--ncat", type=int, default=64, help="catalogs of the batch64 workload")
@@ -820,6 +891,9 @@ def main():
         return
     if args.workload == "pk512":
         pk_bench(args)
+        return
+    if args.workload == "fft512":
+        fft_bench(args)
         return
     if args.workload == "batch64":
         batch_bench(args)
@@ -857,7 +931,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "form": "plain (cold start)", "grid": [nx, ny, nz],
+        "config": {"workload": args.workload,
+                   "form": "plain (cold start)" if args.warm_start == "none" else f"{args.warm_start} warm start",
+                   "grid": [nx, ny, nz],
                    "n_data": len(data[0]),
                    "n_randoms": 0 if rand is None else len(rand[0]),
                    "recon": "multigrid" if multigrid else "iterative",
