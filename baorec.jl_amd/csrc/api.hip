@@ -2287,12 +2287,11 @@ int baorec_rfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *in
         x_fwd(c, G, d, o, 1);
         strided_plain(c, G, f, 1, false, o, 1);
         strided_plain(c, G, f, 2, false, o, 1);
-        for (int kz = 0; kz < G.nz; kz++)   // one plane of ny rows per copy
-            hipchk(c, hipMemcpy2DAsync(out_complex + (int64_t)2 * G.P * G.ny * kz, sizeof(float2) * G.P,
-                                       o + G.PZ() * kz, sizeof(float2) * G.PP(), sizeof(float2) * G.P,
-                                       (size_t)G.ny,
-                                       mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
-                                       c->stream), "copy spectrum");
+        const int64_t nc = (int64_t)G.P * G.ny * G.nz;   // compact complex elements
+        float *dout = stage_out(c, mem, out_complex, 2 * nc, "out_spectrum");
+        hipchk(c, launch_spectrum_pack(reinterpret_cast<float2 *>(dout), o, G.P, G.PP(), G.PZ(), G.ny, G.nz, 1,
+                                       c->stream), "pack spectrum");
+        copy_out(c, mem, out_complex, dout, 2 * nc);
         sync(c);
     });
 }
@@ -2303,12 +2302,10 @@ int baorec_irfft3d(baorec_ctx *c, const int32_t grid[3], int mem, const float *i
         if (!in_complex || !out) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "NULL argument");
         FFTBufs f = fft_bufs(c, G);
         float2 *w = f.cw;  // work copy: the caller's spectrum is not modified
-        for (int kz = 0; kz < G.nz; kz++)
-            hipchk(c, hipMemcpy2DAsync(w + G.PZ() * kz, sizeof(float2) * G.PP(),
-                                       in_complex + (int64_t)2 * G.P * G.ny * kz, sizeof(float2) * G.P,
-                                       sizeof(float2) * G.P, (size_t)G.ny,
-                                       mem == BAOREC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                                       c->stream), "copy spectrum");
+        const int64_t nc = (int64_t)G.P * G.ny * G.nz;
+        const float *din = stage_in(c, mem, in_complex, 2 * nc, "in_spectrum");
+        hipchk(c, launch_spectrum_pack(w, reinterpret_cast<const float2 *>(din), G.P, G.PP(), G.PZ(), G.ny, G.nz, 0,
+                                       c->stream), "unpack spectrum");
         Slot *slot = slot_buf(c) + 1;
         strided_plain(c, G, f, 2, true, w, 1);
         strided_plain(c, G, f, 1, true, w, 1);

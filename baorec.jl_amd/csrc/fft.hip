@@ -63,6 +63,33 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double *part, int64
     }
 }
 
+// Half spectra between the padded working layout (kx row pitch PP, kz plane
+// pitch PZ) and the compact (nx/2+1) x ny x nz layout of the C ABI.
+__global__ void k_spectrum_pack(float2 *__restrict__ dst, const float2 *__restrict__ src, int P, int PP, int64_t PZ,
+                                int ny, int nz, int to_compact) {
+    const int64_t n = (int64_t)P * ny * nz;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(i % P);
+        const int64_t row = i / P;
+        const int ky = (int)(row % ny);
+        const int64_t kz = row / ny;
+        const int64_t padded = kz * PZ + (int64_t)ky * PP + kx;
+        if (to_compact)
+            dst[i] = src[padded];
+        else
+            dst[padded] = src[i];
+    }
+}
+
+hipError_t launch_spectrum_pack(float2 *dst, const float2 *src, int P, int PP, int64_t PZ, int ny, int nz,
+                                int to_compact, hipStream_t s) {
+    const int64_t n = (int64_t)P * ny * nz;
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_spectrum_pack, dim3((unsigned)(nb < 16384 ? nb : 16384)), dim3(256), 0, s, dst, src, P, PP,
+                       PZ, ny, nz, to_compact);
+    return hipGetLastError();
+}
+
 hipError_t launch_sum_partials(const double *part, int64_t n, int nb, double *out, int64_t stride, hipStream_t s) {
     hipLaunchKernelGGL(k_sum_partials, dim3(nb), dim3(1024), 0, s, part, n, out, stride);
     return hipGetLastError();

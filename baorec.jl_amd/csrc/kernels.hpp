@@ -108,6 +108,11 @@ struct XFwdArgs {
 };
 // out[b * stride] = sum of part[b * n .. b * n + n) in a fixed order, b < nb.
 hipError_t launch_sum_partials(const double *part, int64_t n, int nb, double *out, int64_t stride, hipStream_t s);
+// Half spectrum between the padded working layout (row pitch PP, plane pitch
+// PZ) and the compact P x ny x nz layout: to_compact != 0 packs src (padded)
+// into dst (compact), else unpacks.
+hipError_t launch_spectrum_pack(float2 *dst, const float2 *src, int P, int PP, int64_t PZ, int ny, int nz,
+                                int to_compact, hipStream_t s);
 
 // Local-LOS iteration, y inverse pass: from G_m = IFFT_z(k_z^m F) (m = 0,1,2)
 // it forms, per (kx, y, z), the three spectra the x pass needs

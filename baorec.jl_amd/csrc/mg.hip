@@ -236,13 +236,15 @@ constexpr int kGsW = kGsTX + 2 * kGsHalo;      // 72 words per LDS row
 constexpr int kGsRows = kGsTY + 2 * kGsHalo;   // 24 rows per plane
 constexpr int kGsPlane = kGsW * kGsRows;
 constexpr int kGsRing = 10;                    // planes P-9 .. P
-constexpr int kGsThreads = 256;
+constexpr int kGsThreads = 512;   // 16 waves per CU (two workgroups)
 constexpr int kGsMaxTz = 128;
 constexpr int kGsChunks = (kGsW / 4) * kGsRows;   // float4 loads per plane (432)
 // colour cells per stage: (8 + e) rows x (32 + e) columns, e = 3 - s
 constexpr int kGsStage0 = 11 * 35, kGsStage1 = kGsStage0 + 10 * 34, kGsStage2 = kGsStage1 + 9 * 33,
               kGsCells = kGsStage2 + 8 * 32;   // 1278
-constexpr int kGsIters = (kGsCells + kGsThreads - 1) / kGsThreads;   // 5
+constexpr int kGsIters = (kGsCells + kGsThreads - 1) / kGsThreads;
+constexpr int kGsLoads = (kGsChunks + kGsThreads - 1) / kGsThreads;   // float4 plane loads per thread
+constexpr int kGsCPT = kGsTX * kGsTY / kGsThreads;                     // cells written per thread (2 or 4)
 
 __device__ __forceinline__ int gs_word(int xl, int yl) { return yl * kGsW + (xl & 1) * (kGsW / 2) + (xl >> 1); }
 __device__ __forceinline__ int gs_wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
@@ -287,11 +289,11 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
 
     // plane loads: chunk q = row * 18 + j covers x0 - 4 + 4j .. +3 of LDS row
     // `row`; threads past the 432 chunks load chunk 0 again and do not store
-    int64_t ld_off[2];
-    int ld_word[2];
-    bool ld_ok[2];
+    int64_t ld_off[kGsLoads];
+    int ld_word[kGsLoads];
+    bool ld_ok[kGsLoads];
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
+    for (int u = 0; u < kGsLoads; u++) {
         const int q = tid + kGsThreads * u;
         ld_ok[u] = q < kGsChunks;
         const int qq = ld_ok[u] ? q : 0;
@@ -299,19 +301,19 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
         ld_off[u] = gs_wrap(x0 - kGsHalo + 4 * j, nx) + sy * gs_wrap(y0 - kGsHalo + row, ny);
         ld_word[u] = row * kGsW + 2 * j;
     }
-    float4 pre[2];
+    float4 pre[kGsLoads];
     // unconditional (a step past the last plane reloads the last plane), so
     // that no branch makes the compiler wait for the loads before the step's
     // arithmetic
     auto fetch = [&](int P) {
         const float *src = vin + sz * gs_wrap(min(P, z0 + tz + kGsHalo - 1), nz);
 #pragma unroll
-        for (int u = 0; u < 2; u++) pre[u] = *reinterpret_cast<const float4 *>(src + ld_off[u]);
+        for (int u = 0; u < kGsLoads; u++) pre[u] = *reinterpret_cast<const float4 *>(src + ld_off[u]);
     };
     auto store = [&](int slot) {
         float *pl = lds + slot * kGsPlane;
 #pragma unroll
-        for (int u = 0; u < 2; u++)
+        for (int u = 0; u < kGsLoads; u++)
             if (ld_ok[u]) {
                 pl[ld_word[u]] = pre[u].x;
                 pl[ld_word[u] + 1] = pre[u].z;
@@ -340,50 +342,68 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
     }
     // stage s updates plane Q = z0 - 5 - 2s + k, needed on z0 - 3 + s .. z0 + tz + 2 - s
     auto active = [&](int s, int k) { return k >= 2 + 3 * s && k <= tz + 7 + s; };
+    // Per iteration and parity par of the stage planes (z0 is even, so par =
+    // (k + 1) & 1): the cell's LDS column and row (xl | yl << 8) and its
+    // offset in a plane of f (32-bit: the grid has < 2^31 cells).
+    int cpos[2][kGsIters];
+    uint32_t foff[2][kGsIters];
+    const uint32_t sz32 = (uint32_t)sz;
+#pragma unroll
+    for (int par = 0; par < 2; par++)
+#pragma unroll
+        for (int it = 0; it < kGsIters; it++) {
+            const int d = desc[it];
+            int xl = kGsHalo, yl = kGsHalo;
+            if (d >= 0) gs_cell(d & 3, (d >> 2) & 63, d >> 8, par, xl, yl);
+            cpos[par][it] = xl | (yl << 8);
+            foff[par][it] = (uint32_t)gs_wrap(x0 - kGsHalo + xl, nx) + (uint32_t)nx * gs_wrap(y0 - kGsHalo + yl, ny);
+        }
     // f of the thread's cells at step k (a valid dummy address where inactive)
-    auto fetch_f = [&](int k, float *fr) {
-        const int par = (z0 + k + 1) & 1;
+    auto fetch_f = [&](int k, auto parc, float *fr) {
+        constexpr int par = decltype(parc)::value;
 #pragma unroll
         for (int it = 0; it < kGsIters; it++) {
             const int d = desc[it];
             const int s = d & 3;
-            int64_t off = 0;
-            if (d >= 0 && active(s, k)) {
-                int xl, yl;
-                gs_cell(s, (d >> 2) & 63, d >> 8, par, xl, yl);
-                off = gs_wrap(x0 - kGsHalo + xl, nx) + sy * gs_wrap(y0 - kGsHalo + yl, ny) +
-                      sz * gs_wrap(z0 - 5 - 2 * s + k, nz);
-            }
+            const int Q = gs_wrap(z0 - 5 - 2 * s + k, nz);
+            const uint32_t off = d >= 0 && active(s, k) ? foff[par][it] + sz32 * (uint32_t)Q : 0u;
             fr[it] = f[off];
         }
     };
     // write-out: thread -> row tid / 16, x 4 (tid % 16) of the 64 x 16 tile
-    const int wo_row = tid >> 4, wo_x = 4 * (tid & 15);
+    constexpr int kWoTPR = kGsTX / kGsCPT;   // threads per tile row
+    const int wo_row = tid / kWoTPR, wo_x = kGsCPT * (tid % kWoTPR);
     const int64_t wo_off = (x0 + wo_x) + sy * (y0 + wo_row);
     const int wo_word = (kGsHalo + wo_row) * kGsW + ((kGsHalo + wo_x) >> 1);
     const int last_load = tz + 7;
 
-    auto step = [&](int k, const float *fcur, float *fnext) {
+    // step k; parc: the parity of its stage planes, (k + 1) & 1
+    auto step = [&](int k, auto parc, const float *fcur, float *fnext) {
+        constexpr int par = decltype(parc)::value;
         if (k <= last_load) store(k % kGsRing);
         fetch(z0 - kGsHalo + k + 1);
         __syncthreads();
         if (k >= 12) {   // plane z0 - 12 + k, finished in step k - 1
             const float *pl = lds + ((k - 8) % kGsRing) * kGsPlane;
-            const float2 ev = *reinterpret_cast<const float2 *>(pl + wo_word);
-            const float2 od = *reinterpret_cast<const float2 *>(pl + wo_word + kGsW / 2);
-            *reinterpret_cast<float4 *>(vout + sz * (z0 - 12 + k) + wo_off) = make_float4(ev.x, od.x, ev.y, od.y);
-            if (part) acc += ((double)ev.x + (double)od.x) + ((double)ev.y + (double)od.y);
+            if constexpr (kGsCPT == 4) {
+                const float2 ev = *reinterpret_cast<const float2 *>(pl + wo_word);
+                const float2 od = *reinterpret_cast<const float2 *>(pl + wo_word + kGsW / 2);
+                *reinterpret_cast<float4 *>(vout + sz * (z0 - 12 + k) + wo_off) = make_float4(ev.x, od.x, ev.y, od.y);
+                if (part) acc += ((double)ev.x + (double)od.x) + ((double)ev.y + (double)od.y);
+            } else {
+                const float ev = pl[wo_word], od = pl[wo_word + kGsW / 2];
+                *reinterpret_cast<float2 *>(vout + sz * (z0 - 12 + k) + wo_off) = make_float2(ev, od);
+                if (part) acc += (double)ev + (double)od;
+            }
         }
-        fetch_f(k + 1, fnext);
-        const int par = (z0 + k + 1) & 1;
+        fetch_f(k + 1, std::integral_constant<int, 1 - par>{}, fnext);
         const int k10 = k % kGsRing;
 #pragma unroll
         for (int it = 0; it < kGsIters; it++) {
             const int d = desc[it];
             const int s = d & 3;
             if (d < 0 || !active(s, k)) continue;
-            int xl, yl;
-            gs_cell(s, (d >> 2) & 63, d >> 8, par, xl, yl);
+            const int xl = cpos[par][it] & 255, yl = cpos[par][it] >> 8;
             const int ql = k - 1 - 2 * s;   // local plane of Q, >= 1
             int m = k10 + kGsRing - 1 - 2 * s;   // ring slot of Q
             m = m >= kGsRing ? m - kGsRing : m;
@@ -398,8 +418,9 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
 #define A0(o) p0[wc + (o)]
 #define AM(o) pm[wc + (o)]
 #define AP(o) pp[wc + (o)]
-            const Coef<float> c = has_los ? coef_p<float, true>(L, L.losn[0], L.losn[1], L.losn[2], beta)
-                                          : coef_p<float, true>(L, ptab[xl], ptab[kGsW + yl], tz_tab[ql], beta);
+            const Coef<float> c =
+                coef_p<float, true>(L, has_los ? L.losn[0] : ptab[xl], has_los ? L.losn[1] : ptab[kGsW + yl],
+                                    has_los ? L.losn[2] : tz_tab[ql], beta);
             const Nbr<float> sn = nbr_of<float>(
                 c, A0(ox), A0(oxm), A0(kGsW), A0(-kGsW), AP(0), AM(0), A0(kGsW + ox), A0(-kGsW + oxm),
                 A0(kGsW + oxm), A0(-kGsW + ox), AP(ox), AM(oxm), AP(oxm), AM(ox), AP(kGsW), AM(-kGsW), AP(-kGsW),
@@ -413,18 +434,20 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
 
     float fa[kGsIters], fb[kGsIters];
     fetch(z0 - kGsHalo);
-    fetch_f(0, fa);
+    fetch_f(0, std::integral_constant<int, 1>{}, fa);
     for (int k = 0; k < tz + 12; k += 2) {   // tz + 12 is even: the f buffers alternate
-        step(k, fa, fb);
-        step(k + 1, fb, fa);
+        step(k, std::integral_constant<int, 1>{}, fa, fb);
+        step(k + 1, std::integral_constant<int, 0>{}, fb, fa);
     }
     if (part) {
         acc = wave_sum64(acc);
         if ((tid & 63) == 0) red[tid >> 6] = acc;
         __syncthreads();
-        if (tid == 0)
-            part[blockIdx.x + gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] =
-                (red[0] + red[1]) + (red[2] + red[3]);
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kGsThreads / 64; w++) t += red[w];
+            part[blockIdx.x + gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] = t;
+        }
     }
 }
 
@@ -438,20 +461,24 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
 // finished in earlier steps, and the stages of one step touch disjoint
 // planes, so one barrier per step orders them.  Level S (the tile itself)
 // goes straight to vout as float4 rows.
-constexpr int kJbTX = 64, kJbTY = 16, kJbThreads = 256, kJbRing = 4, kJbMaxTz = 128;
+constexpr int kJbTX = 64, kJbTY = 16, kJbRing = 4, kJbMaxTz = 128;
 
 template <int S>
 struct JbShape {
+    // 512 threads at S = 3 keep its per-thread cell lists short enough for
+    // two waves per SIMD without spills
+    static constexpr int Threads = S == 3 ? 512 : 256;
+    static constexpr int CPT = kJbTX * kJbTY / Threads;   // level-S cells per thread (consecutive x)
     static constexpr int W = kJbTX + 2 * S, H = kJbTY + 2 * S, Plane = W * H;
     // cells of the levels below S: level t covers (64 + 2e) x (16 + 2e), e = S - t
     static constexpr int cells(int t) { return (kJbTX + 2 * (S - t)) * (kJbTY + 2 * (S - t)); }
     static constexpr int Inner = S == 2 ? cells(1) : cells(1) + cells(2);
-    static constexpr int Iters = (Inner + kJbThreads - 1) / kJbThreads;
+    static constexpr int Iters = (Inner + Threads - 1) / Threads;
     static constexpr int Chunks = (W + 3) / 4 * H;   // float4 columns per plane row (last may overhang)
 };
 
 template <int S>
-__global__ void __launch_bounds__(kJbThreads) k_mg_jacobi_tb(float *__restrict__ vout, const float *__restrict__ vin,
+__global__ void __launch_bounds__(JbShape<S>::Threads, 2) k_mg_jacobi_tb(float *__restrict__ vout, const float *__restrict__ vin,
                                                              const float *__restrict__ f, MGLevel L, float beta,
                                                              float omega, int has_los, int tz) {
     static_assert(S == 2 || S == 3, "2 or 3 sweeps per pass");
@@ -464,7 +491,7 @@ __global__ void __launch_bounds__(kJbThreads) k_mg_jacobi_tb(float *__restrict__
     const int tid = threadIdx.x;
     float *pz_tab = ptab + J::W + J::H;
     if (!has_los)
-        for (int q = tid; q < J::W + J::H + tz + 2 * S; q += kJbThreads) {
+        for (int q = tid; q < J::W + J::H + tz + 2 * S; q += J::Threads) {
             int g;
             if (q < J::W)
                 g = gs_wrap(x0 - S + q, nx);
@@ -479,11 +506,11 @@ __global__ void __launch_bounds__(kJbThreads) k_mg_jacobi_tb(float *__restrict__
     };
     // level-0 loads: element q = row * W + col, one float per element (W is
     // not a multiple of 4 at S = 3); threads past the plane reload element 0
-    constexpr int NL = (J::Plane + kJbThreads - 1) / kJbThreads;
+    constexpr int NL = (J::Plane + J::Threads - 1) / J::Threads;
     int64_t ld_off[NL];
 #pragma unroll
     for (int u = 0; u < NL; u++) {
-        const int q = tid + kJbThreads * u, qq = q < J::Plane ? q : 0;
+        const int q = tid + J::Threads * u, qq = q < J::Plane ? q : 0;
         ld_off[u] = gs_wrap(x0 - S + qq % J::W, nx) + sy * gs_wrap(y0 - S + qq / J::W, ny);
     }
     float pre[NL];
@@ -496,16 +523,19 @@ __global__ void __launch_bounds__(kJbThreads) k_mg_jacobi_tb(float *__restrict__
         float *pl = plane(0, Pl);
 #pragma unroll
         for (int u = 0; u < NL; u++) {
-            const int q = tid + kJbThreads * u;
+            const int q = tid + J::Threads * u;
             if (q < J::Plane) pl[q] = pre[u];
         }
     };
     // inner levels: iteration it handles cell jc = tid + 256 it of the list
-    // (level 1 cells, then level 2 at S = 3); desc = t | xl << 2 | yl << 10
-    int desc[J::Iters];
+    // (level 1 cells, then level 2 at S = 3); cpos = xl | yl << 8 | t << 16
+    // (t = 0: no cell) and the cell's offset in a plane of f (32-bit: the
+    // grid has < 2^31 cells)
+    int cpos[J::Iters];
+    uint32_t foff[J::Iters];
 #pragma unroll
     for (int it = 0; it < J::Iters; it++) {
-        const int jc = tid + kJbThreads * it;
+        const int jc = tid + J::Threads * it;
         int t = 1, jj = jc;
         if (S == 3 && jc >= J::cells(1)) {
             t = 2;
@@ -513,35 +543,37 @@ __global__ void __launch_bounds__(kJbThreads) k_mg_jacobi_tb(float *__restrict__
         }
         const int e = S - t, w = kJbTX + 2 * e;
         const int yl = S - e + jj / w, xl = S - e + jj % w;
-        desc[it] = jc < J::Inner ? (t | (xl << 2) | (yl << 10)) : -1;
+        cpos[it] = jc < J::Inner ? (xl | (yl << 8) | (t << 16)) : (S | (S << 8));
+        foff[it] = (uint32_t)gs_wrap(x0 - S + xl, nx) + (uint32_t)nx * gs_wrap(y0 - S + yl, ny);
     }
+    const uint32_t sz32 = (uint32_t)sz;
     // level t is computed on plane Q = z0 - S + k - 2t + 1, needed on
     // z0 - (S - t) .. z0 + tz - 1 + (S - t)
-    auto active = [&](int t, int k) { return k >= 3 * t - 1 && k <= tz + 2 * S + t - 2; };
-    auto f_off = [&](int xl, int yl, int Q) {
-        return gs_wrap(x0 - S + xl, nx) + sy * gs_wrap(y0 - S + yl, ny) + sz * gs_wrap(Q, nz);
-    };
+    auto active = [&](int t, int k) { return t > 0 && k >= 3 * t - 1 && k <= tz + 2 * S + t - 2; };
     auto fetch_f = [&](int k, float *fr) {
 #pragma unroll
         for (int it = 0; it < J::Iters; it++) {
-            const int d = desc[it], t = d & 3;
-            const int64_t off = d >= 0 && active(t, k) ? f_off((d >> 2) & 255, d >> 10, z0 - S + k - 2 * t + 1) : 0;
+            const int t = cpos[it] >> 16;
+            const uint32_t off =
+                active(t, k) ? foff[it] + sz32 * (uint32_t)gs_wrap(z0 - S + k - 2 * t + 1, nz) : 0u;
             fr[it] = f[off];
         }
     };
-    // level S: 4 consecutive cells, row tid / 16, x 4 (tid % 16)
-    const int oy = tid >> 4, ox = 4 * (tid & 15);
+    // level S: CPT consecutive cells of row tid / (64 / CPT)
+    using FV = typename std::conditional<J::CPT == 4, float4, float2>::type;
+    constexpr int TPR = kJbTX / J::CPT;   // threads per row
+    const int oy = tid / TPR, ox = J::CPT * (tid % TPR);
     const int64_t o_off = (x0 + ox) + sy * (y0 + oy);
-    auto fetch_fo = [&](int k, float4 &fo) {
+    auto fetch_fo = [&](int k, FV &fo) {
         const int Q = z0 - S + k - 2 * S + 1;
-        fo = *reinterpret_cast<const float4 *>(f + sz * (active(S, k) ? Q : z0) + o_off);
+        fo = *reinterpret_cast<const FV *>(f + sz * (active(S, k) ? Q : z0) + o_off);
     };
     // one cell of level t from level t - 1 (local cell xl, yl, local plane ql)
     auto cell = [&](int t, int xl, int yl, int ql, float fv) {
         const float *pm = plane(t - 1, ql - 1), *p0 = plane(t - 1, ql), *pp = plane(t - 1, ql + 1);
         const int wc = yl * J::W + xl;
-        const Coef<float> c = has_los ? coef_p<float>(L, L.losn[0], L.losn[1], L.losn[2], beta)
-                                      : coef_p<float>(L, ptab[xl], ptab[J::W + yl], pz_tab[ql], beta);
+        const Coef<float> c = coef_p<float>(L, has_los ? L.losn[0] : ptab[xl], has_los ? L.losn[1] : ptab[J::W + yl],
+                                            has_los ? L.losn[2] : pz_tab[ql], beta);
 #define A0(o) p0[wc + (o)]
 #define AM(o) pm[wc + (o)]
 #define AP(o) pp[wc + (o)]
@@ -555,7 +587,7 @@ __global__ void __launch_bounds__(kJbThreads) k_mg_jacobi_tb(float *__restrict__
         return v;
     };
 
-    auto step = [&](int k, const float *fcur, float *fnext, const float4 &focur, float4 &fonext) {
+    auto step = [&](int k, const float *fcur, float *fnext, const FV &focur, FV &fonext) {
         if (k <= tz + 2 * S - 1) store(k);
         fetch(z0 - S + k + 1);
         fetch_f(k + 1, fnext);
@@ -563,23 +595,24 @@ __global__ void __launch_bounds__(kJbThreads) k_mg_jacobi_tb(float *__restrict__
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < J::Iters; it++) {
-            const int d = desc[it], t = d & 3;
-            if (d < 0 || !active(t, k)) continue;
-            const int xl = (d >> 2) & 255, yl = d >> 10, ql = k - 2 * t + 1;
+            const int t = cpos[it] >> 16;
+            if (!active(t, k)) continue;
+            const int xl = cpos[it] & 255, yl = (cpos[it] >> 8) & 255, ql = k - 2 * t + 1;
             plane(t, ql)[yl * J::W + xl] = cell(t, xl, yl, ql, fcur[it]);
         }
         if (active(S, k)) {
             const int ql = k - 2 * S + 1;
-            const float fo[4] = {focur.x, focur.y, focur.z, focur.w};
-            float o[4];
+            FV o;
+            const float *fo = reinterpret_cast<const float *>(&focur);
+            float *op = reinterpret_cast<float *>(&o);
 #pragma unroll
-            for (int j = 0; j < 4; j++) o[j] = cell(S, S + ox + j, S + oy, ql, fo[j]);
-            *reinterpret_cast<float4 *>(vout + sz * (z0 - S + ql) + o_off) = make_float4(o[0], o[1], o[2], o[3]);
+            for (int j = 0; j < J::CPT; j++) op[j] = cell(S, S + ox + j, S + oy, ql, fo[j]);
+            *reinterpret_cast<FV *>(vout + sz * (z0 - S + ql) + o_off) = o;
         }
     };
 
     float fa[J::Iters], fb[J::Iters];
-    float4 foa, fob;
+    FV foa, fob;
     fetch(z0 - S);
     fetch_f(0, fa);
     fetch_fo(0, foa);
@@ -697,9 +730,9 @@ __global__ void __launch_bounds__(kRfThreads)
         for (int j = 0; j < 4; j++) {
             const int cy = cy0 + 4 * j;
             const int xl = cx + 1, yl = cy + 1, wc = yl * kRfW + xl;
-            const Coef<double> c = has_los ? coef_p<double>(L, (double)L.losn[0], (double)L.losn[1],
-                                                            (double)L.losn[2], (double)beta)
-                                           : coef_p<double>(L, ptab[xl], ptab[kRfW + yl], pz_tab[ql], (double)beta);
+            const Coef<double> c = coef_p<double>(L, has_los ? (double)L.losn[0] : ptab[xl],
+                                                  has_los ? (double)L.losn[1] : ptab[kRfW + yl],
+                                                  has_los ? (double)L.losn[2] : pz_tab[ql], (double)beta);
 #define A0(o) p0[wc + (o)]
 #define AM(o) pm[wc + (o)]
 #define AP(o) pp[wc + (o)]
@@ -1078,7 +1111,10 @@ static int gs4_tz(const MGLevel &L) {
     return tz;
 }
 
-bool mg_gs4_ok(const MGLevel &L) { return L.n[0] % kGsTX == 0 && L.n[1] % kGsTY == 0 && L.n[2] % 8 == 0; }
+bool mg_gs4_ok(const MGLevel &L) {
+    return L.n[0] % kGsTX == 0 && L.n[1] % kGsTY == 0 && L.n[2] % 8 == 0 &&
+           (int64_t)L.n[0] * L.n[1] * L.n[2] < ((int64_t)1 << 31);
+}
 
 int64_t mg_gs4_blocks(const MGLevel &L) {
     return (int64_t)(L.n[0] / kGsTX) * (L.n[1] / kGsTY) * (L.n[2] / gs4_tz(L));
@@ -1102,7 +1138,8 @@ static int jacobi_tb_tz(const MGLevel &L) {
 }
 
 bool mg_jacobi_tb_ok(const MGLevel &L) {
-    return L.n[0] % kJbTX == 0 && L.n[1] % kJbTY == 0 && L.n[2] % 8 == 0 && L.n[2] >= 8;
+    return L.n[0] % kJbTX == 0 && L.n[1] % kJbTY == 0 && L.n[2] % 8 == 0 && L.n[2] >= 8 &&
+           (int64_t)L.n[0] * L.n[1] * L.n[2] < ((int64_t)1 << 31);
 }
 
 hipError_t launch_mg_jacobi_tb(float *vout, const float *vin, const float *f, const MGLevel &L, float beta,
@@ -1111,9 +1148,11 @@ hipError_t launch_mg_jacobi_tb(float *vout, const float *vin, const float *f, co
     const int tz = jacobi_tb_tz(L);
     const dim3 grid((unsigned)(L.n[0] / kJbTX), (unsigned)(L.n[1] / kJbTY), (unsigned)(L.n[2] / tz));
     if (sweeps == 2)
-        hipLaunchKernelGGL(k_mg_jacobi_tb<2>, grid, dim3(kJbThreads), 0, s, vout, vin, f, L, beta, omega, has_los, tz);
+        hipLaunchKernelGGL(k_mg_jacobi_tb<2>, grid, dim3(JbShape<2>::Threads), 0, s, vout, vin, f, L, beta, omega,
+                           has_los, tz);
     else
-        hipLaunchKernelGGL(k_mg_jacobi_tb<3>, grid, dim3(kJbThreads), 0, s, vout, vin, f, L, beta, omega, has_los, tz);
+        hipLaunchKernelGGL(k_mg_jacobi_tb<3>, grid, dim3(JbShape<3>::Threads), 0, s, vout, vin, f, L, beta, omega,
+                           has_los, tz);
     return hipGetLastError();
 }
 
