@@ -64,6 +64,16 @@ struct baorec_ctx {
     int force_pair = -1;
     // learned warm start: the network of baorec_warmstart_init / _load
     baorec::UNet *net = nullptr;
+    // HIP graphs of baorec_run_positions_graph, keyed by its arguments; valid
+    // while no library buffer has been (re)allocated since the capture
+    struct GraphEntry {
+        std::string key;
+        uint64_t buf_gen;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t buf_gen = 0;   // counts buffer (re)allocations
+    int capturing = 0;      // inside a stream capture: no host synchronisation
     // comoving-distance table of the last cosmology used
     int cosmo_valid = 0;
     baorec_cosmology cosmo{};
@@ -94,6 +104,8 @@ void hipchk(baorec_ctx *c, hipError_t e, const char *what) {
 void *buf(baorec_ctx *c, const std::string &name, size_t bytes) {
     DevBuf &b = c->bufs[name];
     if (b.bytes < bytes) {
+        if (c->capturing) fail(c, BAOREC_ERR_HIP, "buffer allocation inside a graph capture");
+        c->buf_gen++;
         if (b.p) {
             hipchk(c, hipStreamSynchronize(c->stream), "sync before realloc");
             (void)hipFree(b.p);
@@ -201,6 +213,7 @@ void prof_resolve(baorec_ctx *c) {
 }
 
 void sync(baorec_ctx *c) {
+    if (c->capturing) return;   // the graph's launcher synchronises after the replay
     hipchk(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     if (!c->pending.empty()) prof_resolve(c);
 }
@@ -279,6 +292,7 @@ void slot_from_params(baorec_ctx *c, Slot *s, const baorec_params *p, double cou
 }
 
 Slot read_slot(baorec_ctx *c, const Slot *s) {
+    if (c->capturing) fail(c, BAOREC_ERR_UNSUPPORTED, "a host read of device scalars inside a graph capture");
     Slot h;
     hipchk(c, hipMemcpyAsync(&h, s, sizeof(Slot), hipMemcpyDeviceToHost, c->stream), "D2H slot");
     sync(c);
@@ -359,7 +373,7 @@ void scatter(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *slot, f
     a.rho = rho;
     a.rstride = G.R();
     a.wmax = bufT<uint32_t>(c, "sc_wmax", 64);
-    hipchk(c, hipMemsetAsync(a.wmax, 0, sizeof(uint32_t) * 64, c->stream), "memset");
+    hipchk(c, launch_zero_words(a.wmax, 64, c->stream), "zero");
     (void)tag;
     if (sort_fits(a)) {
         const int nblk = sort_nblk(n);
@@ -447,7 +461,7 @@ void scatter_pair(baorec_ctx *c, const Grid &G, const baorec_params *p, Slot *sl
     a.rho = rho;
     a.rstride = G.R();
     a.wmax = bufT<uint32_t>(c, "sc_wmax", 64);
-    hipchk(c, hipMemsetAsync(a.wmax, 0, sizeof(uint32_t) * 64, c->stream), "memset");
+    hipchk(c, launch_zero_words(a.wmax, 64, c->stream), "zero");
     uint32_t *H = bufT<uint32_t>(c, "sc_hist", (size_t)nblk * T);
     {
         ProfScope ps(c, "scatter_bin", 28.0 * ntot + 16.0 * ntot * 1.2);
@@ -894,7 +908,7 @@ void mg_jacobi_n(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int n) 
 void zero_level(baorec_ctx *c, MGPyr &P, int l) {
     const int64_t n = P.g[l].R();
     if (P.dbl[l]) {
-        hipchk(c, hipMemsetAsync(vcurd(P, l), 0, sizeof(double) * n, c->stream), "memset");
+        hipchk(c, launch_zero_words(vcurd(P, l), 2 * n, c->stream), "zero");
     } else {
         hipchk(c, launch_fill(vcur(P, l), n, 0.f, c->stream), "fill");
     }
@@ -1310,13 +1324,13 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     // FP64 scalars: [0] N alpha, [1] sum f^2, [2] |r|^2, [3] |dr|^2, [4] sum e,
     // [5] |r0|^2, [6] sum r, [7] sum f
     double *red = bufT<double>(c, "mg_red", 8);
-    hipchk(c, hipMemsetAsync(red, 0, sizeof(double) * 8, c->stream), "memset");
+    hipchk(c, launch_zero_words(red, 16, c->stream), "zero");
     hipchk(c, launch_sum<float>(f0, N, red + 7, red_part(c), c->stream), "sum f");
     hipchk(c, hipMemcpyAsync(red, red + 7, sizeof(double), hipMemcpyDeviceToDevice, c->stream), "copy");
     hipchk(c, launch_sumsq<float>(f0, N, red + 1, red_part(c), c->stream), "norm f");
     float *rb[2] = {bufT<float>(c, "mg_r0", N), bufT<float>(c, "mg_r1", N)};
     double *phi = bufT<double>(c, "mg_phi64", N);   // FP64 solution
-    hipchk(c, hipMemsetAsync(phi, 0, sizeof(double) * N, c->stream), "memset");
+    hipchk(c, launch_zero_words(phi, 2 * N, c->stream), "zero");
     MGPyr P = mg_build(c, G, p, slot, v0, rb[0], p->fp64_coarse_max);
     // With sides that are multiples of 64, 16 and 8 the refinement step
     // (phi += e - mean(e), r = f - alpha - A phi, sum r) is one z-marching
@@ -1456,6 +1470,7 @@ Inputs stage_catalogs(baorec_ctx *c, int mem, float *x, float *y, float *z, cons
 }
 
 void check_err_flag(baorec_ctx *c, const Slot *slot) {
+    if (c->capturing) return;   // checked after the replay
     Slot h = read_slot(c, slot);
     if (h.err) fail(c, BAOREC_ERR_OUT_OF_BOX, "a particle indexes outside the grid (BoundsError)");
 }
@@ -1623,7 +1638,7 @@ void batch_group(baorec_ctx *c, const Grid &G, const baorec_params *p, const int
     a.rho = rho;
     a.rstride = R;
     a.wmax = bufT<uint32_t>(c, "bt_wmax", maxnb);
-    hipchk(c, hipMemsetAsync(a.wmax, 0, sizeof(uint32_t) * nb, c->stream), "memset");
+    hipchk(c, launch_zero_words(a.wmax, nb, c->stream), "zero");
     const double pb = (double)ntot * (a.mutate ? 40.0 : 28.0) + 16.0 * ntot * 1.2;
     if (sort_fits(a)) {
         uint32_t *H = bufT<uint32_t>(c, "bt_hist", (size_t)std::max(g.nblk, 1) * T);
@@ -1793,7 +1808,7 @@ CosmoTable cosmo_table(baorec_ctx *c, const baorec_cosmology *cp) {
 
 int *cosmo_err(baorec_ctx *c) {
     int *e = bufT<int>(c, "cosmo_err", 1);
-    hipchk(c, hipMemsetAsync(e, 0, sizeof(int), c->stream), "memset");
+    hipchk(c, launch_zero_words(e, 1, c->stream), "zero");
     return e;
 }
 
@@ -1960,6 +1975,7 @@ int baorec_destroy(baorec_ctx *c) {
     if (!c) return BAOREC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     for (auto &kv : c->bufs)
         if (kv.second.p) (void)hipFree(kv.second.p);
     unet_destroy(c->net);
@@ -1973,10 +1989,9 @@ const char *baorec_last_error(const baorec_ctx *c) {
     return c->err.c_str();
 }
 
-static int run_impl(baorec_ctx *c, int kind, baorec_params *p, const int32_t grid[3], int mem,
-                    float *x, float *y, float *z, const float *w, int64_t n, const float *rx,
-                    const float *ry, const float *rz, const float *rw, int64_t nr) {
-    return guarded(c, [&] {
+static void run_body(baorec_ctx *c, int kind, baorec_params *p, const int32_t grid[3], int mem, float *x, float *y,
+                     float *z, const float *w, int64_t n, const float *rx, const float *ry, const float *rz,
+                     const float *rw, int64_t nr) {
         check_params(c, p);
         Grid G = check_grid(c, grid, true);
         if (mem != BAOREC_MEM_HOST && mem != BAOREC_MEM_DEVICE)
@@ -2004,7 +2019,12 @@ static int run_impl(baorec_ctx *c, int kind, baorec_params *p, const int32_t gri
         c->result_gen++;
         // the spectrum of the iterative result is left in cwork[0]
         if (iter && p->n_iter > 0) c->spec_gen = c->result_gen;
-    });
+}
+
+static int run_impl(baorec_ctx *c, int kind, baorec_params *p, const int32_t grid[3], int mem,
+                    float *x, float *y, float *z, const float *w, int64_t n, const float *rx,
+                    const float *ry, const float *rz, const float *rw, int64_t nr) {
+    return guarded(c, [&] { run_body(c, kind, p, grid, mem, x, y, z, w, n, rx, ry, rz, rw, nr); });
 }
 
 int baorec_run_iterative(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int mem, float *x,
@@ -2104,10 +2124,9 @@ int baorec_reconstructed_potential(baorec_ctx *c, baorec_params *p, const int32_
     });
 }
 
-static int shifts_impl(baorec_ctx *c, const baorec_params *p, int mem, const float *x, const float *y,
-                       const float *z, int64_t n, const float *field3d, const int32_t grid[3], int kind,
-                       int field, float *ox, float *oy, float *oz, int positions) {
-    return guarded(c, [&] {
+static void shifts_body(baorec_ctx *c, const baorec_params *p, int mem, const float *x, const float *y,
+                        const float *z, int64_t n, const float *field3d, const int32_t grid[3], int kind,
+                        int field, float *ox, float *oy, float *oz, int positions) {
         check_params(c, p);
         check_box(c, p);
         if (field < 0 || field > 2) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "field must be disp, rsd or sum");
@@ -2179,6 +2198,13 @@ static int shifts_impl(baorec_ctx *c, const baorec_params *p, int mem, const flo
         copy_out(c, mem, oy, o2, n);
         copy_out(c, mem, oz, o3, n);
         sync(c);
+}
+
+static int shifts_impl(baorec_ctx *c, const baorec_params *p, int mem, const float *x, const float *y,
+                       const float *z, int64_t n, const float *field3d, const int32_t grid[3], int kind,
+                       int field, float *ox, float *oy, float *oz, int positions) {
+    return guarded(c, [&] {
+        shifts_body(c, p, mem, x, y, z, n, field3d, grid, kind, field, ox, oy, oz, positions);
     });
 }
 
@@ -2193,6 +2219,109 @@ int baorec_reconstructed_positions(baorec_ctx *c, const baorec_params *p, int me
                                    const int32_t grid[3], int kind, int field, float *ox, float *oy,
                                    float *oz) {
     return shifts_impl(c, p, mem, x, y, z, n, field3d, grid, kind, field, ox, oy, oz, 1);
+}
+
+// The arguments that fix the launch sequence of baorec_run_positions_graph.
+static std::string graph_key(const baorec_params *p, const int32_t grid[3], const void *const ptr[7], int64_t n,
+                             const baorec_ctx *c) {
+    std::string k;
+    auto put = [&](const void *v, size_t b) { k.append(static_cast<const char *>(v), b); };
+    put(grid, 3 * sizeof(int32_t));
+    put(ptr, 7 * sizeof(void *));
+    put(&n, sizeof n);
+    put(&p->bias, sizeof(float));
+    put(&p->f, sizeof(float));
+    put(&p->smoothing_radius, sizeof(float));
+    put(p->box_size, sizeof p->box_size);
+    put(p->box_min, sizeof p->box_min);
+    put(&p->has_los, sizeof(int32_t));
+    put(p->los, sizeof p->los);
+    put(&p->n_iter, sizeof(int32_t));
+    put(&p->beta, sizeof(float));
+    put(&p->mas, sizeof(int32_t));
+    put(&p->ran_min, sizeof(float));
+    put(&p->mutate_inputs, sizeof(int32_t));
+    // autotuned choices change the kernels the sequence launches
+    for (const auto &kv : c->tune_tile) {
+        put(kv.first.data(), sizeof(int64_t) * kv.first.size());
+        put(kv.second.data(), sizeof(int) * kv.second.size());
+    }
+    for (const auto &kv : c->tune_pair) {
+        put(kv.first.data(), sizeof(int) * kv.first.size());
+        put(&kv.second, sizeof(int));
+    }
+    return k;
+}
+
+int baorec_run_positions_graph(baorec_ctx *c, baorec_params *p, const int32_t grid[3], float *x, float *y, float *z,
+                               const float *w, int64_t n, float *ox, float *oy, float *oz) {
+    return guarded(c, [&] {
+        if (!p || !grid) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "NULL argument");
+        if (n <= 0 || !x || !y || !z || !w || !ox || !oy || !oz)
+            fail(c, BAOREC_ERR_INVALID_ARGUMENT, "positions, weights and outputs (device) are required");
+        check_params(c, p);
+        check_box(c, p);
+        const void *const ptr[7] = {x, y, z, w, ox, oy, oz};
+        const std::string key = graph_key(p, grid, ptr, n, c);
+        auto run = [&] {
+            run_body(c, BAOREC_ITERATIVE, p, grid, BAOREC_MEM_DEVICE, x, y, z, w, n, nullptr, nullptr, nullptr,
+                     nullptr, 0);
+            shifts_body(c, p, BAOREC_MEM_DEVICE, x, y, z, n, nullptr, grid, BAOREC_ITERATIVE, 2, ox, oy, oz, 1);
+        };
+        for (size_t i = 0; i < c->graphs.size(); i++) {
+            auto &g = c->graphs[i];
+            if (g.key != key) continue;
+            if (g.buf_gen != c->buf_gen) {   // a buffer it points into was reallocated
+                (void)hipGraphExecDestroy(g.exec);
+                c->graphs.erase(c->graphs.begin() + (long)i);
+                break;
+            }
+            hipchk(c, hipGraphLaunch(g.exec, c->stream), "hipGraphLaunch");
+            // the bookkeeping run_body and shifts_body do on the host
+            check_grid(c, grid, true);
+            Slot *slot = slot_buf(c);
+            check_err_flag(c, slot);   // synchronises
+            c->has_result = 1;
+            c->result_kind = BAOREC_ITERATIVE;
+            for (int a = 0; a < 3; a++) c->result_grid[a] = grid[a];
+            c->result_gen++;
+            if (p->n_iter > 0) c->spec_gen = c->result_gen;
+            c->disp_valid = 1;
+            c->disp_gen = c->result_gen;
+            const float kb[6] = {p->box_size[0], p->box_size[1], p->box_size[2],
+                                 p->box_min[0], p->box_min[1], p->box_min[2]};
+            memcpy(c->disp_box, kb, sizeof kb);
+            return;
+        }
+        // first call with these arguments: run eagerly (this allocates every
+        // buffer the sequence uses), then capture the same sequence
+        run();
+        const int prof = c->prof_on;
+        c->prof_on = 0;
+        c->capturing = 1;
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            try {
+                run();
+            } catch (const Fail &) {
+                (void)hipStreamEndCapture(c->stream, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                c->capturing = 0;
+                c->prof_on = prof;
+                throw;
+            }
+            e = hipStreamEndCapture(c->stream, &graph);
+        }
+        c->capturing = 0;
+        c->prof_on = prof;
+        hipchk(c, e, "stream capture");
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hipchk(c, e, "hipGraphInstantiate");
+        c->graphs.push_back({key, c->buf_gen, exec});
+    });
 }
 
 int baorec_setup_box(baorec_ctx *c, int mem, const float *x, const float *y, const float *z, int64_t n,
@@ -2778,7 +2907,7 @@ int baorec_power_multipoles(baorec_ctx *c, const baorec_pk_params *pk, const int
             hipchk(c, launch_pk(a, dout, c->stream), "power multipoles");
         }
         // FP64 totals sum(w), sum(w^2) for the normalisation and shot noise
-        hipchk(c, hipMemsetAsync(dout + 5 * nb, 0, 2 * sizeof(double), c->stream), "memset");
+        hipchk(c, launch_zero_words(dout + 5 * nb, 4, c->stream), "zero");
         hipchk(c, launch_sum<float>(wd, n, dout + 5 * nb, red_part(c), c->stream), "sum w");
         hipchk(c, launch_sumsq<float>(wd, n, dout + 5 * nb + 1, red_part(c), c->stream), "sum w2");
         std::vector<double> h((size_t)5 * nb + 2);
@@ -3051,7 +3180,7 @@ int baorec_run_batch(baorec_ctx *c, baorec_params *p, const int32_t grid[3], int
         hipchk(c, hipMemcpyAsync(d32, t32.data(), sizeof(int) * t32.size(), hipMemcpyHostToDevice, c->stream),
                "H2D");
         int32_t *err = bufT<int32_t>(c, "bt_err", 1);
-        hipchk(c, hipMemsetAsync(err, 0, sizeof(int32_t), c->stream), "memset");
+        hipchk(c, launch_zero_words(err, 1, c->stream), "zero");
         Inputs in = stage_catalogs(c, mem, x, y, z, w, n, nullptr, nullptr, nullptr, nullptr, 0);
         float *o1 = stage_out(c, mem, ox, n, "bt_ox");
         float *o2 = stage_out(c, mem, oy, n, "bt_oy");

@@ -186,6 +186,8 @@ hipError_t launch_scatter(ScatterArgs &a, uint32_t *scan_tmp, hipStream_t s);
 // accumulate: 0 overwrite, 1 add into rho, 2 rho already zero (empty tiles skipped)
 hipError_t launch_scatter_accum(const ScatterArgs &a, int accumulate, hipStream_t s);
 // Zero n floats with a fill kernel (graph-capture safe).
+// Zero nwords 32-bit words (a 4-byte aligned buffer) with a kernel.
+hipError_t launch_zero_words(void *a, int64_t nwords, hipStream_t s);
 hipError_t launch_zero(float *a, int64_t n, hipStream_t s);
 // Blocked counting sort (no global atomics); valid when sort_fits(a).
 bool sort_fits(const ScatterArgs &a);

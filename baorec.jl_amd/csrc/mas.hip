@@ -797,8 +797,23 @@ __global__ void k_zero4(float4 *a, int64_t n4) {
         a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+__global__ void k_zero_words(uint32_t *a, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) a[i] = 0u;
+}
+
+// Zero fills are kernels, not hipMemsetAsync: memset nodes in captured
+// graphs have given wrong results on replays.
+hipError_t launch_zero_words(void *a, int64_t nwords, hipStream_t s) {
+    if (nwords <= 0) return hipSuccess;
+    int64_t nb = (nwords + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_zero_words, dim3((unsigned)nb), dim3(256), 0, s, static_cast<uint32_t *>(a), nwords);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero(float *a, int64_t n, hipStream_t s) {
-    if ((reinterpret_cast<uintptr_t>(a) & 15) || (n & 3)) return hipMemsetAsync(a, 0, sizeof(float) * n, s);
+    if ((reinterpret_cast<uintptr_t>(a) & 15) || (n & 3)) return launch_zero_words(a, n, s);
     const int64_t n4 = n / 4;
     int64_t nb = (n4 + 255) / 256;
     if (nb > 8192) nb = 8192;
@@ -816,7 +831,7 @@ int64_t scatter_capacity(int64_t n, int tsc) {
 
 hipError_t launch_scatter(ScatterArgs &a, uint32_t * /*scan_tmp*/, hipStream_t s) {
     const int64_t ntiles = (int64_t)a.ntx * a.nty * a.ntz * a.ncat;
-    hipError_t e = hipMemsetAsync(a.count, 0, sizeof(uint32_t) * ntiles, s);
+    hipError_t e = launch_zero_words(a.count, ntiles, s);
     if (e != hipSuccess) return e;
     if (a.ntotal > 0) {
         const unsigned nb = (unsigned)((a.ntotal + 255) / 256);

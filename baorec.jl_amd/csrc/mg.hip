@@ -248,6 +248,21 @@ constexpr int kGsCPT = kGsTX * kGsTY / kGsThreads;                     // cells 
 
 __device__ __forceinline__ int gs_word(int xl, int yl) { return yl * kGsW + (xl & 1) * (kGsW / 2) + (xl >> 1); }
 __device__ __forceinline__ int gs_wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// The tile of this workgroup, XCD-aware: workgroup i (x fastest) runs on XCD
+// i % 8, so the tiles are renumbered for each XCD to take a contiguous run of
+// them (neighbouring tiles share its L2 and their halo rows).  Returns the
+// tile's linear index; (tx, ty, tz) its coordinates in the launch grid.
+__device__ __forceinline__ int mg_tile(int &tx, int &ty, int &tz) {
+    const int nb = gridDim.x * gridDim.y * gridDim.z;
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int q = nb >> 3, r = nb & 7, x = bid & 7;
+    const int t = x * q + (x < r ? x : r) + (bid >> 3);
+    tx = t % gridDim.x;
+    ty = (t / gridDim.x) % gridDim.y;
+    tz = t / (gridDim.x * gridDim.y);
+    return t;
+}
 __device__ __forceinline__ double wave_sum64(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -271,7 +286,9 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
     double acc = 0.0;
     __shared__ float ptab[kGsW + kGsRows + kGsMaxTz + 8];   // p of LDS column | row | local plane
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
-    const int x0 = blockIdx.x * kGsTX, y0 = blockIdx.y * kGsTY, z0 = blockIdx.z * tz;
+    int tile_x, tile_y, tile_z;
+    const int tile = mg_tile(tile_x, tile_y, tile_z);
+    const int x0 = tile_x * kGsTX, y0 = tile_y * kGsTY, z0 = tile_z * tz;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     const int tid = threadIdx.x;
     float *tz_tab = ptab + kGsW + kGsRows;
@@ -446,7 +463,7 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
         if (tid == 0) {
             double t = 0.0;
             for (int w = 0; w < kGsThreads / 64; w++) t += red[w];
-            part[blockIdx.x + gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] = t;
+            part[tile] = t;
         }
     }
 }
@@ -486,7 +503,9 @@ __global__ void __launch_bounds__(JbShape<S>::Threads, 2) k_mg_jacobi_tb(float *
     __shared__ __align__(16) float lds[S * kJbRing * J::Plane];
     __shared__ float ptab[J::W + J::H + kJbMaxTz + 2 * S];
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
-    const int x0 = blockIdx.x * kJbTX, y0 = blockIdx.y * kJbTY, z0 = blockIdx.z * tz;
+    int tile_x, tile_y, tile_z;
+    mg_tile(tile_x, tile_y, tile_z);
+    const int x0 = tile_x * kJbTX, y0 = tile_y * kJbTY, z0 = tile_z * tz;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     const int tid = threadIdx.x;
     float *pz_tab = ptab + J::W + J::H;
@@ -647,7 +666,9 @@ __global__ void __launch_bounds__(kRfThreads)
     __shared__ double ptab[kRfW + kRfRows + kRfMaxTz + 2];   // p of LDS column | row | local plane
     __shared__ double red[kRfThreads / 64];
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
-    const int x0 = blockIdx.x * kRfTX, y0 = blockIdx.y * kRfTY, z0 = blockIdx.z * tz;
+    int tile_x, tile_y, tile_z;
+    const int tile = mg_tile(tile_x, tile_y, tile_z);
+    const int x0 = tile_x * kRfTX, y0 = tile_y * kRfTY, z0 = tile_z * tz;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     const double count = (double)sz * nz;
     const double me = e ? *esum / count : 0.0;
@@ -753,8 +774,98 @@ __global__ void __launch_bounds__(kRfThreads)
     if ((tid & 63) == 0) red[tid >> 6] = acc;
     __syncthreads();
     if (tid == 0)
-        part[blockIdx.x + gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] =
-            (red[0] + red[1]) + (red[2] + red[3]);
+        part[tile] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// r = f - A v in FP32 (k_mg_residual<float>'s arithmetic, so the same bits)
+// as one z-marching pass: a workgroup owns a 64 x 16 column of tz planes,
+// loads each plane of v once into a ring of 4 LDS planes of 66 x 18 (a halo
+// of 1) and computes the plane below.  f is staged through LDS with the
+// same one-step lead; r goes out in coalesced rows.
+__global__ void __launch_bounds__(kRfThreads)
+    k_mg_residual_zm(float *__restrict__ r, const float *__restrict__ v, const float *__restrict__ f, MGLevel L,
+                     float beta, int has_los, int tz) {
+    __shared__ float ring[kRfRing * kRfPlane];
+    __shared__ float ptab[kRfW + kRfRows + kRfMaxTz + 2];
+    __shared__ __align__(16) float ftile[2][kRfTX * kRfTY];
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    int tile_x, tile_y, tile_z;
+    mg_tile(tile_x, tile_y, tile_z);
+    const int x0 = tile_x * kRfTX, y0 = tile_y * kRfTY, z0 = tile_z * tz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int tid = threadIdx.x;
+    float *pz_tab = ptab + kRfW + kRfRows;
+    if (!has_los)
+        for (int q = tid; q < kRfW + kRfRows + tz + 2; q += kRfThreads) {
+            int g;
+            if (q < kRfW)
+                g = gs_wrap(x0 - 1 + q, nx);
+            else if (q < kRfW + kRfRows)
+                g = nx + gs_wrap(y0 - 1 + q - kRfW, ny);
+            else
+                g = nx + ny + gs_wrap(z0 - 1 + q - kRfW - kRfRows, nz);
+            ptab[q] = L.pv[g];
+        }
+    int64_t ld_off[kRfLoads];
+#pragma unroll
+    for (int u = 0; u < kRfLoads; u++) {
+        const int q = tid + kRfThreads * u, qq = q < kRfPlane ? q : 0;
+        ld_off[u] = gs_wrap(x0 - 1 + qq % kRfW, nx) + sy * gs_wrap(y0 - 1 + qq / kRfW, ny);
+    }
+    float pre[kRfLoads];
+    auto fetch = [&](int P) {
+        const int64_t zo = sz * gs_wrap(min(P, z0 + tz), nz);
+#pragma unroll
+        for (int u = 0; u < kRfLoads; u++) pre[u] = v[zo + ld_off[u]];
+    };
+    auto store = [&](int P) {
+        float *pl = ring + (P - z0 + 1) % kRfRing * kRfPlane;
+#pragma unroll
+        for (int u = 0; u < kRfLoads; u++) {
+            const int q = tid + kRfThreads * u;
+            if (q < kRfPlane) pl[q] = pre[u];
+        }
+    };
+    const int64_t f_off = (x0 + 4 * (tid & 15)) + sy * (y0 + (tid >> 4));
+    float4 fpre = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto fetch_f = [&](int Q) { fpre = *reinterpret_cast<const float4 *>(f + sz * min(Q, z0 + tz - 1) + f_off); };
+    const int cx = tid & 63, cy0 = tid >> 6;
+    const int64_t c_off = (x0 + cx) + sy * (y0 + cy0);
+    fetch(z0 - 1);
+    fetch_f(z0);
+    for (int k = 0; k < tz + 2; k++) {
+        const int P = z0 - 1 + k;
+        store(P);
+        if (k >= 2) *reinterpret_cast<float4 *>(&ftile[k & 1][4 * tid]) = fpre;   // f of plane P - 1
+        fetch(P + 1);
+        if (k >= 2) fetch_f(P);
+        __syncthreads();
+        if (k < 2) continue;
+        const int Q = P - 1, ql = Q - z0 + 1;
+        const float *pm = ring + (ql - 1) % kRfRing * kRfPlane;
+        const float *p0 = ring + ql % kRfRing * kRfPlane;
+        const float *pp = ring + (ql + 1) % kRfRing * kRfPlane;
+        const float *ft = ftile[k & 1];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int cy = cy0 + 4 * j;
+            const int xl = cx + 1, yl = cy + 1, wc = yl * kRfW + xl;
+            const Coef<float> c = coef_p<float>(L, has_los ? L.losn[0] : ptab[xl], has_los ? L.losn[1] : ptab[kRfW + yl],
+                                                has_los ? L.losn[2] : pz_tab[ql], beta);
+#define A0(o) p0[wc + (o)]
+#define AM(o) pm[wc + (o)]
+#define AP(o) pp[wc + (o)]
+            const Nbr<float> sn = nbr_of<float>(
+                c, A0(1), A0(-1), A0(kRfW), A0(-kRfW), AP(0), AM(0), A0(kRfW + 1), A0(-kRfW - 1), A0(kRfW - 1),
+                A0(-kRfW + 1), AP(1), AM(-1), AP(-1), AM(1), AP(kRfW), AM(-kRfW), AP(-kRfW), AM(kRfW));
+            float t = 2.0f * (c.gpx2 + c.gpy2 + c.gpz2) * A0(0) - (sn.A + sn.B + sn.C + sn.X);
+#undef A0
+#undef AM
+#undef AP
+            if (!has_los) t -= sn.G;
+            r[sz * Q + c_off + sy * 4 * j] = ft[cy * kRfTX + cx] - t;
+        }
+    }
 }
 
 // r += (float)(-(*rsum) / n) when rsum (the mean removal of k_add_const),
@@ -881,6 +992,21 @@ __global__ void k_mg_prolong(T *__restrict__ fine, const U *__restrict__ v, int 
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     if (ix >= 2 * cx) return;
     cell_prolong<T, U>(fine, v, cx, cy, cz, add, ix, blockIdx.y, blockIdx.z);
+}
+
+// Trilinear prolongation (+add) with two fine x cells per thread (an odd and
+// an even index: no divergence between the cases of cell_prolong, whose
+// arithmetic it keeps) and a 4-plane z loop per workgroup.
+template <typename T, typename U>
+__global__ void k_mg_prolong2(T *__restrict__ fine, const U *__restrict__ v, int cx, int cy, int cz, int add) {
+    const int fx = 2 * cx;
+    const int ix = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+    if (ix >= fx) return;
+    const int iy = blockIdx.y;
+    for (int iz = 4 * blockIdx.z; iz < 4 * blockIdx.z + 4; iz++) {
+        cell_prolong<T, U>(fine, v, cx, cy, cz, add, ix, iy, iz);
+        cell_prolong<T, U>(fine, v, cx, cy, cz, add, ix + 1, iy, iz);
+    }
 }
 
 // ------------------------------------------------------------ coarse levels
@@ -1060,6 +1186,17 @@ __global__ void k_fill(float *a, int64_t n, float v) {
 
 // ------------------------------------------------------------ launchers
 
+// z planes per workgroup of k_mg_refine: at least 1024 workgroups
+static int refine_tz(const MGLevel &L) {
+    const int64_t cols = (int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY);
+    int tz = kRfMaxTz;
+    while (tz > 8 && (L.n[2] % tz || cols * (L.n[2] / tz) < 1024)) tz /= 2;
+    return tz;
+}
+
+bool mg_refine_ok(const MGLevel &L) { return L.n[0] % kRfTX == 0 && L.n[1] % kRfTY == 0 && L.n[2] % 8 == 0; }
+
+
 static inline dim3 cell_grid(int nx, int ny, int nz, int bx) {
     return dim3((unsigned)((nx + bx - 1) / bx), (unsigned)ny, (unsigned)nz);
 }
@@ -1095,6 +1232,14 @@ hipError_t launch_mg_residual(float *r, const float *v, const float *f, const MG
 template <typename T>
 hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, float beta,
                                 int has_los, hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {   // z-marching on grids of >= 128^3 cells
+        if (mg_refine_ok(L) && (int64_t)L.n[0] * L.n[1] * L.n[2] >= ((int64_t)1 << 21)) {
+            const int tz = refine_tz(L);
+            const dim3 grid((unsigned)(L.n[0] / kRfTX), (unsigned)(L.n[1] / kRfTY), (unsigned)(L.n[2] / tz));
+            hipLaunchKernelGGL(k_mg_residual_zm, grid, dim3(kRfThreads), 0, s, r, v, f, L, beta, has_los, tz);
+            return hipGetLastError();
+        }
+    }
     dim3 grid, block;
     stencil_shape(L, grid, block);
     hipLaunchKernelGGL(k_mg_residual<T>, grid, block, 0, s, r, v, f, L, beta, has_los);
@@ -1155,16 +1300,6 @@ hipError_t launch_mg_jacobi_tb(float *vout, const float *vin, const float *f, co
                            has_los, tz);
     return hipGetLastError();
 }
-
-// z planes per workgroup of k_mg_refine: at least 1024 workgroups
-static int refine_tz(const MGLevel &L) {
-    const int64_t cols = (int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY);
-    int tz = kRfMaxTz;
-    while (tz > 8 && (L.n[2] % tz || cols * (L.n[2] / tz) < 1024)) tz /= 2;
-    return tz;
-}
-
-bool mg_refine_ok(const MGLevel &L) { return L.n[0] % kRfTX == 0 && L.n[1] % kRfTY == 0 && L.n[2] % 8 == 0; }
 
 int64_t mg_refine_blocks(const MGLevel &L) {
     return (int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY) * (L.n[2] / refine_tz(L));
@@ -1229,6 +1364,13 @@ template <typename T, typename U>
 hipError_t launch_mg_prolong_t(T *fine, const U *coarse, const int coarse_n[3], int add,
                                hipStream_t s) {
     const int fx = 2 * coarse_n[0], fy = 2 * coarse_n[1], fz = 2 * coarse_n[2];
+    if (fz % 4 == 0 && (int64_t)fx * fy * fz >= ((int64_t)1 << 21)) {   // two cells per thread, 4 planes
+        const int b2 = bx_for(fx / 2);
+        hipLaunchKernelGGL((k_mg_prolong2<T, U>), dim3((unsigned)((fx / 2 + b2 - 1) / b2), (unsigned)fy,
+                                                       (unsigned)(fz / 4)),
+                           dim3(b2), 0, s, fine, coarse, coarse_n[0], coarse_n[1], coarse_n[2], add);
+        return hipGetLastError();
+    }
     const int bx = bx_for(fx);
     hipLaunchKernelGGL((k_mg_prolong<T, U>), cell_grid(fx, fy, fz, bx), dim3(bx), 0, s, fine, coarse,
                        coarse_n[0], coarse_n[1], coarse_n[2], add);

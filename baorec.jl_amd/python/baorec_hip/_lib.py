@@ -128,6 +128,7 @@ SIGNATURES = {
     "baorec_x_vec": (INT, [I32P, FP, FP, FP, FP, FP]),
     "baorec_mg_jacobi": (INT, [P, PP, I32P, INT, FP, FP]),
     "baorec_mg_gauss_seidel": (INT, [P, PP, I32P, INT, FP, FP, INT, INT]),
+    "baorec_run_positions_graph": (INT, [P, PP, I32P, FP, FP, FP, FP, I64, FP, FP, FP]),
     "baorec_mg_residual": (INT, [P, PP, I32P, INT, FP, FP, FP]),
     "baorec_mg_reduce": (INT, [P, I32P, INT, FP, FP]),
     "baorec_mg_prolong": (INT, [P, I32P, INT, FP, FP, INT]),

@@ -429,6 +429,26 @@ def run_batch(recon: IterativeRecon, grid_size, x, y, z, w, offsets):
     return tuple(outs)
 
 
+def run_positions_graph(recon: IterativeRecon, grid_size, x, y, z, w, out=None):
+    """Extension (no reference counterpart): run!(recon, grid_size, x,y,z,w)
+    followed by reconstructed_positions(recon, x,y,z; field=:sum), replayed
+    as a HIP graph when called again with the same device arrays, sizes and
+    parameters (the first call runs eagerly and captures).  Device arrays
+    only; periodic box (recon.box_size and recon.box_min set).  Returns the
+    reconstructed positions (the arrays of `out` when given)."""
+    if recon.kind != L.ITERATIVE:
+        raise ValueError("run_positions_graph reconstructs with IterativeRecon")
+    if not all(_is_device(a) for a in (x, y, z, w)):
+        raise ValueError("run_positions_graph takes device arrays")
+    ctx = recon.context()
+    n = int(x.shape[0])
+    outs = list(out) if out is not None else [_dev_empty((n,), x) for _ in range(3)]
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_run_positions_graph(ctx.handle, ctypes.byref(p), _grid(grid_size), _ptr(x), _ptr(y),
+                                                 _ptr(z), _ptr(w), n, *(_ptr(o) for o in outs)))
+    return tuple(outs)
+
+
 def autotune(recon: _Recon, grid_size, x, y, z, w):
     """Extension (no reference counterpart): times the library's scatter
     tile shapes and strided-pass kernels on this catalog and keeps the

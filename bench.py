@@ -222,7 +222,7 @@ def kernel_symbol(label, grid, rand):
             "gather": "k_gather", "mg_jacobi": "k_mg_jacobi", "mg_residual": "k_mg_residual<float>",
             "mg_reduce": "k_mg_reduce<float, float>", "mg_prolong": "k_mg_prolong<float, float>",
             "mg_mcgs": "k_mg_mcgs<float>", "mg_mcgs64": "k_mg_mcgs<double>", "mg_gs4": "k_mg_gs4",
-            "mg_refine": "k_mg_refine", "mg_residual_fp64": "k_mg_residual_mixed",
+            "mg_refine": "k_mg_refine", "mg_residual_fp64": "k_mg_residual_mixed", "mg_jacobi_tb": "k_mg_jacobi_tb",
             "mg_correct": "k_add_correction<double>",
             "mg_coarse": "k_mg_coarse<float>", "mg_coarse64": "k_mg_coarse<double>"}.get(label)
 
@@ -265,6 +265,40 @@ def roofline(prof, grid, workload, rand=True):
             "algorithmic_bytes_per_launch": round(per_launch), "profiler_labels": st["labels"]}
 
 
+def graph_bench(args, data, cfg, steps, ctx):
+    """Configuration 1 through baorec_run_positions_graph: run! and
+    reconstructed_positions(:sum) replayed as one HIP graph per step
+    (device-resident catalog; the first call captures).  Returns seconds."""
+    from baorec_hip import _lib as L
+    from baorec_hip.devmem import DeviceArray, hip
+    lib = ctx.lib
+    p = L.Params()
+    lib.baorec_params_default(ctypes.byref(p), cfg["bias"], cfg["f"], cfg["R"])
+    p.has_los = 1
+    for a in range(3):
+        p.los[a] = cfg["los"][a]
+        p.box_size[a], p.box_min[a] = 1000.0, 0.0
+    grid = (ctypes.c_int32 * 3)(*cfg["grid"])
+    d = [DeviceArray.from_numpy(a) for a in data]
+    n = len(data[0])
+    out = [DeviceArray((n,)) for _ in range(3)]
+    vp = lambda a: ctypes.c_void_p(a.ptr)  # noqa: E731
+
+    def step():
+        ctx.check(lib.baorec_run_positions_graph(ctx.handle, ctypes.byref(p), grid, *(vp(a) for a in d), n,
+                                                 *(vp(o) for o in out)))
+
+    h = hip()
+    for _ in range(max(args.warmup, 2)):
+        step()
+    h.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    h.hipDeviceSynchronize()
+    return time.perf_counter() - t0
+
+
 def cpu_baseline(args, data, rand, cfg):
     """FP32 CPU restatement (oracle) of the reference path, one whole
     reconstruction at the full size: setup (CIC + smoothing + normalisation),
@@ -304,6 +338,37 @@ def cpu_baseline(args, data, rand, cfg):
     return dict(value=1.0 / total, unit="recons/s", cores=workers, kind="port",
                 sample=(f"one whole reconstruction by the FP32 numpy/C oracle at the full {nx}^3 workload: "
                         f"setup {t_setup:.1f} s + {rec.n_iter} iterations {its} s + read-out {t_read:.1f} s"))
+
+
+def cpu_baseline_multigrid(args, data, rand, cfg):
+    """The reference's own multigrid reconstruction on the host: the oracle's
+    FP32 restatement of run!(::MultigridRecon) (setup_box, the randoms'
+    overdensity, fmg with its fixed schedule of 6 V-cycles per level and 5 + 5
+    damped Jacobi sweeps, src/recon.jl:184-261, src/multigrid.jl:722-752) and
+    reconstructed_positions(:sum), all timed at the full 512^3 workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import baorec_oracle as orc
+    workers = int(os.environ.get("BAOREC_ORACLE_WORKERS", os.cpu_count() or 1))
+    os.environ["BAOREC_ORACLE_WORKERS"] = str(workers)
+    x, y, z, w = (a.copy() for a in data)
+    rec = orc.MultigridRecon(bias=cfg["bias"], f=cfg["f"], smoothing_radius=cfg["R"], los=cfg["los"])
+    t0 = time.perf_counter()
+    if rand is not None:
+        orc.run_multigrid(rec, cfg["grid"], x, y, z, w, *rand, box_pad=cfg["pad"])
+    else:
+        rec.box_size, rec.box_min = [1000.0] * 3, [0.0] * 3
+        orc.run_multigrid(rec, cfg["grid"], x, y, z, w)
+    t_run = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    orc.reconstructed_positions(rec, x, y, z, kind="sum")
+    t_read = time.perf_counter() - t0
+    total = t_run + t_read
+    log(f"cpu oracle (multigrid): run! {t_run:.1f} s, read-out {t_read:.1f} s")
+    return dict(value=1.0 / total, unit="recons/s", cores=workers, kind="port",
+                sample=(f"one whole multigrid reconstruction by the FP32 numpy/C oracle at the full "
+                        f"{cfg['grid'][0]}^3 workload ({len(data[0])} tracers"
+                        f"{'' if rand is None else f', {len(rand[0])} randoms'}) with the reference's own FMG "
+                        f"schedule: run! {t_run:.1f} s + read-out {t_read:.1f} s"))
 
 
 def readout_bench(args):
@@ -826,10 +891,14 @@ def warm_bench(args):
         o = gpu_bench(args, d2, r2, c2, steps=nsteps, host_leg=False, ctx=ctx)
         others[wl] = {"recons_per_s": round(nsteps / o["dt"], 2), "ms_per_step": round(o["dt"] / nsteps * 1e3, 3),
                       "steps": nsteps}
+        if wl == "box128":   # the same step replayed as a HIP graph
+            dt = graph_bench(args, d2, c2, nsteps, ctx)
+            others["box128_graph"] = {"recons_per_s": round(nsteps / dt, 2), "ms_per_step": round(dt / nsteps * 1e3, 3),
+                                      "steps": nsteps, "entry": "baorec_run_positions_graph"}
     args.workload = "lc512_warm"
     prof = g["prof"]
     roof = roofline(prof, cfg["grid"], "mg_lc512", True)
-    cpu = None if args.no_cpu else cpu_baseline(args, data, rand, dict(cfg, recon="iterative"))
+    cpu = None if args.no_cpu else cpu_baseline_multigrid(args, data, rand, cfg)
     total_ms = sum(v["ms"] for v in prof.values())
     total_bytes = sum(v["bytes"] for v in prof.values())
     plain = comp["none"]
@@ -889,6 +958,8 @@ def main():
                     help="spectral Richardson steps of the learned and refined starts")
     ap.add_argument("--autotune", action="store_true",
                     help="run baorec_autotune on the workload's catalog before timing")
+    ap.add_argument("--graph", action="store_true",
+                    help="box128: time baorec_run_positions_graph (run! + read-out replayed as a HIP graph)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run 1 GiB device-to-device copies (byte-counter calibration)")
     args = ap.parse_args()
@@ -923,6 +994,18 @@ def main():
         warm_bench(args)
         return
     data, rand, cfg = make_workload(args)
+    if args.graph:
+        if args.workload != "box128":
+            raise SystemExit("--graph times the box128 workload")
+        import baorec_hip as bh
+        dt = graph_bench(args, data, cfg, args.steps, bh.Context(0))
+        print(json.dumps({"metric": "recons/s, box128 (run! + reconstructed_positions as a HIP graph)",
+                          "value": round(args.steps / dt, 2), "unit": "recons/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
+                          "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": "box128", "grid": list(cfg["grid"]), "n_data": len(data[0])}}),
+              flush=True)
+        return
     g = gpu_bench(args, data, rand, cfg, warm=args.warm_start)
     prof = g["prof"]
     roof = roofline(prof, cfg["grid"], args.workload, rand is not None)
@@ -930,7 +1013,10 @@ def main():
     total_bytes = sum(v["bytes"] for v in prof.values())
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args, data, rand, cfg)
+        if cfg.get("recon") == "multigrid":
+            cpu = cpu_baseline_multigrid(args, data, rand, cfg)
+        else:
+            cpu = cpu_baseline(args, data, rand, cfg)
     ms = g["dt"] / args.steps * 1e3
     nx, ny, nz = cfg["grid"]
     multigrid = cfg.get("recon") == "multigrid"

@@ -236,6 +236,20 @@ int baorec_reconstructed_positions(baorec_ctx *ctx, const baorec_params *p, int 
 
 /* ------------------------------------------------------------- primitives */
 
+/* run!(::IterativeRecon, grid, x, y, z, w) followed by
+ * reconstructed_positions(recon, x, y, z; field = :sum) (src/recon.jl:134-180,
+ * 366-388), replayed as a HIP graph (an extension; the reference's drivers
+ * call the two per catalog, examples/simulation.jl:22-56).  The first call with
+ * a given set of arguments (device pointers, n, grid, the input fields of p,
+ * the autotuned choices) runs eagerly and captures the launch sequence;
+ * later calls with the same arguments replay it, without the per-launch
+ * host cost.  A replay is discarded when a library buffer it uses has been
+ * reallocated since.  Periodic box only (no randoms: the box comes from p),
+ * device memory only; results and errors as the two calls. */
+int baorec_run_positions_graph(baorec_ctx *ctx, baorec_params *p, const int32_t grid[3],
+                               float *x, float *y, float *z, const float *w, int64_t n,
+                               float *ox, float *oy, float *oz);
+
 /* setup_box(x, y, z, pad), src/utils.jl:100-109. */
 int baorec_setup_box(baorec_ctx *ctx, int mem, const float *x, const float *y,
                      const float *z, int64_t n, float pad, float box_size[3],

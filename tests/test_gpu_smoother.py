@@ -83,3 +83,28 @@ def test_blocked_jacobi_bitwise_single_sweeps(ctx, shape, los, sweeps):
         bh.jacobi_(exp, f, rec)
     assert np.all(np.isfinite(got))
     np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("shape,los", [((128, 128, 128), None), ((64, 64, 512), [0.0, 0.0, 1.0]),
+                                       ((32, 256, 256), [0.6, 0.0, 0.8])])
+def test_zmarching_residual_and_prolongation_match_oracle(ctx, shape, los):
+    """On grids of >= 128^3 cells the FP32 residual runs as a z-marching
+    pass and the prolongation two cells per thread; both keep the reference's
+    FP32 operation order (residual!, prolong!: src/multigrid.jl:378-395,
+    398-518)."""
+    v, f = fields(shape, 3)
+    rec = recon(shape, los, ctx)
+    o = orc.MultigridRecon(bias=2.0, f=0.8, smoothing_radius=15.0, box_size=rec.box_size, box_min=rec.box_min,
+                           los=los)
+    r = np.zeros_like(v)
+    bh.residual_(r, v, f, rec)
+    exp = orc.mg_residual(v, f, o.box_size, o.box_min, o.beta, los=o.los)
+    print("residual max |diff| / max |r|:", np.max(np.abs(r - exp)) / np.abs(exp).max())
+    assert np.max(np.abs(r - exp)) < 1e-6 * np.abs(exp).max()
+    c = v[::2, ::2, ::2].copy()
+    fine = np.zeros(shape, F32)
+    bh.prolong_(fine, c, add=False, ctx=ctx)
+    np.testing.assert_array_equal(fine, orc.mg_prolong(c))
+    fine2 = v.copy()
+    bh.prolong_(fine2, c, add=True, ctx=ctx)
+    np.testing.assert_allclose(fine2, v + orc.mg_prolong(c), rtol=0, atol=1e-6)
