@@ -948,6 +948,32 @@ void mg_coarse(baorec_ctx *c, MGPyr &P, const baorec_params *p, int fmg, int gs)
     // number (pre + post), so P.cur is unchanged
 }
 
+// The FP32 residual of level l restricted to the right-hand side of level
+// l + 1 (residual! + reduce!, src/multigrid.jl:378-395, 641-651): on grids of
+// >= 128^3 cells one fused pass with the same bits (the residual is not
+// stored), else the two kernels through vother(P, l).  coarse = NULL: level
+// l + 1 is FP64 (P.Fd[l + 1]).
+static void mg_residual_reduce(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, float *coarse) {
+    const double R = (double)P.g[l].R();
+    if (coarse && R >= (double)(1 << 21) && mg_residual_restrict_ok(P.L[l])) {
+        ProfScope ps(c, "mg_residual_restrict", 8.5 * R);
+        hipchk(c, launch_mg_residual_restrict(coarse, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
+               "residual + restriction");
+        return;
+    }
+    float *r = vother(P, l);
+    {
+        ProfScope ps(c, "mg_residual", 12.0 * R);
+        hipchk(c, launch_mg_residual(r, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream), "residual");
+    }
+    const int fn[3] = {P.g[l].nx, P.g[l].ny, P.g[l].nz};
+    ProfScope ps(c, "mg_reduce", 4.5 * R);
+    if (coarse)
+        hipchk(c, launch_mg_reduce(coarse, r, fn, c->stream), "reduce");
+    else
+        hipchk(c, launch_mg_reduce_t<double, float>(P.Fd[l + 1], r, fn, c->stream), "reduce");
+}
+
 // vcycle! (src/multigrid.jl:689-719), reference schedule, FP32
 void mg_vcycle_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
     if (l == P.lc) {
@@ -956,17 +982,7 @@ void mg_vcycle_ref(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p) {
     }
     mg_jacobi_n(c, P, l, p, p->jacobi_niterations);
     if (l + 1 < (int)P.g.size()) {
-        float *r = vother(P, l);
-        {
-            ProfScope ps(c, "mg_residual", 12.0 * P.g[l].R());
-            hipchk(c, launch_mg_residual(r, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
-                   "residual");
-        }
-        const int fn[3] = {P.g[l].nx, P.g[l].ny, P.g[l].nz};
-        {
-            ProfScope ps(c, "mg_reduce", 4.5 * P.g[l].R());
-            hipchk(c, launch_mg_reduce(P.F[l + 1], r, fn, c->stream), "reduce");
-        }
+        mg_residual_reduce(c, P, l, p, P.F[l + 1]);
         zero_level(c, P, l + 1);
         mg_vcycle_ref(c, P, l + 1, p);
         const int cn[3] = {P.g[l + 1].nx, P.g[l + 1].ny, P.g[l + 1].nz};
@@ -1056,17 +1072,7 @@ void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, double
             ProfScope ps(c, "mg_reduce64", 1.125 * w * R);
             hipchk(c, launch_mg_reduce_t<double, double>(P.Fd[l + 1], r, fn, c->stream), "reduce64");
         } else {
-            float *r = vother(P, l);
-            {
-                ProfScope ps(c, "mg_residual", 12.0 * R);
-                hipchk(c, launch_mg_residual(r, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
-                       "residual");
-            }
-            ProfScope ps(c, "mg_reduce", 4.5 * R);
-            if (P.dbl[l + 1])
-                hipchk(c, launch_mg_reduce_t<double, float>(P.Fd[l + 1], r, fn, c->stream), "reduce");
-            else
-                hipchk(c, launch_mg_reduce(P.F[l + 1], r, fn, c->stream), "reduce");
+            mg_residual_reduce(c, P, l, p, P.dbl[l + 1] ? nullptr : P.F[l + 1]);
         }
         zero_level(c, P, l + 1);
         mg_vcycle_rb(c, P, l + 1, p);
@@ -1329,8 +1335,7 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     hipchk(c, hipMemcpyAsync(red, red + 7, sizeof(double), hipMemcpyDeviceToDevice, c->stream), "copy");
     hipchk(c, launch_sumsq<float>(f0, N, red + 1, red_part(c), c->stream), "norm f");
     float *rb[2] = {bufT<float>(c, "mg_r0", N), bufT<float>(c, "mg_r1", N)};
-    double *phi = bufT<double>(c, "mg_phi64", N);   // FP64 solution
-    hipchk(c, launch_zero_words(phi, 2 * N, c->stream), "zero");
+    double *phi = bufT<double>(c, "mg_phi64", N);   // FP64 solution (set by the first residual call)
     MGPyr P = mg_build(c, G, p, slot, v0, rb[0], p->fp64_coarse_max);
     // With sides that are multiples of 64, 16 and 8 the refinement step
     // (phi += e - mean(e), r = f - alpha - A phi, sum r) is one z-marching
@@ -1345,15 +1350,17 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     double *epart = P.lc != 0 && mg_use_gs4(P, 0) ? bufT<double>(c, "mg_gs4_part", mg_gs4_blocks(P.L[0])) : nullptr;
     // phi += e - mean(e) (e != NULL), r = residual, alpha += mean(r) and
     // r -= mean(r) (local LOS), norms[0] = |r|^2 and, with rprev, norms[1] =
-    // |r - rprev|^2
-    auto residual = [&](float *r, const float *e, const float *rprev, double *norms) {
+    // |r - rprev|^2.  The first call (start) sets phi to the FP32 seed, or to
+    // zero without one.
+    auto residual = [&](float *r, const float *e, const float *rprev, double *norms, bool start = false,
+                        const float *seed = nullptr) {
         if (fused) {
             {
-                ProfScope ps(c, "mg_refine", (e ? 28.0 : 16.0) * N);
-                hipchk(c, launch_mg_refine(phib, phi, e, red + 4, f0, red, r, rpart, P.L[0], p->beta, p->has_los,
-                                           c->stream), "refine");
+                ProfScope ps(c, "mg_refine", (e ? 28.0 : (start ? (seed ? 24.0 : 16.0) : 16.0)) * N);
+                hipchk(c, launch_mg_refine(phib, start ? nullptr : phi, seed, e, red + 4, f0, red, r, rpart, P.L[0],
+                                           p->beta, p->has_los, c->stream), "refine");
             }
-            if (e) std::swap(phi, phib);
+            if (e || start) std::swap(phi, phib);
             ProfScope ps(c, "mg_shift_norms", (p->has_los ? 4.0 : 8.0) * N + (rprev ? 4.0 : 0.0) * N);
             if (!p->has_los) {
                 hipchk(c, launch_sum_partials(rpart, mg_refine_blocks(P.L[0]), 1, red + 6, 1, c->stream), "sum r");
@@ -1362,6 +1369,12 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
             hipchk(c, launch_shift_norms(r, rprev, N, p->has_los ? nullptr : red + 6, norms, npart, c->stream),
                    "shift, norms");
             return;
+        }
+        if (start) {
+            if (seed)
+                hipchk(c, launch_convert<double, float>(phi, seed, N, c->stream), "seed to FP64");
+            else
+                hipchk(c, launch_zero_words(phi, 2 * N, c->stream), "zero");
         }
         if (e) {
             ProfScope ps(c, "mg_correct", 20.0 * N);
@@ -1381,12 +1394,13 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
         hipchk(c, launch_sumsq<float>(r, N, norms, red_part(c), c->stream), "norm r");
         if (rprev) hipchk(c, launch_sumsq_diff<float>(r, rprev, N, norms + 1, red_part(c), c->stream), "norm dr");
     };
+    float *seed = nullptr;
     if (p->mg_warm_start) {
         if (p->mg_warm_start < 1 || p->mg_warm_start > 3)
             fail(c, BAOREC_ERR_INVALID_ARGUMENT, "unknown mg_warm_start");
         if (!fft_side_ok(G.nx) || !fft_side_ok(G.ny) || !fft_side_ok(G.nz))
             fail(c, BAOREC_ERR_UNSUPPORTED, "the warm starts need grid sides that are powers of two in [16, 1024]");
-        float *seed = bufT<float>(c, "mg_seed", N);
+        seed = bufT<float>(c, "mg_seed", N);
         if (p->mg_warm_start == 1) {
             mg_spectral_seed(c, G, p, slot, P.L[0], f0, seed, false);
         } else {
@@ -1403,9 +1417,8 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
                 mg_spectral_seed(c, G, p, slot, P.L[0], rs, seed, true);   // seed += A_c^-1 rs
             }
         }
-        hipchk(c, launch_convert<double, float>(phi, seed, N, c->stream), "seed to FP64");
     }
-    residual(rb[0], nullptr, nullptr, red + 5);   // of the start (f minus its mean for v = 0)
+    residual(rb[0], nullptr, nullptr, red + 5, true, seed);   // of the start (f minus its mean for v = 0)
     double h[8];
     hipchk(c, hipMemcpyAsync(h, red, sizeof(double) * 8, hipMemcpyDeviceToHost, c->stream), "D2H");
     sync(c);
@@ -2573,6 +2586,37 @@ int baorec_mg_residual(baorec_ctx *c, const baorec_params *p, const int32_t grid
         MGLevel L = level_for(c, G, p, slot);
         hipchk(c, launch_mg_residual(dr, dv, df, L, p->beta, p->has_los, c->stream), "residual");
         copy_out(c, mem, r, dr, G.R());
+        sync(c);
+    });
+}
+
+int baorec_mg_residual_restrict(baorec_ctx *c, const baorec_params *p, const int32_t grid[3], int mem,
+                                float *coarse, const float *v, const float *f, int fused) {
+    return guarded(c, [&] {
+        check_params(c, p);
+        check_box(c, p);
+        Grid G = check_grid(c, grid, false);
+        if (G.nx % 2 || G.ny % 2 || G.nz % 2) fail(c, BAOREC_ERR_INVALID_ARGUMENT, "grid must be even");
+        Slot *slot = slot_buf(c) + 1;
+        slot_from_params(c, slot, p, 0.0);
+        const int64_t nc = G.R() / 8;
+        const float *dv = stage_in(c, mem, v, G.R(), "mgp_v");
+        const float *df = stage_in(c, mem, f, G.R(), "mgp_f");
+        float *dc = stage_out(c, mem, coarse, nc, "mgp_c");
+        MGLevel L = level_for(c, G, p, slot);
+        if (fused) {
+            if (!mg_residual_restrict_ok(L))
+                fail(c, BAOREC_ERR_UNSUPPORTED, "fused residual + restriction needs sides that are multiples of "
+                                                "64, 16 and 2 and fewer than 2^31 cells");
+            hipchk(c, launch_mg_residual_restrict(dc, dv, df, L, p->beta, p->has_los, c->stream),
+                   "residual + restriction");
+        } else {
+            float *dr = bufT<float>(c, "mgp_r", G.R());
+            hipchk(c, launch_mg_residual(dr, dv, df, L, p->beta, p->has_los, c->stream), "residual");
+            const int fn[3] = {G.nx, G.ny, G.nz};
+            hipchk(c, launch_mg_reduce(dc, dr, fn, c->stream), "reduce");
+        }
+        copy_out(c, mem, coarse, dc, nc);
         sync(c);
     });
 }

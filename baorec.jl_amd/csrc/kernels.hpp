@@ -312,6 +312,11 @@ hipError_t launch_mg_jacobi(float *vout, const float *vin, const float *f, const
 hipError_t launch_mg_residual(float *r, const float *v, const float *f, const MGLevel &L,
                               float beta, int has_los, hipStream_t s);
 hipError_t launch_mg_reduce(float *coarse, const float *fine, const int fine_n[3], hipStream_t s);
+// coarse = restriction of (f - A v) in one pass, the bits of launch_mg_residual
+// followed by launch_mg_reduce (sides % (64, 16, 2), < 2^31 cells)
+bool mg_residual_restrict_ok(const MGLevel &L);
+hipError_t launch_mg_residual_restrict(float *coarse, const float *v, const float *f, const MGLevel &L, float beta,
+                                       int has_los, hipStream_t s);
 hipError_t launch_mg_prolong(float *fine, const float *coarse, const int coarse_n[3], int add,
                              hipStream_t s);
 // xv[0 .. n) cell centres (x | y | z), xv[n .. 2n) centres / cell size, n = nx + ny + nz
@@ -364,15 +369,17 @@ hipError_t launch_mg_residual_mixed(float *r, const double *v, const float *f, c
 // phi += e - esum/n (sum in FP64, stored in phi's type)
 template <typename T>
 hipError_t launch_add_correction(T *phi, const float *e, int64_t n, const double *esum, hipStream_t s);
-// The FP64 refinement pass of the RBGS solve in one launch: phi_out = phi_in
-// + (e - sum(e)/N) (e may be NULL: phi_in as is, phi_out unused), r =
-// (float)((f - *fsum/N) - A phi_out), and the per-workgroup FP64 sums of r in
-// part[mg_refine_blocks(L)] (fixed order).  Sides multiples of 64, 16, 8.
+// The FP64 refinement pass of the RBGS solve in one launch: phi = phi_in +
+// (e - sum(e)/N); without e, phi = phi_in, or (double)seed when seed is
+// given, or 0 when phi_in is NULL; phi goes to phi_out unless it is phi_in
+// unchanged.  r = (float)((f - *fsum/N) - A phi), and the per-workgroup FP64
+// sums of r in part[mg_refine_blocks(L)] (fixed order).  Sides multiples of
+// 64, 16, 8.
 bool mg_refine_ok(const MGLevel &L);
 int64_t mg_refine_blocks(const MGLevel &L);
-hipError_t launch_mg_refine(double *phi_out, const double *phi_in, const float *e, const double *esum, const float *f,
-                            const double *fsum, float *r, double *part, const MGLevel &L, float beta, int has_los,
-                            hipStream_t s);
+hipError_t launch_mg_refine(double *phi_out, const double *phi_in, const float *seed, const float *e,
+                            const double *esum, const float *f, const double *fsum, float *r, double *part,
+                            const MGLevel &L, float beta, int has_los, hipStream_t s);
 // r += (float)(-*rsum / n) (rsum may be NULL), then out[0] = sum r^2 and,
 // with rprev, out[1] = sum (r - rprev)^2, FP64 in a fixed order; part holds
 // 2 x 1024 partials.

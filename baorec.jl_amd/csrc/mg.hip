@@ -645,7 +645,8 @@ __global__ void __launch_bounds__(JbShape<S>::Threads, 2) k_mg_jacobi_tb(float *
 }
 
 // One FP64 refinement pass of the RBGS solve (z-marching, LDS ring):
-//   phi_out = phi_in + (e - sum(e) / N)          (with e; without, phi_in is used)
+//   phi_out = phi_in + (e - sum(e) / N)          (with e; without, phi_in is used;
+//                                                 seed: phi = (double)seed; neither: phi = 0)
 //   r = (float)((f - alpha) - A phi_out),  alpha = *fsum / N,
 // the arithmetic of k_add_correction followed by k_mg_residual_mixed, plus
 // each workgroup's FP64 sum of r in a fixed order (part[block]).  A
@@ -659,9 +660,9 @@ constexpr int kRfRing = 4, kRfThreads = 256, kRfMaxTz = 128;
 constexpr int kRfLoads = (kRfPlane + kRfThreads - 1) / kRfThreads;   // 5
 
 __global__ void __launch_bounds__(kRfThreads)
-    k_mg_refine(double *__restrict__ phi_out, const double *__restrict__ phi_in, const float *__restrict__ e,
-                const double *esum, const float *__restrict__ f, const double *fsum, float *__restrict__ r,
-                double *__restrict__ part, MGLevel L, float beta, int has_los, int tz) {
+    k_mg_refine(double *__restrict__ phi_out, const double *__restrict__ phi_in, const float *__restrict__ seed,
+                const float *__restrict__ e, const double *esum, const float *__restrict__ f, const double *fsum,
+                float *__restrict__ r, double *__restrict__ part, MGLevel L, float beta, int has_los, int tz) {
     __shared__ double ring[kRfRing * kRfPlane];
     __shared__ double ptab[kRfW + kRfRows + kRfMaxTz + 2];   // p of LDS column | row | local plane
     __shared__ double red[kRfThreads / 64];
@@ -706,10 +707,11 @@ __global__ void __launch_bounds__(kRfThreads)
         const int64_t zo = sz * gs_wrap(min(P, z0 + tz), nz);
 #pragma unroll
         for (int u = 0; u < kRfLoads; u++) {
-            pre[u] = phi_in[zo + ld_off[u]];
+            pre[u] = seed ? (double)seed[zo + ld_off[u]] : (phi_in ? phi_in[zo + ld_off[u]] : 0.0);
             if (e) pre_e[u] = e[zo + ld_off[u]];
         }
     };
+    const bool write_phi = e || seed || !phi_in;   // phi_out receives the pass's phi
     auto store = [&](int P) {
         double *pl = ring + (P - z0 + 1) % kRfRing * kRfPlane;
         const int64_t zo = sz * gs_wrap(P, nz);
@@ -720,7 +722,7 @@ __global__ void __launch_bounds__(kRfThreads)
             if (q >= kRfPlane) continue;
             const double v = e ? (double)pre[u] + ((double)pre_e[u] - me) : pre[u];
             pl[q] = v;
-            if (e && own && ld_in[u]) phi_out[zo + ld_off[u]] = v;
+            if (write_phi && own && ld_in[u]) phi_out[zo + ld_off[u]] = v;
         }
     };
     // f of the tile's plane Q: float4 chunk tid (row tid / 16, x 4 (tid % 16)),
@@ -994,6 +996,150 @@ __global__ void k_mg_prolong(T *__restrict__ fine, const U *__restrict__ v, int 
     cell_prolong<T, U>(fine, v, cx, cy, cz, add, ix, blockIdx.y, blockIdx.z);
 }
 
+// The V-cycle's residual and restriction in one z-marching pass:
+// coarse = reduce(f - A v), the arithmetic of k_mg_residual<float> followed by
+// k_mg_reduce<float, float> (so the same bits) without writing r.  A
+// workgroup owns a 32 x 8 x tzc block of coarse cells; it streams planes of v
+// (67 x 19 with the halo) through a ring of 4, computes r on the 65 x 17 fine
+// cells the block's 27-point sums read (x0 .. x0 + 64: the + 1 side wraps like
+// reduce's ixp) into a ring of 4 r planes, and a coarse plane every second
+// step, once its three fine planes are in.
+constexpr int kRrCX = 32, kRrCY = 8, kRrThreads = 256, kRrMaxTzc = 16;
+constexpr int kRrVW = 2 * kRrCX + 3, kRrVH = 2 * kRrCY + 3, kRrVPlane = kRrVW * kRrVH;   // 67 x 19
+constexpr int kRrW = 2 * kRrCX + 1, kRrH = 2 * kRrCY + 1, kRrPlane = kRrW * kRrH;          // 65 x 17
+constexpr int kRrIters = (kRrPlane + kRrThreads - 1) / kRrThreads;                          // 5
+constexpr int kRrLoads = (kRrVPlane + kRrThreads - 1) / kRrThreads;                          // 5
+
+__global__ void __launch_bounds__(kRrThreads)
+    k_mg_residual_restrict(float *__restrict__ coarse, const float *__restrict__ v, const float *__restrict__ f,
+                           MGLevel L, float beta, int has_los, int tzc) {
+    __shared__ float vring[4 * kRrVPlane];
+    __shared__ float rring[4 * kRrPlane];
+    __shared__ float ptab[kRrVW + kRrVH + 2 * kRrMaxTzc + 3];
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    int tile_x, tile_y, tile_z;
+    mg_tile(tile_x, tile_y, tile_z);
+    const int x0 = 2 * kRrCX * tile_x, y0 = 2 * kRrCY * tile_y, z0 = 2 * tzc * tile_z;   // fine origin
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int cx = nx / 2, cy = ny / 2;
+    const int tid = threadIdx.x;
+    float *pz_tab = ptab + kRrVW + kRrVH;   // local plane q = z - (z0 - 1)
+    if (!has_los)
+        for (int q = tid; q < kRrVW + kRrVH + 2 * tzc + 3; q += kRrThreads) {
+            int g;
+            if (q < kRrVW)
+                g = gs_wrap(x0 - 1 + q, nx);
+            else if (q < kRrVW + kRrVH)
+                g = nx + gs_wrap(y0 - 1 + q - kRrVW, ny);
+            else
+                g = nx + ny + gs_wrap(z0 - 1 + q - kRrVW - kRrVH, nz);
+            ptab[q] = L.pv[g];
+        }
+    // v plane loads: element q = row * 67 + col (threads past the plane reload 0)
+    int64_t ld_off[kRrLoads];
+#pragma unroll
+    for (int u = 0; u < kRrLoads; u++) {
+        const int q = tid + kRrThreads * u, qq = q < kRrVPlane ? q : 0;
+        ld_off[u] = gs_wrap(x0 - 1 + qq % kRrVW, nx) + sy * gs_wrap(y0 - 1 + qq / kRrVW, ny);
+    }
+    float pre[kRrLoads];
+    const int last_v = z0 + 2 * tzc + 1;
+    auto fetch = [&](int P) {
+        const int64_t zo = sz * gs_wrap(min(P, last_v), nz);
+#pragma unroll
+        for (int u = 0; u < kRrLoads; u++) pre[u] = v[zo + ld_off[u]];
+    };
+    auto store = [&](int P) {
+        float *pl = vring + ((P - z0 + 1) & 3) * kRrVPlane;
+#pragma unroll
+        for (int u = 0; u < kRrLoads; u++) {
+            const int q = tid + kRrThreads * u;
+            if (q < kRrVPlane) pl[q] = pre[u];
+        }
+    };
+    // r cells: iteration it handles cell q = tid + 256 it of the 65 x 17 plane
+    int rpos[kRrIters];
+    uint32_t foff[kRrIters];
+#pragma unroll
+    for (int it = 0; it < kRrIters; it++) {
+        const int q = tid + kRrThreads * it;
+        const int xr = q < kRrPlane ? q % kRrW : 0, yr = q < kRrPlane ? q / kRrW : 0;
+        rpos[it] = q < kRrPlane ? (xr | (yr << 8)) : -1;
+        foff[it] = (uint32_t)gs_wrap(x0 + xr, nx) + (uint32_t)nx * gs_wrap(y0 + yr, ny);
+    }
+    const uint32_t sz32 = (uint32_t)sz;
+    float fpre[kRrIters];
+    auto fetch_f = [&](int Q) {   // f of the r cells on fine plane Q
+        const uint32_t zo = sz32 * (uint32_t)gs_wrap(min(Q, z0 + 2 * tzc), nz);
+#pragma unroll
+        for (int it = 0; it < kRrIters; it++) fpre[it] = f[foff[it] + zo];
+    };
+    // coarse cell of this thread: (cx0 + tid % 32, cy0 + tid / 32)
+    const int kxl = tid % kRrCX, kyl = tid / kRrCX;
+    const int64_t c_off = (x0 / 2 + kxl) + (int64_t)cx * (y0 / 2 + kyl);
+    fetch(z0 - 1);
+    for (int k = 0; k < 2 * tzc + 4; k++) {
+        const int P = z0 - 1 + k;   // v plane loaded this step
+        if (P <= last_v) store(P);
+        float fcur[kRrIters];
+#pragma unroll
+        for (int it = 0; it < kRrIters; it++) fcur[it] = fpre[it];
+        fetch(P + 1);
+        fetch_f(P);   // f of the r plane of the next step
+        __syncthreads();
+        const int Q = P - 1;   // r plane computed this step
+        if (Q >= z0 && Q <= z0 + 2 * tzc) {
+            const int vq = Q - z0 + 1;   // local v plane of Q
+            const float *pm = vring + ((vq - 1) & 3) * kRrVPlane, *p0 = vring + (vq & 3) * kRrVPlane,
+                        *pp = vring + ((vq + 1) & 3) * kRrVPlane;
+            float *rp = rring + ((Q - z0) & 3) * kRrPlane;
+#pragma unroll
+            for (int it = 0; it < kRrIters; it++) {
+                if (rpos[it] < 0) continue;
+                const int xr = rpos[it] & 255, yr = rpos[it] >> 8;
+                const int xl = xr + 1, yl = yr + 1, wc = yl * kRrVW + xl;
+                const Coef<float> c = coef_p<float>(L, has_los ? L.losn[0] : ptab[xl],
+                                                    has_los ? L.losn[1] : ptab[kRrVW + yl],
+                                                    has_los ? L.losn[2] : pz_tab[vq], beta);
+#define A0(o) p0[wc + (o)]
+#define AM(o) pm[wc + (o)]
+#define AP(o) pp[wc + (o)]
+                const Nbr<float> sn = nbr_of<float>(
+                    c, A0(1), A0(-1), A0(kRrVW), A0(-kRrVW), AP(0), AM(0), A0(kRrVW + 1), A0(-kRrVW - 1),
+                    A0(kRrVW - 1), A0(-kRrVW + 1), AP(1), AM(-1), AP(-1), AM(1), AP(kRrVW), AM(-kRrVW),
+                    AP(-kRrVW), AM(kRrVW));
+                float t = 2.0f * (c.gpx2 + c.gpy2 + c.gpz2) * A0(0) - (sn.A + sn.B + sn.C + sn.X);
+#undef A0
+#undef AM
+#undef AP
+                if (!has_los) t -= sn.G;
+                rp[yr * kRrW + xr] = fcur[it] - t;
+            }
+        }
+        // coarse plane j once fine r planes 2j .. 2j + 2 (local) are in: Q = z0 + 2j + 3
+        const int jq = Q - z0 - 3;
+        if (jq >= 0 && (jq & 1) == 0 && (jq >> 1) < tzc) {
+            const int j = jq >> 1;
+            const float *rm = rring + ((2 * j) & 3) * kRrPlane, *r0 = rring + ((2 * j + 1) & 3) * kRrPlane,
+                        *rq = rring + ((2 * j + 2) & 3) * kRrPlane;
+            const int ix = 2 * kxl + 1, iy = 2 * kyl + 1;   // local fine indices of the centre
+#define W0(a, b) r0[(iy + (b)) * kRrW + ix + (a)]
+#define WM(a, b) rm[(iy + (b)) * kRrW + ix + (a)]
+#define WP(a, b) rq[(iy + (b)) * kRrW + ix + (a)]
+            const float sum = 8.0f * W0(0, 0) +
+                              4.0f * (W0(1, 0) + W0(-1, 0) + W0(0, 1) + W0(0, -1) + WP(0, 0) + WM(0, 0)) +
+                              2.0f * (W0(1, 1) + W0(-1, 1) + W0(1, -1) + W0(-1, -1) + WP(1, 0) + WP(-1, 0) +
+                                      WM(1, 0) + WM(-1, 0) + WP(0, 1) + WP(0, -1) + WM(0, 1) + WM(0, -1)) +
+                              WP(1, 1) + WP(-1, 1) + WP(1, -1) + WP(-1, -1) + WM(1, 1) + WM(-1, 1) + WM(1, -1) +
+                              WM(-1, -1);
+#undef W0
+#undef WM
+#undef WP
+            coarse[c_off + (int64_t)cx * cy * (z0 / 2 + j)] = (float)((double)sum / 64.0);
+        }
+    }
+}
+
 // Trilinear prolongation (+add) with two fine x cells per thread (an odd and
 // an even index: no divergence between the cases of cell_prolong, whose
 // arithmetic it keeps) and a 4-plane z loop per workgroup.
@@ -1246,6 +1392,29 @@ hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, 
     return hipGetLastError();
 }
 
+// coarse planes per workgroup of k_mg_residual_restrict: up to 16, fewer so
+// that a launch has at least 1024 workgroups
+static int rr_tzc(const MGLevel &L) {
+    const int64_t cols = (int64_t)(L.n[0] / (2 * kRrCX)) * (L.n[1] / (2 * kRrCY));
+    int t = kRrMaxTzc;
+    while (t > 1 && ((L.n[2] / 2) % t || cols * (L.n[2] / 2 / t) < 1024)) t /= 2;
+    return t;
+}
+
+bool mg_residual_restrict_ok(const MGLevel &L) {
+    return L.n[0] % (2 * kRrCX) == 0 && L.n[1] % (2 * kRrCY) == 0 && L.n[2] % 2 == 0 &&
+           (int64_t)L.n[0] * L.n[1] * L.n[2] < ((int64_t)1 << 31);
+}
+
+hipError_t launch_mg_residual_restrict(float *coarse, const float *v, const float *f, const MGLevel &L, float beta,
+                                       int has_los, hipStream_t s) {
+    if (!mg_residual_restrict_ok(L)) return hipErrorInvalidValue;
+    const int tzc = rr_tzc(L);
+    const dim3 grid((unsigned)(L.n[0] / (2 * kRrCX)), (unsigned)(L.n[1] / (2 * kRrCY)), (unsigned)(L.n[2] / 2 / tzc));
+    hipLaunchKernelGGL(k_mg_residual_restrict, grid, dim3(kRrThreads), 0, s, coarse, v, f, L, beta, has_los, tzc);
+    return hipGetLastError();
+}
+
 // z planes per workgroup of k_mg_gs4: 128, or fewer (a divisor of nz, at
 // least 8) so that a launch has at least 512 workgroups, two per CU; each
 // workgroup also loads 8 halo planes and runs 12 pipeline steps beyond tz.
@@ -1305,14 +1474,14 @@ int64_t mg_refine_blocks(const MGLevel &L) {
     return (int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY) * (L.n[2] / refine_tz(L));
 }
 
-hipError_t launch_mg_refine(double *phi_out, const double *phi_in, const float *e, const double *esum, const float *f,
-                            const double *fsum, float *r, double *part, const MGLevel &L, float beta, int has_los,
-                            hipStream_t s) {
-    if (!mg_refine_ok(L) || (e && phi_out == phi_in)) return hipErrorInvalidValue;
+hipError_t launch_mg_refine(double *phi_out, const double *phi_in, const float *seed, const float *e,
+                            const double *esum, const float *f, const double *fsum, float *r, double *part,
+                            const MGLevel &L, float beta, int has_los, hipStream_t s) {
+    if (!mg_refine_ok(L) || ((e || seed || !phi_in) && phi_out == phi_in) || (seed && e)) return hipErrorInvalidValue;
     const int tz = refine_tz(L);
     const dim3 grid((unsigned)(L.n[0] / kRfTX), (unsigned)(L.n[1] / kRfTY), (unsigned)(L.n[2] / tz));
-    hipLaunchKernelGGL(k_mg_refine, grid, dim3(kRfThreads), 0, s, phi_out, phi_in, e, esum, f, fsum, r, part, L,
-                       beta, has_los, tz);
+    hipLaunchKernelGGL(k_mg_refine, grid, dim3(kRfThreads), 0, s, phi_out, phi_in, seed, e, esum, f, fsum, r, part,
+                       L, beta, has_los, tz);
     return hipGetLastError();
 }
 

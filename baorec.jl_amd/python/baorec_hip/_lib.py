@@ -131,6 +131,7 @@ SIGNATURES = {
     "baorec_run_positions_graph": (INT, [P, PP, I32P, FP, FP, FP, FP, I64, FP, FP, FP]),
     "baorec_mg_residual": (INT, [P, PP, I32P, INT, FP, FP, FP]),
     "baorec_mg_reduce": (INT, [P, I32P, INT, FP, FP]),
+    "baorec_mg_residual_restrict": (INT, [P, PP, I32P, INT, FP, FP, FP, INT]),
     "baorec_mg_prolong": (INT, [P, I32P, INT, FP, FP, INT]),
     "baorec_mg_vcycle": (INT, [P, PP, I32P, INT, FP, FP]),
     "baorec_mg_fmg": (INT, [P, PP, I32P, INT, FP, FP]),

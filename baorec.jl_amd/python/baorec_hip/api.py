@@ -612,6 +612,19 @@ def reduce_(coarse, fine, ctx: Optional[Context] = None):
     return coarse
 
 
+def residual_restrict_(coarse, v, f, recon: MultigridRecon, fused=True):
+    """reduce!(residual!(v, f)), the V-cycle's step down (src/multigrid.jl:
+    700-704), into coarse (half the sides of v).  fused=True runs one pass
+    that does not store the residual (sides multiples of 64, 16 and 2), bit
+    for bit the two-kernel path of fused=False."""
+    ctx = recon.context()
+    p = recon.params()
+    ctx.check(ctx.lib.baorec_mg_residual_restrict(ctx.handle, ctypes.byref(p), _grid(v.shape[::-1]),
+                                                  _mem(coarse, v, f), _ptr(coarse), _ptr(_f32(v)), _ptr(_f32(f)),
+                                                  int(bool(fused))))
+    return coarse
+
+
 def prolong_(fine, coarse, add=False, ctx: Optional[Context] = None):
     """prolong!, src/multigrid.jl:511-518 (add=True adds into fine)."""
     ctx = ctx or default_context()

@@ -315,6 +315,14 @@ int baorec_mg_residual(baorec_ctx *ctx, const baorec_params *p,
                        const float *f);
 int baorec_mg_reduce(baorec_ctx *ctx, const int32_t fine_grid[3], int mem,
                      float *coarse, const float *fine);
+/* The V-cycle's step down (src/multigrid.jl:700-704): coarse = reduce!(
+ * residual!(v, f)) on a grid of even sides, coarse of grid/2.  fused = 0: the
+ * two kernels of baorec_mg_residual and baorec_mg_reduce through a scratch
+ * grid; fused = 1: one pass that does not store the residual (sides multiples
+ * of 64, 16 and 2 and fewer than 2^31 cells, else BAOREC_ERR_UNSUPPORTED),
+ * bit for bit the same result. */
+int baorec_mg_residual_restrict(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],
+                                int mem, float *coarse, const float *v, const float *f, int fused);
 int baorec_mg_prolong(baorec_ctx *ctx, const int32_t coarse_grid[3], int mem,
                       float *fine, const float *coarse, int add);
 int baorec_mg_vcycle(baorec_ctx *ctx, const baorec_params *p, const int32_t grid[3],

@@ -1,4 +1,4 @@
-"""The multigrid smoothers on the GPU: the one-launch 4-colour sweep
+"""The multigrid smoothers and fused level passes on the GPU: the one-launch 4-colour sweep
 (k_mg_gs4) against the per-colour launches (bit for bit) and against the
 oracle's multicolour Gauss-Seidel (FP64, within FP32 rounding); the
 temporally blocked damped Jacobi against single sweeps (bit for bit)."""
@@ -108,3 +108,39 @@ def test_zmarching_residual_and_prolongation_match_oracle(ctx, shape, los):
     fine2 = v.copy()
     bh.prolong_(fine2, c, add=True, ctx=ctx)
     np.testing.assert_allclose(fine2, v + orc.mg_prolong(c), rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("shape,los", [
+    ((2, 16, 64), None),
+    ((24, 32, 64), [0.0, 0.0, 1.0]),
+    ((40, 48, 192), None),
+    ((128, 128, 128), None),
+    ((64, 64, 512), [0.6, 0.0, 0.8]),
+    ((512, 512, 512), None),
+])
+def test_fused_residual_restriction_bitwise(ctx, shape, los):
+    """The V-cycle's step down, reduce!(residual!(v, f)), as one pass: the
+    same bits as the two kernels, on one-plane-pair grids, halos wrapping
+    onto the tile itself, non-cubic sides and the solver's 128^3 and 512^3
+    levels."""
+    v, f = fields(shape, 11 + shape[0])
+    rec = recon(shape, los, ctx)
+    half = tuple(s // 2 for s in shape)
+    a = bh.residual_restrict_(np.full(half, np.nan, F32), v, f, rec, fused=True)
+    b = bh.residual_restrict_(np.full(half, np.nan, F32), v, f, rec, fused=False)
+    assert np.all(np.isfinite(a))
+    np.testing.assert_array_equal(a, b)
+    if shape[0] * shape[1] * shape[2] <= 64 * 64 * 512:
+        o = orc.MultigridRecon(bias=2.0, f=0.8, smoothing_radius=15.0, box_size=rec.box_size,
+                               box_min=rec.box_min, los=los)
+        exp = orc.mg_reduce(orc.mg_residual(v, f, o.box_size, o.box_min, o.beta, los=o.los))
+        assert np.max(np.abs(a - exp)) < 1e-6 * np.abs(exp).max()
+
+
+def test_fused_residual_restriction_rejects_unsupported_shape(ctx):
+    shape = (8, 16, 96)   # x side not a multiple of 64
+    v, f = fields(shape, 2)
+    c = np.zeros((4, 8, 48), F32)
+    with pytest.raises(bh.BaorecError):
+        bh.residual_restrict_(c, v, f, recon(shape, None, ctx), fused=True)
+    bh.residual_restrict_(c, v, f, recon(shape, None, ctx), fused=False)
