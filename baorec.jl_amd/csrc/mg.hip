@@ -232,21 +232,29 @@ __global__ void k_mg_mcgs(T *__restrict__ v, const T *__restrict__ f, MGLevel L,
 // colour cells of a row, every second x, and their x neighbours are
 // consecutive words.
 constexpr int kGsTX = 64, kGsTY = 16, kGsHalo = 4;
-constexpr int kGsW = kGsTX + 2 * kGsHalo;      // 72 words per LDS row
+constexpr int kGsLW = kGsTX + 2 * kGsHalo;     // 72 columns per plane row
+// LDS rows: the even-x half (36 words + 4 spare) at 0, the odd-x half at
+// kGsH = 40, a pitch of 82 words.  A half-wave of the cell list is one 32-cell
+// row (32 consecutive words), and a colour's consecutive rows, 164 words
+// apart, fall 4 banks further on, so the narrow columns beyond 32 spread over
+// the banks too.
+constexpr int kGsW = 82, kGsH = 40;
 constexpr int kGsRows = kGsTY + 2 * kGsHalo;   // 24 rows per plane
 constexpr int kGsPlane = kGsW * kGsRows;
 constexpr int kGsRing = 10;                    // planes P-9 .. P
 constexpr int kGsThreads = 512;   // 16 waves per CU (two workgroups)
 constexpr int kGsMaxTz = 128;
-constexpr int kGsChunks = (kGsW / 4) * kGsRows;   // float4 loads per plane (432)
-// colour cells per stage: (8 + e) rows x (32 + e) columns, e = 3 - s
-constexpr int kGsStage0 = 11 * 35, kGsStage1 = kGsStage0 + 10 * 34, kGsStage2 = kGsStage1 + 9 * 33,
-              kGsCells = kGsStage2 + 8 * 32;   // 1278
+constexpr int kGsChunks = (kGsLW / 4) * kGsRows;   // float4 loads per plane (432)
+// colour cells per stage: (8 + e) rows x (32 + e) columns, e = 3 - s.  The
+// cell list takes columns 0..31 of stages 0..3 row by row (a half-wave per
+// row), then the e extra columns of stages 0..2.
+constexpr int kGsA1 = 11 * 32, kGsA2 = kGsA1 + 10 * 32, kGsA3 = kGsA2 + 9 * 32, kGsB0 = kGsA3 + 8 * 32,
+              kGsB1 = kGsB0 + 11 * 3, kGsB2 = kGsB1 + 10 * 2, kGsCells = kGsB2 + 9;   // 1278
 constexpr int kGsIters = (kGsCells + kGsThreads - 1) / kGsThreads;
 constexpr int kGsLoads = (kGsChunks + kGsThreads - 1) / kGsThreads;   // float4 plane loads per thread
 constexpr int kGsCPT = kGsTX * kGsTY / kGsThreads;                     // cells written per thread (2 or 4)
 
-__device__ __forceinline__ int gs_word(int xl, int yl) { return yl * kGsW + (xl & 1) * (kGsW / 2) + (xl >> 1); }
+__device__ __forceinline__ int gs_word(int xl, int yl) { return yl * kGsW + (xl & 1) * kGsH + (xl >> 1); }
 __device__ __forceinline__ int gs_wrap(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 // The tile of this workgroup, XCD-aware: workgroup i (x fastest) runs on XCD
@@ -284,7 +292,7 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
     __shared__ __align__(16) float lds[kGsRing * kGsPlane];
     __shared__ double red[kGsThreads / 64];
     double acc = 0.0;
-    __shared__ float ptab[kGsW + kGsRows + kGsMaxTz + 8];   // p of LDS column | row | local plane
+    __shared__ float ptab[kGsW + kGsRows + kGsMaxTz + 8];   // p of LDS column word | row | local plane
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     int tile_x, tile_y, tile_z;
     const int tile = mg_tile(tile_x, tile_y, tile_z);
@@ -295,8 +303,8 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
     if (!has_los)
         for (int q = tid; q < kGsW + kGsRows + tz + 8; q += kGsThreads) {
             int g;
-            if (q < kGsW)
-                g = gs_wrap(x0 - kGsHalo + q, nx);
+            if (q < kGsW)   // column word q: x = 2q (even half) or 2(q - kGsH) + 1
+                g = gs_wrap(x0 - kGsHalo + min(q < kGsH ? 2 * q : 2 * (q - kGsH) + 1, kGsLW - 1), nx);
             else if (q < kGsW + kGsRows)
                 g = nx + gs_wrap(y0 - kGsHalo + q - kGsW, ny);
             else
@@ -314,7 +322,7 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
         const int q = tid + kGsThreads * u;
         ld_ok[u] = q < kGsChunks;
         const int qq = ld_ok[u] ? q : 0;
-        const int row = qq / (kGsW / 4), j = qq % (kGsW / 4);
+        const int row = qq / (kGsLW / 4), j = qq % (kGsLW / 4);
         ld_off[u] = gs_wrap(x0 - kGsHalo + 4 * j, nx) + sy * gs_wrap(y0 - kGsHalo + row, ny);
         ld_word[u] = row * kGsW + 2 * j;
     }
@@ -334,35 +342,36 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
             if (ld_ok[u]) {
                 pl[ld_word[u]] = pre[u].x;
                 pl[ld_word[u] + 1] = pre[u].z;
-                pl[ld_word[u] + kGsW / 2] = pre[u].y;
-                pl[ld_word[u] + kGsW / 2 + 1] = pre[u].w;
+                pl[ld_word[u] + kGsH] = pre[u].y;
+                pl[ld_word[u] + kGsH + 1] = pre[u].w;
             }
     };
-    // the thread's cells: iteration it handles cell jc = tid + 256 it of the
-    // stage list; desc = s | r << 2 | i << 8, or -1
+    // the thread's cells: iteration it handles cell jc = tid + 512 it of the
+    // cell list; desc = s | r << 2 | i << 8 (stage, its row and column), or -1
     int desc[kGsIters];
 #pragma unroll
     for (int it = 0; it < kGsIters; it++) {
         const int jc = tid + kGsThreads * it;
-        int s, jj, w, magic;   // stage, index in the stage, columns, ceil(2^16 / w)
-        if (jc < kGsStage0) {
-            s = 0; jj = jc; w = 35; magic = 1873;
-        } else if (jc < kGsStage1) {
-            s = 1; jj = jc - kGsStage0; w = 34; magic = 1928;
-        } else if (jc < kGsStage2) {
-            s = 2; jj = jc - kGsStage1; w = 33; magic = 1986;
-        } else {
-            s = 3; jj = jc - kGsStage2; w = 32; magic = 2048;
+        int s, r, i;
+        if (jc < kGsB0) {   // columns 0..31
+            s = jc < kGsA1 ? 0 : (jc < kGsA2 ? 1 : (jc < kGsA3 ? 2 : 3));
+            const int jj = jc - (s == 0 ? 0 : (s == 1 ? kGsA1 : (s == 2 ? kGsA2 : kGsA3)));
+            r = jj >> 5;
+            i = jj & 31;
+        } else {            // columns 32 .. 34 - s of stages 0..2
+            s = jc < kGsB1 ? 0 : (jc < kGsB2 ? 1 : 2);
+            const int jj = jc - (s == 0 ? kGsB0 : (s == 1 ? kGsB1 : kGsB2)), e = 3 - s;
+            r = jj / e;
+            i = 32 + jj - r * e;
         }
-        const int r = (jj * magic) >> 16, i = jj - r * w;   // exact for jj < 400
         desc[it] = jc < kGsCells ? (s | (r << 2) | (i << 8)) : -1;
     }
     // stage s updates plane Q = z0 - 5 - 2s + k, needed on z0 - 3 + s .. z0 + tz + 2 - s
     auto active = [&](int s, int k) { return k >= 2 + 3 * s && k <= tz + 7 + s; };
     // Per iteration and parity par of the stage planes (z0 is even, so par =
-    // (k + 1) & 1): the cell's LDS column and row (xl | yl << 8) and its
-    // offset in a plane of f (32-bit: the grid has < 2^31 cells).
-    int cpos[2][kGsIters];
+    // (k + 1) & 1): the cell's LDS word, its column word and row (cw | yl << 8)
+    // and its offset in a plane of f (32-bit: the grid has < 2^31 cells).
+    int cword[2][kGsIters], cpos[2][kGsIters];
     uint32_t foff[2][kGsIters];
     const uint32_t sz32 = (uint32_t)sz;
 #pragma unroll
@@ -372,7 +381,8 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
             const int d = desc[it];
             int xl = kGsHalo, yl = kGsHalo;
             if (d >= 0) gs_cell(d & 3, (d >> 2) & 63, d >> 8, par, xl, yl);
-            cpos[par][it] = xl | (yl << 8);
+            cword[par][it] = gs_word(xl, yl);
+            cpos[par][it] = gs_word(xl, 0) | (yl << 8);
             foff[par][it] = (uint32_t)gs_wrap(x0 - kGsHalo + xl, nx) + (uint32_t)nx * gs_wrap(y0 - kGsHalo + yl, ny);
         }
     // f of the thread's cells at step k (a valid dummy address where inactive)
@@ -404,11 +414,11 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
             const float *pl = lds + ((k - 8) % kGsRing) * kGsPlane;
             if constexpr (kGsCPT == 4) {
                 const float2 ev = *reinterpret_cast<const float2 *>(pl + wo_word);
-                const float2 od = *reinterpret_cast<const float2 *>(pl + wo_word + kGsW / 2);
+                const float2 od = *reinterpret_cast<const float2 *>(pl + wo_word + kGsH);
                 *reinterpret_cast<float4 *>(vout + sz * (z0 - 12 + k) + wo_off) = make_float4(ev.x, od.x, ev.y, od.y);
                 if (part) acc += ((double)ev.x + (double)od.x) + ((double)ev.y + (double)od.y);
             } else {
-                const float ev = pl[wo_word], od = pl[wo_word + kGsW / 2];
+                const float ev = pl[wo_word], od = pl[wo_word + kGsH];
                 *reinterpret_cast<float2 *>(vout + sz * (z0 - 12 + k) + wo_off) = make_float2(ev, od);
                 if (part) acc += (double)ev + (double)od;
             }
@@ -418,33 +428,37 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
 #pragma unroll
         for (int it = 0; it < kGsIters; it++) {
             const int d = desc[it];
-            const int s = d & 3;
-            if (d < 0 || !active(s, k)) continue;
-            const int xl = cpos[par][it] & 255, yl = cpos[par][it] >> 8;
-            const int ql = k - 1 - 2 * s;   // local plane of Q, >= 1
-            int m = k10 + kGsRing - 1 - 2 * s;   // ring slot of Q
+            const int st = d & 3;
+            if (d < 0 || !active(st, k)) continue;
+            const int cw = cpos[par][it] & 255, yl = cpos[par][it] >> 8;   // column word, row
+            const int ql = k - 1 - 2 * st;   // local plane of Q, >= 1
+            int m = k10 + kGsRing - 1 - 2 * st;   // ring slot of Q
             m = m >= kGsRing ? m - kGsRing : m;
             const int mm = m == 0 ? kGsRing - 1 : m - 1, mp = m == kGsRing - 1 ? 0 : m + 1;
             const float *pm = lds + mm * kGsPlane;
             float *p0 = lds + m * kGsPlane;
             const float *pp = lds + mp * kGsPlane;
-            const int wc = gs_word(xl, yl);
-            // x neighbours sit in the other parity half: x+1 at the same or next index
-            const int ox = (xl & 1) ? -(kGsW / 2) + 1 : kGsW / 2;     // word(x+1) - word(x)
-            const int oxm = (xl & 1) ? -(kGsW / 2) : kGsW / 2 - 1;    // word(x-1) - word(x)
+            const int wc = cword[par][it];
+            // x neighbours sit in the other parity half: x - 1 at word ob, x + 1 at ob + 1
+            const int ob = wc + (cw >= kGsH ? -kGsH : kGsH - 1);
 #define A0(o) p0[wc + (o)]
 #define AM(o) pm[wc + (o)]
 #define AP(o) pp[wc + (o)]
+#define B0(o) p0[ob + (o)]
+#define BM(o) pm[ob + (o)]
+#define BP(o) pp[ob + (o)]
             const Coef<float> c =
-                coef_p<float, true>(L, has_los ? L.losn[0] : ptab[xl], has_los ? L.losn[1] : ptab[kGsW + yl],
+                coef_p<float, true>(L, has_los ? L.losn[0] : ptab[cw], has_los ? L.losn[1] : ptab[kGsW + yl],
                                     has_los ? L.losn[2] : tz_tab[ql], beta);
-            const Nbr<float> sn = nbr_of<float>(
-                c, A0(ox), A0(oxm), A0(kGsW), A0(-kGsW), AP(0), AM(0), A0(kGsW + ox), A0(-kGsW + oxm),
-                A0(kGsW + oxm), A0(-kGsW + ox), AP(ox), AM(oxm), AP(oxm), AM(ox), AP(kGsW), AM(-kGsW), AP(-kGsW),
-                AM(kGsW));
+            const Nbr<float> sn = nbr_of<float>(c, B0(1), B0(0), A0(kGsW), A0(-kGsW), AP(0), AM(0), B0(kGsW + 1),
+                                                B0(-kGsW), B0(kGsW), B0(-kGsW + 1), BP(1), BM(0), BP(0), BM(1),
+                                                AP(kGsW), AM(-kGsW), AP(-kGsW), AM(kGsW));
 #undef A0
 #undef AM
 #undef AP
+#undef B0
+#undef BM
+#undef BP
             p0[wc] = mcgs_value<float>(c, fcur[it], sn, has_los);
         }
     };
