@@ -147,23 +147,88 @@ __device__ __forceinline__ void cell_residual_mixed(float *__restrict__ r, const
 // colour updates in place without races.
 __device__ __forceinline__ int mg_color(int ix, int iy, int iz) { return ((ix + iy) & 1) + 2 * ((iy + iz) & 1); }
 
-// The Gauss-Seidel value of one cell: (f + neighbour sums) / diagonal, with
-// the smoother's reciprocal (mg_div<T, true>); its coefficients come from
-// coef<T, true>.
+// The Gauss-Seidel value of one cell (the RBGS schedule's smoother, an
+// extension: the reference smooths with jacobi!, src/multigrid.jl:193-210):
+// the operator of src/multigrid.jl:161-185 solved for the cell, (f +
+// neighbour sums) / diagonal, evaluated with fused multiply-adds and the
+// smoother's reciprocal.  The smoother only approximates the solve; the
+// FP64-refined residual decides convergence.  Per axis the coefficients take
+// p (the LOS component in cell units), p2 = p * p and q = cell^2 * p2.
+__device__ __forceinline__ float mg_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double mg_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T>
-__device__ __forceinline__ T mcgs_value(const Coef<T> &c, T f, const Nbr<T> &s, int has_los) {
-    T j = f + s.A + s.B + s.C + s.X;
-    if (!has_los) j += s.G;
-    return mg_div<T, true>(j, (T)2 * (c.gpx2 + c.gpy2 + c.gpz2));
+struct GsCoef {
+    T px, py, pz, g, gx, gy, gz;
+};
+
+template <typename T>
+__device__ __forceinline__ GsCoef<T> gs_coef(const MGLevel &L, T px, T py, T pz, T px2, T py2, T pz2, T qx, T qy,
+                                             T qz, T beta) {
+    GsCoef<T> c;
+    c.px = px;
+    c.py = py;
+    c.pz = pz;
+    c.g = mg_div<T, true>(beta, (qx + qy) + qz);
+    c.gx = mg_fma(c.g, px2, (T)L.icell2[0]);
+    c.gy = mg_fma(c.g, py2, (T)L.icell2[1]);
+    c.gz = mg_fma(c.g, pz2, (T)L.icell2[2]);
+    return c;
+}
+
+// the same from p alone (fixed LOS, or p of the cell)
+template <typename T>
+__device__ __forceinline__ GsCoef<T> gs_coef_p(const MGLevel &L, T px, T py, T pz, T beta) {
+    const T px2 = px * px, py2 = py * py, pz2 = pz * pz;
+    return gs_coef<T>(L, px, py, pz, px2, py2, pz2, (T)L.cell2[0] * px2, (T)L.cell2[1] * py2, (T)L.cell2[2] * pz2,
+                      beta);
+}
+
+// the new value from f and the 18 neighbour values (argument order of nbr_of)
+template <typename T>
+__device__ __forceinline__ T gs_update(const GsCoef<T> &c, T f, T vxp, T vxm, T vyp, T vym, T vzp, T vzm, T xpyp,
+                                       T xmym, T xmyp, T xpym, T xpzp, T xmzm, T xmzp, T xpzm, T ypzp, T ymzm, T ymzp,
+                                       T ypzm, int has_los) {
+    T j = mg_fma(c.gz, vzp + vzm, mg_fma(c.gy, vyp + vym, mg_fma(c.gx, vxp + vxm, f)));
+    const T exy = (xpyp + xmym) - (xmyp + xpym), exz = (xpzp + xmzm) - (xmzp + xpzm),
+            eyz = (ypzp + ymzm) - (ymzp + ypzm);
+    const T x = mg_fma(c.py * c.pz, eyz, mg_fma(c.px * c.pz, exz, (c.px * c.py) * exy));
+    j = mg_fma((T)0.5 * c.g, x, j);
+    if (!has_los) j = mg_fma(c.g, mg_fma(c.pz, vzp - vzm, mg_fma(c.py, vyp - vym, c.px * (vxp - vxm))), j);
+    return mg_div<T, true>(j, (T)2 * ((c.gx + c.gy) + c.gz));
 }
 
 template <typename T>
 __device__ __forceinline__ void cell_mcgs(T *v, const T *__restrict__ f, const MGLevel &L, int ix, int iy, int iz,
                                           float beta, int has_los) {
-    const Coef<T> c = coef<T, true>(L, ix, iy, iz, (T)beta, has_los);
-    const Nbr<T> s = nbr<T>(v, L, ix, iy, iz, c);
-    const int64_t I = ix + (int64_t)L.n[0] * (iy + (int64_t)L.n[1] * iz);
-    v[I] = mcgs_value<T>(c, f[I], s, has_los);
+    T px, py, pz;
+    if (has_los) {
+        px = (T)L.losn[0];
+        py = (T)L.losn[1];
+        pz = (T)L.losn[2];
+    } else if constexpr (std::is_same<T, float>::value) {
+        px = L.pv[ix];
+        py = L.pv[L.n[0] + iy];
+        pz = L.pv[L.n[0] + L.n[1] + iz];
+    } else {
+        px = (T)L.xv[ix] / (T)L.cell[0];
+        py = (T)L.xv[L.n[0] + iy] / (T)L.cell[1];
+        pz = (T)L.xv[L.n[0] + L.n[1] + iz] / (T)L.cell[2];
+    }
+    const GsCoef<T> c = gs_coef_p<T>(L, px, py, pz, (T)beta);
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const int ixp = ix + 1 == nx ? 0 : ix + 1, ixm = ix == 0 ? nx - 1 : ix - 1;
+    const int iyp = iy + 1 == ny ? 0 : iy + 1, iym = iy == 0 ? ny - 1 : iy - 1;
+    const int izp = iz + 1 == nz ? 0 : iz + 1, izm = iz == 0 ? nz - 1 : iz - 1;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+#define V(a, b, d) v[(a) + sy * (b) + sz * (d)]
+    const T nv = gs_update<T>(c, f[ix + sy * iy + sz * iz], V(ixp, iy, iz), V(ixm, iy, iz), V(ix, iyp, iz),
+                              V(ix, iym, iz), V(ix, iy, izp), V(ix, iy, izm), V(ixp, iyp, iz), V(ixm, iym, iz),
+                              V(ixm, iyp, iz), V(ixp, iym, iz), V(ixp, iy, izp), V(ixm, iy, izm), V(ixm, iy, izp),
+                              V(ixp, iy, izm), V(ix, iyp, izp), V(ix, iym, izm), V(ix, iym, izp), V(ix, iyp, izm),
+                              has_los);
+#undef V
+    v[ix + sy * iy + sz * iz] = nv;
 }
 
 __global__ void k_mg_jacobi(float *__restrict__ vout, const float *__restrict__ vin,
@@ -238,7 +303,7 @@ constexpr int kGsLW = kGsTX + 2 * kGsHalo;     // 72 columns per plane row
 // row (32 consecutive words), and a colour's consecutive rows, 164 words
 // apart, fall 4 banks further on, so the narrow columns beyond 32 spread over
 // the banks too.
-constexpr int kGsW = 82, kGsH = 40;
+constexpr int kGsW = 82, kGsH = 40, kGsXWords = kGsH + kGsLW / 2;   // column words in use: 76
 constexpr int kGsRows = kGsTY + 2 * kGsHalo;   // 24 rows per plane
 constexpr int kGsPlane = kGsW * kGsRows;
 constexpr int kGsRing = 10;                    // planes P-9 .. P
@@ -292,24 +357,32 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
     __shared__ __align__(16) float lds[kGsRing * kGsPlane];
     __shared__ double red[kGsThreads / 64];
     double acc = 0.0;
-    __shared__ float ptab[kGsW + kGsRows + kGsMaxTz + 8];   // p of LDS column word | row | local plane
+    // (p, p^2, cell^2 p^2) of each LDS column word (76) and row; (p, p^2) of each local plane
+    __shared__ float4 xytab[kGsXWords + kGsRows];
+    __shared__ float2 ztab[kGsMaxTz + 8];
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     int tile_x, tile_y, tile_z;
     const int tile = mg_tile(tile_x, tile_y, tile_z);
     const int x0 = tile_x * kGsTX, y0 = tile_y * kGsTY, z0 = tile_z * tz;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     const int tid = threadIdx.x;
-    float *tz_tab = ptab + kGsW + kGsRows;
     if (!has_los)
-        for (int q = tid; q < kGsW + kGsRows + tz + 8; q += kGsThreads) {
-            int g;
-            if (q < kGsW)   // column word q: x = 2q (even half) or 2(q - kGsH) + 1
-                g = gs_wrap(x0 - kGsHalo + min(q < kGsH ? 2 * q : 2 * (q - kGsH) + 1, kGsLW - 1), nx);
-            else if (q < kGsW + kGsRows)
-                g = nx + gs_wrap(y0 - kGsHalo + q - kGsW, ny);
-            else
-                g = nx + ny + gs_wrap(z0 - kGsHalo + q - kGsW - kGsRows, nz);
-            ptab[q] = L.pv[g];
+        for (int q = tid; q < kGsXWords + kGsRows + tz + 8; q += kGsThreads) {
+            if (q < kGsXWords + kGsRows) {
+                int g, a;
+                if (q < kGsXWords) {   // column word q: x = 2q (even half) or 2(q - kGsH) + 1
+                    g = gs_wrap(x0 - kGsHalo + min(q < kGsH ? 2 * q : 2 * (q - kGsH) + 1, kGsLW - 1), nx);
+                    a = 0;
+                } else {
+                    g = nx + gs_wrap(y0 - kGsHalo + q - kGsXWords, ny);
+                    a = 1;
+                }
+                const float p = L.pv[g], p2 = p * p;
+                xytab[q] = make_float4(p, p2, L.cell2[a] * p2, 0.f);
+            } else {
+                const float p = L.pv[nx + ny + gs_wrap(z0 - kGsHalo + q - kGsXWords - kGsRows, nz)];
+                ztab[q - kGsXWords - kGsRows] = make_float2(p, p * p);
+            }
         }
 
     // plane loads: chunk q = row * 18 + j covers x0 - 4 + 4j .. +3 of LDS row
@@ -425,11 +498,8 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
         }
         fetch_f(k + 1, std::integral_constant<int, 1 - par>{}, fnext);
         const int k10 = k % kGsRing;
-#pragma unroll
-        for (int it = 0; it < kGsIters; it++) {
-            const int d = desc[it];
-            const int st = d & 3;
-            if (d < 0 || !active(st, k)) continue;
+        // the update of the thread's cell of iteration it, in stage st
+        auto update = [&](int it, int st) {
             const int cw = cpos[par][it] & 255, yl = cpos[par][it] >> 8;   // column word, row
             const int ql = k - 1 - 2 * st;   // local plane of Q, >= 1
             int m = k10 + kGsRing - 1 - 2 * st;   // ring slot of Q
@@ -447,19 +517,29 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
 #define B0(o) p0[ob + (o)]
 #define BM(o) pm[ob + (o)]
 #define BP(o) pp[ob + (o)]
-            const Coef<float> c =
-                coef_p<float, true>(L, has_los ? L.losn[0] : ptab[cw], has_los ? L.losn[1] : ptab[kGsW + yl],
-                                    has_los ? L.losn[2] : tz_tab[ql], beta);
-            const Nbr<float> sn = nbr_of<float>(c, B0(1), B0(0), A0(kGsW), A0(-kGsW), AP(0), AM(0), B0(kGsW + 1),
-                                                B0(-kGsW), B0(kGsW), B0(-kGsW + 1), BP(1), BM(0), BP(0), BM(1),
-                                                AP(kGsW), AM(-kGsW), AP(-kGsW), AM(kGsW));
+            GsCoef<float> c;
+            if (has_los) {
+                c = gs_coef_p<float>(L, L.losn[0], L.losn[1], L.losn[2], beta);
+            } else {
+                const float4 tx = xytab[cw], ty = xytab[kGsXWords + yl];
+                const float2 tq = ztab[ql];
+                c = gs_coef<float>(L, tx.x, ty.x, tq.x, tx.y, ty.y, tq.y, tx.z, ty.z, L.cell2[2] * tq.y, beta);
+            }
+            p0[wc] = gs_update<float>(c, fcur[it], B0(1), B0(0), A0(kGsW), A0(-kGsW), AP(0), AM(0), B0(kGsW + 1),
+                                      B0(-kGsW), B0(kGsW), B0(-kGsW + 1), BP(1), BM(0), BP(0), BM(1), AP(kGsW),
+                                      AM(-kGsW), AP(-kGsW), AM(kGsW), has_los);
 #undef A0
 #undef AM
 #undef AP
 #undef B0
 #undef BM
 #undef BP
-            p0[wc] = mcgs_value<float>(c, fcur[it], sn, has_los);
+        };
+#pragma unroll
+        for (int it = 0; it < kGsIters; it++) {
+            const int d = desc[it];
+            if (d < 0 || !active(d & 3, k)) continue;
+            update(it, d & 3);
         }
     };
 
