@@ -289,7 +289,7 @@ __global__ void k_mg_mcgs(T *__restrict__ v, const T *__restrict__ f, MGLevel L,
 // plane P = z0 - 4 + k and updates colour s on plane P - 1 - 2s (s = 0..3):
 // the four stages of a step read and write disjoint (plane, colour) sets, so
 // one barrier per step orders everything, and the tile of plane P - 8
-// (finished in the step before) is written out as float4 rows.
+// (finished in the step before) is written out in coalesced rows.
 //
 // Latency: the next plane's float4 loads and the next step's f values are
 // in flight during a step's arithmetic; the coefficients' cell positions
@@ -571,7 +571,7 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
 // on plane P - 2t + 1: level t - 1 there and on both neighbouring planes was
 // finished in earlier steps, and the stages of one step touch disjoint
 // planes, so one barrier per step orders them.  Level S (the tile itself)
-// goes straight to vout as float4 rows.
+// goes straight to vout, a row per wave.
 constexpr int kJbTX = 64, kJbTY = 16, kJbRing = 4, kJbMaxTz = 128;
 
 template <int S>
@@ -579,7 +579,8 @@ struct JbShape {
     // 512 threads at S = 3 keep its per-thread cell lists short enough for
     // two waves per SIMD without spills
     static constexpr int Threads = S == 3 ? 512 : 256;
-    static constexpr int CPT = kJbTX * kJbTY / Threads;   // level-S cells per thread (consecutive x)
+    static constexpr int CPT = kJbTX * kJbTY / Threads;   // level-S cells per thread (one column)
+    static constexpr int RS = Threads / kJbTX;             // rows between them
     static constexpr int W = kJbTX + 2 * S, H = kJbTY + 2 * S, Plane = W * H;
     // cells of the levels below S: level t covers (64 + 2e) x (16 + 2e), e = S - t
     static constexpr int cells(int t) { return (kJbTX + 2 * (S - t)) * (kJbTY + 2 * (S - t)); }
@@ -672,14 +673,18 @@ __global__ void __launch_bounds__(JbShape<S>::Threads, 2) k_mg_jacobi_tb(float *
             fr[it] = f[off];
         }
     };
-    // level S: CPT consecutive cells of row tid / (64 / CPT)
-    using FV = typename std::conditional<J::CPT == 4, float4, float2>::type;
-    constexpr int TPR = kJbTX / J::CPT;   // threads per row
-    const int oy = tid / TPR, ox = J::CPT * (tid % TPR);
+    // level S: cells (ox, oy + RS j), j < CPT, of the tile: a wave takes a
+    // row, a column per lane, so its LDS reads are consecutive words
+    struct FV {
+        float v[J::CPT];
+    };
+    const int ox = tid % kJbTX, oy = tid / kJbTX;
     const int64_t o_off = (x0 + ox) + sy * (y0 + oy);
     auto fetch_fo = [&](int k, FV &fo) {
         const int Q = z0 - S + k - 2 * S + 1;
-        fo = *reinterpret_cast<const FV *>(f + sz * (active(S, k) ? Q : z0) + o_off);
+        const float *src = f + sz * (active(S, k) ? Q : z0) + o_off;
+#pragma unroll
+        for (int j = 0; j < J::CPT; j++) fo.v[j] = src[sy * J::RS * j];
     };
     // one cell of level t from level t - 1 (local cell xl, yl, local plane ql)
     auto cell = [&](int t, int xl, int yl, int ql, float fv) {
@@ -715,12 +720,9 @@ __global__ void __launch_bounds__(JbShape<S>::Threads, 2) k_mg_jacobi_tb(float *
         }
         if (active(S, k)) {
             const int ql = k - 2 * S + 1;
-            FV o;
-            const float *fo = reinterpret_cast<const float *>(&focur);
-            float *op = reinterpret_cast<float *>(&o);
+            float *dst = vout + sz * (z0 - S + ql) + o_off;
 #pragma unroll
-            for (int j = 0; j < J::CPT; j++) op[j] = cell(S, S + ox + j, S + oy, ql, fo[j]);
-            *reinterpret_cast<FV *>(vout + sz * (z0 - S + ql) + o_off) = o;
+            for (int j = 0; j < J::CPT; j++) dst[sy * J::RS * j] = cell(S, S + ox, S + oy + J::RS * j, ql, focur.v[j]);
         }
     };
 
@@ -1097,10 +1099,12 @@ __global__ void k_mg_prolong(T *__restrict__ fine, const U *__restrict__ v, int 
 // (67 x 19 with the halo) through a ring of 4, computes r on the 65 x 17 fine
 // cells the block's 27-point sums read (x0 .. x0 + 64: the + 1 side wraps like
 // reduce's ixp) into a ring of 4 r planes, and a coarse plane every second
-// step, once its three fine planes are in.
+// step, once its three fine planes are in.  r rows keep even x (33) then odd
+// x (32), so the restriction's reads, every second x, are consecutive words.
 constexpr int kRrCX = 32, kRrCY = 8, kRrThreads = 256, kRrMaxTzc = 16;
 constexpr int kRrVW = 2 * kRrCX + 3, kRrVH = 2 * kRrCY + 3, kRrVPlane = kRrVW * kRrVH;   // 67 x 19
 constexpr int kRrW = 2 * kRrCX + 1, kRrH = 2 * kRrCY + 1, kRrPlane = kRrW * kRrH;          // 65 x 17
+constexpr int kRrEven = kRrCX + 1;                                                          // 33 even x
 constexpr int kRrIters = (kRrPlane + kRrThreads - 1) / kRrThreads;                          // 5
 constexpr int kRrLoads = (kRrVPlane + kRrThreads - 1) / kRrThreads;                          // 5
 
@@ -1207,7 +1211,7 @@ __global__ void __launch_bounds__(kRrThreads)
 #undef AM
 #undef AP
                 if (!has_los) t -= sn.G;
-                rp[yr * kRrW + xr] = fcur[it] - t;
+                rp[yr * kRrW + (xr & 1) * kRrEven + (xr >> 1)] = fcur[it] - t;
             }
         }
         // coarse plane j once fine r planes 2j .. 2j + 2 (local) are in: Q = z0 + 2j + 3
@@ -1216,10 +1220,13 @@ __global__ void __launch_bounds__(kRrThreads)
             const int j = jq >> 1;
             const float *rm = rring + ((2 * j) & 3) * kRrPlane, *r0 = rring + ((2 * j + 1) & 3) * kRrPlane,
                         *rq = rring + ((2 * j + 2) & 3) * kRrPlane;
-            const int ix = 2 * kxl + 1, iy = 2 * kyl + 1;   // local fine indices of the centre
-#define W0(a, b) r0[(iy + (b)) * kRrW + ix + (a)]
-#define WM(a, b) rm[(iy + (b)) * kRrW + ix + (a)]
-#define WP(a, b) rq[(iy + (b)) * kRrW + ix + (a)]
+            // the centre (local 2 kxl + 1, odd) is word kRrEven + kxl of its row, x -/+ 1
+            // (even) words kxl and kxl + 1
+            const int iy = 2 * kyl + 1, wc = kRrEven + kxl, we = kxl;
+#define RX(a) ((a) == 0 ? wc : we + ((a) > 0))
+#define W0(a, b) r0[(iy + (b)) * kRrW + RX(a)]
+#define WM(a, b) rm[(iy + (b)) * kRrW + RX(a)]
+#define WP(a, b) rq[(iy + (b)) * kRrW + RX(a)]
             const float sum = 8.0f * W0(0, 0) +
                               4.0f * (W0(1, 0) + W0(-1, 0) + W0(0, 1) + W0(0, -1) + WP(0, 0) + WM(0, 0)) +
                               2.0f * (W0(1, 1) + W0(-1, 1) + W0(1, -1) + W0(-1, -1) + WP(1, 0) + WP(-1, 0) +
@@ -1229,6 +1236,7 @@ __global__ void __launch_bounds__(kRrThreads)
 #undef W0
 #undef WM
 #undef WP
+#undef RX
             coarse[c_off + (int64_t)cx * cy * (z0 / 2 + j)] = (float)((double)sum / 64.0);
         }
     }
