@@ -1434,12 +1434,55 @@ __global__ void k_fill(float *a, int64_t n, float v) {
 
 // ------------------------------------------------------------ launchers
 
-// z planes per workgroup of k_mg_refine: at least 1024 workgroups
+// Workgroups of a kernel resident on one CU (occupancy query, once per
+// kernel) and the CU count of the current device.
+template <typename K>
+static int blocks_per_cu(K kernel, int threads) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) n = 1;
+    return n;
+}
+static int device_cus() {
+    static const int n = [] {
+        int d = 0, c = 0;
+        if (hipGetDevice(&d) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d) !=
+                                                  hipSuccess)
+            c = 0;
+        return c > 0 ? c : 256;
+    }();
+    return n;
+}
+
+// z planes per workgroup of a z-marching kernel over `cols` tile columns of
+// nz planes: the power of two in [min_tz, max_tz] dividing nz that minimises
+// (rounds of resident workgroups) x (tz + the pipeline's extra steps), so
+// that a launch neither ends on a thinly occupied last round nor pays the
+// halo planes of short columns too often.
+static int zm_tz(int64_t cols, int nz, int max_tz, int min_tz, int per_cu, int extra) {
+    const int64_t slots = (int64_t)device_cus() * per_cu;
+    int best = min_tz;
+    int64_t best_cost = -1;
+    for (int tz = max_tz; tz >= min_tz; tz /= 2) {
+        if (nz % tz) continue;
+        const int64_t rounds = (cols * (nz / tz) + slots - 1) / slots, cost = rounds * (tz + extra);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = tz;
+        }
+    }
+    return best;
+}
+
+// z planes per workgroup of k_mg_refine (tz + 2 steps)
 static int refine_tz(const MGLevel &L) {
-    const int64_t cols = (int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY);
-    int tz = kRfMaxTz;
-    while (tz > 8 && (L.n[2] % tz || cols * (L.n[2] / tz) < 1024)) tz /= 2;
-    return tz;
+    static const int per_cu = blocks_per_cu(k_mg_refine, kRfThreads);
+    return zm_tz((int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY), L.n[2], kRfMaxTz, 8, per_cu, 2);
+}
+
+// the same for k_mg_residual_zm
+static int residual_zm_tz(const MGLevel &L) {
+    static const int per_cu = blocks_per_cu(k_mg_residual_zm, kRfThreads);
+    return zm_tz((int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY), L.n[2], kRfMaxTz, 8, per_cu, 2);
 }
 
 bool mg_refine_ok(const MGLevel &L) { return L.n[0] % kRfTX == 0 && L.n[1] % kRfTY == 0 && L.n[2] % 8 == 0; }
@@ -1482,7 +1525,7 @@ hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, 
                                 int has_los, hipStream_t s) {
     if constexpr (sizeof(T) == 4) {   // z-marching on grids of >= 128^3 cells
         if (mg_refine_ok(L) && (int64_t)L.n[0] * L.n[1] * L.n[2] >= ((int64_t)1 << 21)) {
-            const int tz = refine_tz(L);
+            const int tz = residual_zm_tz(L);
             const dim3 grid((unsigned)(L.n[0] / kRfTX), (unsigned)(L.n[1] / kRfTY), (unsigned)(L.n[2] / tz));
             hipLaunchKernelGGL(k_mg_residual_zm, grid, dim3(kRfThreads), 0, s, r, v, f, L, beta, has_los, tz);
             return hipGetLastError();
@@ -1494,13 +1537,10 @@ hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, 
     return hipGetLastError();
 }
 
-// coarse planes per workgroup of k_mg_residual_restrict: up to 16, fewer so
-// that a launch has at least 1024 workgroups
+// coarse planes per workgroup of k_mg_residual_restrict (2 tzc + 4 steps)
 static int rr_tzc(const MGLevel &L) {
-    const int64_t cols = (int64_t)(L.n[0] / (2 * kRrCX)) * (L.n[1] / (2 * kRrCY));
-    int t = kRrMaxTzc;
-    while (t > 1 && ((L.n[2] / 2) % t || cols * (L.n[2] / 2 / t) < 1024)) t /= 2;
-    return t;
+    static const int per_cu = blocks_per_cu(k_mg_residual_restrict, kRrThreads);
+    return zm_tz((int64_t)(L.n[0] / (2 * kRrCX)) * (L.n[1] / (2 * kRrCY)), L.n[2] / 2, kRrMaxTzc, 1, per_cu, 2);
 }
 
 bool mg_residual_restrict_ok(const MGLevel &L) {
@@ -1517,14 +1557,11 @@ hipError_t launch_mg_residual_restrict(float *coarse, const float *v, const floa
     return hipGetLastError();
 }
 
-// z planes per workgroup of k_mg_gs4: 128, or fewer (a divisor of nz, at
-// least 8) so that a launch has at least 512 workgroups, two per CU; each
-// workgroup also loads 8 halo planes and runs 12 pipeline steps beyond tz.
+// z planes per workgroup of k_mg_gs4 (at least 8; each workgroup also loads
+// 8 halo planes and runs 12 pipeline steps beyond tz)
 static int gs4_tz(const MGLevel &L) {
-    const int64_t cols = (int64_t)(L.n[0] / kGsTX) * (L.n[1] / kGsTY);
-    int tz = kGsMaxTz;
-    while (tz > 8 && (L.n[2] % tz || cols * (L.n[2] / tz) < 512)) tz /= 2;
-    return tz;
+    static const int per_cu = blocks_per_cu(k_mg_gs4, kGsThreads);
+    return zm_tz((int64_t)(L.n[0] / kGsTX) * (L.n[1] / kGsTY), L.n[2], kGsMaxTz, 8, per_cu, 12);
 }
 
 bool mg_gs4_ok(const MGLevel &L) {
@@ -1545,12 +1582,11 @@ hipError_t launch_mg_gs4(float *vout, const float *vin, const float *f, const MG
     return hipGetLastError();
 }
 
-// z planes per workgroup of k_mg_jacobi_tb: at least 512 workgroups
+// z planes per workgroup of k_mg_jacobi_tb<S> (tz + 3 S - 1 steps)
+template <int S>
 static int jacobi_tb_tz(const MGLevel &L) {
-    const int64_t cols = (int64_t)(L.n[0] / kJbTX) * (L.n[1] / kJbTY);
-    int tz = kJbMaxTz;
-    while (tz > 8 && (L.n[2] % tz || cols * (L.n[2] / tz) < 512)) tz /= 2;
-    return tz;
+    static const int per_cu = blocks_per_cu(k_mg_jacobi_tb<S>, JbShape<S>::Threads);
+    return zm_tz((int64_t)(L.n[0] / kJbTX) * (L.n[1] / kJbTY), L.n[2], kJbMaxTz, 8, per_cu, 3 * S - 1);
 }
 
 bool mg_jacobi_tb_ok(const MGLevel &L) {
@@ -1561,7 +1597,7 @@ bool mg_jacobi_tb_ok(const MGLevel &L) {
 hipError_t launch_mg_jacobi_tb(float *vout, const float *vin, const float *f, const MGLevel &L, float beta,
                                float omega, int has_los, int sweeps, hipStream_t s) {
     if (!mg_jacobi_tb_ok(L) || vout == vin || (sweeps != 2 && sweeps != 3)) return hipErrorInvalidValue;
-    const int tz = jacobi_tb_tz(L);
+    const int tz = sweeps == 2 ? jacobi_tb_tz<2>(L) : jacobi_tb_tz<3>(L);
     const dim3 grid((unsigned)(L.n[0] / kJbTX), (unsigned)(L.n[1] / kJbTY), (unsigned)(L.n[2] / tz));
     if (sweeps == 2)
         hipLaunchKernelGGL(k_mg_jacobi_tb<2>, grid, dim3(JbShape<2>::Threads), 0, s, vout, vin, f, L, beta, omega,
