@@ -1051,10 +1051,8 @@ __global__ void k_mg_reduce(T *__restrict__ coarse, const U *__restrict__ w, int
 // Trilinear prolongation (src/multigrid.jl:456-509) evaluated per fine cell:
 // odd fine index 2c+1 copies coarse c, even index 2c+2 averages c and c+1;
 // the sums keep the reference's term order for each of the 8 cases.
-template <typename T, typename U>
-__device__ __forceinline__ void cell_prolong(T *__restrict__ fine, const U *__restrict__ v, int cx, int cy, int cz,
-                                             int add, int ix, int iy, int iz) {
-    const int fx = 2 * cx, fy = 2 * cy;
+template <typename U>
+__device__ __forceinline__ U prolong_value(const U *__restrict__ v, int cx, int cy, int cz, int ix, int iy, int iz) {
     const int tx = (ix & 1) == 0, ty = (iy & 1) == 0, tz = (iz & 1) == 0;
     const int ax = tx ? (ix == 0 ? cx - 1 : ix / 2 - 1) : (ix - 1) / 2;
     const int ay = ty ? (iy == 0 ? cy - 1 : iy / 2 - 1) : (iy - 1) / 2;
@@ -1080,7 +1078,14 @@ __device__ __forceinline__ void cell_prolong(T *__restrict__ fine, const U *__re
             break;
     }
 #undef C
-    const int64_t I = ix + (int64_t)fx * (iy + (int64_t)fy * iz);
+    return val;
+}
+
+template <typename T, typename U>
+__device__ __forceinline__ void cell_prolong(T *__restrict__ fine, const U *__restrict__ v, int cx, int cy, int cz,
+                                             int add, int ix, int iy, int iz) {
+    const U val = prolong_value<U>(v, cx, cy, cz, ix, iy, iz);
+    const int64_t I = ix + (int64_t)(2 * cx) * (iy + (int64_t)(2 * cy) * iz);
     fine[I] = add ? (T)(fine[I] + (T)val) : (T)val;
 }
 
@@ -1251,9 +1256,22 @@ __global__ void k_mg_prolong2(T *__restrict__ fine, const U *__restrict__ v, int
     const int ix = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
     if (ix >= fx) return;
     const int iy = blockIdx.y;
+    using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
     for (int iz = 4 * blockIdx.z; iz < 4 * blockIdx.z + 4; iz++) {
-        cell_prolong<T, U>(fine, v, cx, cy, cz, add, ix, iy, iz);
-        cell_prolong<T, U>(fine, v, cx, cy, cz, add, ix + 1, iy, iz);
+        const U a = prolong_value<U>(v, cx, cy, cz, ix, iy, iz), b = prolong_value<U>(v, cx, cy, cz, ix + 1, iy, iz);
+        // ix is even: the pair is one aligned 2-element access
+        T2 *p = reinterpret_cast<T2 *>(fine + (ix + (int64_t)fx * (iy + (int64_t)(2 * cy) * iz)));
+        if (add) {
+            T2 o = *p;
+            o.x = (T)(o.x + (T)a);
+            o.y = (T)(o.y + (T)b);
+            *p = o;
+        } else {
+            T2 o;
+            o.x = (T)a;
+            o.y = (T)b;
+            *p = o;
+        }
     }
 }
 
@@ -1671,7 +1689,8 @@ template <typename T, typename U>
 hipError_t launch_mg_prolong_t(T *fine, const U *coarse, const int coarse_n[3], int add,
                                hipStream_t s) {
     const int fx = 2 * coarse_n[0], fy = 2 * coarse_n[1], fz = 2 * coarse_n[2];
-    if (fz % 4 == 0 && (int64_t)fx * fy * fz >= ((int64_t)1 << 21)) {   // two cells per thread, 4 planes
+    if (fz % 4 == 0 && (int64_t)fx * fy * fz >= ((int64_t)1 << 21) &&
+        ((uintptr_t)fine % (2 * sizeof(T))) == 0) {   // two cells per thread (one aligned pair), 4 planes
         const int b2 = bx_for(fx / 2);
         hipLaunchKernelGGL((k_mg_prolong2<T, U>), dim3((unsigned)((fx / 2 + b2 - 1) / b2), (unsigned)fy,
                                                        (unsigned)(fz / 4)),
