@@ -198,9 +198,9 @@ __device__ __forceinline__ T gs_update(const GsCoef<T> &c, T f, T vxp, T vxm, T 
     return mg_div<T, true>(j, (T)2 * ((c.gx + c.gy) + c.gz));
 }
 
+// the coefficients of cell (ix, iy, iz)
 template <typename T>
-__device__ __forceinline__ void cell_mcgs(T *v, const T *__restrict__ f, const MGLevel &L, int ix, int iy, int iz,
-                                          float beta, int has_los) {
+__device__ __forceinline__ GsCoef<T> gs_coef_at(const MGLevel &L, int ix, int iy, int iz, float beta, int has_los) {
     T px, py, pz;
     if (has_los) {
         px = (T)L.losn[0];
@@ -215,7 +215,13 @@ __device__ __forceinline__ void cell_mcgs(T *v, const T *__restrict__ f, const M
         py = (T)L.xv[L.n[0] + iy] / (T)L.cell[1];
         pz = (T)L.xv[L.n[0] + L.n[1] + iz] / (T)L.cell[2];
     }
-    const GsCoef<T> c = gs_coef_p<T>(L, px, py, pz, (T)beta);
+    return gs_coef_p<T>(L, px, py, pz, (T)beta);
+}
+
+template <typename T>
+__device__ __forceinline__ void cell_mcgs(T *v, const T *__restrict__ f, const MGLevel &L, int ix, int iy, int iz,
+                                          float beta, int has_los) {
+    const GsCoef<T> c = gs_coef_at<T>(L, ix, iy, iz, beta, has_los);
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     const int ixp = ix + 1 == nx ? 0 : ix + 1, ixm = ix == 0 ? nx - 1 : ix - 1;
     const int iyp = iy + 1 == ny ? 0 : iy + 1, iym = iy == 0 ? ny - 1 : iy - 1;
@@ -1307,6 +1313,41 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(MGCoarse<T> a) {
                 }
                 return;
             }
+        }
+        const int nx = a.L[l].n[0], ny = a.L[l].n[1], nz = a.L[l].n[2], nc = nx * ny * nz;
+        if (sizeof(T) == sizeof(double) && nc <= 64) {
+            // an FP64 level of at most 64 cells (the RBGS solve's coarsest,
+            // with its many sweeps) in the first wave: a cell per lane, its
+            // neighbours' values by cross-lane shuffles, no barrier between
+            // colour passes; the same per-cell arithmetic as cell_mcgs on the
+            // same values
+            if (threadIdx.x < 64) {
+                const int c = threadIdx.x, cc = c < nc ? c : 0;
+                const int ix = cc % nx, iy = (cc / nx) % ny, iz = cc / (nx * ny);
+                const int ixp = ix + 1 == nx ? 0 : ix + 1, ixm = ix == 0 ? nx - 1 : ix - 1;
+                const int iyp = iy + 1 == ny ? 0 : iy + 1, iym = iy == 0 ? ny - 1 : iy - 1;
+                const int izp = iz + 1 == nz ? 0 : iz + 1, izm = iz == 0 ? nz - 1 : iz - 1;
+                auto id = [&](int x, int y, int z) { return x + nx * (y + ny * z); };
+                T *v = vc(l);
+                T val = v[cc];
+                const T fv = a.F[l][cc];
+                const GsCoef<T> co = gs_coef_at<T>(a.L[l], ix, iy, iz, a.beta, a.has_los);
+                const int mine = c < nc ? mg_color(ix, iy, iz) : -1;
+#define V(x, y, z) __shfl(val, id(x, y, z), 64)
+                for (int i = 0; i < n; i++)
+                    for (int col = 0; col < 4; col++) {
+                        const T nv = gs_update<T>(co, fv, V(ixp, iy, iz), V(ixm, iy, iz), V(ix, iyp, iz),
+                                                  V(ix, iym, iz), V(ix, iy, izp), V(ix, iy, izm), V(ixp, iyp, iz),
+                                                  V(ixm, iym, iz), V(ixm, iyp, iz), V(ixp, iym, iz), V(ixp, iy, izp),
+                                                  V(ixm, iy, izm), V(ixm, iy, izp), V(ixp, iy, izm), V(ix, iyp, izp),
+                                                  V(ix, iym, izm), V(ix, iym, izp), V(ix, iyp, izm), a.has_los);
+                        if (mine == col) val = nv;
+                    }
+#undef V
+                if (c < nc) v[c] = val;
+            }
+            __syncthreads();
+            return;
         }
         for (int i = 0; i < n; i++)
             for (int col = 0; col < 4; col++) {
