@@ -939,7 +939,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 50 for lc512_warm, about 3 s per form; 10 otherwise)")
+                    help="timed steps (default 120 for lc512_warm: at least 3 s per form; 10 otherwise)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="lc512_warm",
                     help="lc512_warm (default: the 512^3 lightcone multigrid recon, learned warm start against "
@@ -964,7 +964,7 @@ def main():
                     help="only run 1 GiB device-to-device copies (byte-counter calibration)")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 50 if args.workload == "lc512_warm" else 10
+        args.steps = 120 if args.workload == "lc512_warm" else 10
     if args.calibrate:
         copy_calibration()
         return
