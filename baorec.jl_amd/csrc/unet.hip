@@ -63,33 +63,46 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     for (int m = 0; m < MB; m++)
 #pragma unroll
         for (int nb = 0; nb < NB; nb++) acc[m][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < KB; kb++) {
+    // operands of chunk kb: B (weights) and A (the input at the lane's tap and
+    // channel group), 32-bit element offsets (the launcher checks the sizes)
+    auto load_chunk = [&](int kb, bf16x8 *bv, bf16x8 *av) {
         const int kc = kb * 32 + g * 8;
         int t = kc / CIN;
         const int ci = kc - t * CIN;
         if (t > 26) t = 26;   // padding columns of K: their weights are zero
         const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
-        const int xs = (x + dx) & (nx - 1), zs = (z + dz) & (nz - 1);
-        bf16x8 bv[NB];
+        const uint32_t xs = (uint32_t)((x + dx) & (nx - 1)), zs = (uint32_t)((z + dz) & (nz - 1));
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++) bv[nb] = load8(a.w + ((size_t)((kb * NB + nb) * 4 + g) * 16 + r16) * 8);
+        for (int nb = 0; nb < NB; nb++) bv[nb] = load8(a.w + ((uint32_t)((kb * NB + nb) * 4 + g) * 16 + r16) * 8);
 #pragma unroll
         for (int m = 0; m < MB; m++) {
-            const int ys = (y0 + m + dy) & (ny - 1);
+            const uint32_t ys = (uint32_t)((y0 + m + dy) & (ny - 1));
             const uint16_t *src;
             if (C2 == 0 || ci < C1) {
-                src = a.in1 + ((((size_t)b * nz + zs) * ny + ys) * nx + xs) * C1 + ci;
+                src = a.in1 + ((((uint32_t)b * nz + zs) * ny + ys) * nx + xs) * C1 + ci;
             } else if (UP2) {
-                const int cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
-                src = a.in2 + ((((size_t)b * cz + (zs >> 1)) * cy + (ys >> 1)) * cx + (xs >> 1)) * C2 + (ci - C1);
+                const uint32_t cx = nx >> 1, cy = ny >> 1, cz = nz >> 1;
+                src = a.in2 + ((((uint32_t)b * cz + (zs >> 1)) * cy + (ys >> 1)) * cx + (xs >> 1)) * C2 + (ci - C1);
             } else {
-                src = a.in2 + ((((size_t)b * nz + zs) * ny + ys) * nx + xs) * C2 + (ci - C1);
+                src = a.in2 + ((((uint32_t)b * nz + zs) * ny + ys) * nx + xs) * C2 + (ci - C1);
             }
-            const bf16x8 av = load8(src);
+            av[m] = load8(src);
+        }
+    };
+    // chunk kb + 1's loads are in flight during chunk kb's products
+    bf16x8 bv[NB], av[MB], bn[NB], an[MB];
+    load_chunk(0, bn, an);
+    for (int kb = 0; kb < KB; kb++) {
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) bv[nb] = bn[nb];
+#pragma unroll
+        for (int m = 0; m < MB; m++) av[m] = an[m];
+        if (kb + 1 < KB) load_chunk(kb + 1, bn, an);
+#pragma unroll
+        for (int m = 0; m < MB; m++)
 #pragma unroll
             for (int nb = 0; nb < NB; nb++)
-                acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[nb], acc[m][nb], 0, 0, 0);
-        }
+                acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[nb], acc[m][nb], 0, 0, 0);
     }
     // lane holds rows (voxels) 4g .. 4g+3 of column (channel) r16
 #pragma unroll
@@ -514,6 +527,9 @@ using namespace unet;
 hipError_t launch_unet_conv(const ConvArgs &a, int c1, int c2, int co, bool relu, bool up2, int batch,
                             hipStream_t s) {
     const int nx = 1 << a.lx, ny = 1 << a.ly, nz = 1 << a.lz;
+    // k_conv addresses its inputs with 32-bit element offsets
+    const int cmax = c1 > c2 ? c1 : c2;
+    if ((int64_t)batch * nx * ny * nz * cmax >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
     dim3 grid((unsigned)((nx + 15) / 16), (unsigned)((ny + 7) / 8), (unsigned)(nz * batch));
 #define LAUNCH(C1_, C2_, CO_, UP_)                                                                        \
     if (c1 == C1_ && c2 == C2_ && co == CO_ && up2 == UP_) {                                              \
