@@ -622,6 +622,49 @@ def test_mg_gauss_seidel_check_catches_planted_colouring(monkeypatch):
         test_mg_gauss_seidel_is_textbook_colour_ordered_gauss_seidel(None)
 
 
+def centre_los_inverse(n, L, bmin, beta):
+    """The warm start's spectral preconditioner: the exact inverse of the
+    operator's Fourier symbol with the LOS fixed at the box centre (no
+    first-derivative term), zero on the k = 0 mode."""
+    d = np.full(3, bmin + L / 2)
+    d /= np.linalg.norm(d)
+    S = mg_symbol((n,) * 3, (L,) * 3, beta, d)
+    S[0, 0, 0] = np.inf
+    return lambda r: np.fft.ifftn(np.fft.fftn(r) / S).real
+
+
+def test_spectral_richardson_keeps_contracting_in_fp64():
+    """Richardson steps v += M^-1 (f - A v) with the centre-LOS spectral
+    inverse M^-1 on the local-LOS operator of a lightcone box (an octant shell
+    r in [1000, 1800] Mpc/h, 64^3 cells of 31 Mpc/h): in FP64 every step cuts
+    the relative residual (mean removed, as the consistency shift does) by
+    more than 4x, below 3e-6 in six steps.  The 1e-4 floor of the FP32 steps
+    at 512^3 is the rounding of the FP32 residual, not the preconditioner."""
+    n, L, bmin, beta = 64, 1980.0, -90.0, 0.757 / 2.2
+    rng = np.random.default_rng(3)
+    c = bmin + (np.arange(n) + 0.5) * (L / n)
+    r = np.sqrt(c[None, None, :] ** 2 + c[None, :, None] ** 2 + c[:, None, None] ** 2)
+    f = orc.smooth(rng.standard_normal((n, n, n)), 60.0, [L] * 3) * ((r > 1000) & (r < 1800))
+    f -= f.mean()
+    kw = dict(box_size=[L] * 3, box_min=[bmin] * 3, beta=beta, los=None)
+    Minv = centre_los_inverse(n, L, bmin, beta)
+
+    def rel(v):
+        res = orc.mg_residual(v, f, **kw)
+        return np.linalg.norm(res - res.mean()) / np.linalg.norm(f)
+
+    v = Minv(f)
+    prev = rel(v)
+    for step in range(6):
+        res = orc.mg_residual(v, f, **kw)
+        v = v + Minv(res - res.mean())
+        cur = rel(v)
+        print(f"step {step + 1}: {cur:.2e}")
+        assert cur < prev / 4, (step, prev, cur)
+        prev = cur
+    assert prev < 3e-6
+
+
 # ------------------------------------------------------------------ planted mistake
 
 def test_planted_sign_error_is_caught(monkeypatch):
