@@ -45,11 +45,17 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
 
 // ------------------------------------------------------------ convolution
 
+// Output-channel blocks of 16 per workgroup of k_conv: all of them up to 32
+// channels; one from 64 on (the coarse levels' grids have few voxels, so
+// splitting the channels over workgroups fills the GPU: at 16^3 a launch has
+// 32 voxel workgroups).  Each output keeps its K order, so the same bits.
+__host__ __device__ constexpr int conv_split(int co) { return co >= 64 ? co / 16 : 1; }
+
 template <int C1, int C2, int CO, bool RELU, bool UP2>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     constexpr int CIN = C1 + C2;
     constexpr int KB = (27 * CIN + 31) / 32;
-    constexpr int NB = CO / 16;
+    constexpr int NBT = CO / 16, NSPLIT = conv_split(CO), NB = NBT / NSPLIT;
     constexpr int MB = 2;
     static_assert(C1 % 8 == 0 && C2 % 8 == 0 && CO % 16 == 0, "channel blocks");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -57,7 +63,8 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     const int nx = 1 << a.lx, ny = 1 << a.ly, nz = 1 << a.lz;
     const int x = (blockIdx.x * 16 + r16) & (nx - 1);
     const int y0 = (blockIdx.y * 4 + wave) * MB;
-    const int z = blockIdx.z & (nz - 1), b = blockIdx.z >> a.lz;
+    const int zb = blockIdx.z / NSPLIT, nb0 = (blockIdx.z % NSPLIT) * NB;   // first output block
+    const int z = zb & (nz - 1), b = zb >> a.lz;
     f32x4 acc[MB][NB];
 #pragma unroll
     for (int m = 0; m < MB; m++)
@@ -73,7 +80,7 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
         const int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
         const uint32_t xs = (uint32_t)((x + dx) & (nx - 1)), zs = (uint32_t)((z + dz) & (nz - 1));
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++) bv[nb] = load8(a.w + ((uint32_t)((kb * NB + nb) * 4 + g) * 16 + r16) * 8);
+        for (int nb = 0; nb < NB; nb++) bv[nb] = load8(a.w + ((uint32_t)((kb * NBT + nb0 + nb) * 4 + g) * 16 + r16) * 8);
 #pragma unroll
         for (int m = 0; m < MB; m++) {
             const uint32_t ys = (uint32_t)((y0 + m + dy) & (ny - 1));
@@ -89,20 +96,25 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
             av[m] = load8(src);
         }
     };
-    // chunk kb + 1's loads are in flight during chunk kb's products
-    bf16x8 bv[NB], av[MB], bn[NB], an[MB];
-    load_chunk(0, bn, an);
-    for (int kb = 0; kb < KB; kb++) {
+    // the loads of chunks kb + 1 .. kb + PF are in flight during chunk kb's
+    // products (a ring of PF + 1 operand sets, indexed statically)
+    constexpr int PF = 3, RING = PF + 1;
+    bf16x8 bq[RING][NB], aq[RING][MB];
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++) bv[nb] = bn[nb];
+    for (int i = 0; i < PF; i++)
+        if (i < KB) load_chunk(i, bq[i], aq[i]);
+    for (int kb0 = 0; kb0 < KB; kb0 += RING) {
 #pragma unroll
-        for (int m = 0; m < MB; m++) av[m] = an[m];
-        if (kb + 1 < KB) load_chunk(kb + 1, bn, an);
+        for (int u = 0; u < RING; u++) {
+            const int kb = kb0 + u;
+            if (kb >= KB) break;
+            if (kb + PF < KB) load_chunk(kb + PF, bq[(u + PF) % RING], aq[(u + PF) % RING]);
 #pragma unroll
-        for (int m = 0; m < MB; m++)
+            for (int m = 0; m < MB; m++)
 #pragma unroll
-            for (int nb = 0; nb < NB; nb++)
-                acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[m], bv[nb], acc[m][nb], 0, 0, 0);
+                for (int nb = 0; nb < NB; nb++)
+                    acc[m][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aq[u][m], bq[u][nb], acc[m][nb], 0, 0, 0);
+        }
     }
     // lane holds rows (voxels) 4g .. 4g+3 of column (channel) r16
 #pragma unroll
@@ -111,7 +123,7 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
         if (y >= ny) continue;
 #pragma unroll
         for (int nb = 0; nb < NB; nb++) {
-            const int co = nb * 16 + r16;
+            const int co = (nb0 + nb) * 16 + r16;
             const float bias = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -530,7 +542,7 @@ hipError_t launch_unet_conv(const ConvArgs &a, int c1, int c2, int co, bool relu
     // k_conv addresses its inputs with 32-bit element offsets
     const int cmax = c1 > c2 ? c1 : c2;
     if ((int64_t)batch * nx * ny * nz * cmax >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
-    dim3 grid((unsigned)((nx + 15) / 16), (unsigned)((ny + 7) / 8), (unsigned)(nz * batch));
+    dim3 grid((unsigned)((nx + 15) / 16), (unsigned)((ny + 7) / 8), (unsigned)(nz * batch * conv_split(co)));
 #define LAUNCH(C1_, C2_, CO_, UP_)                                                                        \
     if (c1 == C1_ && c2 == C2_ && co == CO_ && up2 == UP_) {                                              \
         if (relu)                                                                                         \
