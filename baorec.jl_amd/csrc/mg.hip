@@ -1253,32 +1253,66 @@ __global__ void __launch_bounds__(kRrThreads)
     }
 }
 
-// Trilinear prolongation (+add) with two fine x cells per thread (an odd and
-// an even index: no divergence between the cases of cell_prolong, whose
-// arithmetic it keeps) and a 4-plane z loop per workgroup.
+// The value of prolong_value for a fine cell of parities (EX, EY, EZ) (1:
+// even index) from the 2 x 2 x 2 coarse cells V[x][y][z] of its block (index
+// 0: the coarse cell below the block's, with the wrap; 1: the block's own):
+// an even index reads its "a" cell from 0 and its "b" cell from 1, an odd
+// index its "a" cell from 1.  The same sums in the same order.
+template <typename U, int EX, int EY, int EZ>
+__device__ __forceinline__ U prolong_block_value(const U (&V)[2][2][2]) {
+    constexpr int Ax = EX ? 0 : 1, Ay = EY ? 0 : 1, Az = EZ ? 0 : 1;
+    constexpr int code = EX + 2 * EY + 4 * EZ;
+    const U c000 = V[Ax][Ay][Az];
+    if constexpr (code == 0) return c000;
+    else if constexpr (code == 1) return (c000 + V[1][Ay][Az]) / (U)2;
+    else if constexpr (code == 2) return (c000 + V[Ax][1][Az]) / (U)2;
+    else if constexpr (code == 4) return (c000 + V[Ax][Ay][1]) / (U)2;
+    else if constexpr (code == 3) return (c000 + V[1][Ay][Az] + V[Ax][1][Az] + V[1][1][Az]) / (U)4;
+    else if constexpr (code == 6) return (c000 + V[Ax][1][Az] + V[Ax][Ay][1] + V[Ax][1][1]) / (U)4;
+    else if constexpr (code == 5) return (c000 + V[1][Ay][Az] + V[Ax][Ay][1] + V[1][Ay][1]) / (U)4;
+    else
+        return (c000 + V[1][Ay][Az] + V[Ax][1][Az] + V[Ax][Ay][1] + V[1][1][Az] + V[1][Ay][1] + V[Ax][1][1] +
+                V[1][1][1]) / (U)8;
+}
+
+// Trilinear prolongation (+add) by 2 x 2 x 2 fine blocks: a thread loads the
+// 8 coarse cells its block reads, then reads, adds and writes the block as
+// four aligned pairs along x; prolong_value's arithmetic, no branches.
 template <typename T, typename U>
 __global__ void k_mg_prolong2(T *__restrict__ fine, const U *__restrict__ v, int cx, int cy, int cz, int add) {
-    const int fx = 2 * cx;
-    const int ix = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-    if (ix >= fx) return;
-    const int iy = blockIdx.y;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;   // coarse x of the block
+    if (c >= cx) return;
+    const int cyb = blockIdx.y, czb = blockIdx.z;
+    const int xs[2] = {c == 0 ? cx - 1 : c - 1, c}, ys[2] = {cyb == 0 ? cy - 1 : cyb - 1, cyb},
+              zs[2] = {czb == 0 ? cz - 1 : czb - 1, czb};
+    const int64_t sy = cx, sz = (int64_t)cx * cy;
+    U V[2][2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) V[i][j][k] = v[xs[i] + sy * ys[j] + sz * zs[k]];
     using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
-    for (int iz = 4 * blockIdx.z; iz < 4 * blockIdx.z + 4; iz++) {
-        const U a = prolong_value<U>(v, cx, cy, cz, ix, iy, iz), b = prolong_value<U>(v, cx, cy, cz, ix + 1, iy, iz);
-        // ix is even: the pair is one aligned 2-element access
-        T2 *p = reinterpret_cast<T2 *>(fine + (ix + (int64_t)fx * (iy + (int64_t)(2 * cy) * iz)));
+    const int64_t fsy = 2 * (int64_t)cx, fsz = fsy * 2 * cy;
+    T2 *base = reinterpret_cast<T2 *>(fine + (2 * c + fsy * 2 * cyb + fsz * 2 * czb));   // 2c is even: aligned
+    auto row = [&](T2 *p, U a, U b) {
+        T2 o;
         if (add) {
-            T2 o = *p;
+            o = *p;
             o.x = (T)(o.x + (T)a);
             o.y = (T)(o.y + (T)b);
-            *p = o;
         } else {
-            T2 o;
             o.x = (T)a;
             o.y = (T)b;
-            *p = o;
         }
-    }
+        *p = o;
+    };
+    const int64_t ry = fsy / 2, rz = fsz / 2;   // in pairs
+    row(base, prolong_block_value<U, 1, 1, 1>(V), prolong_block_value<U, 0, 1, 1>(V));
+    row(base + ry, prolong_block_value<U, 1, 0, 1>(V), prolong_block_value<U, 0, 0, 1>(V));
+    row(base + rz, prolong_block_value<U, 1, 1, 0>(V), prolong_block_value<U, 0, 1, 0>(V));
+    row(base + ry + rz, prolong_block_value<U, 1, 0, 0>(V), prolong_block_value<U, 0, 0, 0>(V));
 }
 
 // ------------------------------------------------------------ coarse levels
@@ -1730,11 +1764,11 @@ template <typename T, typename U>
 hipError_t launch_mg_prolong_t(T *fine, const U *coarse, const int coarse_n[3], int add,
                                hipStream_t s) {
     const int fx = 2 * coarse_n[0], fy = 2 * coarse_n[1], fz = 2 * coarse_n[2];
-    if (fz % 4 == 0 && (int64_t)fx * fy * fz >= ((int64_t)1 << 21) &&
-        ((uintptr_t)fine % (2 * sizeof(T))) == 0) {   // two cells per thread (one aligned pair), 4 planes
-        const int b2 = bx_for(fx / 2);
-        hipLaunchKernelGGL((k_mg_prolong2<T, U>), dim3((unsigned)((fx / 2 + b2 - 1) / b2), (unsigned)fy,
-                                                       (unsigned)(fz / 4)),
+    if ((int64_t)fx * fy * fz >= ((int64_t)1 << 21) &&
+        ((uintptr_t)fine % (2 * sizeof(T))) == 0) {   // 2 x 2 x 2 fine blocks, aligned pairs
+        const int b2 = bx_for(coarse_n[0]);
+        hipLaunchKernelGGL((k_mg_prolong2<T, U>), dim3((unsigned)((coarse_n[0] + b2 - 1) / b2),
+                                                       (unsigned)coarse_n[1], (unsigned)coarse_n[2]),
                            dim3(b2), 0, s, fine, coarse, coarse_n[0], coarse_n[1], coarse_n[2], add);
         return hipGetLastError();
     }
