@@ -1351,30 +1351,16 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     // phi += e - mean(e) (e != NULL), r = residual, alpha += mean(r) and
     // r -= mean(r) (local LOS), norms[0] = |r|^2 and, with rprev, norms[1] =
     // |r - rprev|^2.  The first call (start) sets phi to the FP32 seed, or to
-    // zero without one.  The fused pass does not store that start: phi stays
-    // "the seed" (phi_seed, or zero) until the first correction reads the
-    // seed in its place.
-    bool phi_is_seed = false;
-    const float *phi_seed = nullptr;
+    // zero without one.
     auto residual = [&](float *r, const float *e, const float *rprev, double *norms, bool start = false,
                         const float *seed = nullptr) {
         if (fused) {
-            if (start) {
-                phi_is_seed = true;
-                phi_seed = seed;
-            }
-            const bool from_seed = phi_is_seed;
             {
-                ProfScope ps(c, "mg_refine", (e ? (from_seed ? (phi_seed ? 24.0 : 20.0) : 28.0)
-                                                : (from_seed ? (phi_seed ? 12.0 : 8.0) : 16.0)) * N);
-                hipchk(c, launch_mg_refine(e ? phib : nullptr, from_seed ? nullptr : phi, from_seed ? phi_seed : nullptr,
-                                           e, red + 4, f0, red, r, rpart, P.L[0], p->beta, p->has_los, c->stream),
-                       "refine");
+                ProfScope ps(c, "mg_refine", (e ? 28.0 : (start ? (seed ? 24.0 : 16.0) : 16.0)) * N);
+                hipchk(c, launch_mg_refine(phib, start ? nullptr : phi, seed, e, red + 4, f0, red, r, rpart, P.L[0],
+                                           p->beta, p->has_los, c->stream), "refine");
             }
-            if (e) {
-                std::swap(phi, phib);
-                phi_is_seed = false;
-            }
+            if (e || start) std::swap(phi, phib);
             ProfScope ps(c, "mg_shift_norms", (p->has_los ? 4.0 : 8.0) * N + (rprev ? 4.0 : 0.0) * N);
             if (!p->has_los) {
                 hipchk(c, launch_sum_partials(rpart, mg_refine_blocks(P.L[0]), 1, red + 6, 1, c->stream), "sum r");
@@ -1470,14 +1456,7 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
     hipchk(c, hipMemcpyAsync(h, red, sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
     sync(c);
     p->rhs_shift = h[0] / (double)N;
-    if (phi_is_seed) {   // no correction ran: phi is the seed (or zero)
-        if (phi_seed)
-            hipchk(c, hipMemcpyAsync(v0, phi_seed, sizeof(float) * N, hipMemcpyDeviceToDevice, c->stream), "copy");
-        else
-            hipchk(c, launch_zero_words(v0, N, c->stream), "zero");
-    } else {
-        hipchk(c, launch_convert<float, double>(v0, phi, N, c->stream), "phi to FP32");
-    }
+    hipchk(c, launch_convert<float, double>(v0, phi, N, c->stream), "phi to FP32");
 }
 
 // ------------------------------------------------------------ entry helpers

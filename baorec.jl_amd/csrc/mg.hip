@@ -747,10 +747,9 @@ __global__ void __launch_bounds__(JbShape<S>::Threads, 2) k_mg_jacobi_tb(float *
 }
 
 // One FP64 refinement pass of the RBGS solve (z-marching, LDS ring):
-//   phi = p0 + (e - sum(e) / N)   (without e: p0),  p0 = (double)seed, else phi_in, else 0
-//   r = (float)((f - alpha) - A phi),  alpha = *fsum / N,
-// phi is written to phi_out unless it is just phi_in or phi_out is NULL (the
-// caller keeps phi as the seed and passes it again with the next e),
+//   phi_out = phi_in + (e - sum(e) / N)          (with e; without, phi_in is used;
+//                                                 seed: phi = (double)seed; neither: phi = 0)
+//   r = (float)((f - alpha) - A phi_out),  alpha = *fsum / N,
 // the arithmetic of k_add_correction followed by k_mg_residual_mixed, plus
 // each workgroup's FP64 sum of r in a fixed order (part[block]).  A
 // workgroup owns a 64 x 16 column of tz planes; step k loads plane z0-1+k of
@@ -814,7 +813,7 @@ __global__ void __launch_bounds__(kRfThreads)
             if (e) pre_e[u] = e[zo + ld_off[u]];
         }
     };
-    const bool write_phi = phi_out && (e || seed || !phi_in);   // phi_out receives the pass's phi
+    const bool write_phi = e || seed || !phi_in;   // phi_out receives the pass's phi
     auto store = [&](int P) {
         double *pl = ring + (P - z0 + 1) % kRfRing * kRfPlane;
         const int64_t zo = sz * gs_wrap(P, nz);
@@ -1709,7 +1708,7 @@ int64_t mg_refine_blocks(const MGLevel &L) {
 hipError_t launch_mg_refine(double *phi_out, const double *phi_in, const float *seed, const float *e,
                             const double *esum, const float *f, const double *fsum, float *r, double *part,
                             const MGLevel &L, float beta, int has_los, hipStream_t s) {
-    if (!mg_refine_ok(L) || (phi_out && phi_out == phi_in) || (seed && phi_in)) return hipErrorInvalidValue;
+    if (!mg_refine_ok(L) || ((e || seed || !phi_in) && phi_out == phi_in) || (seed && e)) return hipErrorInvalidValue;
     const int tz = refine_tz(L);
     const dim3 grid((unsigned)(L.n[0] / kRfTX), (unsigned)(L.n[1] / kRfTY), (unsigned)(L.n[2] / tz));
     hipLaunchKernelGGL(k_mg_refine, grid, dim3(kRfThreads), 0, s, phi_out, phi_in, seed, e, esum, f, fsum, r, part,
