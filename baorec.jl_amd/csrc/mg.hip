@@ -357,15 +357,22 @@ __device__ __forceinline__ void gs_cell(int s, int r, int i, int par, int &xl, i
 
 // part (may be NULL): per workgroup the FP64 sum of the values it writes,
 // in a fixed order (the total of vout without another pass over it).
+// LOS: a fixed line of sight (has_los); PART: part != NULL; both template
+// constants.
+template <int LOS, int PART>
 __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout, const float *__restrict__ vin,
-                                                       const float *__restrict__ f, MGLevel L, float beta,
-                                                       int has_los, int tz, double *part) {
+                                                       const float *__restrict__ f, MGLevel L, float beta, int tz,
+                                                       double *part) {
+    constexpr int has_los = LOS;
     __shared__ __align__(16) float lds[kGsRing * kGsPlane];
     __shared__ double red[kGsThreads / 64];
     double acc = 0.0;
     // (p, p^2, cell^2 p^2) of each LDS column word (76) and row; (p, p^2) of each local plane
     __shared__ float4 xytab[kGsXWords + kGsRows];
     __shared__ float2 ztab[kGsMaxTz + 8];
+    // byte offsets of ring slots (j - 1, j, j + 1) mod 10 for j < 20: a cell's
+    // three planes from one LDS read instead of per-lane wrap arithmetic
+    __shared__ int4 slot_tab[2 * kGsRing];
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     int tile_x, tile_y, tile_z;
     const int tile = mg_tile(tile_x, tile_y, tile_z);
@@ -390,6 +397,11 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
                 ztab[q - kGsXWords - kGsRows] = make_float2(p, p * p);
             }
         }
+    if (tid < 2 * kGsRing) {
+        constexpr int PB = kGsPlane * (int)sizeof(float);
+        const int j = tid % kGsRing;
+        slot_tab[tid] = make_int4((j == 0 ? kGsRing - 1 : j - 1) * PB, j * PB, (j == kGsRing - 1 ? 0 : j + 1) * PB, 0);
+    }
 
     // plane loads: chunk q = row * 18 + j covers x0 - 4 + 4j .. +3 of LDS row
     // `row`; threads past the 432 chunks load chunk 0 again and do not store
@@ -448,11 +460,18 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
     // stage s updates plane Q = z0 - 5 - 2s + k, needed on z0 - 3 + s .. z0 + tz + 2 - s
     auto active = [&](int s, int k) { return k >= 2 + 3 * s && k <= tz + 7 + s; };
     // Per iteration and parity par of the stage planes (z0 is even, so par =
-    // (k + 1) & 1): the cell's LDS word, its column word and row (cw | yl << 8)
-    // and its offset in a plane of f (32-bit: the grid has < 2^31 cells).
-    int cword[2][kGsIters], cpos[2][kGsIters];
-    uint32_t foff[2][kGsIters];
-    const uint32_t sz32 = (uint32_t)sz;
+    // (k + 1) & 1): the byte offset in a plane of the LDS word one row below
+    // the cell (every neighbour then sits at a non-negative offset from it),
+    // the cell's column word and row (cw | yl << 8) and its offset in a plane
+    // of f (32-bit: the grid has < 2^31 cells).
+    //
+    // fetch_f(k, par) below runs for k = 1 - par, 3 - par, ... in turn and
+    // reads plane Q = z0 - 5 - 2s + k of f, so fo[par][it], the offset in f
+    // of the cell's next read, starts at k = 1 - par and advances two planes
+    // per call, wrapping past the last plane.
+    int cwb[2][kGsIters], cpos[2][kGsIters];
+    uint32_t fo[2][kGsIters];
+    const uint32_t sz32 = (uint32_t)sz, ncell = (uint32_t)(sz * nz);
 #pragma unroll
     for (int par = 0; par < 2; par++)
 #pragma unroll
@@ -460,9 +479,11 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
             const int d = desc[it];
             int xl = kGsHalo, yl = kGsHalo;
             if (d >= 0) gs_cell(d & 3, (d >> 2) & 63, d >> 8, par, xl, yl);
-            cword[par][it] = gs_word(xl, yl);
+            cwb[par][it] = (gs_word(xl, yl) - kGsW) * (int)sizeof(float);
             cpos[par][it] = gs_word(xl, 0) | (yl << 8);
-            foff[par][it] = (uint32_t)gs_wrap(x0 - kGsHalo + xl, nx) + (uint32_t)nx * gs_wrap(y0 - kGsHalo + yl, ny);
+            const int q0 = ((z0 - 5 - 2 * (d & 3) + 1 - par) % nz + nz) % nz;
+            fo[par][it] = (uint32_t)gs_wrap(x0 - kGsHalo + xl, nx) + (uint32_t)nx * gs_wrap(y0 - kGsHalo + yl, ny) +
+                          sz32 * (uint32_t)q0;
         }
     // f of the thread's cells at step k (a valid dummy address where inactive)
     auto fetch_f = [&](int k, auto parc, float *fr) {
@@ -470,10 +491,10 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
 #pragma unroll
         for (int it = 0; it < kGsIters; it++) {
             const int d = desc[it];
-            const int s = d & 3;
-            const int Q = gs_wrap(z0 - 5 - 2 * s + k, nz);
-            const uint32_t off = d >= 0 && active(s, k) ? foff[par][it] + sz32 * (uint32_t)Q : 0u;
+            const uint32_t off = d >= 0 && active(d & 3, k) ? fo[par][it] : 0u;
             fr[it] = f[off];
+            const uint32_t nxt = fo[par][it] + 2 * sz32;   // < 2^32: fo < ncell < 2^31
+            fo[par][it] = min(nxt, nxt - ncell);             // nxt - ncell wraps above nxt unless nxt >= ncell
         }
     };
     // write-out: thread -> row tid / 16, x 4 (tid % 16) of the 64 x 16 tile
@@ -482,6 +503,9 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
     const int64_t wo_off = (x0 + wo_x) + sy * (y0 + wo_row);
     const int wo_word = (kGsHalo + wo_row) * kGsW + ((kGsHalo + wo_x) >> 1);
     const int last_load = tz + 7;
+
+    GsCoef<float> cfix{};   // the coefficients of a fixed LOS
+    if constexpr (LOS) cfix = gs_coef_p<float>(L, L.losn[0], L.losn[1], L.losn[2], beta);
 
     // step k; parc: the parity of its stage planes, (k + 1) & 1
     auto step = [&](int k, auto parc, const float *fcur, float *fnext) {
@@ -495,11 +519,11 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
                 const float2 ev = *reinterpret_cast<const float2 *>(pl + wo_word);
                 const float2 od = *reinterpret_cast<const float2 *>(pl + wo_word + kGsH);
                 *reinterpret_cast<float4 *>(vout + sz * (z0 - 12 + k) + wo_off) = make_float4(ev.x, od.x, ev.y, od.y);
-                if (part) acc += ((double)ev.x + (double)od.x) + ((double)ev.y + (double)od.y);
+                if constexpr (PART) acc += ((double)ev.x + (double)od.x) + ((double)ev.y + (double)od.y);
             } else {
                 const float ev = pl[wo_word], od = pl[wo_word + kGsH];
                 *reinterpret_cast<float2 *>(vout + sz * (z0 - 12 + k) + wo_off) = make_float2(ev, od);
-                if (part) acc += (double)ev + (double)od;
+                if constexpr (PART) acc += (double)ev + (double)od;
             }
         }
         fetch_f(k + 1, std::integral_constant<int, 1 - par>{}, fnext);
@@ -508,32 +532,36 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
         auto update = [&](int it, int st) {
             const int cw = cpos[par][it] & 255, yl = cpos[par][it] >> 8;   // column word, row
             const int ql = k - 1 - 2 * st;   // local plane of Q, >= 1
-            int m = k10 + kGsRing - 1 - 2 * st;   // ring slot of Q
-            m = m >= kGsRing ? m - kGsRing : m;
-            const int mm = m == 0 ? kGsRing - 1 : m - 1, mp = m == kGsRing - 1 ? 0 : m + 1;
-            const float *pm = lds + mm * kGsPlane;
-            float *p0 = lds + m * kGsPlane;
-            const float *pp = lds + mp * kGsPlane;
-            const int wc = cword[par][it];
-            // x neighbours sit in the other parity half: x - 1 at word ob, x + 1 at ob + 1
-            const int ob = wc + (cw >= kGsH ? -kGsH : kGsH - 1);
-#define A0(o) p0[wc + (o)]
-#define AM(o) pm[wc + (o)]
-#define AP(o) pp[wc + (o)]
-#define B0(o) p0[ob + (o)]
-#define BM(o) pm[ob + (o)]
-#define BP(o) pp[ob + (o)]
+            const int4 Tc = slot_tab[k10 + kGsRing - 1 - 2 * st];   // ring slots of Q - 1, Q, Q + 1
+            char *lb = reinterpret_cast<char *>(lds);
+            const int wb = cwb[par][it];
+            // x neighbours sit in the other parity half: x - 1 at word ob, x + 1
+            // at ob + 1, ob = wc + (cw >= kGsH ? -kGsH : kGsH - 1)
+            const int db = (cw >= kGsH ? -kGsH : kGsH - 1) * (int)sizeof(float);
+            // bases one row below the cell (a*) and below its x - 1 neighbour (b*)
+            const float *am = reinterpret_cast<const float *>(lb + (Tc.x + wb));
+            float *a0 = reinterpret_cast<float *>(lb + (Tc.y + wb));
+            const float *ap = reinterpret_cast<const float *>(lb + (Tc.z + wb));
+            const float *bm = reinterpret_cast<const float *>(lb + (Tc.x + wb + db));
+            const float *b0 = reinterpret_cast<const float *>(lb + (Tc.y + wb + db));
+            const float *bp = reinterpret_cast<const float *>(lb + (Tc.z + wb + db));
+#define A0(o) a0[kGsW + (o)]
+#define AM(o) am[kGsW + (o)]
+#define AP(o) ap[kGsW + (o)]
+#define B0(o) b0[kGsW + (o)]
+#define BM(o) bm[kGsW + (o)]
+#define BP(o) bp[kGsW + (o)]
             GsCoef<float> c;
-            if (has_los) {
-                c = gs_coef_p<float>(L, L.losn[0], L.losn[1], L.losn[2], beta);
+            if constexpr (LOS) {
+                c = cfix;
             } else {
                 const float4 tx = xytab[cw], ty = xytab[kGsXWords + yl];
                 const float2 tq = ztab[ql];
                 c = gs_coef<float>(L, tx.x, ty.x, tq.x, tx.y, ty.y, tq.y, tx.z, ty.z, L.cell2[2] * tq.y, beta);
             }
-            p0[wc] = gs_update<float>(c, fcur[it], B0(1), B0(0), A0(kGsW), A0(-kGsW), AP(0), AM(0), B0(kGsW + 1),
-                                      B0(-kGsW), B0(kGsW), B0(-kGsW + 1), BP(1), BM(0), BP(0), BM(1), AP(kGsW),
-                                      AM(-kGsW), AP(-kGsW), AM(kGsW), has_los);
+            A0(0) = gs_update<float>(c, fcur[it], B0(1), B0(0), A0(kGsW), A0(-kGsW), AP(0), AM(0), B0(kGsW + 1),
+                                     B0(-kGsW), B0(kGsW), B0(-kGsW + 1), BP(1), BM(0), BP(0), BM(1), AP(kGsW),
+                                     AM(-kGsW), AP(-kGsW), AM(kGsW), has_los);
 #undef A0
 #undef AM
 #undef AP
@@ -556,7 +584,7 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
         step(k, std::integral_constant<int, 1>{}, fa, fb);
         step(k + 1, std::integral_constant<int, 0>{}, fb, fa);
     }
-    if (part) {
+    if constexpr (PART) {
         acc = wave_sum64(acc);
         if ((tid & 63) == 0) red[tid >> 6] = acc;
         __syncthreads();
@@ -1653,7 +1681,7 @@ hipError_t launch_mg_residual_restrict(float *coarse, const float *v, const floa
 // z planes per workgroup of k_mg_gs4 (at least 8; each workgroup also loads
 // 8 halo planes and runs 12 pipeline steps beyond tz)
 static int gs4_tz(const MGLevel &L) {
-    static const int per_cu = blocks_per_cu(k_mg_gs4, kGsThreads);
+    static const int per_cu = blocks_per_cu(k_mg_gs4<0, 1>, kGsThreads);
     return zm_tz((int64_t)(L.n[0] / kGsTX) * (L.n[1] / kGsTY), L.n[2], kGsMaxTz, 8, per_cu, 12);
 }
 
@@ -1671,7 +1699,8 @@ hipError_t launch_mg_gs4(float *vout, const float *vin, const float *f, const MG
     if (!mg_gs4_ok(L) || vout == vin) return hipErrorInvalidValue;
     const int tz = gs4_tz(L);
     const dim3 grid((unsigned)(L.n[0] / kGsTX), (unsigned)(L.n[1] / kGsTY), (unsigned)(L.n[2] / tz));
-    hipLaunchKernelGGL(k_mg_gs4, grid, dim3(kGsThreads), 0, s, vout, vin, f, L, beta, has_los, tz, part);
+    const auto kernel = has_los ? (part ? k_mg_gs4<1, 1> : k_mg_gs4<1, 0>) : (part ? k_mg_gs4<0, 1> : k_mg_gs4<0, 0>);
+    hipLaunchKernelGGL(kernel, grid, dim3(kGsThreads), 0, s, vout, vin, f, L, beta, tz, part);
     return hipGetLastError();
 }
 
