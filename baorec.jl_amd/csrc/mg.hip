@@ -789,10 +789,15 @@ constexpr int kRfW = kRfTX + 2, kRfRows = kRfTY + 2, kRfPlane = kRfW * kRfRows; 
 constexpr int kRfRing = 4, kRfThreads = 256, kRfMaxTz = 128;
 constexpr int kRfLoads = (kRfPlane + kRfThreads - 1) / kRfThreads;   // 5
 
+// The pass's form, a template constant: which of phi_in, e and seed it reads.
+enum RfMode { kRfPhi = 0, kRfPhiE = 1, kRfSeed = 2, kRfZero = 3 };
+
+template <int LOS, int MODE>
 __global__ void __launch_bounds__(kRfThreads)
     k_mg_refine(double *__restrict__ phi_out, const double *__restrict__ phi_in, const float *__restrict__ seed,
                 const float *__restrict__ e, const double *esum, const float *__restrict__ f, const double *fsum,
-                float *__restrict__ r, double *__restrict__ part, MGLevel L, float beta, int has_los, int tz) {
+                float *__restrict__ r, double *__restrict__ part, MGLevel L, float beta, int tz) {
+    constexpr int has_los = LOS;
     __shared__ double ring[kRfRing * kRfPlane];
     __shared__ double ptab[kRfW + kRfRows + kRfMaxTz + 2];   // p of LDS column | row | local plane
     __shared__ double red[kRfThreads / 64];
@@ -802,7 +807,7 @@ __global__ void __launch_bounds__(kRfThreads)
     const int x0 = tile_x * kRfTX, y0 = tile_y * kRfTY, z0 = tile_z * tz;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     const double count = (double)sz * nz;
-    const double me = e ? *esum / count : 0.0;
+    const double me = MODE == kRfPhiE ? *esum / count : 0.0;
     const double alpha = fsum ? *fsum / count : 0.0;
     const int tid = threadIdx.x;
     double *pz_tab = ptab + kRfW + kRfRows;
@@ -837,11 +842,16 @@ __global__ void __launch_bounds__(kRfThreads)
         const int64_t zo = sz * gs_wrap(min(P, z0 + tz), nz);
 #pragma unroll
         for (int u = 0; u < kRfLoads; u++) {
-            pre[u] = seed ? (double)seed[zo + ld_off[u]] : (phi_in ? phi_in[zo + ld_off[u]] : 0.0);
-            if (e) pre_e[u] = e[zo + ld_off[u]];
+            if constexpr (MODE == kRfSeed)
+                pre[u] = (double)seed[zo + ld_off[u]];
+            else if constexpr (MODE == kRfZero)
+                pre[u] = 0.0;
+            else
+                pre[u] = phi_in[zo + ld_off[u]];
+            if constexpr (MODE == kRfPhiE) pre_e[u] = e[zo + ld_off[u]];
         }
     };
-    const bool write_phi = e || seed || !phi_in;   // phi_out receives the pass's phi
+    constexpr bool write_phi = MODE != kRfPhi;   // phi_out receives the pass's phi
     auto store = [&](int P) {
         double *pl = ring + (P - z0 + 1) % kRfRing * kRfPlane;
         const int64_t zo = sz * gs_wrap(P, nz);
@@ -850,7 +860,7 @@ __global__ void __launch_bounds__(kRfThreads)
         for (int u = 0; u < kRfLoads; u++) {
             const int q = tid + kRfThreads * u;
             if (q >= kRfPlane) continue;
-            const double v = e ? (double)pre[u] + ((double)pre_e[u] - me) : pre[u];
+            const double v = MODE == kRfPhiE ? (double)pre[u] + ((double)pre_e[u] - me) : pre[u];
             pl[q] = v;
             if (write_phi && own && ld_in[u]) phi_out[zo + ld_off[u]] = v;
         }
@@ -1596,7 +1606,7 @@ static int zm_tz(int64_t cols, int nz, int max_tz, int min_tz, int per_cu, int e
 
 // z planes per workgroup of k_mg_refine (tz + 2 steps)
 static int refine_tz(const MGLevel &L) {
-    static const int per_cu = blocks_per_cu(k_mg_refine, kRfThreads);
+    static const int per_cu = blocks_per_cu(k_mg_refine<0, kRfPhiE>, kRfThreads);
     return zm_tz((int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY), L.n[2], kRfMaxTz, 8, per_cu, 2);
 }
 
@@ -1737,11 +1747,17 @@ int64_t mg_refine_blocks(const MGLevel &L) {
 hipError_t launch_mg_refine(double *phi_out, const double *phi_in, const float *seed, const float *e,
                             const double *esum, const float *f, const double *fsum, float *r, double *part,
                             const MGLevel &L, float beta, int has_los, hipStream_t s) {
-    if (!mg_refine_ok(L) || ((e || seed || !phi_in) && phi_out == phi_in) || (seed && e)) return hipErrorInvalidValue;
+    if (!mg_refine_ok(L) || ((e || seed || !phi_in) && phi_out == phi_in) || (seed && e) || (e && !phi_in))
+        return hipErrorInvalidValue;
     const int tz = refine_tz(L);
     const dim3 grid((unsigned)(L.n[0] / kRfTX), (unsigned)(L.n[1] / kRfTY), (unsigned)(L.n[2] / tz));
-    hipLaunchKernelGGL(k_mg_refine, grid, dim3(kRfThreads), 0, s, phi_out, phi_in, seed, e, esum, f, fsum, r, part,
-                       L, beta, has_los, tz);
+    using K = decltype(&k_mg_refine<0, kRfPhi>);
+    static const K kernels[2][4] = {
+        {k_mg_refine<0, kRfPhi>, k_mg_refine<0, kRfPhiE>, k_mg_refine<0, kRfSeed>, k_mg_refine<0, kRfZero>},
+        {k_mg_refine<1, kRfPhi>, k_mg_refine<1, kRfPhiE>, k_mg_refine<1, kRfSeed>, k_mg_refine<1, kRfZero>}};
+    const int mode = seed ? kRfSeed : (e ? kRfPhiE : (phi_in ? kRfPhi : kRfZero));
+    hipLaunchKernelGGL(kernels[has_los ? 1 : 0][mode], grid, dim3(kRfThreads), 0, s, phi_out, phi_in, seed, e, esum,
+                       f, fsum, r, part, L, beta, tz);
     return hipGetLastError();
 }
 
