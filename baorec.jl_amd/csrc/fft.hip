@@ -467,10 +467,12 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
     return x * q + (x < r ? x : r) + (bid >> 3);
 }
 
+// MK >= 0: the multiplier kind as a template constant (a.mul is not read)
+template <int MK = -1>
 __device__ __forceinline__ float2 apply_mul(float2 F, float kx, float ky, float kz, const StridedArgs &a) {
     const float R2 = a.R2, l0 = a.los[0], l1 = a.los[1], l2 = a.los[2];
     const float k2 = (kx * kx + ky * ky) + kz * kz;
-    switch (a.mul) {
+    switch (MK >= 0 ? MK : a.mul) {
         case MUL_GAUSS: {
             const double g = exp(-0.5 * (double)R2 * (double)k2);
             return make_float2((float)((double)F.x * g), (float)((double)F.y * g));
@@ -595,7 +597,9 @@ k_strided(StridedArgs a) {
 // one float4 per thread, 128 contiguous bytes per row, and the two columns
 // share twiddles and barriers.  Columns past Pv (inside the row pitch) are
 // transformed and stored like the others; nothing reads them.
-template <int M>
+// MODE: a.mode as a template constant; MK: the multiplier kind of mode 2, or
+// -1 to read a.mul
+template <int M, int MODE, int MK>
 __global__ void __launch_bounds__(M) __attribute__((amdgpu_waves_per_eu(4)))
 k_strided2(StridedArgs a) {
     constexpr int B = 16;
@@ -634,7 +638,7 @@ k_strided2(StridedArgs a) {
         *reinterpret_cast<float4 *>(dst + base + pos * rs) = make_float4(x0.x, x0.y, x1.x, x1.y);
     };
 
-    if (a.mode == 0) {
+    if constexpr (MODE == 0) {
 #pragma unroll
         for (int i = 0; i < 8; i++) load(J.in, PF::in_pos(t, i), i);
         fft_line2<M, false>(v, lds, lay, t, g_tw);
@@ -644,7 +648,7 @@ k_strided2(StridedArgs a) {
         }
         return;
     }
-    if (a.mode == 2) {
+    if constexpr (MODE == 2) {
 #pragma unroll
         for (int i = 0; i < 8; i++) load(J.in, PF::in_pos(t, i), i);
         fft_line2<M, false>(v, lds, lay, t, g_tw);
@@ -656,8 +660,8 @@ k_strided2(StridedArgs a) {
             const float kal = kax[PF::out_pos(t, i)];
             const float ky = a.axis == 1 ? kal : kother;
             const float kz = a.axis == 1 ? kother : kal;
-            v[0][i] = apply_mul(v[0][i], kx0v, ky, kz, a);
-            v[1][i] = apply_mul(v[1][i], kx1v, ky, kz, a);
+            v[0][i] = apply_mul<MK>(v[0][i], kx0v, ky, kz, a);
+            v[1][i] = apply_mul<MK>(v[1][i], kx1v, ky, kz, a);
         }
     } else {
 #pragma unroll
@@ -885,7 +889,14 @@ static hipError_t strided_m(const StridedArgs &a, int njobs, int nbatch, hipStre
         if (strided_uses_pair(a, njobs)) {
             const int ncg = (a.Pv + 15) / 16;
             dim3 grid((unsigned)(ncg * nother), njobs, nbatch);
-            hipLaunchKernelGGL(k_strided2<M>, grid, dim3(M), 0, s, a);
+            using K = decltype(&k_strided2<M, 0, -1>);
+            static const K fused[8] = {k_strided2<M, 2, MUL_NONE>,     k_strided2<M, 2, MUL_GAUSS>,
+                                       k_strided2<M, 2, MUL_INVK2>,    k_strided2<M, 2, MUL_FIXEDLOS>,
+                                       k_strided2<M, 2, MUL_I_INVK2>,  k_strided2<M, 2, MUL_I>,
+                                       k_strided2<M, 2, MUL_MG_INV>,   k_strided2<M, 2, MUL_GAUSS1>};
+            if (a.mode == 2 && (a.mul < 0 || a.mul > 7)) return hipErrorInvalidValue;
+            const K kernel = a.mode == 0 ? k_strided2<M, 0, -1> : (a.mode == 2 ? fused[a.mul] : k_strided2<M, 1, -1>);
+            hipLaunchKernelGGL(kernel, grid, dim3(M), 0, s, a);
             return hipGetLastError();
         }
     }
