@@ -510,12 +510,9 @@ __device__ __forceinline__ float2 apply_mul(float2 F, float kx, float ky, float 
     }
 }
 
-// MODE: a.mode as a template constant, or -1 to read a.mode; MK: the
-// multiplier kind of mode 2, or -1 to read a.mul
-template <int M, int MODE, int MK>
+template <int M>
 __global__ void __launch_bounds__(SShape<M>::NT) __attribute__((amdgpu_waves_per_eu(4)))
 k_strided(StridedArgs a) {
-    const int mode = MODE >= 0 ? MODE : a.mode;
     constexpr int B = SShape<M>::B;
     using PF = Plan<M, false>;
     using PI = Plan<M, true>;
@@ -543,7 +540,7 @@ k_strided(StridedArgs a) {
     LayStrided lay{col, B};
     float2 v[8];
 
-    if (mode == 0) {
+    if (a.mode == 0) {
 #pragma unroll
         for (int i = 0; i < 8; i++)
             v[i] = valid ? J.in[base + PF::in_pos(t, i) * rs] : make_float2(0.f, 0.f);
@@ -554,7 +551,7 @@ k_strided(StridedArgs a) {
         }
         return;
     }
-    if (mode == 2) {
+    if (a.mode == 2) {
 #pragma unroll
         for (int i = 0; i < 8; i++)
             v[i] = valid ? J.in[base + PF::in_pos(t, i) * rs] : make_float2(0.f, 0.f);
@@ -566,7 +563,7 @@ k_strided(StridedArgs a) {
             const float kal = kax[PF::out_pos(t, i)];
             const float ky = a.axis == 1 ? kal : kother;
             const float kz = a.axis == 1 ? kother : kal;
-            v[i] = apply_mul<MK>(v[i], kxv, ky, kz, a);
+            v[i] = apply_mul(v[i], kxv, ky, kz, a);
         }
     } else {
 #pragma unroll
@@ -905,14 +902,7 @@ static hipError_t strided_m(const StridedArgs &a, int njobs, int nbatch, hipStre
     }
     const int ncg = (a.Pv + S::B - 1) / S::B;
     dim3 grid((unsigned)(ncg * nother), njobs, nbatch);
-    using K = decltype(&k_strided<M, -1, -1>);
-    static const K fused[8] = {k_strided<M, 2, MUL_NONE>,    k_strided<M, 2, MUL_GAUSS>,  k_strided<M, 2, MUL_INVK2>,
-                               k_strided<M, 2, MUL_FIXEDLOS>, k_strided<M, 2, MUL_I_INVK2>, k_strided<M, 2, MUL_I>,
-                               k_strided<M, 2, MUL_MG_INV>,  k_strided<M, 2, MUL_GAUSS1>};
-    if (a.mode == 2 && (a.mul < 0 || a.mul > 7)) return hipErrorInvalidValue;
-    // modes 0 and 1 in one kernel: specialised, they spill at n = 512
-    const K kernel = a.mode == 2 ? fused[a.mul] : k_strided<M, -1, -1>;
-    hipLaunchKernelGGL(kernel, grid, dim3(S::NT), 0, s, a);
+    hipLaunchKernelGGL(k_strided<M>, grid, dim3(S::NT), 0, s, a);
     return hipGetLastError();
 }
 
