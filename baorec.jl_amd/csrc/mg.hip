@@ -623,11 +623,13 @@ struct JbShape {
     static constexpr int Chunks = (W + 3) / 4 * H;   // float4 columns per plane row (last may overhang)
 };
 
-template <int S>
+// LOS: a fixed line of sight (has_los), as a template constant
+template <int S, int LOS>
 __global__ void __launch_bounds__(JbShape<S>::Threads, 2) k_mg_jacobi_tb(float *__restrict__ vout, const float *__restrict__ vin,
                                                              const float *__restrict__ f, MGLevel L, float beta,
-                                                             float omega, int has_los, int tz) {
+                                                             float omega, int tz) {
     static_assert(S == 2 || S == 3, "2 or 3 sweeps per pass");
+    constexpr int has_los = LOS;
     using J = JbShape<S>;
     __shared__ __align__(16) float lds[S * kJbRing * J::Plane];
     __shared__ float ptab[J::W + J::H + kJbMaxTz + 2 * S];
@@ -1717,7 +1719,7 @@ hipError_t launch_mg_gs4(float *vout, const float *vin, const float *f, const MG
 // z planes per workgroup of k_mg_jacobi_tb<S> (tz + 3 S - 1 steps)
 template <int S>
 static int jacobi_tb_tz(const MGLevel &L) {
-    static const int per_cu = blocks_per_cu(k_mg_jacobi_tb<S>, JbShape<S>::Threads);
+    static const int per_cu = blocks_per_cu(k_mg_jacobi_tb<S, 0>, JbShape<S>::Threads);
     return zm_tz((int64_t)(L.n[0] / kJbTX) * (L.n[1] / kJbTY), L.n[2], kJbMaxTz, 8, per_cu, 3 * S - 1);
 }
 
@@ -1732,11 +1734,11 @@ hipError_t launch_mg_jacobi_tb(float *vout, const float *vin, const float *f, co
     const int tz = sweeps == 2 ? jacobi_tb_tz<2>(L) : jacobi_tb_tz<3>(L);
     const dim3 grid((unsigned)(L.n[0] / kJbTX), (unsigned)(L.n[1] / kJbTY), (unsigned)(L.n[2] / tz));
     if (sweeps == 2)
-        hipLaunchKernelGGL(k_mg_jacobi_tb<2>, grid, dim3(JbShape<2>::Threads), 0, s, vout, vin, f, L, beta, omega,
-                           has_los, tz);
+        hipLaunchKernelGGL((has_los ? k_mg_jacobi_tb<2, 1> : k_mg_jacobi_tb<2, 0>), grid, dim3(JbShape<2>::Threads), 0,
+                           s, vout, vin, f, L, beta, omega, tz);
     else
-        hipLaunchKernelGGL(k_mg_jacobi_tb<3>, grid, dim3(JbShape<3>::Threads), 0, s, vout, vin, f, L, beta, omega,
-                           has_los, tz);
+        hipLaunchKernelGGL((has_los ? k_mg_jacobi_tb<3, 1> : k_mg_jacobi_tb<3, 0>), grid, dim3(JbShape<3>::Threads), 0,
+                           s, vout, vin, f, L, beta, omega, tz);
     return hipGetLastError();
 }
 
