@@ -793,6 +793,7 @@ struct MGPyr {
     std::vector<MGLevel> L;
     std::vector<int> cur;              // 0: A holds v, 1: B holds v
     std::vector<int> dbl;
+    std::vector<int> zero;             // 1: v is zero and not stored (its next fused sweep reads none)
     int lc = 0;                        // levels >= lc run in one k_mg_coarse launch
 };
 
@@ -863,6 +864,7 @@ MGPyr mg_build(baorec_ctx *c, const Grid &G0, const baorec_params *p, Slot *slot
         off += 2 * (q.nx + q.ny + q.nz);
         P.L.push_back(L);
         P.cur.push_back(0);
+        P.zero.push_back(0);
     }
     // first level from which every level is small and of one precision
     P.lc = nl;
@@ -1019,16 +1021,31 @@ static bool mg_use_gs4(const MGPyr &P, int l) {
     return !P.dbl[l] && mg_gs4_ok(P.L[l]) && P.g[l].R() >= ((int64_t)1 << 21);
 }
 
+// v = 0 on level l before an RBGS V-cycle: on levels of fused sweeps no
+// fill, the first sweep reads no v (mg_smooth_rb); otherwise zero_level.
+void zero_level_rb(baorec_ctx *c, MGPyr &P, int l) {
+    if (l != P.lc && mg_use_gs4(P, l))
+        P.zero[l] = 1;
+    else
+        zero_level(c, P, l);
+}
+
 // sum_part (may be NULL; one-launch sweeps only): the last sweep leaves the
-// per-workgroup sums of the result there (mg_gs4_blocks partials)
+// per-workgroup sums of the result there (mg_gs4_blocks partials).  Leaves
+// v stored (P.zero[l] cleared).
 void mg_smooth_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, int sweeps, double *sum_part = nullptr) {
     if (mg_use_gs4(P, l)) {
         for (int s = 0; s < sweeps; s++) {
-            ProfScope ps(c, "mg_gs4", 12.0 * P.g[l].R());
-            hipchk(c, launch_mg_gs4(vother(P, l), vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream,
-                                    s + 1 == sweeps ? sum_part : nullptr),
+            ProfScope ps(c, "mg_gs4", (P.zero[l] ? 8.0 : 12.0) * P.g[l].R());
+            hipchk(c, launch_mg_gs4(vother(P, l), P.zero[l] ? nullptr : vcur(P, l), P.F[l], P.L[l], p->beta,
+                                    p->has_los, c->stream, s + 1 == sweeps ? sum_part : nullptr),
                    "gs4");
             P.cur[l] ^= 1;
+            P.zero[l] = 0;
+        }
+        if (P.zero[l]) {   // no sweep: store the zero
+            zero_level(c, P, l);
+            P.zero[l] = 0;
         }
         return;
     }
@@ -1074,7 +1091,7 @@ void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, double
         } else {
             mg_residual_reduce(c, P, l, p, P.dbl[l + 1] ? nullptr : P.F[l + 1]);
         }
-        zero_level(c, P, l + 1);
+        zero_level_rb(c, P, l + 1);
         mg_vcycle_rb(c, P, l + 1, p);
         ProfScope ps(c, P.dbl[l] ? "mg_prolong64" : "mg_prolong", (2.0 + 0.125) * w * R);
         if (P.dbl[l])
@@ -1434,7 +1451,7 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
         const double rel_prev = rel;
         float *r = rb[cyc & 1], *rn = rb[(cyc + 1) & 1];
         P.F[0] = r;
-        zero_level(c, P, 0);
+        zero_level_rb(c, P, 0);
         mg_vcycle_rb(c, P, 0, p, epart);
         // v += e - mean(e): v stays zero-mean (A annihilates constants), so an
         // inconsistent f does not make it drift by a constant every cycle

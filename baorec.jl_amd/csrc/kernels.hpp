@@ -335,6 +335,7 @@ hipError_t launch_mg_jacobi_tb(float *vout, const float *vin, const float *f, co
 // in-place launch_mg_rbgs colour passes; for levels with mg_gs4_ok (sides
 // multiples of 64, 16 and 8).
 // part (may be NULL): mg_gs4_blocks(L) FP64 partial sums of vout, fixed order.
+// vin == NULL: the sweep starts from v = 0 (nothing is read for it).
 bool mg_gs4_ok(const MGLevel &L);
 int64_t mg_gs4_blocks(const MGLevel &L);
 hipError_t launch_mg_gs4(float *vout, const float *vin, const float *f, const MGLevel &L, float beta, int has_los,

@@ -357,9 +357,10 @@ __device__ __forceinline__ void gs_cell(int s, int r, int i, int par, int &xl, i
 
 // part (may be NULL): per workgroup the FP64 sum of the values it writes,
 // in a fixed order (the total of vout without another pass over it).
-// LOS: a fixed line of sight (has_los); PART: part != NULL; both template
-// constants.
-template <int LOS, int PART>
+// LOS: a fixed line of sight (has_los); PART: part != NULL; ZIN: vin is
+// zero and not read (vin == NULL: the first sweep of a correction V-cycle,
+// whose level starts from v = 0); all three template constants.
+template <int LOS, int PART, int ZIN>
 __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout, const float *__restrict__ vin,
                                                        const float *__restrict__ f, MGLevel L, float beta, int tz,
                                                        double *part) {
@@ -422,6 +423,11 @@ __global__ void __launch_bounds__(kGsThreads) k_mg_gs4(float *__restrict__ vout,
     // that no branch makes the compiler wait for the loads before the step's
     // arithmetic
     auto fetch = [&](int P) {
+        if constexpr (ZIN) {
+#pragma unroll
+            for (int u = 0; u < kGsLoads; u++) pre[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            return;
+        }
         const float *src = vin + sz * gs_wrap(min(P, z0 + tz + kGsHalo - 1), nz);
 #pragma unroll
         for (int u = 0; u < kGsLoads; u++) pre[u] = *reinterpret_cast<const float4 *>(src + ld_off[u]);
@@ -1693,7 +1699,7 @@ hipError_t launch_mg_residual_restrict(float *coarse, const float *v, const floa
 // z planes per workgroup of k_mg_gs4 (at least 8; each workgroup also loads
 // 8 halo planes and runs 12 pipeline steps beyond tz)
 static int gs4_tz(const MGLevel &L) {
-    static const int per_cu = blocks_per_cu(k_mg_gs4<0, 1>, kGsThreads);
+    static const int per_cu = blocks_per_cu(k_mg_gs4<0, 1, 0>, kGsThreads);
     return zm_tz((int64_t)(L.n[0] / kGsTX) * (L.n[1] / kGsTY), L.n[2], kGsMaxTz, 8, per_cu, 12);
 }
 
@@ -1708,10 +1714,13 @@ int64_t mg_gs4_blocks(const MGLevel &L) {
 
 hipError_t launch_mg_gs4(float *vout, const float *vin, const float *f, const MGLevel &L, float beta, int has_los,
                          hipStream_t s, double *part) {
-    if (!mg_gs4_ok(L) || vout == vin) return hipErrorInvalidValue;
+    if (!mg_gs4_ok(L) || vout == vin || !vout) return hipErrorInvalidValue;
     const int tz = gs4_tz(L);
     const dim3 grid((unsigned)(L.n[0] / kGsTX), (unsigned)(L.n[1] / kGsTY), (unsigned)(L.n[2] / tz));
-    const auto kernel = has_los ? (part ? k_mg_gs4<1, 1> : k_mg_gs4<1, 0>) : (part ? k_mg_gs4<0, 1> : k_mg_gs4<0, 0>);
+    using K = decltype(&k_mg_gs4<0, 0, 0>);
+    static const K kernels[8] = {k_mg_gs4<0, 0, 0>, k_mg_gs4<0, 0, 1>, k_mg_gs4<0, 1, 0>, k_mg_gs4<0, 1, 1>,
+                                 k_mg_gs4<1, 0, 0>, k_mg_gs4<1, 0, 1>, k_mg_gs4<1, 1, 0>, k_mg_gs4<1, 1, 1>};
+    const K kernel = kernels[(has_los ? 4 : 0) + (part ? 2 : 0) + (vin ? 0 : 1)];
     hipLaunchKernelGGL(kernel, grid, dim3(kGsThreads), 0, s, vout, vin, f, L, beta, tz, part);
     return hipGetLastError();
 }
