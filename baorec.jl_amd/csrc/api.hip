@@ -955,11 +955,15 @@ void mg_coarse(baorec_ctx *c, MGPyr &P, const baorec_params *p, int fmg, int gs)
 // >= 128^3 cells one fused pass with the same bits (the residual is not
 // stored), else the two kernels through vother(P, l).  coarse = NULL: level
 // l + 1 is FP64 (P.Fd[l + 1]).
-static void mg_residual_reduce(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, float *coarse) {
+// fast: the fused pass computes the coefficient with the hardware
+// reciprocal (the RBGS solve; the reference schedule keeps its division)
+static void mg_residual_reduce(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, float *coarse,
+                               int fast = 0) {
     const double R = (double)P.g[l].R();
     if (coarse && R >= (double)(1 << 21) && mg_residual_restrict_ok(P.L[l])) {
         ProfScope ps(c, "mg_residual_restrict", 8.5 * R);
-        hipchk(c, launch_mg_residual_restrict(coarse, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream),
+        hipchk(c, launch_mg_residual_restrict(coarse, vcur(P, l), P.F[l], P.L[l], p->beta, p->has_los, c->stream,
+                                              fast),
                "residual + restriction");
         return;
     }
@@ -1089,7 +1093,7 @@ void mg_vcycle_rb(baorec_ctx *c, MGPyr &P, int l, const baorec_params *p, double
             ProfScope ps(c, "mg_reduce64", 1.125 * w * R);
             hipchk(c, launch_mg_reduce_t<double, double>(P.Fd[l + 1], r, fn, c->stream), "reduce64");
         } else {
-            mg_residual_reduce(c, P, l, p, P.dbl[l + 1] ? nullptr : P.F[l + 1]);
+            mg_residual_reduce(c, P, l, p, P.dbl[l + 1] ? nullptr : P.F[l + 1], 1);
         }
         zero_level_rb(c, P, l + 1);
         mg_vcycle_rb(c, P, l + 1, p);
@@ -1430,7 +1434,8 @@ void multigrid_solve(baorec_ctx *c, const Grid &G, baorec_params *p, Slot *slot,
             // spectral Richardson steps: seed += A_c^-1 (f - A seed)
             float *rs = bufT<float>(c, "mg_seed_r", N);
             for (int k = 0; k < p->mg_seed_steps; k++) {
-                hipchk(c, launch_mg_residual(rs, seed, f0, P.L[0], p->beta, p->has_los, c->stream), "seed residual");
+                hipchk(c, launch_mg_residual_fast(rs, seed, f0, P.L[0], p->beta, p->has_los, c->stream),
+                       "seed residual");
                 mg_spectral_seed(c, G, p, slot, P.L[0], rs, seed, true);   // seed += A_c^-1 rs
             }
         }

@@ -311,12 +311,17 @@ hipError_t launch_mg_jacobi(float *vout, const float *vin, const float *f, const
                             float beta, float omega, int has_los, hipStream_t s);
 hipError_t launch_mg_residual(float *r, const float *v, const float *f, const MGLevel &L,
                               float beta, int has_los, hipStream_t s);
+// The same residual with the coefficient from the hardware reciprocal (1 ulp)
+// on z-marching shapes (otherwise launch_mg_residual), for the RBGS solve.
+hipError_t launch_mg_residual_fast(float *r, const float *v, const float *f, const MGLevel &L, float beta,
+                                   int has_los, hipStream_t s);
 hipError_t launch_mg_reduce(float *coarse, const float *fine, const int fine_n[3], hipStream_t s);
 // coarse = restriction of (f - A v) in one pass, the bits of launch_mg_residual
 // followed by launch_mg_reduce (sides % (64, 16, 2), < 2^31 cells)
 bool mg_residual_restrict_ok(const MGLevel &L);
+// fast: the coefficient from the hardware reciprocal, as launch_mg_residual_fast.
 hipError_t launch_mg_residual_restrict(float *coarse, const float *v, const float *f, const MGLevel &L, float beta,
-                                       int has_los, hipStream_t s);
+                                       int has_los, hipStream_t s, int fast = 0);
 hipError_t launch_mg_prolong(float *fine, const float *coarse, const int coarse_n[3], int add,
                              hipStream_t s);
 // xv[0 .. n) cell centres (x | y | z), xv[n .. 2n) centres / cell size, n = nx + ny + nz

@@ -932,6 +932,10 @@ __global__ void __launch_bounds__(kRfThreads)
 // loads each plane of v once into a ring of 4 LDS planes of 66 x 18 (a halo
 // of 1) and computes the plane below.  f is staged through LDS with the
 // same one-step lead; r goes out in coalesced rows.
+// FAST: the coefficient g with the hardware reciprocal (mg_div<float, true>,
+// 1 ulp) instead of the reference's IEEE division, for the residuals inside
+// the RBGS solve
+template <int FAST>
 __global__ void __launch_bounds__(kRfThreads)
     k_mg_residual_zm(float *__restrict__ r, const float *__restrict__ v, const float *__restrict__ f, MGLevel L,
                      float beta, int has_los, int tz) {
@@ -1000,8 +1004,9 @@ __global__ void __launch_bounds__(kRfThreads)
         for (int j = 0; j < 4; j++) {
             const int cy = cy0 + 4 * j;
             const int xl = cx + 1, yl = cy + 1, wc = yl * kRfW + xl;
-            const Coef<float> c = coef_p<float>(L, has_los ? L.losn[0] : ptab[xl], has_los ? L.losn[1] : ptab[kRfW + yl],
-                                                has_los ? L.losn[2] : pz_tab[ql], beta);
+            const Coef<float> c = coef_p<float, FAST != 0>(L, has_los ? L.losn[0] : ptab[xl],
+                                                           has_los ? L.losn[1] : ptab[kRfW + yl],
+                                                           has_los ? L.losn[2] : pz_tab[ql], beta);
 #define A0(o) p0[wc + (o)]
 #define AM(o) pm[wc + (o)]
 #define AP(o) pp[wc + (o)]
@@ -1165,6 +1170,7 @@ constexpr int kRrEven = kRrCX + 1;                                              
 constexpr int kRrIters = (kRrPlane + kRrThreads - 1) / kRrThreads;                          // 5
 constexpr int kRrLoads = (kRrVPlane + kRrThreads - 1) / kRrThreads;                          // 5
 
+template <int FAST>   // as k_mg_residual_zm
 __global__ void __launch_bounds__(kRrThreads)
     k_mg_residual_restrict(float *__restrict__ coarse, const float *__restrict__ v, const float *__restrict__ f,
                            MGLevel L, float beta, int has_los, int tzc) {
@@ -1253,9 +1259,9 @@ __global__ void __launch_bounds__(kRrThreads)
                 if (rpos[it] < 0) continue;
                 const int xr = rpos[it] & 255, yr = rpos[it] >> 8;
                 const int xl = xr + 1, yl = yr + 1, wc = yl * kRrVW + xl;
-                const Coef<float> c = coef_p<float>(L, has_los ? L.losn[0] : ptab[xl],
-                                                    has_los ? L.losn[1] : ptab[kRrVW + yl],
-                                                    has_los ? L.losn[2] : pz_tab[vq], beta);
+                const Coef<float> c = coef_p<float, FAST != 0>(L, has_los ? L.losn[0] : ptab[xl],
+                                                               has_los ? L.losn[1] : ptab[kRrVW + yl],
+                                                               has_los ? L.losn[2] : pz_tab[vq], beta);
 #define A0(o) p0[wc + (o)]
 #define AM(o) pm[wc + (o)]
 #define AP(o) pp[wc + (o)]
@@ -1620,7 +1626,7 @@ static int refine_tz(const MGLevel &L) {
 
 // the same for k_mg_residual_zm
 static int residual_zm_tz(const MGLevel &L) {
-    static const int per_cu = blocks_per_cu(k_mg_residual_zm, kRfThreads);
+    static const int per_cu = blocks_per_cu(k_mg_residual_zm<0>, kRfThreads);
     return zm_tz((int64_t)(L.n[0] / kRfTX) * (L.n[1] / kRfTY), L.n[2], kRfMaxTz, 8, per_cu, 2);
 }
 
@@ -1659,6 +1665,17 @@ hipError_t launch_mg_residual(float *r, const float *v, const float *f, const MG
     return launch_mg_residual_t<float>(r, v, f, L, beta, has_los, s);
 }
 
+hipError_t launch_mg_residual_fast(float *r, const float *v, const float *f, const MGLevel &L, float beta,
+                                   int has_los, hipStream_t s) {
+    if (mg_refine_ok(L) && (int64_t)L.n[0] * L.n[1] * L.n[2] >= ((int64_t)1 << 21)) {
+        const int tz = residual_zm_tz(L);
+        const dim3 grid((unsigned)(L.n[0] / kRfTX), (unsigned)(L.n[1] / kRfTY), (unsigned)(L.n[2] / tz));
+        hipLaunchKernelGGL(k_mg_residual_zm<1>, grid, dim3(kRfThreads), 0, s, r, v, f, L, beta, has_los, tz);
+        return hipGetLastError();
+    }
+    return launch_mg_residual_t<float>(r, v, f, L, beta, has_los, s);
+}
+
 template <typename T>
 hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, float beta,
                                 int has_los, hipStream_t s) {
@@ -1666,7 +1683,7 @@ hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, 
         if (mg_refine_ok(L) && (int64_t)L.n[0] * L.n[1] * L.n[2] >= ((int64_t)1 << 21)) {
             const int tz = residual_zm_tz(L);
             const dim3 grid((unsigned)(L.n[0] / kRfTX), (unsigned)(L.n[1] / kRfTY), (unsigned)(L.n[2] / tz));
-            hipLaunchKernelGGL(k_mg_residual_zm, grid, dim3(kRfThreads), 0, s, r, v, f, L, beta, has_los, tz);
+            hipLaunchKernelGGL(k_mg_residual_zm<0>, grid, dim3(kRfThreads), 0, s, r, v, f, L, beta, has_los, tz);
             return hipGetLastError();
         }
     }
@@ -1678,7 +1695,7 @@ hipError_t launch_mg_residual_t(T *r, const T *v, const T *f, const MGLevel &L, 
 
 // coarse planes per workgroup of k_mg_residual_restrict (2 tzc + 4 steps)
 static int rr_tzc(const MGLevel &L) {
-    static const int per_cu = blocks_per_cu(k_mg_residual_restrict, kRrThreads);
+    static const int per_cu = blocks_per_cu(k_mg_residual_restrict<0>, kRrThreads);
     return zm_tz((int64_t)(L.n[0] / (2 * kRrCX)) * (L.n[1] / (2 * kRrCY)), L.n[2] / 2, kRrMaxTzc, 1, per_cu, 2);
 }
 
@@ -1688,11 +1705,12 @@ bool mg_residual_restrict_ok(const MGLevel &L) {
 }
 
 hipError_t launch_mg_residual_restrict(float *coarse, const float *v, const float *f, const MGLevel &L, float beta,
-                                       int has_los, hipStream_t s) {
+                                       int has_los, hipStream_t s, int fast) {
     if (!mg_residual_restrict_ok(L)) return hipErrorInvalidValue;
     const int tzc = rr_tzc(L);
     const dim3 grid((unsigned)(L.n[0] / (2 * kRrCX)), (unsigned)(L.n[1] / (2 * kRrCY)), (unsigned)(L.n[2] / 2 / tzc));
-    hipLaunchKernelGGL(k_mg_residual_restrict, grid, dim3(kRrThreads), 0, s, coarse, v, f, L, beta, has_los, tzc);
+    hipLaunchKernelGGL(fast ? k_mg_residual_restrict<1> : k_mg_residual_restrict<0>, grid, dim3(kRrThreads), 0, s,
+                       coarse, v, f, L, beta, has_los, tzc);
     return hipGetLastError();
 }
 
